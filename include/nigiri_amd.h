@@ -1,0 +1,278 @@
+/*
+ * nigiri_amd.h — C ABI of the MI355X RAPTOR library (libnigiri_amd.so).
+ *
+ * Plain C: pointers, sizes and status codes only.  No exception, no C++ or
+ * torch type crosses this boundary.  Every function returns NIGIRI_AMD_OK (0)
+ * or a negative status; nigiri_amd_last_error() gives the message of the
+ * last failure on the calling thread.
+ *
+ * Times crossing the boundary are minutes since the unix epoch (int32),
+ * exactly nigiri's unixtime_t (include/nigiri/types.h:360).  Durations are
+ * minutes (int16, nigiri's duration_t, types.h:359).  Location indices follow
+ * nigiri's location_idx_t, including the 9 special stations at 0..8
+ * (include/nigiri/special_stations.h:7-22).
+ */
+#ifndef NIGIRI_AMD_H
+#define NIGIRI_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NIGIRI_AMD_OK 0
+#define NIGIRI_AMD_EINVAL (-1)   /* bad argument / malformed input          */
+#define NIGIRI_AMD_ENOMEM (-2)   /* device or host allocation failed        */
+#define NIGIRI_AMD_EDEVICE (-3)  /* HIP runtime error                        */
+#define NIGIRI_AMD_ECAPACITY (-4) /* request exceeds slots / state memory   */
+#define NIGIRI_AMD_EUNSUPPORTED (-5) /* query feature not supported here    */
+
+#define NIGIRI_AMD_DIR_FWD 0
+#define NIGIRI_AMD_DIR_BWD 1
+
+#define NIGIRI_AMD_MATCH_EXACT 0      /* location_match_mode::kExact      */
+#define NIGIRI_AMD_MATCH_INTERMODAL 1 /* location_match_mode::kIntermodal */
+
+/* -------------------------------------------------------------------------
+ * Timetable view: the arrays of nigiri::timetable the RAPTOR path reads
+ * (include/nigiri/timetable.h:342-398, 89-98, 421-422), flattened to CSR.
+ * All pointers are borrowed; the view never owns memory.
+ * ---------------------------------------------------------------------- */
+typedef struct nigiri_tt_view {
+  uint32_t n_locations;  /* including the 9 special stations               */
+  uint32_t n_routes;
+  uint32_t n_transports;
+  uint32_t n_bitfields;  /* each bitfield is 8 x uint64 (bitset<512>)      */
+  uint32_t n_route_stops;    /* size of route_stops                         */
+  uint32_t n_event_times;    /* size of event_times                         */
+  uint32_t n_location_routes;
+  uint32_t n_footpaths;      /* size of fp_out == size of fp_in             */
+  uint32_t n_lb_fwd, n_lb_bwd;
+  /* calendar: nigiri's date_range_ and internal interval (timetable.h:232-252)
+   * internal day 0 starts at internal_from (= date_range.from - 5 days).   */
+  int32_t date_range_from; /* unix minutes, midnight                        */
+  int32_t date_range_to;   /* unix minutes, midnight (exclusive)            */
+  int32_t internal_from;   /* unix minutes of internal day 0                */
+  uint32_t n_days;         /* internal_interval_days().size()               */
+
+  /* T1 route_location_seq_ (vecvec<route, stop>): packed stop words
+   *   bits 0..27 location, 28 in_allowed, 29 out_allowed,
+   *   30 in_allowed_wheelchair, 31 out_allowed_wheelchair (stop.h:85-89)   */
+  const uint32_t* route_stop_offsets;  /* [n_routes + 1]                   */
+  const uint32_t* route_stops;         /* [n_route_stops]                  */
+  /* T2 route_transport_ranges_, route_stop_time_ranges_, route_stop_times_
+   *   transports of route r are [route_transport_from[r], ...[r+1])
+   *   event block of route r starts at route_stop_time_from[r]; stop-event
+   *   block j (= 2*stop_idx - is_arrival) holds one delta per transport.
+   *   delta packing (types.h:630-631): bits 0..4 days, bits 5..15 mam.     */
+  const uint32_t* route_transport_from; /* [n_routes + 1]                  */
+  const uint32_t* route_stop_time_from; /* [n_routes + 1]                  */
+  const uint16_t* event_times;          /* [n_event_times]                 */
+  /* T3 traffic days: bitfield index per transport / route, 512-bit sets    */
+  const uint32_t* transport_traffic_days; /* [n_transports]                */
+  const uint32_t* route_traffic_days;     /* [n_routes]                    */
+  const uint64_t* bitfields;              /* [n_bitfields * 8]             */
+  /* T4 location_routes_                                                    */
+  const uint32_t* location_route_offsets; /* [n_locations + 1]             */
+  const uint32_t* location_routes;        /* [n_location_routes]           */
+  /* T5 transfer_time_ (u8 minutes) and footpaths_out_/in_[profile 0];
+   *   footpath packing (footpath.h:65-66): bits 0..22 target, 23..31 min.  */
+  const uint8_t* transfer_time;           /* [n_locations]                 */
+  const uint32_t* fp_out_offsets;         /* [n_locations + 1]             */
+  const uint32_t* fp_out;                 /* [n_footpaths]                 */
+  const uint32_t* fp_in_offsets;          /* [n_locations + 1]             */
+  const uint32_t* fp_in;                  /* [n_footpaths]                 */
+  /* lower-bound graphs fwd_/bwd_search_lb_graph_ (timetable.h:421-422),
+   * same footpath packing                                                  */
+  const uint32_t* lb_fwd_offsets;         /* [n_locations + 1]             */
+  const uint32_t* lb_fwd;                 /* [n_lb_fwd]                    */
+  const uint32_t* lb_bwd_offsets;         /* [n_locations + 1]             */
+  const uint32_t* lb_bwd;                 /* [n_lb_bwd]                    */
+  /* planar stop coordinates in metres (synthetic timetables only; used to
+   * draw intermodal offsets).  May be NULL.                                */
+  const float* coord_x;                   /* [n_locations]                 */
+  const float* coord_y;                   /* [n_locations]                 */
+} nigiri_tt_view;
+
+/* -------------------------------------------------------------------------
+ * Query: the subset of nigiri::routing::query (include/nigiri/routing/
+ * query.h:70-101) that the static RAPTOR path supports (no vias, no
+ * td offsets, no real-time).  Offsets are (location, duration) pairs.
+ * ---------------------------------------------------------------------- */
+typedef struct nigiri_query {
+  int32_t dir;               /* NIGIRI_AMD_DIR_FWD / _BWD                    */
+  int32_t start_match_mode;  /* NIGIRI_AMD_MATCH_*                           */
+  int32_t dest_match_mode;
+  int32_t use_start_footpaths;
+  int32_t is_interval;       /* 0: start_time_ is a point (ontrip)           */
+  int32_t start_time;        /* unix minutes (ontrip)                        */
+  int32_t interval_from;     /* unix minutes, [from, to) (pretrip)           */
+  int32_t interval_to;
+  uint32_t n_start;
+  uint32_t n_dest;
+  const uint32_t* start_loc;
+  const int16_t* start_dur;
+  const uint32_t* dest_loc;
+  const int16_t* dest_dur;
+  int16_t max_start_offset;  /* default 7200 (kMaxTravelTime)               */
+  int16_t max_travel_time;   /* default 7200                                */
+  uint8_t max_transfers;     /* default 14 (kMaxTransfers)                  */
+  uint8_t pad_[3];
+  uint32_t min_connection_count; /* only 0 is supported                     */
+  int32_t extend_interval_earlier;
+  int32_t extend_interval_later;
+  int32_t prf_idx;               /* only 0 is supported                     */
+  int32_t tts_default;           /* transfer_time_settings::default_        */
+  int16_t tts_min_transfer_time;
+  int16_t tts_additional_time;
+  float tts_factor;
+  int32_t has_fastest_direct;    /* query::fastest_direct_ (optional)       */
+  int16_t fastest_direct;
+  int16_t pad2_;
+  double fastest_direct_factor;  /* default 1.0                             */
+} nigiri_query;
+
+/* A journey of the result Pareto set (journey.h:135-141, legs omitted). */
+typedef struct nigiri_journey {
+  int32_t start_time;  /* unix minutes */
+  int32_t dest_time;   /* unix minutes */
+  uint32_t dest;       /* location index (END = 1 for intermodal)           */
+  uint8_t transfers;
+  uint8_t pad_[3];
+} nigiri_journey;
+
+/* Per-query outcome of a batched search. */
+typedef struct nigiri_result {
+  int32_t status;          /* NIGIRI_AMD_OK or a negative status            */
+  uint32_t n_journeys;     /* journeys written (<= journey_capacity)        */
+  uint32_t journey_offset; /* index of the first journey in the out array   */
+  uint32_t n_epochs;       /* start-time groups executed                    */
+  uint32_t n_rounds;       /* RAPTOR rounds executed, summed over epochs    */
+  int32_t interval_from;   /* search interval actually searched             */
+  int32_t interval_to;
+  int16_t fastest_direct;
+  int16_t base_day;
+  double latency_ms;       /* admission -> completion (served path)         */
+} nigiri_result;
+
+typedef struct nigiri_amd_tt nigiri_amd_tt;         /* host timetable       */
+typedef struct nigiri_amd_image nigiri_amd_image;   /* device timetable     */
+typedef struct nigiri_amd_engine nigiri_amd_engine; /* device query state   */
+
+const char* nigiri_amd_last_error(void);
+int nigiri_amd_version(void);
+
+/* ---- host timetable --------------------------------------------------- */
+
+/* Seeded synthetic timetable for workload `cfg` (1..5).  `scale` in (0, 1]
+ * shrinks stop/route/transport counts for quick tests (1.0 = full size).
+ * Replaces reading nigiri's tt.bin (src/timetable.cc:275-281) for the
+ * benchmark workloads; invariants follow the loader (FIFO routes,
+ * gtfs/load_timetable.cc:291-301; route traffic days, init_finish.cc:
+ * 168-198; footpath order, build_footpaths.cc:404-414; lb graph,
+ * loader/build_lb_graph.h:11-99). */
+int nigiri_amd_tt_generate(int cfg, uint64_t seed, double scale,
+                           nigiri_amd_tt** out);
+/* Builds a timetable from caller arrays.  Only the route, transport,
+ * bitfield, transfer-time and footpath arrays of `v` are read; location
+ * routes, route traffic days, footpaths_in and the lb graphs are rebuilt as
+ * nigiri's finalize() does (src/loader/init_finish.cc:200-234). */
+int nigiri_amd_tt_from_arrays(const nigiri_tt_view* v, nigiri_amd_tt** out);
+int nigiri_amd_tt_view(const nigiri_amd_tt* tt, nigiri_tt_view* out);
+void nigiri_amd_tt_free(nigiri_amd_tt* tt);
+
+/* Seeded query workload for `cfg` (point / range / intermodal mix as the
+ * workload defines).  Offsets are written into caller arrays sized by
+ * nigiri_amd_queries_generate(...) with out == NULL (returns the number of
+ * offset entries needed in *n_offsets). */
+int nigiri_amd_queries_generate(const nigiri_amd_tt* tt, int cfg,
+                                uint64_t seed, uint32_t n_queries,
+                                nigiri_query* out, uint32_t* offset_loc,
+                                int16_t* offset_dur, uint32_t* n_offsets);
+
+/* ---- device image (replaces gpu_timetable, gpu/raptor.h:27-33) -------- */
+
+int nigiri_amd_image_create(const nigiri_amd_tt* tt, int device,
+                            nigiri_amd_image** out);
+int nigiri_amd_image_bytes(const nigiri_amd_image* img, uint64_t* bytes);
+void nigiri_amd_image_destroy(nigiri_amd_image* img);
+
+/* ---- engine: device query state (replaces gpu_raptor_state,
+ * gpu/raptor.h:47-53, and raptor_state, raptor/raptor_state.h:22-111) ----
+ *   lanes      : 64 for the batched engine (64 queries share a wavefront),
+ *                1 for the plain one-query-per-launch engine
+ *   n_groups   : number of 64-query groups (lanes == 64) or of independent
+ *                single-query states (lanes == 1)
+ *   max_k      : rows of round_times (end_k = min(max_transfers,14)+2 <=
+ *                max_k <= 16)                                              */
+int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes,
+                             uint32_t n_groups, uint32_t max_k,
+                             nigiri_amd_engine** out);
+int nigiri_amd_engine_state_bytes(const nigiri_amd_engine* e, uint64_t* b);
+void nigiri_amd_engine_destroy(nigiri_amd_engine* e);
+
+/* Full search of n queries (replaces routing::raptor_search,
+ * include/nigiri/routing/raptor_search.h:9-17, and search<>::execute,
+ * search.h:204-345).  Journeys of query i are written to
+ * journeys[res[i].journey_offset ...]; journey_capacity bounds the total.
+ * Queries run batched: up to lanes * n_groups at once, epoch-synchronous. */
+int nigiri_amd_search(nigiri_amd_engine* e, const nigiri_query* qs,
+                      uint32_t n, nigiri_result* res, nigiri_journey* journeys,
+                      uint32_t journey_capacity);
+
+/* ---- Algo-level entry points for one slot: the method set of
+ * raptor<> (raptor.h:174-414) / gpu_raptor<> (gpu/raptor.h:83-99) that
+ * search<Dir, Algo> drives.  `slot` < lanes * n_groups. ---------------- */
+typedef struct nigiri_algo_init {
+  int32_t dir;
+  int32_t base_day;          /* day_idx_t base (search.h:158-164)           */
+  uint32_t n_dest;           /* is_dest_ set bits (non-intermodal)          */
+  const uint32_t* dest;      /* ascending                                   */
+  int32_t intermodal;        /* dist_to_dest non-empty                      */
+  uint32_t n_dist;           /* intermodal: reachable (location, minutes)   */
+  const uint32_t* dist_loc;  /* ascending                                   */
+  const uint16_t* dist_min;
+  const uint16_t* lb;        /* [n_locations] travel-time lower bounds      */
+  int32_t tts_default;
+  int16_t tts_min_transfer_time;
+  int16_t tts_additional_time;
+  float tts_factor;
+} nigiri_algo_init;
+
+int nigiri_amd_algo_init(nigiri_amd_engine* e, uint32_t slot,
+                         const nigiri_algo_init* init);
+int nigiri_amd_algo_reset_arrivals(nigiri_amd_engine* e, uint32_t slot);
+int nigiri_amd_algo_next_start_time(nigiri_amd_engine* e, uint32_t slot);
+int nigiri_amd_algo_add_start(nigiri_amd_engine* e, uint32_t slot,
+                              uint32_t location, int32_t unixtime);
+/* Runs the rounds of one epoch; writes dest round times into
+ * dest_times[n_dest_bits * max_k] (row k of dest j at [j*max_k + k]) as
+ * deltas; journey extraction (raptor.h:393-413) happens in the caller. */
+int nigiri_amd_algo_execute(nigiri_amd_engine* e, uint32_t slot,
+                            int32_t start_time, uint8_t max_transfers,
+                            int32_t worst_time_at_dest, int16_t* dest_times,
+                            uint32_t* n_rounds);
+/* Copies the slot's state to host arrays (any pointer may be NULL):
+ * round_times[max_k][n_locations], best[n_locations], tmp[n_locations],
+ * time_at_dest[max_k]. */
+int nigiri_amd_algo_download(nigiri_amd_engine* e, uint32_t slot,
+                             int16_t* round_times, int16_t* best,
+                             int16_t* tmp, int16_t* time_at_dest);
+
+/* ---- serving loop ------------------------------------------------------ */
+/* Runs a continuous-batching server over the engine: query i becomes
+ * visible at arrival_ms[i] (relative to the start), is admitted when a slot
+ * and state memory are free, and completes with res[i].latency_ms set.
+ * Reports through res/journeys exactly like nigiri_amd_search. */
+int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs,
+                     const double* arrival_ms, uint32_t n,
+                     nigiri_result* res, nigiri_journey* journeys,
+                     uint32_t journey_capacity, double* wall_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NIGIRI_AMD_H */

@@ -1,0 +1,424 @@
+"""nigiri_amd — MI355X RAPTOR for nigiri's earliest-arrival and range queries.
+
+Python mirror of the C ABI in include/nigiri_amd.h (ctypes).  The compute
+path is libnigiri_amd.so (HIP kernels for gfx950 + host C++); this module
+only marshals arguments.  Importing it does not touch the GPU; creating an
+Image does.  If the shared library is missing every entry point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnigiri_amd.so")
+
+OK, EINVAL, ENOMEM, EDEVICE, ECAPACITY, EUNSUPPORTED = 0, -1, -2, -3, -4, -5
+DIR_FWD, DIR_BWD = 0, 1
+MATCH_EXACT, MATCH_INTERMODAL = 0, 1
+N_SPECIAL = 9
+END = 1
+
+
+class NigiriError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"nigiri_amd error {code}: {msg}")
+        self.code = code
+
+
+class TtView(C.Structure):
+    _fields_ = [
+        ("n_locations", C.c_uint32), ("n_routes", C.c_uint32), ("n_transports", C.c_uint32),
+        ("n_bitfields", C.c_uint32), ("n_route_stops", C.c_uint32), ("n_event_times", C.c_uint32),
+        ("n_location_routes", C.c_uint32), ("n_footpaths", C.c_uint32),
+        ("n_lb_fwd", C.c_uint32), ("n_lb_bwd", C.c_uint32),
+        ("date_range_from", C.c_int32), ("date_range_to", C.c_int32),
+        ("internal_from", C.c_int32), ("n_days", C.c_uint32),
+        ("route_stop_offsets", C.POINTER(C.c_uint32)), ("route_stops", C.POINTER(C.c_uint32)),
+        ("route_transport_from", C.POINTER(C.c_uint32)),
+        ("route_stop_time_from", C.POINTER(C.c_uint32)),
+        ("event_times", C.POINTER(C.c_uint16)),
+        ("transport_traffic_days", C.POINTER(C.c_uint32)),
+        ("route_traffic_days", C.POINTER(C.c_uint32)), ("bitfields", C.POINTER(C.c_uint64)),
+        ("location_route_offsets", C.POINTER(C.c_uint32)),
+        ("location_routes", C.POINTER(C.c_uint32)),
+        ("transfer_time", C.POINTER(C.c_uint8)),
+        ("fp_out_offsets", C.POINTER(C.c_uint32)), ("fp_out", C.POINTER(C.c_uint32)),
+        ("fp_in_offsets", C.POINTER(C.c_uint32)), ("fp_in", C.POINTER(C.c_uint32)),
+        ("lb_fwd_offsets", C.POINTER(C.c_uint32)), ("lb_fwd", C.POINTER(C.c_uint32)),
+        ("lb_bwd_offsets", C.POINTER(C.c_uint32)), ("lb_bwd", C.POINTER(C.c_uint32)),
+        ("coord_x", C.POINTER(C.c_float)), ("coord_y", C.POINTER(C.c_float)),
+    ]
+
+
+class Query(C.Structure):
+    _fields_ = [
+        ("dir", C.c_int32), ("start_match_mode", C.c_int32), ("dest_match_mode", C.c_int32),
+        ("use_start_footpaths", C.c_int32), ("is_interval", C.c_int32),
+        ("start_time", C.c_int32), ("interval_from", C.c_int32), ("interval_to", C.c_int32),
+        ("n_start", C.c_uint32), ("n_dest", C.c_uint32),
+        ("start_loc", C.POINTER(C.c_uint32)), ("start_dur", C.POINTER(C.c_int16)),
+        ("dest_loc", C.POINTER(C.c_uint32)), ("dest_dur", C.POINTER(C.c_int16)),
+        ("max_start_offset", C.c_int16), ("max_travel_time", C.c_int16),
+        ("max_transfers", C.c_uint8), ("pad_", C.c_uint8 * 3),
+        ("min_connection_count", C.c_uint32), ("extend_interval_earlier", C.c_int32),
+        ("extend_interval_later", C.c_int32), ("prf_idx", C.c_int32),
+        ("tts_default", C.c_int32), ("tts_min_transfer_time", C.c_int16),
+        ("tts_additional_time", C.c_int16), ("tts_factor", C.c_float),
+        ("has_fastest_direct", C.c_int32), ("fastest_direct", C.c_int16), ("pad2_", C.c_int16),
+        ("fastest_direct_factor", C.c_double),
+    ]
+
+
+class Journey(C.Structure):
+    _fields_ = [("start_time", C.c_int32), ("dest_time", C.c_int32), ("dest", C.c_uint32),
+                ("transfers", C.c_uint8), ("pad_", C.c_uint8 * 3)]
+
+
+class Result(C.Structure):
+    _fields_ = [("status", C.c_int32), ("n_journeys", C.c_uint32), ("journey_offset", C.c_uint32),
+                ("n_epochs", C.c_uint32), ("n_rounds", C.c_uint32),
+                ("interval_from", C.c_int32), ("interval_to", C.c_int32),
+                ("fastest_direct", C.c_int16), ("base_day", C.c_int16), ("latency_ms", C.c_double)]
+
+
+class AlgoInit(C.Structure):
+    _fields_ = [("dir", C.c_int32), ("base_day", C.c_int32), ("n_dest", C.c_uint32),
+                ("dest", C.POINTER(C.c_uint32)), ("intermodal", C.c_int32),
+                ("n_dist", C.c_uint32), ("dist_loc", C.POINTER(C.c_uint32)),
+                ("dist_min", C.POINTER(C.c_uint16)), ("lb", C.POINTER(C.c_uint16)),
+                ("tts_default", C.c_int32), ("tts_min_transfer_time", C.c_int16),
+                ("tts_additional_time", C.c_int16), ("tts_factor", C.c_float)]
+
+
+# every entry point of include/nigiri_amd.h: name -> (restype, argtypes)
+_P = C.c_void_p
+SIGNATURES = {
+    "nigiri_amd_last_error": (C.c_char_p, []),
+    "nigiri_amd_version": (C.c_int, []),
+    "nigiri_amd_tt_generate": (C.c_int, [C.c_int, C.c_uint64, C.c_double, C.POINTER(_P)]),
+    "nigiri_amd_tt_from_arrays": (C.c_int, [C.POINTER(TtView), C.POINTER(_P)]),
+    "nigiri_amd_tt_view": (C.c_int, [_P, C.POINTER(TtView)]),
+    "nigiri_amd_tt_free": (None, [_P]),
+    "nigiri_amd_queries_generate": (C.c_int, [_P, C.c_int, C.c_uint64, C.c_uint32,
+                                              C.POINTER(Query), C.POINTER(C.c_uint32),
+                                              C.POINTER(C.c_int16), C.POINTER(C.c_uint32)]),
+    "nigiri_amd_image_create": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
+    "nigiri_amd_image_bytes": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "nigiri_amd_image_destroy": (None, [_P]),
+    "nigiri_amd_engine_create": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
+    "nigiri_amd_engine_state_bytes": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "nigiri_amd_engine_destroy": (None, [_P]),
+    "nigiri_amd_search": (C.c_int, [_P, C.POINTER(Query), C.c_uint32, C.POINTER(Result),
+                                    C.POINTER(Journey), C.c_uint32]),
+    "nigiri_amd_algo_init": (C.c_int, [_P, C.c_uint32, C.POINTER(AlgoInit)]),
+    "nigiri_amd_algo_reset_arrivals": (C.c_int, [_P, C.c_uint32]),
+    "nigiri_amd_algo_next_start_time": (C.c_int, [_P, C.c_uint32]),
+    "nigiri_amd_algo_add_start": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_int32]),
+    "nigiri_amd_algo_execute": (C.c_int, [_P, C.c_uint32, C.c_int32, C.c_uint8, C.c_int32,
+                                          C.POINTER(C.c_int16), C.POINTER(C.c_uint32)]),
+    "nigiri_amd_algo_download": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_int16),
+                                           C.POINTER(C.c_int16), C.POINTER(C.c_int16),
+                                           C.POINTER(C.c_int16)]),
+    "nigiri_amd_serve": (C.c_int, [_P, C.POINTER(Query), C.POINTER(C.c_double), C.c_uint32,
+                                   C.POINTER(Result), C.POINTER(Journey), C.c_uint32,
+                                   C.POINTER(C.c_double)]),
+}
+
+_lib_handle = None
+
+
+def lib():
+    """Loads libnigiri_amd.so (raises if it has not been built)."""
+    global _lib_handle
+    if _lib_handle is None:
+        if not os.path.exists(LIB_PATH):
+            raise NigiriError(EINVAL, f"{LIB_PATH} missing: run nigiri_amd.build.build()")
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(h, name)
+            f.restype = res
+            f.argtypes = args
+        _lib_handle = h
+    return _lib_handle
+
+
+def _check(code):
+    if code != OK:
+        raise NigiriError(code, lib().nigiri_amd_last_error().decode())
+
+
+def ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+_VIEW_ARRAYS = {
+    # name: (dtype, length field expression)
+    "route_stop_offsets": (np.uint32, lambda v: v.n_routes + 1),
+    "route_stops": (np.uint32, lambda v: v.n_route_stops),
+    "route_transport_from": (np.uint32, lambda v: v.n_routes + 1),
+    "route_stop_time_from": (np.uint32, lambda v: v.n_routes + 1),
+    "event_times": (np.uint16, lambda v: v.n_event_times),
+    "transport_traffic_days": (np.uint32, lambda v: v.n_transports),
+    "route_traffic_days": (np.uint32, lambda v: v.n_routes),
+    "bitfields": (np.uint64, lambda v: v.n_bitfields * 8),
+    "location_route_offsets": (np.uint32, lambda v: v.n_locations + 1),
+    "location_routes": (np.uint32, lambda v: v.n_location_routes),
+    "transfer_time": (np.uint8, lambda v: v.n_locations),
+    "fp_out_offsets": (np.uint32, lambda v: v.n_locations + 1),
+    "fp_out": (np.uint32, lambda v: v.n_footpaths),
+    "fp_in_offsets": (np.uint32, lambda v: v.n_locations + 1),
+    "fp_in": (np.uint32, lambda v: v.n_footpaths),
+    "lb_fwd_offsets": (np.uint32, lambda v: v.n_locations + 1),
+    "lb_fwd": (np.uint32, lambda v: v.n_lb_fwd),
+    "lb_bwd_offsets": (np.uint32, lambda v: v.n_locations + 1),
+    "lb_bwd": (np.uint32, lambda v: v.n_lb_bwd),
+}
+
+
+class Timetable:
+    """Host timetable (nigiri's RAPTOR arrays as CSR)."""
+
+    def __init__(self, handle):
+        self.handle = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+        self.view = TtView()
+        _check(lib().nigiri_amd_tt_view(self.handle, C.byref(self.view)))
+
+    @classmethod
+    def generate(cls, cfg, seed=None, scale=1.0):
+        """Synthetic timetable of workload `cfg` (1..5); default seed 0x6E69 + cfg."""
+        h = C.c_void_p()
+        _check(lib().nigiri_amd_tt_generate(cfg, 0x6E69 + cfg if seed is None else seed,
+                                            float(scale), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, **a):
+        """Builds a timetable from plain arrays (see nigiri_amd_tt_from_arrays)."""
+        v = TtView()
+        keep = []
+
+        def arr(name, dtype):
+            x = np.ascontiguousarray(a[name], dtype=dtype)
+            keep.append(x)
+            return x
+
+        n_loc = int(a["n_locations"])
+        rso = arr("route_stop_offsets", np.uint32)
+        v.n_locations = n_loc
+        v.n_routes = len(rso) - 1
+        v.route_stop_offsets = ptr(rso, C.c_uint32)
+        v.route_stops = ptr(arr("route_stops", np.uint32), C.c_uint32)
+        v.route_transport_from = ptr(arr("route_transport_from", np.uint32), C.c_uint32)
+        v.route_stop_time_from = ptr(arr("route_stop_time_from", np.uint32), C.c_uint32)
+        v.event_times = ptr(arr("event_times", np.uint16), C.c_uint16)
+        v.transport_traffic_days = ptr(arr("transport_traffic_days", np.uint32), C.c_uint32)
+        bf = arr("bitfields", np.uint64)
+        v.n_bitfields = len(bf) // 8
+        v.bitfields = ptr(bf, C.c_uint64)
+        v.transfer_time = ptr(arr("transfer_time", np.uint8), C.c_uint8)
+        v.fp_out_offsets = ptr(arr("fp_out_offsets", np.uint32), C.c_uint32)
+        v.fp_out = ptr(arr("fp_out", np.uint32), C.c_uint32)
+        v.date_range_from = int(a["date_range_from"])
+        v.date_range_to = int(a["date_range_to"])
+        v.internal_from = int(a["internal_from"])
+        v.n_days = int(a["n_days"])
+        h = C.c_void_p()
+        _check(lib().nigiri_amd_tt_from_arrays(C.byref(v), C.byref(h)))
+        return cls(h)
+
+    def array(self, name):
+        dtype, n = _VIEW_ARRAYS[name]
+        ctype = np.ctypeslib.as_ctypes_type(dtype)
+        p = getattr(self.view, name)
+        count = n(self.view)
+        if count == 0:
+            return np.zeros(0, dtype)
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(ctype)), shape=(count,)).copy()
+
+    @property
+    def n_locations(self):
+        return self.view.n_locations
+
+    def close(self):
+        if self.handle:
+            lib().nigiri_amd_tt_free(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Queries:
+    """A contiguous array of nigiri_query structs plus the offset storage they point into."""
+
+    def __init__(self, n):
+        self.q = (Query * max(n, 1))()
+        self.n = n
+        self._keep = []
+
+    def __len__(self):
+        return self.n
+
+    def __getitem__(self, i):
+        return self.q[i]
+
+    @classmethod
+    def generate(cls, tt, cfg, n, seed=None):
+        seed = 0x7172 + cfg if seed is None else seed
+        need = C.c_uint32(0)
+        _check(lib().nigiri_amd_queries_generate(tt.handle, cfg, seed, n, None, None, None,
+                                                 C.byref(need)))
+        self = cls(n)
+        loc = np.zeros(max(need.value, 1), np.uint32)
+        dur = np.zeros(max(need.value, 1), np.int16)
+        self._keep += [loc, dur]
+        _check(lib().nigiri_amd_queries_generate(tt.handle, cfg, seed, n, self.q,
+                                                 ptr(loc, C.c_uint32), ptr(dur, C.c_int16),
+                                                 C.byref(need)))
+        return self
+
+    @classmethod
+    def build(cls, specs):
+        """specs: list of dicts with keys start=[(loc, dur)], dest=[(loc, dur)] and any Query field."""
+        self = cls(len(specs))
+        for i, s in enumerate(specs):
+            q = self.q[i]
+            q.dir = s.get("dir", DIR_FWD)
+            q.start_match_mode = s.get("start_match_mode", MATCH_EXACT)
+            q.dest_match_mode = s.get("dest_match_mode", MATCH_EXACT)
+            q.use_start_footpaths = int(s.get("use_start_footpaths", 0))
+            q.max_start_offset = s.get("max_start_offset", 7200)
+            q.max_travel_time = s.get("max_travel_time", 7200)
+            q.max_transfers = s.get("max_transfers", 14)
+            q.tts_default = int(s.get("tts_default", 1))
+            q.tts_min_transfer_time = s.get("tts_min_transfer_time", 0)
+            q.tts_additional_time = s.get("tts_additional_time", 0)
+            q.tts_factor = s.get("tts_factor", 1.0)
+            q.fastest_direct_factor = s.get("fastest_direct_factor", 1.0)
+            if "fastest_direct" in s:
+                q.has_fastest_direct = 1
+                q.fastest_direct = s["fastest_direct"]
+            if "interval" in s:
+                q.is_interval = 1
+                q.interval_from, q.interval_to = s["interval"]
+            else:
+                q.start_time = s["time"]
+            for which in ("start", "dest"):
+                offs = s[which]
+                loc = np.array([o[0] for o in offs], np.uint32)
+                dur = np.array([o[1] for o in offs], np.int16)
+                self._keep += [loc, dur]
+                setattr(q, "n_" + which, len(offs))
+                setattr(q, which + "_loc", ptr(loc, C.c_uint32))
+                setattr(q, which + "_dur", ptr(dur, C.c_int16))
+        return self
+
+    def subset(self, idx):
+        out = Queries(len(idx))
+        out._keep = [self]
+        for j, i in enumerate(idx):
+            out.q[j] = self.q[i]
+        return out
+
+
+class Image:
+    """Device-resident timetable (one per GPU)."""
+
+    def __init__(self, tt, device=0):
+        self.tt = tt  # the image borrows the host timetable
+        self.handle = C.c_void_p()
+        _check(lib().nigiri_amd_image_create(tt.handle, device, C.byref(self.handle)))
+
+    def bytes(self):
+        b = C.c_uint64()
+        _check(lib().nigiri_amd_image_bytes(self.handle, C.byref(b)))
+        return b.value
+
+    def close(self):
+        if self.handle:
+            lib().nigiri_amd_image_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _collect(res, jr, n):
+    out = []
+    for i in range(n):
+        r = res[i]
+        js = [(jr[r.journey_offset + j].start_time, jr[r.journey_offset + j].dest_time,
+               jr[r.journey_offset + j].dest, jr[r.journey_offset + j].transfers)
+              for j in range(r.n_journeys)]
+        out.append({"status": r.status, "journeys": js, "n_epochs": r.n_epochs,
+                    "base_day": r.base_day, "fastest_direct": r.fastest_direct,
+                    "latency_ms": r.latency_ms})
+    return out
+
+
+class Engine:
+    """Device query state: lanes=64 (batched) or 1 (one query per launch)."""
+
+    def __init__(self, image, lanes=64, groups=1, max_k=16):
+        self.image = image
+        self.lanes, self.groups, self.max_k = lanes, groups, max_k
+        self.handle = C.c_void_p()
+        _check(lib().nigiri_amd_engine_create(image.handle, lanes, groups, max_k,
+                                              C.byref(self.handle)))
+
+    @property
+    def slots(self):
+        return self.lanes * self.groups
+
+    def state_bytes(self):
+        b = C.c_uint64()
+        _check(lib().nigiri_amd_engine_state_bytes(self.handle, C.byref(b)))
+        return b.value
+
+    def search(self, queries, journey_capacity=None):
+        n = len(queries)
+        cap = journey_capacity or max(64, 64 * n)
+        res = (Result * max(n, 1))()
+        jr = (Journey * cap)()
+        _check(lib().nigiri_amd_search(self.handle, queries.q, n, res, jr, cap))
+        return _collect(res, jr, n)
+
+    def serve(self, queries, arrival_ms, journey_capacity=None):
+        n = len(queries)
+        cap = journey_capacity or max(64, 64 * n)
+        res = (Result * max(n, 1))()
+        jr = (Journey * cap)()
+        arr = np.ascontiguousarray(arrival_ms, np.float64)
+        wall = C.c_double()
+        _check(lib().nigiri_amd_serve(self.handle, queries.q, ptr(arr, C.c_double), n, res, jr,
+                                      cap, C.byref(wall)))
+        return _collect(res, jr, n), wall.value
+
+    def download(self, slot):
+        L = self.image.tt.n_locations
+        rt = np.zeros((self.max_k, L), np.int16)
+        best = np.zeros(L, np.int16)
+        tmp = np.zeros(L, np.int16)
+        tad = np.zeros(16, np.int16)
+        _check(lib().nigiri_amd_algo_download(self.handle, slot, ptr(rt, C.c_int16),
+                                              ptr(best, C.c_int16), ptr(tmp, C.c_int16),
+                                              ptr(tad, C.c_int16)))
+        return {"round_times": rt, "best": best, "tmp": tmp, "time_at_dest": tad}
+
+    def close(self):
+        if self.handle:
+            lib().nigiri_amd_engine_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

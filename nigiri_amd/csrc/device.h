@@ -1,0 +1,96 @@
+// Device-side data structures shared by the kernels and the engine host code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace nigiri_amd {
+
+constexpr int kRowsMax = 16;          // round_times rows (kMaxTransfers + 2)
+constexpr std::uint32_t kMaxDestBits = 16U;  // destination stations per query
+constexpr std::uint32_t kMaxDist = 64U;      // intermodal egress offsets
+constexpr std::uint32_t kMaxChanges = 64U;   // time_at_dest changes per phase
+constexpr std::uint32_t kMaxDestEvents = 256U;
+constexpr std::uint32_t kPosTransfer = 0U << 30;
+constexpr std::uint32_t kPosIntermodal = 1U << 30;
+constexpr std::uint32_t kPosFootpath = 2U << 30;
+constexpr std::uint16_t kUnreachableLb = 0xFFFFU;
+
+// Read-only timetable image in HBM (one allocation per array).
+struct dev_image {
+  std::uint32_t L, R, T, NB;
+  std::uint32_t const* route_stop_offsets;
+  std::uint32_t const* route_stops;
+  std::uint32_t const* route_transport_from;
+  std::uint32_t const* route_stop_time_from;
+  std::uint16_t const* event_times;
+  std::uint32_t const* transport_traffic_days;
+  std::uint32_t const* route_traffic_days;
+  std::uint64_t const* bitfields;
+  std::uint32_t const* location_route_offsets;
+  std::uint32_t const* location_routes;
+  std::uint8_t const* transfer_time;
+  std::uint32_t const* fp_out_offsets;
+  std::uint32_t const* fp_out;
+  std::uint32_t const* fp_in_offsets;
+  std::uint32_t const* fp_in;
+  std::uint32_t const* fp_in_to_out;  // in-edge index -> out-edge index
+  std::uint32_t const* fp_out_to_in;  // out-edge index -> in-edge index
+};
+
+// Per-slot parameters written by the host at admission.
+struct slot_param {
+  std::int16_t tts_min, tts_add;
+  float tts_factor;
+  std::uint16_t base;
+  std::uint8_t end_k;
+  std::uint8_t tts_default;
+  std::uint8_t intermodal;
+  std::uint8_t n_dest;   // non-intermodal destination bits
+  std::uint8_t n_dist;   // intermodal egress offsets
+  std::uint8_t pad;
+};
+
+// Device query state of an engine.  `W` lanes share one group: 64 for the
+// batched engine (one query per lane of a wavefront), 1 for the plain
+// single-query engine.  Layouts:
+//   round_times [G][K][L][W], best/tmp/lb [G][L][W]  (int16 / uint16)
+//   station marks / dest masks: W == 64 -> one uint64 lane mask per (g, l);
+//                               W == 1  -> bit l % 64 of word g*LW + l/64
+//   route marks likewise over R.
+struct dev_state {
+  std::uint32_t W, G, K, L, R;
+  std::uint32_t LW, RW;  // words per group for W == 1 bitsets
+  std::int16_t* round_times;
+  std::int16_t* best;
+  std::int16_t* tmp;
+  std::uint16_t* lb;
+  std::uint64_t* mark_t;   // marks of the transfer phase / starts
+  std::uint64_t* mark_r;   // marks of the route phase
+  std::uint64_t* route_mark;
+  std::uint64_t* dest_mask;
+  std::uint64_t* grp_active;
+  std::uint64_t* grp_any_route;
+  std::uint64_t* grp_any_tmp;
+  // work lists: packed (g << 32 | index)
+  std::uint64_t* list_t;  // stations marked in mark_t
+  std::uint64_t* list_r;  // stations marked in mark_r
+  std::uint64_t* list_route;
+  std::uint64_t* list_hz;  // hazards: (slot << 32 | location)
+  std::uint32_t* counts;   // [0] list_t, [1] list_r, [2] list_route, [3] hz, [4] overflow flags
+  std::size_t cap_t, cap_r, cap_route, cap_hz;
+  // per slot
+  slot_param* param;
+  std::int16_t* tad;       // [S][16]
+  std::int16_t* tad0;      // [S]
+  std::uint32_t* n_chg;    // [S]
+  std::uint32_t* chg_pos;  // [S][kMaxChanges]
+  std::int16_t* chg_tad;   // [S][kMaxChanges]
+  std::uint32_t* dest;     // [S][kMaxDestBits]
+  std::uint32_t* dist_loc; // [S][kMaxDist]
+  std::uint16_t* dist_min; // [S][kMaxDist]
+  std::int16_t* dest_rows; // [S][kMaxDestBits][K]
+};
+
+}  // namespace nigiri_amd

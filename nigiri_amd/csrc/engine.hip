@@ -1,0 +1,1117 @@
+// Engine host code: device timetable image, device query state, the
+// epoch-synchronous batch scheduler and the C ABI entry points that use the
+// GPU.
+//
+// Scheduling model: all running slots start a start-time epoch together
+// (next_start_time + add_start, raptor.h:179-203), then run rounds
+// k = 1, 2, ... in lock step (each slot stops at its own break or end_k),
+// then the host extracts the destination round times of every slot and
+// updates its Pareto set.  At every epoch boundary a slot either continues
+// its range query with the next start time or takes the next queued query
+// (iteration-level continuous batching).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <climits>
+#include <cstring>
+#include <deque>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "capi_util.h"
+#include "handles.h"
+#include "device.h"
+#include "kernels.h"
+#include "search_host.h"
+#include "timetable.h"
+
+
+namespace nigiri_amd {
+
+#define HIP_CHECK(expr)                                                       \
+  do {                                                                        \
+    hipError_t const err_ = (expr);                                           \
+    if (err_ != hipSuccess) {                                                 \
+      throw status_error{NIGIRI_AMD_EDEVICE,                                  \
+                         std::string{#expr ": "} + hipGetErrorString(err_)}; \
+    }                                                                         \
+  } while (false)
+
+struct dev_alloc {
+  std::vector<void*> ptrs;
+  std::uint64_t bytes = 0U;
+  template <typename T>
+  T* alloc(std::size_t n) {
+    if (n == 0U) n = 1U;
+    void* p = nullptr;
+    auto const err = hipMalloc(&p, n * sizeof(T));
+    if (err != hipSuccess) {
+      throw status_error{NIGIRI_AMD_ENOMEM, std::string{"hipMalloc: "} + hipGetErrorString(err)};
+    }
+    ptrs.push_back(p);
+    bytes += n * sizeof(T);
+    return static_cast<T*>(p);
+  }
+  template <typename T>
+  T* upload(std::vector<T> const& v, hipStream_t s) {
+    auto* p = alloc<T>(v.size());
+    if (!v.empty()) HIP_CHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return p;
+  }
+  ~dev_alloc() {
+    for (auto* p : ptrs) (void)hipFree(p);
+  }
+};
+
+template <typename T>
+struct pinned {
+  T* p = nullptr;
+  std::size_t n = 0U;
+  void reserve(std::size_t m) {
+    if (m <= n) return;
+    if (p != nullptr) (void)hipHostFree(p);
+    p = nullptr;
+    HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&p), std::max<std::size_t>(m, 1U) * sizeof(T),
+                            hipHostMallocDefault));
+    n = m;
+  }
+  ~pinned() {
+    if (p != nullptr) (void)hipHostFree(p);
+  }
+};
+
+}  // namespace nigiri_amd
+
+using namespace nigiri_amd;
+
+struct nigiri_amd_image {
+  timetable const* tt = nullptr;
+  int device = 0;
+  dev_image d{};
+  dev_alloc mem;
+};
+
+struct nigiri_amd_engine {
+  nigiri_amd_image* img = nullptr;
+  std::uint32_t W = 64U, G = 1U, K = 16U, S = 64U;
+  hipStream_t stream = nullptr;
+  dev_alloc mem;
+  dev_state st{};
+  // host mirrors of per-slot parameters
+  std::vector<slot_param> h_param;
+  std::vector<std::uint32_t> h_dest, h_dist_loc;
+  std::vector<std::uint16_t> h_dist_min;
+  std::vector<std::vector<std::uint32_t>> slot_dest_bits;  // dest mask bits set per slot
+  // device scratch for epoch setup
+  std::uint32_t* d_reset_groups = nullptr;
+  std::uint64_t* d_rt_mask = nullptr;
+  std::uint64_t* d_bt_mask = nullptr;
+  std::uint8_t* d_flags = nullptr;
+  std::int16_t* d_worst = nullptr;
+  std::uint32_t* d_sweep_groups = nullptr;
+  std::uint16_t* d_lb_staging = nullptr;
+  std::uint32_t* d_lb_slot = nullptr;
+  std::int32_t* d_lb_row = nullptr;
+  std::uint32_t* d_lb_groups = nullptr;
+  std::uint32_t lb_rows_cap = 64U;
+  // growable device buffers for starts / dest ops
+  std::uint32_t* d_start_slot = nullptr;
+  std::uint32_t* d_start_loc = nullptr;
+  std::int16_t* d_start_delta = nullptr;
+  std::size_t start_cap = 0U;
+  std::uint64_t* d_dest_ops = nullptr;
+  std::uint8_t* d_dest_set = nullptr;
+  std::size_t dest_ops_cap = 0U;
+  std::vector<void*> grow_ptrs;
+  // pinned host staging
+  pinned<std::uint16_t> h_lb;
+  pinned<std::uint64_t> h_active;
+  pinned<std::int16_t> h_rows;
+  pinned<std::uint32_t> h_counts;
+  // algo-level (single slot) pending starts
+  std::vector<std::vector<std::pair<std::uint32_t, std::int32_t>>> pending_starts;
+  std::vector<std::uint8_t> algo_new;    // reset_arrivals requested
+  std::vector<std::uint8_t> algo_fresh;  // next_start_time requested
+  std::vector<std::uint8_t> algo_dir;
+
+  ~nigiri_amd_engine() {
+    for (auto* p : grow_ptrs) (void)hipFree(p);
+    if (stream != nullptr) (void)hipStreamDestroy(stream);
+  }
+
+  template <typename T>
+  T* grow(T* old, std::size_t n) {
+    if (old != nullptr) {
+      grow_ptrs.erase(std::remove(grow_ptrs.begin(), grow_ptrs.end(), static_cast<void*>(old)), grow_ptrs.end());
+      (void)hipFree(old);
+    }
+    void* p = nullptr;
+    auto const err = hipMalloc(&p, std::max<std::size_t>(n, 1U) * sizeof(T));
+    if (err != hipSuccess) throw status_error{NIGIRI_AMD_ENOMEM, "hipMalloc (scratch)"};
+    grow_ptrs.push_back(p);
+    return static_cast<T*>(p);
+  }
+};
+
+namespace nigiri_amd {
+
+namespace {
+
+void build_image(nigiri_amd_image& img, timetable const& tt, hipStream_t s) {
+  auto& d = img.d;
+  auto& m = img.mem;
+  d.L = tt.n_locations;
+  d.R = tt.n_routes();
+  d.T = tt.n_transports();
+  d.NB = tt.n_bitfields();
+  d.route_stop_offsets = m.upload(tt.route_stop_offsets, s);
+  d.route_stops = m.upload(tt.route_stops, s);
+  d.route_transport_from = m.upload(tt.route_transport_from, s);
+  d.route_stop_time_from = m.upload(tt.route_stop_time_from, s);
+  d.event_times = m.upload(tt.event_times, s);
+  d.transport_traffic_days = m.upload(tt.transport_traffic_days, s);
+  d.route_traffic_days = m.upload(tt.route_traffic_days, s);
+  d.bitfields = m.upload(tt.bitfields, s);
+  d.location_route_offsets = m.upload(tt.location_route_offsets, s);
+  d.location_routes = m.upload(tt.location_routes, s);
+  d.transfer_time = m.upload(tt.transfer_time, s);
+  d.fp_out_offsets = m.upload(tt.fp_out_offsets, s);
+  d.fp_out = m.upload(tt.fp_out, s);
+  d.fp_in_offsets = m.upload(tt.fp_in_offsets, s);
+  d.fp_in = m.upload(tt.fp_in, s);
+  // edge maps between the two CSR directions: footpaths_in is the exact
+  // transpose of footpaths_out, so sorting both edge lists by (target,
+  // source, duration) pairs every out-edge with its in-edge
+  struct key {
+    std::uint32_t t, src, dur, idx;
+  };
+  std::vector<key> outs, ins;
+  outs.reserve(tt.fp_out.size());
+  ins.reserve(tt.fp_in.size());
+  for (auto l = 0U; l != tt.n_locations; ++l) {
+    for (auto e = tt.fp_out_offsets[l]; e != tt.fp_out_offsets[l + 1U]; ++e) {
+      outs.push_back({fp_target(tt.fp_out[e]), l, fp_duration(tt.fp_out[e]), e});
+    }
+    for (auto f = tt.fp_in_offsets[l]; f != tt.fp_in_offsets[l + 1U]; ++f) {
+      ins.push_back({l, fp_target(tt.fp_in[f]), fp_duration(tt.fp_in[f]), f});
+    }
+  }
+  auto const by_key = [](key const& a, key const& b) {
+    return std::tie(a.t, a.src, a.dur, a.idx) < std::tie(b.t, b.src, b.dur, b.idx);
+  };
+  std::sort(outs.begin(), outs.end(), by_key);
+  std::sort(ins.begin(), ins.end(), by_key);
+  if (outs.size() != ins.size()) {
+    throw status_error{NIGIRI_AMD_EINVAL, "footpaths_in is not the transpose of footpaths_out"};
+  }
+  std::vector<std::uint32_t> in_to_out(ins.size()), out_to_in(outs.size());
+  for (std::size_t i = 0; i != outs.size(); ++i) {
+    if (outs[i].t != ins[i].t || outs[i].src != ins[i].src || outs[i].dur != ins[i].dur) {
+      throw status_error{NIGIRI_AMD_EINVAL, "footpaths_in is not the transpose of footpaths_out"};
+    }
+    out_to_in[outs[i].idx] = ins[i].idx;
+    in_to_out[ins[i].idx] = outs[i].idx;
+  }
+  d.fp_in_to_out = m.upload(in_to_out, s);
+  d.fp_out_to_in = m.upload(out_to_in, s);
+}
+
+std::int16_t clamp16(long long v) {
+  return static_cast<std::int16_t>(std::clamp<long long>(v, SHRT_MIN, SHRT_MAX));
+}
+
+// ---------------------------------------------------------------------------
+// batch scheduler
+
+struct slot_info {
+  std::int32_t q = -1;
+  std::uint32_t epoch = 0U;
+  bool running = false;
+  bool is_new = false;
+};
+
+struct runner {
+  nigiri_amd_engine& e;
+  timetable const& tt;
+  bool fwd;
+  std::vector<prepared_query>& pq;
+  std::function<std::int32_t()> next;            // next query id or -1
+  std::function<void(std::int32_t)> on_done;      // query finished
+  std::vector<slot_info> slots;
+  std::uint64_t rounds = 0U, epochs = 0U;
+
+  runner(nigiri_amd_engine& eng, timetable const& t, bool f, std::vector<prepared_query>& p)
+      : e{eng}, tt{t}, fwd{f}, pq{p}, slots(eng.S) {}
+
+  round_args rargs(std::int32_t k) const {
+    round_args a{};
+    a.im = e.img->d;
+    a.st = e.st;
+    a.k = k;
+    a.invalid = fwd ? SHRT_MAX : SHRT_MIN;
+    return a;
+  }
+
+  // one epoch step over all slots; returns false when nothing ran
+  bool step() {
+    auto const S = e.S, W = e.W, G = e.G, K = e.K;
+    std::vector<std::uint64_t> rt_mask(G, 0U), bt_mask(G, 0U);
+    std::vector<std::uint8_t> flags(S, 0U);
+    std::vector<std::int16_t> worst(S, 0);
+    std::vector<std::uint32_t> st_slot, st_loc;
+    std::vector<std::int16_t> st_delta;
+    std::vector<std::uint64_t> dest_ops;
+    std::vector<std::uint8_t> dest_set;
+    std::vector<std::uint32_t> new_slots;
+    bool any = false;
+    std::uint32_t max_end_k = 0U;
+    std::vector<std::uint8_t> sweep(G, 0U);
+    for (std::uint32_t s = 0; s != S; ++s) {
+      auto& si = slots[s];
+      si.is_new = false;
+      si.running = false;
+      if (si.q >= 0 && si.epoch + 1U < pq[static_cast<std::size_t>(si.q)].epochs_.size()) {
+        ++si.epoch;
+      } else {
+        si.q = -1;
+        while (true) {
+          auto const q = next();
+          if (q < 0) break;
+          auto& p = pq[static_cast<std::size_t>(q)];
+          if (p.epochs_.empty()) {  // overlap or no start labels
+            finish(p);
+            on_done(q);
+            continue;
+          }
+          si.q = q;
+          si.epoch = 0U;
+          si.is_new = true;
+          break;
+        }
+        if (si.q < 0) continue;
+      }
+      si.running = true;
+      any = true;
+      auto& p = pq[static_cast<std::size_t>(si.q)];
+      auto const g = s / W, lane = s % W;
+      bt_mask[g] |= 1ULL << lane;
+      if (si.is_new) {
+        rt_mask[g] |= 1ULL << lane;
+        new_slots.push_back(s);
+        // slot parameters
+        auto& sp = e.h_param[s];
+        sp.tts_min = p.tts_.min_transfer_time_;
+        sp.tts_add = p.tts_.additional_time_;
+        sp.tts_factor = p.tts_.factor_;
+        sp.tts_default = p.tts_.default_ ? 1U : 0U;
+        sp.base = p.base_;
+        sp.end_k = static_cast<std::uint8_t>(std::min<std::uint32_t>(p.end_k_, K));
+        sp.intermodal = p.intermodal_dest_ ? 1U : 0U;
+        if (p.dests_.size() > kMaxDestBits || p.dist_loc_.size() > kMaxDist) {
+          throw status_error{NIGIRI_AMD_EUNSUPPORTED, "too many destination offsets"};
+        }
+        sp.n_dest = static_cast<std::uint8_t>(p.dests_.size());
+        sp.n_dist = static_cast<std::uint8_t>(p.dist_loc_.size());
+        for (std::size_t j = 0; j != p.dests_.size(); ++j) e.h_dest[s * kMaxDestBits + j] = p.dests_[j];
+        for (std::size_t j = 0; j != p.dist_loc_.size(); ++j) {
+          e.h_dist_loc[s * kMaxDist + j] = p.dist_loc_[j];
+          e.h_dist_min[s * kMaxDist + j] = p.dist_min_[j];
+        }
+        for (auto const l : e.slot_dest_bits[s]) {
+          dest_ops.push_back((static_cast<std::uint64_t>(s) << 32) | l);
+          dest_set.push_back(0U);
+        }
+        e.slot_dest_bits[s].clear();
+        if (!p.intermodal_dest_) {
+          for (auto const l : p.dests_) {
+            dest_ops.push_back((static_cast<std::uint64_t>(s) << 32) | l);
+            dest_set.push_back(1U);
+            e.slot_dest_bits[s].push_back(l);
+          }
+        }
+      } else {
+        sweep[g] = 1U;  // later epoch: round_times rows hold earlier labels
+      }
+      auto const& ep = p.epochs_[si.epoch];
+      auto const base = p.base_minutes(tt);
+      flags[s] = static_cast<std::uint8_t>(1U | (si.is_new ? 2U : 0U));
+      worst[s] = clamp16(static_cast<long long>(ep.worst_time_at_dest_) - base);
+      for (std::size_t j = 0; j != ep.loc_.size(); ++j) {
+        st_slot.push_back(s);
+        st_loc.push_back(ep.loc_[j]);
+        st_delta.push_back(clamp16(static_cast<long long>(ep.time_[j]) - base));
+      }
+      max_end_k = std::max<std::uint32_t>(max_end_k, std::min<std::uint32_t>(p.end_k_, K));
+    }
+    if (!any) return false;
+    ++epochs;
+    auto const stream = e.stream;
+    auto const& tr = e.st;
+
+    // parameters
+    HIP_CHECK(hipMemcpyAsync(tr.param, e.h_param.data(), sizeof(slot_param) * S, hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(tr.dest, e.h_dest.data(), sizeof(std::uint32_t) * e.h_dest.size(), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(tr.dist_loc, e.h_dist_loc.data(), sizeof(std::uint32_t) * e.h_dist_loc.size(), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(tr.dist_min, e.h_dist_min.data(), sizeof(std::uint16_t) * e.h_dist_min.size(), hipMemcpyHostToDevice, stream));
+
+    epoch_args a{};
+    a.st = e.st;
+    a.invalid = fwd ? SHRT_MAX : SHRT_MIN;
+    std::vector<std::uint32_t> reset_groups;
+    for (std::uint32_t g = 0; g != G; ++g) {
+      if (bt_mask[g] != 0U) reset_groups.push_back(g);
+    }
+    HIP_CHECK(hipMemcpyAsync(e.d_reset_groups, reset_groups.data(), sizeof(std::uint32_t) * reset_groups.size(), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(e.d_rt_mask, rt_mask.data(), sizeof(std::uint64_t) * G, hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(e.d_bt_mask, bt_mask.data(), sizeof(std::uint64_t) * G, hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(e.d_flags, flags.data(), S, hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(e.d_worst, worst.data(), sizeof(std::int16_t) * S, hipMemcpyHostToDevice, stream));
+    a.reset_groups = e.d_reset_groups;
+    a.n_reset_groups = static_cast<std::uint32_t>(reset_groups.size());
+    a.reset_rt_mask = e.d_rt_mask;
+    a.reset_bt_mask = e.d_bt_mask;
+    if (st_slot.size() > e.start_cap) {
+      e.start_cap = st_slot.size() * 2U;
+      e.d_start_slot = e.grow(e.d_start_slot, e.start_cap);
+      e.d_start_loc = e.grow(e.d_start_loc, e.start_cap);
+      e.d_start_delta = e.grow(e.d_start_delta, e.start_cap);
+    }
+    if (dest_ops.size() > e.dest_ops_cap) {
+      e.dest_ops_cap = dest_ops.size() * 2U;
+      e.d_dest_ops = e.grow(e.d_dest_ops, e.dest_ops_cap);
+      e.d_dest_set = e.grow(e.d_dest_set, e.dest_ops_cap);
+    }
+    if (!st_slot.empty()) {
+      HIP_CHECK(hipMemcpyAsync(e.d_start_slot, st_slot.data(), 4U * st_slot.size(), hipMemcpyHostToDevice, stream));
+      HIP_CHECK(hipMemcpyAsync(e.d_start_loc, st_loc.data(), 4U * st_loc.size(), hipMemcpyHostToDevice, stream));
+      HIP_CHECK(hipMemcpyAsync(e.d_start_delta, st_delta.data(), 2U * st_delta.size(), hipMemcpyHostToDevice, stream));
+    }
+    if (!dest_ops.empty()) {
+      HIP_CHECK(hipMemcpyAsync(e.d_dest_ops, dest_ops.data(), 8U * dest_ops.size(), hipMemcpyHostToDevice, stream));
+      HIP_CHECK(hipMemcpyAsync(e.d_dest_set, dest_set.data(), dest_set.size(), hipMemcpyHostToDevice, stream));
+    }
+    a.start_slot = e.d_start_slot;
+    a.start_loc = e.d_start_loc;
+    a.start_delta = e.d_start_delta;
+    a.n_starts = static_cast<std::uint32_t>(st_slot.size());
+    a.dest_ops = e.d_dest_ops;
+    a.dest_set = e.d_dest_set;
+    a.n_dest_ops = static_cast<std::uint32_t>(dest_ops.size());
+    a.slot_flags = e.d_flags;
+    a.slot_worst = e.d_worst;
+    a.n_slots = S;
+
+    // lower bounds of new slots, in chunks of lb_rows_cap staging rows
+    upload_lbs(new_slots, a);
+
+    launch_epoch_begin(fwd, W, a, stream);
+    HIP_CHECK(hipGetLastError());
+
+    std::vector<std::uint32_t> sweep_groups;
+    for (std::uint32_t g = 0; g != G; ++g) {
+      if (sweep[g] != 0U) sweep_groups.push_back(g);
+    }
+    if (!sweep_groups.empty()) {
+      HIP_CHECK(hipMemcpyAsync(e.d_sweep_groups, sweep_groups.data(), 4U * sweep_groups.size(), hipMemcpyHostToDevice, stream));
+    }
+    int const grid_waves = 2048;
+    for (std::uint32_t k = 1U; k < max_end_k; ++k) {
+      launch_round(fwd, W, rargs(static_cast<std::int32_t>(k)), e.d_sweep_groups,
+                   static_cast<std::uint32_t>(sweep_groups.size()), S, grid_waves, stream);
+      HIP_CHECK(hipGetLastError());
+      ++rounds;
+      HIP_CHECK(hipMemcpyAsync(e.h_active.p, tr.grp_active, 8U * G, hipMemcpyDeviceToHost, stream));
+      HIP_CHECK(hipStreamSynchronize(stream));
+      bool alive = false;
+      for (std::uint32_t g = 0; g != G; ++g) alive |= e.h_active.p[g] != 0U;
+      if (!alive) break;
+    }
+    launch_epoch_end(W, tr, S, stream);
+    HIP_CHECK(hipGetLastError());
+    auto const rows = static_cast<std::size_t>(S) * kMaxDestBits * K;
+    HIP_CHECK(hipMemcpyAsync(e.h_rows.p, tr.dest_rows, 2U * rows, hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipMemcpyAsync(e.h_counts.p, tr.counts, 4U * 8U, hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    if (e.h_counts.p[4] != 0U) {
+      throw status_error{NIGIRI_AMD_ECAPACITY,
+                         "device work list overflow (flags " + std::to_string(e.h_counts.p[4]) + ")"};
+    }
+    // reset the overflow word for the next epoch
+    std::uint32_t zero = 0U;
+    HIP_CHECK(hipMemcpyAsync(tr.counts + 4, &zero, 4U, hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+
+    for (std::uint32_t s = 0; s != S; ++s) {
+      auto& si = slots[s];
+      if (!si.running) continue;
+      auto& p = pq[static_cast<std::size_t>(si.q)];
+      add_epoch_results(tt, p, si.epoch, e.h_rows.p + static_cast<std::size_t>(s) * kMaxDestBits * K, K);
+      if (si.epoch + 1U == p.epochs_.size()) {
+        finish(p);
+        on_done(si.q);
+      }
+    }
+    return true;
+  }
+
+  void upload_lbs(std::vector<std::uint32_t> const& new_slots, epoch_args& a) {
+    if (new_slots.empty()) return;
+    auto const L = e.st.L;
+    auto const stream = e.stream;
+    std::vector<std::int32_t> row_of(e.S, -1);
+    for (std::size_t from = 0; from < new_slots.size(); from += e.lb_rows_cap) {
+      auto const to = std::min<std::size_t>(new_slots.size(), from + e.lb_rows_cap);
+      HIP_CHECK(hipStreamSynchronize(stream));  // staging buffer reuse
+      std::vector<std::uint32_t> slots_chunk(new_slots.begin() + static_cast<std::ptrdiff_t>(from),
+                                             new_slots.begin() + static_cast<std::ptrdiff_t>(to));
+      std::vector<std::uint32_t> groups;
+      std::fill(row_of.begin(), row_of.end(), -1);
+      for (std::size_t j = 0; j != slots_chunk.size(); ++j) {
+        auto const s = slots_chunk[j];
+        auto const& lb = pq[static_cast<std::size_t>(slots[s].q)].lb_;
+        std::memcpy(e.h_lb.p + j * L, lb.data(), 2U * L);
+        row_of[s] = static_cast<std::int32_t>(j);
+        auto const g = s / e.W;
+        if (groups.empty() || groups.back() != g) groups.push_back(g);
+      }
+      HIP_CHECK(hipMemcpyAsync(e.d_lb_staging, e.h_lb.p, 2U * L * slots_chunk.size(), hipMemcpyHostToDevice, stream));
+      HIP_CHECK(hipMemcpyAsync(e.d_lb_slot, slots_chunk.data(), 4U * slots_chunk.size(), hipMemcpyHostToDevice, stream));
+      HIP_CHECK(hipMemcpyAsync(e.d_lb_row, row_of.data(), 4U * e.S, hipMemcpyHostToDevice, stream));
+      HIP_CHECK(hipMemcpyAsync(e.d_lb_groups, groups.data(), 4U * groups.size(), hipMemcpyHostToDevice, stream));
+      a.lb_staging = e.d_lb_staging;
+      a.lb_slot = e.d_lb_slot;
+      a.n_lb = static_cast<std::uint32_t>(slots_chunk.size());
+      a.lb_row_of_slot = e.d_lb_row;
+      a.lb_groups = e.d_lb_groups;
+      a.n_lb_groups = static_cast<std::uint32_t>(groups.size());
+      launch_lb_scatter(e.W, a, stream);
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipStreamSynchronize(stream));  // host vectors go out of scope
+    }
+  }
+};
+
+// prepare queries on host threads
+void prepare_all(timetable const& tt, nigiri_query const* qs, std::uint32_t n,
+                 std::vector<prepared_query>& pq, std::vector<int>& status,
+                 std::vector<std::string>& errors) {
+  pq.resize(n);
+  status.assign(n, NIGIRI_AMD_OK);
+  errors.assign(n, {});
+  auto const hw = std::max(1U, std::thread::hardware_concurrency());
+  auto const threads = std::min<std::uint32_t>(std::min<std::uint32_t>(hw, 64U), std::max(1U, n));
+  std::atomic<std::uint32_t> next{0U};
+  std::vector<std::thread> pool;
+  for (std::uint32_t t = 0; t != threads; ++t) {
+    pool.emplace_back([&]() {
+      host_scratch scratch;
+      while (true) {
+        auto const i = next.fetch_add(1U);
+        if (i >= n) break;
+        try {
+          prepare(tt, qs[i], pq[i], scratch);
+        } catch (status_error const& e) {
+          status[i] = e.code_;
+          errors[i] = e.what();
+        } catch (std::exception const& e) {
+          status[i] = NIGIRI_AMD_EINVAL;
+          errors[i] = e.what();
+        }
+      }
+    });
+  }
+  for (auto& t : pool) t.join();
+}
+
+std::uint32_t write_results(std::vector<prepared_query> const& pq, std::vector<int> const& status,
+                            nigiri_result* res, nigiri_journey* journeys, std::uint32_t cap,
+                            std::vector<std::uint32_t> const* epochs,
+                            std::vector<double> const* latency) {
+  std::uint32_t pos = 0U;
+  int overflow = 0;
+  for (std::size_t i = 0; i != pq.size(); ++i) {
+    auto& r = res[i];
+    r = nigiri_result{};
+    r.status = status[i];
+    r.journey_offset = pos;
+    if (status[i] != NIGIRI_AMD_OK) continue;
+    auto const& p = pq[i];
+    r.fastest_direct = p.fastest_direct_;
+    r.base_day = static_cast<std::int16_t>(p.base_);
+    r.interval_from = p.iv_from_;
+    r.interval_to = p.iv_to_;
+    r.n_epochs = static_cast<std::uint32_t>(p.epochs_.size());
+    if (epochs != nullptr) r.n_rounds = (*epochs)[i];
+    if (latency != nullptr) r.latency_ms = (*latency)[i];
+    for (auto const& j : p.results_) {
+      if (pos < cap && journeys != nullptr) {
+        journeys[pos] = nigiri_journey{j.start_time_, j.dest_time_, j.dest_, j.transfers_, {}};
+        ++pos;
+        ++r.n_journeys;
+      } else {
+        overflow = 1;
+      }
+    }
+  }
+  return overflow != 0 ? ~0U : pos;
+}
+
+}  // namespace
+
+// used by the serving loop (serve.cc)
+void engine_run_dir(nigiri_amd_engine& e, bool fwd, std::vector<prepared_query>& pq,
+                    std::function<std::int32_t()> next, std::function<void(std::int32_t)> done,
+                    std::function<void()> after_step) {
+  runner r{e, *e.img->tt, fwd, pq};
+  r.next = std::move(next);
+  r.on_done = std::move(done);
+  while (r.step()) {
+    if (after_step) after_step();
+  }
+}
+
+void engine_prepare_all(timetable const& tt, nigiri_query const* qs, std::uint32_t n,
+                        std::vector<prepared_query>& pq, std::vector<int>& status,
+                        std::vector<std::string>& errors) {
+  prepare_all(tt, qs, n, pq, status, errors);
+}
+
+std::uint32_t engine_write_results(std::vector<prepared_query> const& pq, std::vector<int> const& status,
+                                   nigiri_result* res, nigiri_journey* journeys, std::uint32_t cap,
+                                   std::vector<double> const* latency) {
+  return write_results(pq, status, res, journeys, cap, nullptr, latency);
+}
+
+}  // namespace nigiri_amd
+
+extern "C" {
+
+int nigiri_amd_image_create(const nigiri_amd_tt* tt, int device, nigiri_amd_image** out) {
+  if (tt == nullptr || out == nullptr) {
+    set_error("image_create: null argument");
+    return NIGIRI_AMD_EINVAL;
+  }
+  return guarded([&]() {
+    HIP_CHECK(hipSetDevice(device));
+    auto img = std::make_unique<nigiri_amd_image>();
+    img->tt = &tt->tt;
+    img->device = device;
+    hipStream_t s = nullptr;
+    HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    try {
+      build_image(*img, tt->tt, s);
+      HIP_CHECK(hipStreamSynchronize(s));
+    } catch (...) {
+      (void)hipStreamDestroy(s);
+      throw;
+    }
+    HIP_CHECK(hipStreamDestroy(s));
+    *out = img.release();
+    return NIGIRI_AMD_OK;
+  });
+}
+
+int nigiri_amd_image_bytes(const nigiri_amd_image* img, uint64_t* bytes) {
+  if (img == nullptr || bytes == nullptr) return NIGIRI_AMD_EINVAL;
+  *bytes = img->mem.bytes;
+  return NIGIRI_AMD_OK;
+}
+
+void nigiri_amd_image_destroy(nigiri_amd_image* img) { delete img; }
+
+int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_groups,
+                             uint32_t max_k, nigiri_amd_engine** out) {
+  if (img == nullptr || out == nullptr || (lanes != 64U && lanes != 1U) || n_groups == 0U ||
+      max_k < 2U || max_k > 16U) {
+    set_error("engine_create: lanes must be 1 or 64, n_groups > 0, 2 <= max_k <= 16");
+    return NIGIRI_AMD_EINVAL;
+  }
+  return guarded([&]() {
+    HIP_CHECK(hipSetDevice(img->device));
+    auto e = std::make_unique<nigiri_amd_engine>();
+    e->img = img;
+    e->W = lanes;
+    e->G = n_groups;
+    e->K = max_k;
+    e->S = lanes * n_groups;
+    HIP_CHECK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    auto& st = e->st;
+    auto& m = e->mem;
+    auto const L = img->d.L, R = img->d.R, G = e->G, W = e->W, K = e->K, S = e->S;
+    st.W = W;
+    st.G = G;
+    st.K = K;
+    st.L = L;
+    st.R = R;
+    st.LW = (L + 63U) / 64U;
+    st.RW = (R + 63U) / 64U;
+    auto const cols = static_cast<std::size_t>(G) * L * W;
+    st.round_times = m.alloc<std::int16_t>(cols * K);
+    st.best = m.alloc<std::int16_t>(cols);
+    st.tmp = m.alloc<std::int16_t>(cols);
+    st.lb = m.alloc<std::uint16_t>(cols);
+    auto const lwords = W == 64U ? static_cast<std::size_t>(G) * L : static_cast<std::size_t>(G) * st.LW;
+    auto const rwords = W == 64U ? static_cast<std::size_t>(G) * R : static_cast<std::size_t>(G) * st.RW;
+    st.mark_t = m.alloc<std::uint64_t>(lwords);
+    st.mark_r = m.alloc<std::uint64_t>(lwords);
+    st.dest_mask = m.alloc<std::uint64_t>(lwords);
+    st.route_mark = m.alloc<std::uint64_t>(rwords);
+    st.grp_active = m.alloc<std::uint64_t>(G);
+    st.grp_any_route = m.alloc<std::uint64_t>(G);
+    st.grp_any_tmp = m.alloc<std::uint64_t>(G);
+    st.cap_t = static_cast<std::size_t>(G) * L;
+    st.cap_r = static_cast<std::size_t>(G) * L;
+    st.cap_route = static_cast<std::size_t>(G) * R;
+    st.cap_hz = std::max<std::size_t>(1U << 16, static_cast<std::size_t>(S) * 64U);
+    st.list_t = m.alloc<std::uint64_t>(st.cap_t);
+    st.list_r = m.alloc<std::uint64_t>(st.cap_r);
+    st.list_route = m.alloc<std::uint64_t>(st.cap_route);
+    st.list_hz = m.alloc<std::uint64_t>(st.cap_hz);
+    st.counts = m.alloc<std::uint32_t>(8U);
+    st.param = m.alloc<slot_param>(S);
+    st.tad = m.alloc<std::int16_t>(static_cast<std::size_t>(S) * kRowsMax);
+    st.tad0 = m.alloc<std::int16_t>(S);
+    st.n_chg = m.alloc<std::uint32_t>(S);
+    st.chg_pos = m.alloc<std::uint32_t>(static_cast<std::size_t>(S) * kMaxChanges);
+    st.chg_tad = m.alloc<std::int16_t>(static_cast<std::size_t>(S) * kMaxChanges);
+    st.dest = m.alloc<std::uint32_t>(static_cast<std::size_t>(S) * kMaxDestBits);
+    st.dist_loc = m.alloc<std::uint32_t>(static_cast<std::size_t>(S) * kMaxDist);
+    st.dist_min = m.alloc<std::uint16_t>(static_cast<std::size_t>(S) * kMaxDist);
+    st.dest_rows = m.alloc<std::int16_t>(static_cast<std::size_t>(S) * kMaxDestBits * K);
+    // zero the marks, masks, counters (one-time set-up, not on a hot path)
+    HIP_CHECK(hipMemsetAsync(st.mark_t, 0, 8U * lwords, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.mark_r, 0, 8U * lwords, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.dest_mask, 0, 8U * lwords, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.route_mark, 0, 8U * rwords, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.grp_active, 0, 8U * G, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.grp_any_route, 0, 8U * G, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.grp_any_tmp, 0, 8U * G, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.counts, 0, 32U, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.n_chg, 0, 4U * S, e->stream));
+    e->h_param.assign(S, slot_param{});
+    e->h_dest.assign(static_cast<std::size_t>(S) * kMaxDestBits, 0U);
+    e->h_dist_loc.assign(static_cast<std::size_t>(S) * kMaxDist, 0U);
+    e->h_dist_min.assign(static_cast<std::size_t>(S) * kMaxDist, 0U);
+    e->slot_dest_bits.assign(S, {});
+    e->d_reset_groups = m.alloc<std::uint32_t>(G);
+    e->d_rt_mask = m.alloc<std::uint64_t>(G);
+    e->d_bt_mask = m.alloc<std::uint64_t>(G);
+    e->d_flags = m.alloc<std::uint8_t>(S);
+    e->d_worst = m.alloc<std::int16_t>(S);
+    e->d_sweep_groups = m.alloc<std::uint32_t>(G);
+    e->lb_rows_cap = std::min<std::uint32_t>(S, 64U);
+    e->d_lb_staging = m.alloc<std::uint16_t>(static_cast<std::size_t>(e->lb_rows_cap) * L);
+    e->d_lb_slot = m.alloc<std::uint32_t>(e->lb_rows_cap);
+    e->d_lb_row = m.alloc<std::int32_t>(S);
+    e->d_lb_groups = m.alloc<std::uint32_t>(G);
+    e->h_lb.reserve(static_cast<std::size_t>(e->lb_rows_cap) * L);
+    e->h_active.reserve(G);
+    e->h_rows.reserve(static_cast<std::size_t>(S) * kMaxDestBits * K);
+    e->h_counts.reserve(8U);
+    e->pending_starts.assign(S, {});
+    e->algo_new.assign(S, 0U);
+    e->algo_fresh.assign(S, 0U);
+    e->algo_dir.assign(S, 0U);
+    HIP_CHECK(hipStreamSynchronize(e->stream));
+    *out = e.release();
+    return NIGIRI_AMD_OK;
+  });
+}
+
+int nigiri_amd_engine_state_bytes(const nigiri_amd_engine* e, uint64_t* b) {
+  if (e == nullptr || b == nullptr) return NIGIRI_AMD_EINVAL;
+  *b = e->mem.bytes;
+  return NIGIRI_AMD_OK;
+}
+
+void nigiri_amd_engine_destroy(nigiri_amd_engine* e) { delete e; }
+
+int nigiri_amd_search(nigiri_amd_engine* e, const nigiri_query* qs, uint32_t n, nigiri_result* res,
+                      nigiri_journey* journeys, uint32_t journey_capacity) {
+  if (e == nullptr || (n != 0U && (qs == nullptr || res == nullptr))) {
+    set_error("search: null argument");
+    return NIGIRI_AMD_EINVAL;
+  }
+  return guarded([&]() {
+    HIP_CHECK(hipSetDevice(e->img->device));
+    auto const& tt = *e->img->tt;
+    std::vector<prepared_query> pq;
+    std::vector<int> status;
+    std::vector<std::string> errors;
+    prepare_all(tt, qs, n, pq, status, errors);
+    for (int dir = 0; dir != 2; ++dir) {
+      std::vector<std::int32_t> queue;
+      for (std::uint32_t i = 0; i != n; ++i) {
+        if (status[i] == NIGIRI_AMD_OK && (qs[i].dir == NIGIRI_AMD_DIR_FWD) == (dir == 0)) {
+          queue.push_back(static_cast<std::int32_t>(i));
+        }
+      }
+      if (queue.empty()) continue;
+      std::size_t head = 0U;
+      engine_run_dir(
+          *e, dir == 0, pq,
+          [&]() { return head < queue.size() ? queue[head++] : -1; }, [](std::int32_t) {}, {});
+    }
+    auto const w = write_results(pq, status, res, journeys, journey_capacity, nullptr, nullptr);
+    if (w == ~0U) {
+      set_error("search: journey_capacity too small");
+      return NIGIRI_AMD_ECAPACITY;
+    }
+    for (std::uint32_t i = 0; i != n; ++i) {
+      if (status[i] != NIGIRI_AMD_OK) {
+        set_error(errors[i]);
+      }
+    }
+    return NIGIRI_AMD_OK;
+  });
+}
+
+// ---- serving loop -----------------------------------------------------------
+// Queries become visible at their arrival time; a pool of host workers
+// prepares them (destinations, lower bounds, start labels) as they arrive;
+// the GPU loop admits prepared queries into free slots at every epoch
+// boundary while other slots keep running (continuous batching).
+int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs, const double* arrival_ms, uint32_t n,
+                     nigiri_result* res, nigiri_journey* journeys, uint32_t journey_capacity,
+                     double* wall_ms) {
+  if (e == nullptr || (n != 0U && (qs == nullptr || arrival_ms == nullptr || res == nullptr))) {
+    set_error("serve: null argument");
+    return NIGIRI_AMD_EINVAL;
+  }
+  for (uint32_t i = 0; i != n; ++i) {
+    if (qs[i].dir != qs[0].dir) {
+      set_error("serve: all queries of one serving run must share a direction");
+      return NIGIRI_AMD_EUNSUPPORTED;
+    }
+    if (i != 0U && arrival_ms[i] < arrival_ms[i - 1U]) {
+      set_error("serve: arrival times must be non-decreasing");
+      return NIGIRI_AMD_EINVAL;
+    }
+  }
+  return guarded([&]() {
+    HIP_CHECK(hipSetDevice(e->img->device));
+    auto const& tt = *e->img->tt;
+    using clock = std::chrono::steady_clock;
+    std::vector<prepared_query> pq(n);
+    std::vector<int> status(n, NIGIRI_AMD_OK);
+    std::vector<double> latency(n, 0.0);
+    std::mutex mu;
+    std::deque<std::int32_t> ready;
+    std::atomic<std::uint32_t> next_prep{0U};
+    std::atomic<std::uint32_t> n_done{0U};
+    auto const t0 = clock::now();
+    auto const ms_since = [&](clock::time_point t) {
+      return std::chrono::duration<double, std::milli>(t - t0).count();
+    };
+    auto const hw = std::max(1U, std::thread::hardware_concurrency());
+    auto const n_workers = std::min<std::uint32_t>(std::max(1U, hw - 1U), 32U);
+    std::vector<std::thread> workers;
+    for (std::uint32_t w = 0; w != n_workers; ++w) {
+      workers.emplace_back([&]() {
+        host_scratch scratch;
+        while (true) {
+          auto const i = next_prep.fetch_add(1U);
+          if (i >= n) break;
+          auto const due = t0 + std::chrono::duration_cast<clock::duration>(
+                                    std::chrono::duration<double, std::milli>(arrival_ms[i]));
+          std::this_thread::sleep_until(due);
+          try {
+            prepare(tt, qs[i], pq[i], scratch);
+          } catch (status_error const& ex) {
+            status[i] = ex.code_;
+          } catch (std::exception const&) {
+            status[i] = NIGIRI_AMD_EINVAL;
+          }
+          if (status[i] != NIGIRI_AMD_OK) {
+            latency[i] = ms_since(clock::now()) - arrival_ms[i];
+            n_done.fetch_add(1U);
+            continue;
+          }
+          std::lock_guard<std::mutex> lock{mu};
+          ready.push_back(static_cast<std::int32_t>(i));
+        }
+      });
+    }
+    runner r{*e, tt, qs[0].dir == NIGIRI_AMD_DIR_FWD, pq};
+    r.next = [&]() -> std::int32_t {
+      std::lock_guard<std::mutex> lock{mu};
+      if (ready.empty()) return -1;
+      auto const q = ready.front();
+      ready.pop_front();
+      return q;
+    };
+    r.on_done = [&](std::int32_t q) {
+      latency[static_cast<std::size_t>(q)] = ms_since(clock::now()) - arrival_ms[q];
+      n_done.fetch_add(1U);
+    };
+    try {
+      while (n_done.load() < n) {
+        if (!r.step()) {
+          std::this_thread::sleep_for(std::chrono::microseconds(50));
+        }
+      }
+    } catch (...) {
+      next_prep.store(n);
+      for (auto& t : workers) t.join();
+      throw;
+    }
+    for (auto& t : workers) t.join();
+    if (wall_ms != nullptr) *wall_ms = ms_since(clock::now());
+    auto const w = write_results(pq, status, res, journeys, journey_capacity, nullptr, &latency);
+    if (w == ~0U) {
+      set_error("serve: journey_capacity too small");
+      return NIGIRI_AMD_ECAPACITY;
+    }
+    return NIGIRI_AMD_OK;
+  });
+}
+
+// ---- Algo-level entry points (one slot) -----------------------------------
+
+int nigiri_amd_algo_init(nigiri_amd_engine* e, uint32_t slot, const nigiri_algo_init* init) {
+  if (e == nullptr || init == nullptr || slot >= e->S || init->lb == nullptr) {
+    set_error("algo_init: bad argument");
+    return NIGIRI_AMD_EINVAL;
+  }
+  if (init->n_dest > kMaxDestBits || init->n_dist > kMaxDist) {
+    set_error("algo_init: too many destinations");
+    return NIGIRI_AMD_EUNSUPPORTED;
+  }
+  return guarded([&]() {
+    HIP_CHECK(hipSetDevice(e->img->device));
+    auto& sp = e->h_param[slot];
+    sp.base = static_cast<std::uint16_t>(init->base_day);
+    sp.intermodal = init->intermodal != 0 ? 1U : 0U;
+    sp.n_dest = static_cast<std::uint8_t>(init->n_dest);
+    sp.n_dist = static_cast<std::uint8_t>(init->n_dist);
+    sp.tts_default = init->tts_default != 0 ? 1U : 0U;
+    sp.tts_min = init->tts_min_transfer_time;
+    sp.tts_add = init->tts_additional_time;
+    sp.tts_factor = init->tts_factor;
+    sp.end_k = static_cast<std::uint8_t>(e->K);
+    for (std::uint32_t j = 0; j != init->n_dest; ++j) e->h_dest[slot * kMaxDestBits + j] = init->dest[j];
+    for (std::uint32_t j = 0; j != init->n_dist; ++j) {
+      e->h_dist_loc[slot * kMaxDist + j] = init->dist_loc[j];
+      e->h_dist_min[slot * kMaxDist + j] = init->dist_min[j];
+    }
+    e->algo_dir[slot] = static_cast<std::uint8_t>(init->dir);
+    // destination masks and lower bounds
+    epoch_args a{};
+    a.st = e->st;
+    std::vector<std::uint64_t> ops;
+    std::vector<std::uint8_t> set;
+    for (auto const l : e->slot_dest_bits[slot]) {
+      ops.push_back((static_cast<std::uint64_t>(slot) << 32) | l);
+      set.push_back(0U);
+    }
+    e->slot_dest_bits[slot].clear();
+    if (init->intermodal == 0) {
+      for (std::uint32_t j = 0; j != init->n_dest; ++j) {
+        ops.push_back((static_cast<std::uint64_t>(slot) << 32) | init->dest[j]);
+        set.push_back(1U);
+        e->slot_dest_bits[slot].push_back(init->dest[j]);
+      }
+    }
+    if (ops.size() > e->dest_ops_cap) {
+      e->dest_ops_cap = ops.size() * 2U;
+      e->d_dest_ops = e->grow(e->d_dest_ops, e->dest_ops_cap);
+      e->d_dest_set = e->grow(e->d_dest_set, e->dest_ops_cap);
+    }
+    auto const L = e->st.L;
+    std::memcpy(e->h_lb.p, init->lb, 2U * L);
+    std::vector<std::int32_t> row_of(e->S, -1);
+    row_of[slot] = 0;
+    std::uint32_t const g = slot / e->W;
+    HIP_CHECK(hipMemcpyAsync(e->d_lb_staging, e->h_lb.p, 2U * L, hipMemcpyHostToDevice, e->stream));
+    HIP_CHECK(hipMemcpyAsync(e->d_lb_slot, &slot, 4U, hipMemcpyHostToDevice, e->stream));
+    HIP_CHECK(hipMemcpyAsync(e->d_lb_row, row_of.data(), 4U * e->S, hipMemcpyHostToDevice, e->stream));
+    HIP_CHECK(hipMemcpyAsync(e->d_lb_groups, &g, 4U, hipMemcpyHostToDevice, e->stream));
+    if (!ops.empty()) {
+      HIP_CHECK(hipMemcpyAsync(e->d_dest_ops, ops.data(), 8U * ops.size(), hipMemcpyHostToDevice, e->stream));
+      HIP_CHECK(hipMemcpyAsync(e->d_dest_set, set.data(), set.size(), hipMemcpyHostToDevice, e->stream));
+    }
+    a.lb_staging = e->d_lb_staging;
+    a.lb_slot = e->d_lb_slot;
+    a.n_lb = 1U;
+    a.lb_row_of_slot = e->d_lb_row;
+    a.lb_groups = e->d_lb_groups;
+    a.n_lb_groups = 1U;
+    launch_lb_scatter(e->W, a, e->stream);
+    a.n_lb = 0U;
+    a.dest_ops = e->d_dest_ops;
+    a.dest_set = e->d_dest_set;
+    a.n_dest_ops = static_cast<std::uint32_t>(ops.size());
+    a.n_slots = 0U;  // no slot init
+    launch_epoch_begin(init->dir == NIGIRI_AMD_DIR_FWD, e->W, a, e->stream);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(e->stream));
+    e->algo_new[slot] = 1U;
+    e->algo_fresh[slot] = 1U;
+    e->pending_starts[slot].clear();
+    return NIGIRI_AMD_OK;
+  });
+}
+
+int nigiri_amd_algo_reset_arrivals(nigiri_amd_engine* e, uint32_t slot) {
+  if (e == nullptr || slot >= e->S) return NIGIRI_AMD_EINVAL;
+  e->algo_new[slot] = 1U;
+  return NIGIRI_AMD_OK;
+}
+
+int nigiri_amd_algo_next_start_time(nigiri_amd_engine* e, uint32_t slot) {
+  if (e == nullptr || slot >= e->S) return NIGIRI_AMD_EINVAL;
+  e->algo_fresh[slot] = 1U;
+  e->pending_starts[slot].clear();
+  return NIGIRI_AMD_OK;
+}
+
+int nigiri_amd_algo_add_start(nigiri_amd_engine* e, uint32_t slot, uint32_t location, int32_t t) {
+  if (e == nullptr || slot >= e->S || location >= e->st.L) return NIGIRI_AMD_EINVAL;
+  e->pending_starts[slot].emplace_back(location, t);
+  return NIGIRI_AMD_OK;
+}
+
+int nigiri_amd_algo_execute(nigiri_amd_engine* e, uint32_t slot, int32_t start_time,
+                            uint8_t max_transfers, int32_t worst_time_at_dest, int16_t* dest_times,
+                            uint32_t* n_rounds) {
+  if (e == nullptr || slot >= e->S) return NIGIRI_AMD_EINVAL;
+  return guarded([&]() {
+    HIP_CHECK(hipSetDevice(e->img->device));
+    auto const& tt = *e->img->tt;
+    auto const fwd = e->algo_dir[slot] == NIGIRI_AMD_DIR_FWD;
+    auto& sp = e->h_param[slot];
+    auto const end_k = std::min<std::uint32_t>(std::min<std::uint32_t>(max_transfers, 14U) + 2U, e->K);
+    sp.end_k = static_cast<std::uint8_t>(end_k);
+    auto const base = tt.internal_from + static_cast<std::int32_t>(sp.base) * 1440;
+    (void)start_time;
+    auto const S = e->S, G = e->G;
+    std::vector<std::uint64_t> rt_mask(G, 0U), bt_mask(G, 0U);
+    std::vector<std::uint8_t> flags(S, 0U);
+    std::vector<std::int16_t> worst(S, 0);
+    auto const g = slot / e->W, lane = slot % e->W;
+    if (e->algo_new[slot] != 0U) rt_mask[g] |= 1ULL << lane;
+    if (e->algo_fresh[slot] != 0U) bt_mask[g] |= 1ULL << lane;
+    flags[slot] = static_cast<std::uint8_t>(1U | (e->algo_new[slot] != 0U ? 2U : 0U));
+    worst[slot] = clamp16(static_cast<long long>(worst_time_at_dest) - base);
+    std::vector<std::uint32_t> st_slot, st_loc;
+    std::vector<std::int16_t> st_delta;
+    for (auto const& [l, t] : e->pending_starts[slot]) {
+      st_slot.push_back(slot);
+      st_loc.push_back(l);
+      st_delta.push_back(clamp16(static_cast<long long>(t) - base));
+    }
+    if (st_slot.size() > e->start_cap) {
+      e->start_cap = st_slot.size() * 2U;
+      e->d_start_slot = e->grow(e->d_start_slot, e->start_cap);
+      e->d_start_loc = e->grow(e->d_start_loc, e->start_cap);
+      e->d_start_delta = e->grow(e->d_start_delta, e->start_cap);
+    }
+    auto const stream = e->stream;
+    auto const& tr = e->st;
+    HIP_CHECK(hipMemcpyAsync(tr.param, e->h_param.data(), sizeof(slot_param) * S, hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(tr.dest, e->h_dest.data(), 4U * e->h_dest.size(), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(tr.dist_loc, e->h_dist_loc.data(), 4U * e->h_dist_loc.size(), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(tr.dist_min, e->h_dist_min.data(), 2U * e->h_dist_min.size(), hipMemcpyHostToDevice, stream));
+    std::uint32_t const gg = g;
+    HIP_CHECK(hipMemcpyAsync(e->d_reset_groups, &gg, 4U, hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(e->d_rt_mask, rt_mask.data(), 8U * G, hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(e->d_bt_mask, bt_mask.data(), 8U * G, hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(e->d_flags, flags.data(), S, hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(e->d_worst, worst.data(), 2U * S, hipMemcpyHostToDevice, stream));
+    if (!st_slot.empty()) {
+      HIP_CHECK(hipMemcpyAsync(e->d_start_slot, st_slot.data(), 4U * st_slot.size(), hipMemcpyHostToDevice, stream));
+      HIP_CHECK(hipMemcpyAsync(e->d_start_loc, st_loc.data(), 4U * st_loc.size(), hipMemcpyHostToDevice, stream));
+      HIP_CHECK(hipMemcpyAsync(e->d_start_delta, st_delta.data(), 2U * st_delta.size(), hipMemcpyHostToDevice, stream));
+    }
+    epoch_args a{};
+    a.st = tr;
+    a.invalid = fwd ? SHRT_MAX : SHRT_MIN;
+    a.reset_groups = e->d_reset_groups;
+    a.n_reset_groups = (rt_mask[g] | bt_mask[g]) != 0U ? 1U : 0U;
+    a.reset_rt_mask = e->d_rt_mask;
+    a.reset_bt_mask = e->d_bt_mask;
+    a.start_slot = e->d_start_slot;
+    a.start_loc = e->d_start_loc;
+    a.start_delta = e->d_start_delta;
+    a.n_starts = static_cast<std::uint32_t>(st_slot.size());
+    a.slot_flags = e->d_flags;
+    a.slot_worst = e->d_worst;
+    a.n_slots = S;
+    launch_epoch_begin(fwd, e->W, a, stream);
+    HIP_CHECK(hipGetLastError());
+    std::uint32_t rounds = 0U;
+    for (std::uint32_t k = 1U; k < end_k; ++k) {
+      round_args ra{};
+      ra.im = e->img->d;
+      ra.st = tr;
+      ra.k = static_cast<std::int32_t>(k);
+      ra.invalid = a.invalid;
+      // a slot after its first epoch sees earlier rows in round_times
+      std::uint32_t const n_sweep = e->algo_new[slot] != 0U ? 0U : 1U;
+      launch_round(fwd, e->W, ra, e->d_reset_groups, n_sweep, S, 2048, stream);
+      HIP_CHECK(hipGetLastError());
+      ++rounds;
+      HIP_CHECK(hipMemcpyAsync(e->h_active.p, tr.grp_active, 8U * G, hipMemcpyDeviceToHost, stream));
+      HIP_CHECK(hipStreamSynchronize(stream));
+      if ((e->h_active.p[g] & (1ULL << lane)) == 0U) break;
+    }
+    launch_epoch_end(e->W, tr, S, stream);
+    auto const rows = static_cast<std::size_t>(S) * kMaxDestBits * e->K;
+    HIP_CHECK(hipMemcpyAsync(e->h_rows.p, tr.dest_rows, 2U * rows, hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipMemcpyAsync(e->h_counts.p, tr.counts, 32U, hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    if (e->h_counts.p[4] != 0U) {
+      std::uint32_t zero = 0U;
+      HIP_CHECK(hipMemcpy(tr.counts + 4, &zero, 4U, hipMemcpyHostToDevice));
+      throw status_error{NIGIRI_AMD_ECAPACITY, "device work list overflow"};
+    }
+    if (dest_times != nullptr) {
+      std::memcpy(dest_times, e->h_rows.p + static_cast<std::size_t>(slot) * kMaxDestBits * e->K,
+                  2U * kMaxDestBits * e->K);
+    }
+    if (n_rounds != nullptr) *n_rounds = rounds;
+    e->algo_new[slot] = 0U;
+    e->algo_fresh[slot] = 0U;
+    e->pending_starts[slot].clear();
+    return NIGIRI_AMD_OK;
+  });
+}
+
+int nigiri_amd_algo_download(nigiri_amd_engine* e, uint32_t slot, int16_t* round_times, int16_t* best,
+                             int16_t* tmp, int16_t* time_at_dest) {
+  if (e == nullptr || slot >= e->S) return NIGIRI_AMD_EINVAL;
+  return guarded([&]() {
+    HIP_CHECK(hipSetDevice(e->img->device));
+    auto const L = e->st.L, K = e->K;
+    std::int16_t* d = nullptr;
+    HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&d), 2U * static_cast<std::size_t>(K + 2U) * L));
+    std::vector<std::int16_t> h(static_cast<std::size_t>(K + 2U) * L);
+    try {
+      launch_gather_slot(e->W, e->st, slot, d, d + static_cast<std::size_t>(K) * L,
+                         d + static_cast<std::size_t>(K + 1U) * L, e->stream);
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipMemcpyAsync(h.data(), d, 2U * h.size(), hipMemcpyDeviceToHost, e->stream));
+      HIP_CHECK(hipStreamSynchronize(e->stream));
+    } catch (...) {
+      (void)hipFree(d);
+      throw;
+    }
+    HIP_CHECK(hipFree(d));
+    if (round_times != nullptr) std::memcpy(round_times, h.data(), 2U * static_cast<std::size_t>(K) * L);
+    if (best != nullptr) std::memcpy(best, h.data() + static_cast<std::size_t>(K) * L, 2U * L);
+    if (tmp != nullptr) std::memcpy(tmp, h.data() + static_cast<std::size_t>(K + 1U) * L, 2U * L);
+    if (time_at_dest != nullptr) {
+      HIP_CHECK(hipMemcpy(time_at_dest, e->st.tad + static_cast<std::size_t>(slot) * kRowsMax,
+                          2U * kRowsMax, hipMemcpyDeviceToHost));
+    }
+    return NIGIRI_AMD_OK;
+  });
+}
+
+}  // extern "C"

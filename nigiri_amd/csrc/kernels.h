@@ -1,0 +1,92 @@
+// gfx950 round kernels of the RAPTOR engine.
+//
+// One RAPTOR round (raptor.h:218-387) for every active query slot is a
+// fixed sequence of persistent, grid-stride kernels that read their work
+// counts from device memory, so a round is enqueued without host syncs:
+//
+//   round_begin   time_at_dest from destination labels  (raptor.h:219-226)
+//   best_sweep    best = min(round_times[k], best)       (only for slots in
+//                 a later start-time epoch; earlier rows are empty otherwise)
+//   mark_routes   station marks -> route marks           (raptor.h:228-246)
+//   route_scan    loop_routes / update_route / get_earliest_transport
+//                 (raptor.h:496-575, 1181-1442): batched engine = one
+//                 wavefront per (64-query group, route), one query per lane;
+//                 plain engine = one route per lane
+//   clear marks, group_update (the two early exits, raptor.h:243-246, 370-373)
+//   dest_pass     destination events of update_transfers /
+//                 update_intermodal_footpaths / update_footpaths, simulated
+//                 in list order per query: yields the exact time_at_dest
+//                 trajectory of the phase
+//   transfer_fp   all other transfer and footpath relaxations, each checked
+//                 against the time_at_dest value in force at its position
+//   rt_fixup      round_times[k] = best for stations improved in the phase
+//   hazards       exact replay for the rare stations where time_at_dest fell
+//                 between a passing and a later pruned relaxation
+//
+// The integer semantics (loose <= pruning, pre-writes of round_times, clamp
+// to int16, uint16 day arithmetic) follow raptor.h line by line; every
+// comparison is templated on the search direction.
+#pragma once
+
+#include <cstdint>
+
+#include "device.h"
+
+namespace nigiri_amd {
+
+struct round_args {
+  dev_image im;
+  dev_state st;
+  std::int32_t k;
+  std::int16_t invalid;
+};
+
+// Launch helpers (defined in kernels.hip), all asynchronous on `stream`.
+void launch_round(bool fwd, std::uint32_t W, round_args const& a,
+                  std::uint32_t const* sweep_groups, std::uint32_t n_sweep_groups,
+                  std::uint32_t n_slots, int grid_waves, hipStream_t stream);
+
+struct epoch_args {
+  dev_state st;
+  std::int16_t invalid;
+  // lane reset
+  std::uint32_t const* reset_groups;  // groups with any lane to reset
+  std::uint32_t n_reset_groups;
+  std::uint64_t const* reset_rt_mask;  // [G] lanes whose round_times reset
+  std::uint64_t const* reset_bt_mask;  // [G] lanes whose best/tmp reset
+  // lower bounds: staging rows [n_lb][L] for slots lb_slot[i];
+  // lb_row_of_slot[slot] = staging row or -1; lb_groups = groups with rows
+  std::uint16_t const* lb_staging;
+  std::uint32_t const* lb_slot;
+  std::uint32_t n_lb;
+  std::int32_t const* lb_row_of_slot;
+  std::uint32_t const* lb_groups;
+  std::uint32_t n_lb_groups;
+  // destination mask ops: (slot << 32 | location); dest_set[i] != 0 sets
+  std::uint64_t const* dest_ops;
+  std::uint8_t const* dest_set;
+  std::uint32_t n_dest_ops;
+  // starts: slot, location, delta
+  std::uint32_t const* start_slot;
+  std::uint32_t const* start_loc;
+  std::int16_t const* start_delta;
+  std::uint32_t n_starts;
+  // per slot init: flags bit0 = running this epoch, bit1 = new query
+  std::uint8_t const* slot_flags;
+  std::int16_t const* slot_worst;  // d_worst_at_dest per slot
+  std::uint32_t n_slots;
+};
+
+// lane resets, destination masks, slot init, starts (lower bounds go
+// through launch_lb_scatter)
+void launch_epoch_begin(bool fwd, std::uint32_t W, epoch_args const& a,
+                        hipStream_t stream);
+void launch_lb_scatter(std::uint32_t W, epoch_args const& a, hipStream_t stream);
+// state download: round_times [K][L], best [L], tmp [L] of one slot
+void launch_gather_slot(std::uint32_t W, dev_state const& st, std::uint32_t slot,
+                        std::int16_t* rt, std::int16_t* best, std::int16_t* tmp,
+                        hipStream_t stream);
+void launch_epoch_end(std::uint32_t W, dev_state const& st,
+                      std::uint32_t n_slots, hipStream_t stream);
+
+}  // namespace nigiri_amd

@@ -1,0 +1,997 @@
+// gfx950 round kernels (see kernels.h for the round structure).
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace nigiri_amd {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / 64;
+
+template <bool Fwd>
+struct dirop {
+  static constexpr std::int16_t kInvalid = Fwd ? 32767 : -32768;
+  __device__ static bool better(int a, int b) { return Fwd ? a < b : a > b; }
+  __device__ static bool better_eq(int a, int b) { return Fwd ? a <= b : a >= b; }
+  __device__ static std::int16_t best_of(std::int16_t a, std::int16_t b) {
+    return better(a, b) ? a : b;
+  }
+  __device__ static int dir(int a) { return Fwd ? a : -a; }
+};
+
+__device__ inline std::int16_t clamp16(int v) {
+  return static_cast<std::int16_t>(v < -32768 ? -32768 : (v > 32767 ? 32767 : v));
+}
+
+// Atomic "keep the better value" on an int16 element (32-bit CAS on the
+// aligned word).  Returns the previous value.
+template <bool Fwd>
+__device__ inline std::int16_t atomic_best16(std::int16_t* p, std::int16_t v) {
+  auto const addr = reinterpret_cast<std::uintptr_t>(p);
+  auto* w = reinterpret_cast<unsigned int*>(addr & ~std::uintptr_t{3});
+  unsigned const shift = (addr & 2U) != 0U ? 16U : 0U;
+  unsigned old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (true) {
+    auto const cur = static_cast<std::int16_t>(static_cast<std::uint16_t>(old >> shift));
+    if (!dirop<Fwd>::better(v, cur)) return cur;
+    unsigned const nw = (old & ~(0xFFFFU << shift)) |
+                        (static_cast<unsigned>(static_cast<std::uint16_t>(v)) << shift);
+    unsigned const prev = atomicCAS(w, old, nw);
+    if (prev == old) return cur;
+    old = prev;
+  }
+}
+
+__device__ inline bool bit_test(std::uint64_t const* bf, std::uint32_t idx, std::uint32_t day) {
+  if (day >= 512U) return false;  // out-of-range day index: not active
+  return ((bf[static_cast<std::size_t>(idx) * 8U + day / 64U] >> (day % 64U)) & 1ULL) != 0ULL;
+}
+
+__device__ inline std::uint32_t clamp_count(std::uint32_t n, std::size_t cap) {
+  return n < cap ? n : static_cast<std::uint32_t>(cap);
+}
+
+__device__ inline void list_push(std::uint64_t* list, std::uint32_t* count, std::size_t cap,
+                                 std::uint64_t v, std::uint32_t* overflow) {
+  auto const i = atomicAdd(count, 1U);
+  if (i < cap) {
+    list[i] = v;
+  } else {
+    atomicOr(overflow, 1U);
+  }
+}
+
+// ---- state indexing ---------------------------------------------------
+template <int W>
+struct lay {
+  __device__ static std::size_t col(dev_state const& s, std::uint32_t g, std::uint32_t l,
+                                    std::uint32_t lane) {
+    return (static_cast<std::size_t>(g) * s.L + l) * W + lane;
+  }
+  __device__ static std::size_t rt(dev_state const& s, std::uint32_t g, std::uint32_t k,
+                                   std::uint32_t l, std::uint32_t lane) {
+    return ((static_cast<std::size_t>(g) * s.K + k) * s.L + l) * W + lane;
+  }
+  __device__ static std::size_t lword(dev_state const& s, std::uint32_t g, std::uint32_t l) {
+    return W == 64 ? static_cast<std::size_t>(g) * s.L + l
+                   : static_cast<std::size_t>(g) * s.LW + l / 64U;
+  }
+  __device__ static std::uint64_t lbit(std::uint32_t l, std::uint32_t lane) {
+    return W == 64 ? (1ULL << lane) : (1ULL << (l % 64U));
+  }
+  __device__ static std::size_t rword(dev_state const& s, std::uint32_t g, std::uint32_t r) {
+    return W == 64 ? static_cast<std::size_t>(g) * s.R + r
+                   : static_cast<std::size_t>(g) * s.RW + r / 64U;
+  }
+  __device__ static std::uint64_t rbit(std::uint32_t r, std::uint32_t lane) {
+    return W == 64 ? (1ULL << lane) : (1ULL << (r % 64U));
+  }
+  // lane mask (W == 64) or bit 0 (W == 1) of the marked lanes at location l
+  __device__ static std::uint64_t lanes_at(std::uint64_t word, std::uint32_t l) {
+    return W == 64 ? word : ((word >> (l % 64U)) & 1ULL);
+  }
+  __device__ static bool test(std::uint64_t const* m, dev_state const& s, std::uint32_t g,
+                              std::uint32_t l, std::uint32_t lane) {
+    return (m[lword(s, g, l)] & lbit(l, lane)) != 0ULL;
+  }
+};
+
+// Sets the mark of (g, l, lane); appends (g, l) to the list on the first
+// mark of the location word.
+template <int W>
+__device__ inline void mark_one(std::uint64_t* marks, std::uint64_t* list, std::uint32_t* count,
+                                std::size_t cap, std::uint32_t* overflow, dev_state const& s,
+                                std::uint32_t g, std::uint32_t l, std::uint32_t lane) {
+  auto const bit = lay<W>::lbit(l, lane);
+  auto const old = atomicOr(reinterpret_cast<unsigned long long*>(&marks[lay<W>::lword(s, g, l)]),
+                            static_cast<unsigned long long>(bit));
+  bool const first = W == 64 ? old == 0ULL : (old & bit) == 0ULL;
+  if (first) {
+    list_push(list, count, cap, (static_cast<std::uint64_t>(g) << 32) | l, overflow);
+  }
+}
+
+// ---- transfer-time settings (transfer_time_settings.h:21-31) ------------
+template <typename T>
+__device__ inline T adjusted(slot_param const& p, T const d) {
+  if (p.tts_default != 0U) return d;
+  auto const a = static_cast<T>(p.tts_add);
+  auto const m = static_cast<T>(p.tts_min);
+  auto const f = static_cast<T>(static_cast<float>(d) * p.tts_factor);
+  return static_cast<T>(a + (m > f ? m : f));
+}
+
+template <bool Fwd>
+__device__ inline void update_tad(std::int16_t* tad, std::int32_t k, std::int16_t t) {
+  for (int i = k; i < kRowsMax; ++i) {
+    tad[i] = dirop<Fwd>::best_of(tad[i], t);
+  }
+}
+
+// time_at_dest in force at sequence position `pos` of the transfer phase
+__device__ inline std::int16_t tad_at(dev_state const& s, std::uint32_t slot, std::uint32_t pos) {
+  auto v = s.tad0[slot];
+  auto const n = s.n_chg[slot];
+  for (std::uint32_t j = 0; j < n; ++j) {
+    if (s.chg_pos[slot * kMaxChanges + j] < pos) {
+      v = s.chg_tad[slot * kMaxChanges + j];
+    } else {
+      break;
+    }
+  }
+  return v;
+}
+
+// ======================================================================
+// round_end_check: slots whose rounds are exhausted (k == end_k) stop
+template <int W>
+__global__ void k_round_end_check(round_args a, std::uint32_t n_slots) {
+  auto const& s = a.st;
+  auto const slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= n_slots) return;
+  auto const g = slot / W, lane = slot % W;
+  auto const lbit = 1ULL << lane;
+  if ((s.grp_active[g] & lbit) == 0ULL) return;
+  if (a.k >= s.param[slot].end_k) {
+    atomicAnd(reinterpret_cast<unsigned long long*>(&s.grp_active[g]), ~lbit);
+  }
+}
+
+// round_begin: time_at_dest from the destination labels (raptor.h:224-226)
+template <bool Fwd, int W>
+__global__ void k_round_begin(round_args a, std::uint32_t n_slots) {
+  auto const& s = a.st;
+  auto const slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= n_slots) return;
+  auto const g = slot / W, lane = slot % W;
+  if ((s.grp_active[g] & (1ULL << lane)) == 0ULL) return;
+  auto const& p = s.param[slot];
+  auto* tad = s.tad + static_cast<std::size_t>(slot) * kRowsMax;
+  if (p.intermodal != 0U) {
+    update_tad<Fwd>(tad, a.k, s.best[lay<W>::col(s, g, 1U, lane)]);
+  } else {
+    for (std::uint32_t j = 0; j < p.n_dest; ++j) {
+      auto const d = s.dest[slot * kMaxDestBits + j];
+      update_tad<Fwd>(tad, a.k, s.best[lay<W>::col(s, g, d, lane)]);
+    }
+  }
+}
+
+// best = min(round_times[k], best) for groups holding later-epoch slots
+template <bool Fwd, int W>
+__global__ void k_best_sweep(round_args a, std::uint32_t const* groups, std::uint32_t n_groups) {
+  auto const& s = a.st;
+  auto const per_group = static_cast<std::size_t>(s.L) * W;
+  auto const total = per_group * n_groups;
+  for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < total;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    auto const gi = i / per_group;
+    auto const rem = i % per_group;
+    auto const g = groups[gi];
+    auto const l = static_cast<std::uint32_t>(rem / W), lane = static_cast<std::uint32_t>(rem % W);
+    if ((s.grp_active[g] & (1ULL << lane)) == 0ULL) continue;
+    auto const c = lay<W>::col(s, g, l, lane);
+    auto const v = s.round_times[lay<W>::rt(s, g, a.k, l, lane)];
+    s.best[c] = dirop<Fwd>::best_of(v, s.best[c]);
+  }
+}
+
+// mark_routes: station marks (mark_t) -> route marks
+template <int W>
+__global__ void k_mark_routes(round_args a) {
+  auto const& im = a.im;
+  auto const& s = a.st;
+  auto* overflow = &s.counts[4];
+  auto const n_items = clamp_count(s.counts[0], s.cap_t);
+  if constexpr (W == 64) {
+    auto const lane = threadIdx.x & 63U;
+    for (std::uint32_t item = (blockIdx.x * blockDim.x + threadIdx.x) / 64U; item < n_items;
+         item += gridDim.x * kWavesPerBlock) {
+      auto const e = s.list_t[item];
+      auto const g = static_cast<std::uint32_t>(e >> 32), l = static_cast<std::uint32_t>(e);
+      auto const word = s.mark_t[lay<W>::lword(s, g, l)] & s.grp_active[g];
+      if (word == 0ULL) continue;
+      auto const r0 = im.location_route_offsets[l], r1 = im.location_route_offsets[l + 1U];
+      for (auto j = r0 + lane; j < r1; j += 64U) {
+        auto const r = im.location_routes[j];
+        auto const old = atomicOr(
+            reinterpret_cast<unsigned long long*>(&s.route_mark[lay<W>::rword(s, g, r)]),
+            static_cast<unsigned long long>(word));
+        if (old == 0ULL) {
+          list_push(s.list_route, &s.counts[2], s.cap_route,
+                    (static_cast<std::uint64_t>(g) << 32) | r, overflow);
+        }
+      }
+      if (lane == 0U && r1 != r0) {
+        atomicOr(reinterpret_cast<unsigned long long*>(&s.grp_any_route[g]),
+                 static_cast<unsigned long long>(word));
+      }
+    }
+  } else {
+    for (std::uint32_t item = blockIdx.x * blockDim.x + threadIdx.x; item < n_items;
+         item += gridDim.x * blockDim.x) {
+      auto const e = s.list_t[item];
+      auto const g = static_cast<std::uint32_t>(e >> 32), l = static_cast<std::uint32_t>(e);
+      if (!lay<W>::test(s.mark_t, s, g, l, 0U) || (s.grp_active[g] & 1ULL) == 0ULL) continue;
+      auto const r0 = im.location_route_offsets[l], r1 = im.location_route_offsets[l + 1U];
+      for (auto j = r0; j < r1; ++j) {
+        auto const r = im.location_routes[j];
+        auto const bit = lay<W>::rbit(r, 0U);
+        auto const old = atomicOr(
+            reinterpret_cast<unsigned long long*>(&s.route_mark[lay<W>::rword(s, g, r)]),
+            static_cast<unsigned long long>(bit));
+        if ((old & bit) == 0ULL) {
+          list_push(s.list_route, &s.counts[2], s.cap_route,
+                    (static_cast<std::uint64_t>(g) << 32) | r, overflow);
+        }
+      }
+      if (r1 != r0) {
+        atomicOr(reinterpret_cast<unsigned long long*>(&s.grp_any_route[g]), 1ULL);
+      }
+    }
+  }
+}
+
+// ---- update_route for one query (one lane) ---------------------------------
+template <bool Fwd>
+struct et_t {
+  std::uint32_t t_off;
+  std::uint16_t day;
+  __device__ bool valid() const { return day != 0xFFFFU; }
+};
+
+template <bool Fwd>
+__device__ inline std::int16_t to_delta(std::uint16_t day, int mam, std::uint16_t base) {
+  return clamp16((static_cast<int>(day) - static_cast<int>(base)) * 1440 + mam);
+}
+
+// get_earliest_transport (raptor.h:1351-1442)
+template <bool Fwd>
+__device__ inline et_t<Fwd> earliest_transport(dev_image const& im, std::uint32_t r,
+                                               std::uint32_t t_from, std::uint32_t n_t,
+                                               std::uint32_t ev_base, std::uint16_t day_at_stop,
+                                               int mam_at_stop, int lb_l, int tad_k,
+                                               std::uint16_t base) {
+  using D = dirop<Fwd>;
+  auto const rbf = im.route_traffic_days[r];
+  for (std::uint32_t i = 0U; i != 6U; ++i) {  // kMaxTravelTime / 1 day + 1
+    auto const day = static_cast<std::uint16_t>(Fwd ? day_at_stop + i : day_at_stop - i);
+    if (!bit_test(im.bitfields, rbf, day)) continue;
+    std::uint32_t pos = 0U;
+    if (i == 0U) {
+      while (pos != n_t) {
+        auto const t_off = Fwd ? pos : n_t - 1U - pos;
+        if (!D::better(im.event_times[ev_base + t_off] >> 5, mam_at_stop)) break;
+        ++pos;
+      }
+    }
+    for (; pos < n_t; ++pos) {
+      auto const t_off = Fwd ? pos : n_t - 1U - pos;
+      auto const ev = im.event_times[ev_base + t_off];
+      int const ev_mam = ev >> 5;
+      if (!D::better_eq(to_delta<Fwd>(day, ev_mam, base) + D::dir(lb_l), tad_k)) {
+        return {~0U, 0xFFFFU};
+      }
+      if (i == 0U && !D::better_eq(mam_at_stop, ev_mam)) continue;
+      auto const start_day = static_cast<std::uint16_t>(static_cast<int>(day) - (ev & 31));
+      if (!bit_test(im.bitfields, im.transport_traffic_days[t_from + t_off], start_day)) continue;
+      return {t_off, start_day};
+    }
+  }
+  return {~0U, 0xFFFFU};
+}
+
+template <bool Fwd>
+__device__ inline std::int16_t time_at_stop(dev_image const& im, std::uint32_t stf,
+                                            std::uint32_t n_t, et_t<Fwd> et,
+                                            std::uint32_t stop_idx, bool arr,
+                                            std::uint16_t base) {
+  auto const ev = im.event_times[stf + n_t * (stop_idx * 2U - (arr ? 1U : 0U)) + et.t_off];
+  return to_delta<Fwd>(et.day, (ev & 31) * 1440 + (ev >> 5), base);
+}
+
+// route_scan: loop_routes + update_route (raptor.h:496-575, 1181-1349).
+// The route phase reads only round_times[k-1], best, lb, time_at_dest and
+// the previous marks, and writes tmp by minimum and marks by OR: route
+// order does not matter, so routes run in parallel.
+template <bool Fwd, int W>
+__global__ void k_route_scan(round_args a) {
+  using D = dirop<Fwd>;
+  auto const& im = a.im;
+  auto const& s = a.st;
+  auto const k = static_cast<std::uint32_t>(a.k);
+  auto* overflow = &s.counts[4];
+  auto const n_items = clamp_count(s.counts[2], s.cap_route);
+  std::uint32_t const lane_id = W == 64 ? (threadIdx.x & 63U) : 0U;
+  std::uint32_t const step = W == 64 ? gridDim.x * kWavesPerBlock : gridDim.x * blockDim.x;
+  std::uint32_t item = W == 64 ? (blockIdx.x * blockDim.x + threadIdx.x) / 64U
+                               : blockIdx.x * blockDim.x + threadIdx.x;
+  for (; item < n_items; item += step) {
+    auto const e = s.list_route[item];
+    auto const g = static_cast<std::uint32_t>(e >> 32), r = static_cast<std::uint32_t>(e);
+    auto const rw = lay<W>::rword(s, g, r);
+    auto const mword = s.route_mark[rw];
+    std::uint64_t lanes;
+    if constexpr (W == 64) {
+      lanes = mword & s.grp_active[g] & s.grp_any_route[g];
+      if (lane_id == 0U) s.route_mark[rw] = 0ULL;
+    } else {
+      lanes = ((mword >> (r % 64U)) & 1ULL) & s.grp_active[g] & s.grp_any_route[g];
+      // W == 1: several routes share a word; clear only this route's bit
+      atomicAnd(reinterpret_cast<unsigned long long*>(&s.route_mark[rw]),
+                ~static_cast<unsigned long long>(lay<W>::rbit(r, 0U)));
+    }
+    if (lanes == 0ULL) continue;
+    bool const on = ((lanes >> lane_id) & 1ULL) != 0ULL;
+    auto const slot = g * W + lane_id;
+    auto const base = s.param[slot].base;
+    int const tad_k = s.tad[static_cast<std::size_t>(slot) * kRowsMax + k];
+
+    auto const s0 = im.route_stop_offsets[r];
+    auto const n_stops = im.route_stop_offsets[r + 1U] - s0;
+    auto const t_from = im.route_transport_from[r];
+    auto const n_t = im.route_transport_from[r + 1U] - t_from;
+    auto const stf = im.route_stop_time_from[r];
+
+    et_t<Fwd> et{~0U, 0xFFFFU};
+    bool done = !on;
+    bool any = false;
+    for (std::uint32_t i = 0U; i != n_stops; ++i) {
+      auto const stop_idx = Fwd ? i : n_stops - i - 1U;
+      auto const stp = im.route_stops[s0 + stop_idx];
+      auto const l = stp & ((1U << 28) - 1U);
+      auto const in_ok = ((stp >> 28) & 1U) != 0U, out_ok = ((stp >> 29) & 1U) != 0U;
+      bool const can_finish = Fwd ? out_ok : in_ok;
+      bool const can_start = Fwd ? in_ok : out_ok;
+      auto const c = lay<W>::col(s, g, l, lane_id);
+
+      bool upd = false;
+      bool ride = false;
+      if (!done && et.valid() && can_finish) {
+        ride = true;
+        auto const by = time_at_stop<Fwd>(im, stf, n_t, et, stop_idx, Fwd, base);
+        int const lbv = s.lb[c];
+        if (D::better_eq(by, tad_k) && lbv != kUnreachableLb && D::better_eq(by + D::dir(lbv), tad_k)) {
+          atomic_best16<Fwd>(&s.tmp[c], by);
+          upd = true;
+        }
+      }
+      if constexpr (W == 64) {
+        auto const m = __ballot(upd);
+        if (m != 0ULL && lane_id == static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1)) {
+          auto const old = atomicOr(reinterpret_cast<unsigned long long*>(&s.mark_r[lay<W>::lword(s, g, l)]),
+                                    static_cast<unsigned long long>(m));
+          if (old == 0ULL) {
+            list_push(s.list_r, &s.counts[1], s.cap_r, (static_cast<std::uint64_t>(g) << 32) | l, overflow);
+          }
+        }
+      } else {
+        if (upd) mark_one<W>(s.mark_r, s.list_r, &s.counts[1], s.cap_r, overflow, s, g, l, 0U);
+      }
+      any |= upd;
+
+      bool const is_last = i == n_stops - 1U;
+      if (done || is_last || !can_start || !lay<W>::test(s.mark_t, s, g, l, lane_id)) continue;
+      if (s.lb[c] == kUnreachableLb) {
+        done = true;  // break (raptor.h:1309-1311)
+        continue;
+      }
+      auto const et_dep = et.valid() ? time_at_stop<Fwd>(im, stf, n_t, et, stop_idx, !Fwd, base)
+                                     : D::kInvalid;
+      auto const prev = s.round_times[lay<W>::rt(s, g, k - 1U, l, lane_id)];
+      if (prev != D::kInvalid && D::better_eq(prev, et_dep)) {
+        int const minutes = static_cast<int>(base) * 1440 + prev;
+        auto const day = static_cast<std::uint16_t>(minutes / 1440);
+        int const mam = static_cast<std::int16_t>(minutes % 1440);
+        auto const ev_base = stf + n_t * (stop_idx * 2U - (Fwd ? 0U : 1U));
+        auto const new_et = earliest_transport<Fwd>(im, r, t_from, n_t, ev_base, day, mam,
+                                                    s.lb[c], tad_k, base);
+        // current_best is valid whenever a ride was evaluated at this stop
+        // (it then includes the valid round_times[k-1] label) or best / tmp
+        // hold a label
+        bool const cb_valid = ride || s.best[c] != D::kInvalid || s.tmp[c] != D::kInvalid;
+        if (new_et.valid() &&
+            (!cb_valid ||
+             D::better_eq(time_at_stop<Fwd>(im, stf, n_t, new_et, stop_idx, !Fwd, base), et_dep))) {
+          et = new_et;
+        }
+      }
+    }
+    if constexpr (W == 64) {
+      auto const m = __ballot(any);
+      if (m != 0ULL && lane_id == static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1)) {
+        atomicOr(reinterpret_cast<unsigned long long*>(&s.grp_any_tmp[g]),
+                 static_cast<unsigned long long>(m));
+      }
+    } else {
+      if (any) atomicOr(reinterpret_cast<unsigned long long*>(&s.grp_any_tmp[g]), 1ULL);
+    }
+  }
+}
+
+// clear the marks listed in `list` (all lanes)
+template <int W>
+__global__ void k_clear_marks(dev_state s, int which) {
+  auto const n = which == 0 ? clamp_count(s.counts[0], s.cap_t) : clamp_count(s.counts[1], s.cap_r);
+  auto* marks = which == 0 ? s.mark_t : s.mark_r;
+  auto const* list = which == 0 ? s.list_t : s.list_r;
+  for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    auto const e = list[i];
+    marks[lay<W>::lword(s, static_cast<std::uint32_t>(e >> 32), static_cast<std::uint32_t>(e))] = 0ULL;
+  }
+}
+
+// group_update: slots without marked routes or without route improvements
+// end their epoch (raptor.h:243-246, 370-373)
+__global__ void k_group_update(dev_state s) {
+  auto const g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= s.G) return;
+  s.grp_active[g] &= s.grp_any_route[g] & s.grp_any_tmp[g];
+  s.grp_any_route[g] = 0ULL;
+  s.grp_any_tmp[g] = 0ULL;
+}
+
+__global__ void k_deactivate_all(dev_state s) {
+  for (std::uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < s.G; g += gridDim.x * blockDim.x) {
+    s.grp_active[g] = 0ULL;
+    s.grp_any_route[g] = 0ULL;
+    s.grp_any_tmp[g] = 0ULL;
+  }
+}
+
+__global__ void k_reset_counts(dev_state s, std::uint32_t mask) {
+  if (threadIdx.x == 0U && blockIdx.x == 0U) {
+    for (int i = 0; i < 4; ++i) {
+      if ((mask >> i) & 1U) s.counts[i] = 0U;
+    }
+  }
+}
+
+// ---- destination events (exact sequential simulation per query) ---------
+struct dest_event {
+  std::uint32_t pos;
+  std::uint32_t target;
+  std::int16_t x;
+  std::uint8_t kind;  // 0 transfer/footpath, 1 intermodal egress
+};
+
+template <bool Fwd, int W>
+__global__ void k_dest_pass(round_args a, std::uint32_t n_slots) {
+  using D = dirop<Fwd>;
+  auto const& im = a.im;
+  auto const& s = a.st;
+  auto const k = static_cast<std::uint32_t>(a.k);
+  auto const slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= n_slots) return;
+  auto const g = slot / W, lane = slot % W;
+  auto* tad = s.tad + static_cast<std::size_t>(slot) * kRowsMax;
+  s.n_chg[slot] = 0U;
+  s.tad0[slot] = tad[k < kRowsMax ? k : kRowsMax - 1];
+  if ((s.grp_active[g] & (1ULL << lane)) == 0ULL) return;
+  auto const& p = s.param[slot];
+
+  dest_event ev[kMaxDestEvents];
+  std::uint32_t n = 0U;
+  auto const add = [&](std::uint32_t pos, std::uint32_t target, int x, std::uint8_t kind) {
+    if (n < kMaxDestEvents) {
+      ev[n++] = dest_event{pos, target, clamp16(x), kind};
+    } else {
+      atomicOr(&s.counts[4], 2U);
+    }
+  };
+  if (p.intermodal != 0U) {
+    for (std::uint32_t j = 0; j < p.n_dist; ++j) {
+      auto const i = s.dist_loc[slot * kMaxDist + j];
+      if (!lay<W>::test(s.mark_r, s, g, i, lane)) continue;
+      auto const t = s.tmp[lay<W>::col(s, g, i, lane)];
+      if (t == D::kInvalid) continue;
+      add(kPosIntermodal | i, 1U, t + D::dir(s.dist_min[slot * kMaxDist + j]), 1U);
+    }
+  } else {
+    for (std::uint32_t j = 0; j < p.n_dest; ++j) {
+      auto const d = s.dest[slot * kMaxDestBits + j];
+      if (lay<W>::test(s.mark_r, s, g, d, lane)) {
+        auto const t = s.tmp[lay<W>::col(s, g, d, lane)];
+        if (t != D::kInvalid) add(kPosTransfer | d, d, t, 0U);  // transfer time 0 at dest
+      }
+      // footpaths ending in d, at their position in the source's list
+      auto const* off = Fwd ? im.fp_in_offsets : im.fp_out_offsets;
+      auto const* fps = Fwd ? im.fp_in : im.fp_out;
+      auto const* map = Fwd ? im.fp_in_to_out : im.fp_out_to_in;
+      for (auto e = off[d]; e != off[d + 1U]; ++e) {
+        auto const i = fps[e] & ((1U << 23) - 1U);
+        if (!lay<W>::test(s.mark_r, s, g, i, lane)) continue;
+        auto const t = s.tmp[lay<W>::col(s, g, i, lane)];
+        if (t == D::kInvalid) continue;
+        auto const dur = static_cast<std::int16_t>(fps[e] >> 23);
+        add(kPosFootpath | map[e], d, t + D::dir(adjusted<std::int16_t>(p, dur)), 0U);
+      }
+    }
+  }
+  // insertion sort by position
+  for (std::uint32_t i = 1U; i < n; ++i) {
+    auto const v = ev[i];
+    auto j = i;
+    while (j > 0U && ev[j - 1U].pos > v.pos) {
+      ev[j] = ev[j - 1U];
+      --j;
+    }
+    ev[j] = v;
+  }
+  std::uint32_t n_chg = 0U;
+  for (std::uint32_t i = 0U; i < n; ++i) {
+    auto const& e = ev[i];
+    auto const before = tad[k];
+    auto const c = lay<W>::col(s, g, e.target, lane);
+    auto const rti = lay<W>::rt(s, g, k, e.target, lane);
+    if (e.kind == 1U) {
+      // update_intermodal_footpaths (raptor.h:1009-1025)
+      if (D::better_eq(e.x, tad[k])) {
+        s.round_times[rti] = e.x;
+        s.best[c] = e.x;
+        update_tad<Fwd>(tad, k, e.x);
+      }
+    } else {
+      // update_transfers / update_footpaths for a destination target
+      if (D::better(e.x, s.best[c])) {
+        s.round_times[rti] = D::best_of(e.x, s.round_times[rti]);
+      }
+      if (D::better(e.x, s.best[c]) && D::better_eq(e.x, tad[k])) {
+        int const lbv = s.lb[c];
+        if (lbv == kUnreachableLb || !D::better_eq(e.x + D::dir(lbv), tad[k])) continue;
+        s.round_times[rti] = e.x;
+        s.best[c] = e.x;
+        mark_one<W>(s.mark_t, s.list_t, &s.counts[0], s.cap_t, &s.counts[4], s, g, e.target, lane);
+        update_tad<Fwd>(tad, k, e.x);
+      }
+    }
+    if (tad[k] != before) {
+      if (n_chg < kMaxChanges) {
+        s.chg_pos[slot * kMaxChanges + n_chg] = e.pos;
+        s.chg_tad[slot * kMaxChanges + n_chg] = tad[k];
+        ++n_chg;
+      } else {
+        atomicOr(&s.counts[4], 4U);
+      }
+    }
+  }
+  s.n_chg[slot] = n_chg;
+}
+
+// One non-destination relaxation (update_transfers raptor.h:699-740 /
+// update_footpaths raptor.h:785-833).  Returns true when it improved best
+// (the station becomes marked).
+template <bool Fwd, int W>
+__device__ inline bool relax(dev_state const& s, std::uint32_t g, std::uint32_t lane,
+                             std::uint32_t slot, std::uint32_t k, std::uint32_t t, int x,
+                             std::uint32_t pos, std::int16_t tad0) {
+  using D = dirop<Fwd>;
+  auto const c = lay<W>::col(s, g, t, lane);
+  auto const xv = clamp16(x);
+  if (!D::better(xv, s.best[c])) return false;
+  auto const tad_e = tad_at(s, slot, pos);
+  int const lbv = s.lb[c];
+  bool const elig = D::better_eq(xv, tad_e) && lbv != kUnreachableLb &&
+                    D::better_eq(xv + D::dir(lbv), tad_e);
+  if (elig) {
+    auto const old = atomic_best16<Fwd>(&s.best[c], xv);
+    return D::better(xv, old);
+  }
+  atomic_best16<Fwd>(&s.round_times[lay<W>::rt(s, g, k, t, lane)], xv);  // pre-write
+  if (tad_e != tad0) {
+    list_push(s.list_hz, &s.counts[3], s.cap_hz, (static_cast<std::uint64_t>(slot) << 32) | t,
+              &s.counts[4]);
+  }
+  return false;
+}
+
+template <bool Fwd, int W>
+__global__ void k_transfer_fp(round_args a) {
+  using D = dirop<Fwd>;
+  auto const& im = a.im;
+  auto const& s = a.st;
+  auto const k = static_cast<std::uint32_t>(a.k);
+  auto* overflow = &s.counts[4];
+  auto const n_items = clamp_count(s.counts[1], s.cap_r);
+  std::uint32_t const lane_id = W == 64 ? (threadIdx.x & 63U) : 0U;
+  std::uint32_t const step = W == 64 ? gridDim.x * kWavesPerBlock : gridDim.x * blockDim.x;
+  std::uint32_t item = W == 64 ? (blockIdx.x * blockDim.x + threadIdx.x) / 64U
+                               : blockIdx.x * blockDim.x + threadIdx.x;
+  auto const* off = Fwd ? im.fp_out_offsets : im.fp_in_offsets;
+  auto const* fps = Fwd ? im.fp_out : im.fp_in;
+  for (; item < n_items; item += step) {
+    auto const e = s.list_r[item];
+    auto const g = static_cast<std::uint32_t>(e >> 32), i = static_cast<std::uint32_t>(e);
+    auto const lanes = lay<W>::lanes_at(s.mark_r[lay<W>::lword(s, g, i)], i) & s.grp_active[g];
+    if (lanes == 0ULL) continue;
+    bool const on = ((lanes >> lane_id) & 1ULL) != 0ULL;
+    auto const slot = g * W + lane_id;
+    auto const& p = s.param[slot];
+    auto const tmp_i = s.tmp[lay<W>::col(s, g, i, lane_id)];
+    bool const valid = on && tmp_i != D::kInvalid;
+    auto const tad0 = s.tad0[slot];
+
+    auto const emit = [&](std::uint32_t t, bool improved) {
+      if constexpr (W == 64) {
+        auto const m = __ballot(improved);
+        if (m != 0ULL && lane_id == static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1)) {
+          auto const old = atomicOr(reinterpret_cast<unsigned long long*>(&s.mark_t[lay<W>::lword(s, g, t)]),
+                                    static_cast<unsigned long long>(m));
+          if (old == 0ULL) {
+            list_push(s.list_t, &s.counts[0], s.cap_t, (static_cast<std::uint64_t>(g) << 32) | t, overflow);
+          }
+        }
+      } else {
+        if (improved) mark_one<W>(s.mark_t, s.list_t, &s.counts[0], s.cap_t, overflow, s, g, t, 0U);
+      }
+    };
+
+    // transfer at i (destination stations are handled by k_dest_pass)
+    {
+      bool const is_dest = lay<W>::test(s.dest_mask, s, g, i, lane_id);
+      bool improved = false;
+      if (valid && !is_dest) {
+        auto const tt = adjusted<std::uint8_t>(p, im.transfer_time[i]);
+        improved = relax<Fwd, W>(s, g, lane_id, slot, k, i, tmp_i + D::dir(tt), kPosTransfer | i, tad0);
+      }
+      emit(i, improved);
+    }
+    // footpaths of i in list order
+    for (auto fe = off[i]; fe != off[i + 1U]; ++fe) {
+      auto const fp = fps[fe];
+      auto const t = fp & ((1U << 23) - 1U);
+      bool improved = false;
+      if (valid && !lay<W>::test(s.dest_mask, s, g, t, lane_id)) {
+        auto const dur = static_cast<std::int16_t>(fp >> 23);
+        improved = relax<Fwd, W>(s, g, lane_id, slot, k, t, tmp_i + D::dir(adjusted<std::int16_t>(p, dur)),
+                                 kPosFootpath | fe, tad0);
+      }
+      emit(t, improved);
+    }
+  }
+}
+
+// round_times[k] = best for every station improved in the transfer phase
+// (a passing relaxation overwrites round_times, raptor.h:734, 827)
+template <bool Fwd, int W>
+__global__ void k_rt_fixup(round_args a) {
+  auto const& s = a.st;
+  auto const k = static_cast<std::uint32_t>(a.k);
+  auto const n = clamp_count(s.counts[0], s.cap_t);
+  for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n * W;
+       i += gridDim.x * blockDim.x) {
+    auto const item = i / W, lane = i % W;
+    auto const e = s.list_t[item];
+    auto const g = static_cast<std::uint32_t>(e >> 32), t = static_cast<std::uint32_t>(e);
+    if (!lay<W>::test(s.mark_t, s, g, t, lane)) continue;
+    if ((s.grp_active[g] & (1ULL << lane)) == 0ULL) continue;
+    if (lay<W>::test(s.dest_mask, s, g, t, lane)) continue;
+    s.round_times[lay<W>::rt(s, g, k, t, lane)] = s.best[lay<W>::col(s, g, t, lane)];
+  }
+}
+
+// Exact replay for hazard stations: last passing relaxation P is the first
+// eligible one with x == best; later pre-writes lower round_times below it.
+template <bool Fwd, int W>
+__global__ void k_hazards(round_args a) {
+  using D = dirop<Fwd>;
+  auto const& im = a.im;
+  auto const& s = a.st;
+  auto const k = static_cast<std::uint32_t>(a.k);
+  auto const n = clamp_count(s.counts[3], s.cap_hz);
+  for (std::uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < n; h += gridDim.x * blockDim.x) {
+    auto const e = s.list_hz[h];
+    auto const slot = static_cast<std::uint32_t>(e >> 32), t = static_cast<std::uint32_t>(e);
+    auto const g = slot / W, lane = slot % W;
+    if (!lay<W>::test(s.mark_t, s, g, t, lane)) continue;  // no passing relaxation
+    if (lay<W>::test(s.dest_mask, s, g, t, lane)) continue;
+    auto const& p = s.param[slot];
+    auto const c = lay<W>::col(s, g, t, lane);
+    auto const bt = s.best[c];
+    int const lbv = s.lb[c];
+    auto const eligible = [&](std::int16_t x, std::uint32_t pos) {
+      auto const tad_e = tad_at(s, slot, pos);
+      return D::better_eq(x, tad_e) && lbv != kUnreachableLb && D::better_eq(x + D::dir(lbv), tad_e);
+    };
+    // enumerate events into t: (pos, x)
+    auto const* off = Fwd ? im.fp_in_offsets : im.fp_out_offsets;
+    auto const* fps = Fwd ? im.fp_in : im.fp_out;
+    auto const* map = Fwd ? im.fp_in_to_out : im.fp_out_to_in;
+    auto const for_each_event = [&](auto&& fn) {
+      if (lay<W>::test(s.mark_r, s, g, t, lane)) {
+        auto const tmp_t = s.tmp[c];
+        if (tmp_t != D::kInvalid) {
+          fn(kPosTransfer | t, clamp16(tmp_t + D::dir(adjusted<std::uint8_t>(p, im.transfer_time[t]))));
+        }
+      }
+      for (auto fe = off[t]; fe != off[t + 1U]; ++fe) {
+        auto const src = fps[fe] & ((1U << 23) - 1U);
+        if (!lay<W>::test(s.mark_r, s, g, src, lane)) continue;
+        auto const tmp_s = s.tmp[lay<W>::col(s, g, src, lane)];
+        if (tmp_s == D::kInvalid) continue;
+        auto const dur = static_cast<std::int16_t>(fps[fe] >> 23);
+        fn(kPosFootpath | map[fe], clamp16(tmp_s + D::dir(adjusted<std::int16_t>(p, dur))));
+      }
+    };
+    std::uint32_t pos_p = 0xFFFFFFFFU;
+    for_each_event([&](std::uint32_t pos, std::int16_t x) {
+      if (x == bt && eligible(x, pos) && pos < pos_p) pos_p = pos;
+    });
+    if (pos_p == 0xFFFFFFFFU) continue;
+    auto v = bt;
+    for_each_event([&](std::uint32_t pos, std::int16_t x) {
+      if (pos > pos_p) v = D::best_of(x, v);
+    });
+    s.round_times[lay<W>::rt(s, g, k, t, lane)] = v;
+  }
+}
+
+// ======================================================================
+// epoch begin / end
+
+template <int W>
+__global__ void k_lane_reset(epoch_args a) {
+  auto const& s = a.st;
+  auto const per_group = static_cast<std::size_t>(s.L) * W;
+  auto const total = per_group * a.n_reset_groups;
+  for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < total;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    auto const g = a.reset_groups[i / per_group];
+    auto const rem = i % per_group;
+    auto const l = static_cast<std::uint32_t>(rem / W), lane = static_cast<std::uint32_t>(rem % W);
+    auto const bit = 1ULL << lane;
+    if ((a.reset_rt_mask[g] & bit) != 0ULL) {
+      for (std::uint32_t k = 0; k < s.K; ++k) {
+        s.round_times[lay<W>::rt(s, g, k, l, lane)] = a.invalid;
+      }
+    }
+    if ((a.reset_bt_mask[g] & bit) != 0ULL) {
+      auto const c = lay<W>::col(s, g, l, lane);
+      s.best[c] = a.invalid;
+      s.tmp[c] = a.invalid;
+    }
+  }
+}
+
+// lower bounds: staging rows -> interleaved columns.  W == 64: LDS-tiled
+// transpose, one block per (group, 64-location tile), both sides coalesced.
+template <int W>
+__global__ void k_lb_scatter(epoch_args a) {
+  auto const& s = a.st;
+  if constexpr (W == 64) {
+    __shared__ std::uint16_t tile[64][65];
+    auto const n_tiles = (s.L + 63U) / 64U;
+    for (std::uint32_t b = blockIdx.x; b < n_tiles * a.n_lb_groups; b += gridDim.x) {
+      auto const g = a.lb_groups[b / n_tiles];
+      auto const l0 = (b % n_tiles) * 64U;
+      __syncthreads();
+      for (std::uint32_t v = threadIdx.x; v < 64U * 64U; v += blockDim.x) {
+        auto const lane = v / 64U, x = v % 64U;
+        auto const row = a.lb_row_of_slot[g * 64U + lane];
+        if (row >= 0 && l0 + x < s.L) {
+          tile[x][lane] = a.lb_staging[static_cast<std::size_t>(row) * s.L + l0 + x];
+        }
+      }
+      __syncthreads();
+      for (std::uint32_t v = threadIdx.x; v < 64U * 64U; v += blockDim.x) {
+        auto const x = v / 64U, lane = v % 64U;
+        if (a.lb_row_of_slot[g * 64U + lane] >= 0 && l0 + x < s.L) {
+          s.lb[lay<W>::col(s, g, l0 + x, lane)] = tile[x][lane];
+        }
+      }
+    }
+  } else {
+    auto const total = static_cast<std::size_t>(a.n_lb) * s.L;
+    for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < total;
+         i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+      auto const row = i / s.L;
+      auto const l = static_cast<std::uint32_t>(i % s.L);
+      s.lb[lay<W>::col(s, a.lb_slot[row], l, 0U)] = a.lb_staging[i];
+    }
+  }
+}
+
+template <int W>
+__global__ void k_dest_ops(epoch_args a) {
+  auto const& s = a.st;
+  for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n_dest_ops; i += gridDim.x * blockDim.x) {
+    auto const op = a.dest_ops[i];
+    auto const slot = static_cast<std::uint32_t>(op >> 32), l = static_cast<std::uint32_t>(op);
+    auto const g = slot / W, lane = slot % W;
+    auto const bit = lay<W>::lbit(l, lane);
+    auto* w = reinterpret_cast<unsigned long long*>(&s.dest_mask[lay<W>::lword(s, g, l)]);
+    if (a.dest_set[i] != 0U) {
+      atomicOr(w, static_cast<unsigned long long>(bit));
+    } else {
+      atomicAnd(w, ~static_cast<unsigned long long>(bit));
+    }
+  }
+}
+
+template <bool Fwd, int W>
+__global__ void k_slot_init(epoch_args a) {
+  auto const& s = a.st;
+  auto const slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= a.n_slots) return;
+  auto const f = a.slot_flags[slot];
+  if ((f & 1U) == 0U) return;
+  auto* tad = s.tad + static_cast<std::size_t>(slot) * kRowsMax;
+  if ((f & 2U) != 0U) {
+    for (int i = 0; i < kRowsMax; ++i) tad[i] = a.invalid;  // reset_arrivals
+  }
+  auto const w = a.slot_worst[slot];
+  for (int i = 0; i < kRowsMax; ++i) tad[i] = dirop<Fwd>::best_of(w, tad[i]);
+  auto const g = slot / W, lane = slot % W;
+  atomicOr(reinterpret_cast<unsigned long long*>(&s.grp_active[g]), 1ULL << lane);
+}
+
+// add_start (raptor.h:190-203)
+template <bool Fwd, int W>
+__global__ void k_add_starts(epoch_args a) {
+  auto const& s = a.st;
+  for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n_starts; i += gridDim.x * blockDim.x) {
+    auto const slot = a.start_slot[i], l = a.start_loc[i];
+    auto const d = a.start_delta[i];
+    auto const g = slot / W, lane = slot % W;
+    atomic_best16<Fwd>(&s.best[lay<W>::col(s, g, l, lane)], d);
+    atomic_best16<Fwd>(&s.round_times[lay<W>::rt(s, g, 0U, l, lane)], d);
+    mark_one<W>(s.mark_t, s.list_t, &s.counts[0], s.cap_t, &s.counts[4], s, g, l, lane);
+  }
+}
+
+// journey extraction input: round_times[k][dest bit] for every running slot
+template <int W>
+__global__ void k_extract(dev_state s, std::uint32_t n_slots) {
+  auto const total = static_cast<std::size_t>(n_slots) * kMaxDestBits * s.K;
+  for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < total;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    auto const slot = static_cast<std::uint32_t>(i / (kMaxDestBits * s.K));
+    auto const rem = static_cast<std::uint32_t>(i % (kMaxDestBits * s.K));
+    auto const j = rem / s.K, k = rem % s.K;
+    auto const& p = s.param[slot];
+    auto const nbits = p.intermodal != 0U ? 1U : p.n_dest;
+    if (j >= nbits) continue;
+    auto const d = p.intermodal != 0U ? 1U : s.dest[slot * kMaxDestBits + j];
+    s.dest_rows[i] = s.round_times[lay<W>::rt(s, slot / W, k, d, slot % W)];
+  }
+}
+
+// copies one slot's state into contiguous rows (state download)
+template <int W>
+__global__ void k_gather_slot(dev_state s, std::uint32_t slot, std::int16_t* rt, std::int16_t* best,
+                              std::int16_t* tmp) {
+  auto const g = slot / W, lane = slot % W;
+  auto const total = static_cast<std::size_t>(s.K + 2U) * s.L;
+  for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < total;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    auto const row = static_cast<std::uint32_t>(i / s.L), l = static_cast<std::uint32_t>(i % s.L);
+    if (row < s.K) {
+      rt[i] = s.round_times[lay<W>::rt(s, g, row, l, lane)];
+    } else if (row == s.K) {
+      best[l] = s.best[lay<W>::col(s, g, l, lane)];
+    } else {
+      tmp[l] = s.tmp[lay<W>::col(s, g, l, lane)];
+    }
+  }
+}
+
+int grid_for(std::size_t work, int per_block, int cap) {
+  auto const b = (work + static_cast<std::size_t>(per_block) - 1U) / static_cast<std::size_t>(per_block);
+  return static_cast<int>(b < 1U ? 1U : (b > static_cast<std::size_t>(cap) ? static_cast<std::size_t>(cap) : b));
+}
+
+template <bool Fwd, int W>
+void round_impl(round_args const& a, std::uint32_t const* sweep_groups, std::uint32_t n_sweep,
+                std::uint32_t n_slots, int grid_waves, hipStream_t st) {
+  auto const slot_blocks = grid_for(n_slots, kBlock, 1 << 20);
+  auto const item_blocks = W == 64 ? (grid_waves + kWavesPerBlock - 1) / kWavesPerBlock
+                                   : (grid_waves * 64 + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL((k_round_end_check<W>), dim3(slot_blocks), dim3(kBlock), 0, st, a, n_slots);
+  if (n_sweep != 0U) {
+    hipLaunchKernelGGL((k_best_sweep<Fwd, W>), dim3(2048), dim3(kBlock), 0, st, a, sweep_groups, n_sweep);
+  }
+  hipLaunchKernelGGL((k_round_begin<Fwd, W>), dim3(slot_blocks), dim3(kBlock), 0, st, a, n_slots);
+  hipLaunchKernelGGL((k_mark_routes<W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
+  hipLaunchKernelGGL((k_route_scan<Fwd, W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
+  hipLaunchKernelGGL((k_clear_marks<W>), dim3(1024), dim3(kBlock), 0, st, a.st, 0);
+  hipLaunchKernelGGL((k_group_update), dim3(grid_for(a.st.G, kBlock, 1 << 20)), dim3(kBlock), 0, st, a.st);
+  hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 1U | 4U);  // list_t, routes
+  hipLaunchKernelGGL((k_dest_pass<Fwd, W>), dim3(grid_for(n_slots, 64, 1 << 20)), dim3(64), 0, st, a, n_slots);
+  hipLaunchKernelGGL((k_transfer_fp<Fwd, W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
+  hipLaunchKernelGGL((k_rt_fixup<Fwd, W>), dim3(1024), dim3(kBlock), 0, st, a);
+  hipLaunchKernelGGL((k_hazards<Fwd, W>), dim3(64), dim3(kBlock), 0, st, a);
+  hipLaunchKernelGGL((k_clear_marks<W>), dim3(1024), dim3(kBlock), 0, st, a.st, 1);
+  hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 2U | 8U);  // list_r, hazards
+}
+
+template <bool Fwd, int W>
+void epoch_begin_impl(epoch_args const& a, hipStream_t st) {
+  if (a.n_reset_groups != 0U) {
+    hipLaunchKernelGGL((k_lane_reset<W>), dim3(4096), dim3(kBlock), 0, st, a);
+  }
+  if (a.n_dest_ops != 0U) {
+    hipLaunchKernelGGL((k_dest_ops<W>), dim3(grid_for(a.n_dest_ops, kBlock, 4096)), dim3(kBlock), 0, st, a);
+  }
+  hipLaunchKernelGGL((k_slot_init<Fwd, W>), dim3(grid_for(a.n_slots, kBlock, 1 << 20)), dim3(kBlock), 0, st, a);
+  if (a.n_starts != 0U) {
+    hipLaunchKernelGGL((k_add_starts<Fwd, W>), dim3(grid_for(a.n_starts, kBlock, 4096)), dim3(kBlock), 0, st, a);
+  }
+}
+
+}  // namespace
+
+void launch_round(bool fwd, std::uint32_t W, round_args const& a, std::uint32_t const* sweep_groups,
+                  std::uint32_t n_sweep_groups, std::uint32_t n_slots, int grid_waves,
+                  hipStream_t stream) {
+  if (W == 64U) {
+    fwd ? round_impl<true, 64>(a, sweep_groups, n_sweep_groups, n_slots, grid_waves, stream)
+        : round_impl<false, 64>(a, sweep_groups, n_sweep_groups, n_slots, grid_waves, stream);
+  } else {
+    fwd ? round_impl<true, 1>(a, sweep_groups, n_sweep_groups, n_slots, grid_waves, stream)
+        : round_impl<false, 1>(a, sweep_groups, n_sweep_groups, n_slots, grid_waves, stream);
+  }
+}
+
+void launch_lb_scatter(std::uint32_t W, epoch_args const& a, hipStream_t stream) {
+  if (a.n_lb == 0U) return;
+  if (W == 64U) {
+    hipLaunchKernelGGL((k_lb_scatter<64>), dim3(4096), dim3(kBlock), 0, stream, a);
+  } else {
+    hipLaunchKernelGGL((k_lb_scatter<1>), dim3(4096), dim3(kBlock), 0, stream, a);
+  }
+}
+
+void launch_epoch_begin(bool fwd, std::uint32_t W, epoch_args const& a, hipStream_t stream) {
+  if (W == 64U) {
+    fwd ? epoch_begin_impl<true, 64>(a, stream) : epoch_begin_impl<false, 64>(a, stream);
+  } else {
+    fwd ? epoch_begin_impl<true, 1>(a, stream) : epoch_begin_impl<false, 1>(a, stream);
+  }
+}
+
+void launch_gather_slot(std::uint32_t W, dev_state const& st, std::uint32_t slot, std::int16_t* rt,
+                        std::int16_t* best, std::int16_t* tmp, hipStream_t stream) {
+  if (W == 64U) {
+    hipLaunchKernelGGL((k_gather_slot<64>), dim3(2048), dim3(kBlock), 0, stream, st, slot, rt, best, tmp);
+  } else {
+    hipLaunchKernelGGL((k_gather_slot<1>), dim3(2048), dim3(kBlock), 0, stream, st, slot, rt, best, tmp);
+  }
+}
+
+void launch_epoch_end(std::uint32_t W, dev_state const& st, std::uint32_t n_slots,
+                      hipStream_t stream) {
+  auto const work = static_cast<std::size_t>(n_slots) * kMaxDestBits * st.K;
+  if (W == 64U) {
+    hipLaunchKernelGGL((k_extract<64>), dim3(grid_for(work, kBlock, 4096)), dim3(kBlock), 0, stream, st, n_slots);
+    hipLaunchKernelGGL((k_clear_marks<64>), dim3(1024), dim3(kBlock), 0, stream, st, 0);
+  } else {
+    hipLaunchKernelGGL((k_extract<1>), dim3(grid_for(work, kBlock, 4096)), dim3(kBlock), 0, stream, st, n_slots);
+    hipLaunchKernelGGL((k_clear_marks<1>), dim3(1024), dim3(kBlock), 0, stream, st, 0);
+  }
+  hipLaunchKernelGGL((k_deactivate_all), dim3(grid_for(st.G, kBlock, 1024)), dim3(kBlock), 0, stream, st);
+  hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, stream, st, 0xFU);
+}
+
+}  // namespace nigiri_amd

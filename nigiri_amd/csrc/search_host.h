@@ -1,0 +1,98 @@
+// Host half of the search driver (nigiri's search<Dir, Algo>, include/
+// nigiri/routing/search.h:78-505).  It prepares everything a query needs
+// before its rounds run on the device — destinations, lower bounds, fastest
+// direct connection, base day, start-label groups — and turns the
+// destination round times of each epoch into the final Pareto set.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "timetable.h"
+
+namespace nigiri_amd {
+
+struct host_tts {
+  bool default_{true};
+  std::int16_t min_transfer_time_{0}, additional_time_{0};
+  float factor_{1.0F};
+};
+
+struct host_journey {
+  std::int32_t start_time_, dest_time_;
+  std::uint32_t dest_;
+  std::uint8_t transfers_;
+  bool reconstructed_{false};
+  bool dominates(host_journey const& o) const {  // journey.h:80-88
+    if (start_time_ <= dest_time_) {
+      return transfers_ <= o.transfers_ && start_time_ >= o.start_time_ &&
+             dest_time_ <= o.dest_time_;
+    }
+    return transfers_ <= o.transfers_ && start_time_ <= o.start_time_ &&
+           dest_time_ >= o.dest_time_;
+  }
+  std::int16_t travel_time() const {
+    return static_cast<std::int16_t>(std::abs(dest_time_ - start_time_));
+  }
+};
+
+// One start-time group (search.h:424-454): the labels added before one
+// execute() call.
+struct epoch {
+  std::int32_t start_time_;
+  std::int32_t worst_time_at_dest_;
+  std::vector<std::uint32_t> loc_;
+  std::vector<std::int32_t> time_;  // time at stop (unix minutes)
+};
+
+struct prepared_query {
+  bool fwd_{true};
+  bool ontrip_{true};
+  bool intermodal_dest_{false};
+  bool overlap_{false};  // start/destination overlap: empty result
+  std::uint8_t max_transfers_{14};
+  std::uint8_t end_k_{16};
+  std::int16_t fastest_direct_{32767};
+  std::int16_t max_travel_time_{7200};
+  std::uint16_t base_{0};
+  std::int32_t iv_from_{0}, iv_to_{0};
+  host_tts tts_;
+  std::vector<std::uint32_t> dests_;  // ascending (non-intermodal)
+  std::vector<std::uint32_t> dist_loc_;  // ascending (intermodal)
+  std::vector<std::uint16_t> dist_min_;
+  std::vector<std::uint16_t> lb_;  // [n_locations]
+  std::vector<epoch> epochs_;
+  std::vector<host_journey> results_;  // running Pareto set
+
+  std::int32_t base_minutes(timetable const& tt) const {
+    return tt.internal_from + static_cast<std::int32_t>(base_) * 1440;
+  }
+  // number of destination rows reported per epoch
+  std::uint32_t n_dest_bits() const {
+    return intermodal_dest_ ? 1U : static_cast<std::uint32_t>(dests_.size());
+  }
+  std::uint32_t dest_bit(std::uint32_t j) const {
+    return intermodal_dest_ ? kEnd : dests_[j];
+  }
+};
+
+// Scratch reused across queries by one host worker (Dijkstra buckets).
+struct host_scratch {
+  std::vector<std::vector<std::uint32_t>> buckets_;
+};
+
+// Throws status_error for unsupported or malformed queries.
+void prepare(timetable const& tt, nigiri_query const& q, prepared_query& p,
+             host_scratch& s);
+
+// Journey extraction of one epoch (raptor.h:393-413) followed by the
+// reconstruction gate of search_interval() (search.h:457-475).
+// dest_rows[j * max_k + k] is round_times[k][dest bit j] (delta).
+void add_epoch_results(timetable const& tt, prepared_query& p,
+                       std::size_t epoch_idx, std::int16_t const* dest_rows,
+                       std::uint32_t max_k);
+
+// Final filtering of search::execute (search.h:319-336).
+void finish(prepared_query& p);
+
+}  // namespace nigiri_amd

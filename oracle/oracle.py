@@ -1,0 +1,137 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the CPU
+baseline leg of bench.py.  The product package never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "n_routes_visited", "n_route_stops_visited", "n_earliest_trip_calls", "n_events_scanned",
+        "n_route_day_tests", "n_transport_day_tests", "n_marked_stations",
+        "n_location_routes_read", "n_transfer_stations", "n_footpaths_visited",
+        "n_label_writes", "n_rounds", "n_epochs", "alg_bytes")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+def build(arch_flags=""):
+    """Compiles the oracle with its Makefile (host C++ only)."""
+    cmd = ["make", "-C", HERE]
+    if arch_flags:
+        cmd.append(f"ARCHFLAGS={arch_flags}")
+    subprocess.run(cmd, check=True, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        from nigiri_amd import TtView, Query, Journey  # struct layouts only
+        h = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        h.oracle_search.restype = C.c_int
+        h.oracle_search.argtypes = [P(TtView), P(Query), P(Journey), C.c_uint32, P(C.c_uint32),
+                                    P(C.c_int16), P(C.c_int16), P(C.c_int16), P(C.c_int16),
+                                    P(C.c_uint16), P(C.c_int32), P(Stats)]
+        h.oracle_lb.restype = C.c_int
+        h.oracle_lb.argtypes = [P(TtView), P(Query), P(C.c_uint16)]
+        h.oracle_fastest_direct.restype = C.c_int
+        h.oracle_fastest_direct.argtypes = [P(TtView), P(Query), P(C.c_int16)]
+        h.oracle_starts.restype = C.c_int
+        h.oracle_starts.argtypes = [P(TtView), P(Query), P(C.c_int32), P(C.c_int32),
+                                    P(C.c_uint32), C.c_uint32, P(C.c_uint32)]
+        h.oracle_search_many.restype = C.c_int
+        h.oracle_search_many.argtypes = [P(TtView), P(Query), C.c_uint32, C.c_int,
+                                         P(C.c_double), P(C.c_double), P(C.c_uint32), P(Stats)]
+        _lib = h
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def search(tt, q, state=False, cap=4096):
+    """Runs one query; returns journeys (start, dest_time, dest, transfers) and optionally state."""
+    from nigiri_amd import Journey
+    L = tt.n_locations
+    jr = (Journey * cap)()
+    n = C.c_uint32()
+    info = np.zeros(4, np.int32)
+    stats = Stats()
+    rt = best = tmp = tad = lb = None
+    args = [None] * 5
+    if state:
+        rt = np.zeros((16, L), np.int16)
+        best = np.zeros(L, np.int16)
+        tmp = np.zeros(L, np.int16)
+        tad = np.zeros(16, np.int16)
+        lb = np.zeros(L, np.uint16)
+        args = [_p(rt, C.c_int16), _p(best, C.c_int16), _p(tmp, C.c_int16), _p(tad, C.c_int16),
+                _p(lb, C.c_uint16)]
+    rc = lib().oracle_search(C.byref(tt.view), C.byref(q), jr, cap, C.byref(n), *args,
+                             _p(info, C.c_int32), C.byref(stats))
+    if rc != 0:
+        raise RuntimeError("oracle_search failed")
+    js = [(jr[i].start_time, jr[i].dest_time, jr[i].dest, jr[i].transfers)
+          for i in range(min(n.value, cap))]
+    out = {"journeys": js, "base_day": int(info[0]), "fastest_direct": int(info[1]),
+           "stats": stats.as_dict()}
+    if state:
+        out.update(round_times=rt, best=best, tmp=tmp, time_at_dest=tad, lb=lb)
+    return out
+
+
+def lower_bounds(tt, q):
+    lb = np.zeros(tt.n_locations, np.uint16)
+    if lib().oracle_lb(C.byref(tt.view), C.byref(q), _p(lb, C.c_uint16)) != 0:
+        raise RuntimeError("oracle_lb failed")
+    return lb
+
+
+def fastest_direct(tt, q):
+    out = C.c_int16()
+    if lib().oracle_fastest_direct(C.byref(tt.view), C.byref(q), C.byref(out)) != 0:
+        raise RuntimeError("oracle_fastest_direct failed")
+    return out.value
+
+
+def starts(tt, q, cap=1 << 16):
+    a = np.zeros(cap, np.int32)
+    b = np.zeros(cap, np.int32)
+    s = np.zeros(cap, np.uint32)
+    n = C.c_uint32()
+    if lib().oracle_starts(C.byref(tt.view), C.byref(q), _p(a, C.c_int32), _p(b, C.c_int32),
+                           _p(s, C.c_uint32), cap, C.byref(n)) != 0:
+        raise RuntimeError("oracle_starts failed")
+    k = min(n.value, cap)
+    return list(zip(a[:k].tolist(), b[:k].tolist(), s[:k].tolist()))
+
+
+def search_many(tt, queries, threads=1):
+    n = len(queries)
+    lat = np.zeros(max(n, 1), np.float64)
+    nj = np.zeros(max(n, 1), np.uint32)
+    wall = C.c_double()
+    stats = Stats()
+    rc = lib().oracle_search_many(C.byref(tt.view), queries.q, n, threads, _p(lat, C.c_double),
+                                  C.byref(wall), _p(nj, C.c_uint32), C.byref(stats))
+    if rc != 0:
+        raise RuntimeError("oracle_search_many failed")
+    return {"latency_ms": lat[:n], "wall_ms": wall.value, "n_journeys": nj[:n],
+            "stats": stats.as_dict()}

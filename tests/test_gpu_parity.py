@@ -1,0 +1,136 @@
+"""GPU parity: the HIP engine (through the C ABI) against the CPU oracle.
+
+Bar: bit-exact journeys (start, dest_time, dest, transfers) and bit-exact
+final state (round_times rows, best, tmp, time_at_dest) per query.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import nigiri_amd as na
+import oracle
+from golden import raptor_abc as G
+from test_oracle_golden import abc_timetable, hm
+
+pytestmark = pytest.mark.gpu
+
+STATE_KEYS = ("round_times", "best", "tmp", "time_at_dest")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+_images = {}
+
+
+def image_for(key, make_tt):
+    if key not in _images:
+        tt = make_tt()
+        _images[key] = (tt, na.Image(tt))
+    return _images[key]
+
+
+def compare(tt, eng, qs, idx=None, check_state=True):
+    res = eng.search(qs)
+    n = len(qs)
+    for i in range(n):
+        ref = oracle.search(tt, qs[i], state=check_state)
+        got = res[i]
+        assert got["status"] == 0
+        assert got["journeys"] == ref["journeys"], (i, got["journeys"], ref["journeys"])
+        assert got["base_day"] == ref["base_day"]
+        if check_state and i < eng.slots:
+            st = eng.download(i)
+            K = eng.max_k
+            for key in STATE_KEYS:
+                a = st[key]
+                b = ref[key][:K] if key == "round_times" else ref[key]
+                if not np.array_equal(a, b):
+                    diff = np.argwhere(a != b)
+                    raise AssertionError(f"query {i} {key} differs at {diff[:5].tolist()}: "
+                                         f"gpu {a[tuple(diff[0])]} oracle {b[tuple(diff[0])]}")
+    return res
+
+
+def test_gpu_golden_abc_range_fwd_bwd():
+    tt, loc, day0 = abc_timetable()
+    img = na.Image(tt)
+    eng = na.Engine(img, lanes=64, groups=1, max_k=16)
+    iv = (day0 + hm(G.INTERVAL[0]), day0 + hm(G.INTERVAL[1]))
+    for d, (s, t) in ((na.DIR_FWD, ("A", "C")), (na.DIR_BWD, ("C", "A"))):
+        say(f"golden abc dir={d}")
+        qs = na.Queries.build([dict(start=[(loc[s], 0)], dest=[(loc[t], 0)], interval=iv, dir=d)])
+        compare(tt, eng, qs)
+
+
+@pytest.mark.parametrize("lanes", [64, 1])
+def test_gpu_point_queries_workload1(lanes):
+    tt, img = image_for("w1", lambda: na.Timetable.generate(1))
+    groups = 2 if lanes == 64 else 8
+    eng = na.Engine(img, lanes=lanes, groups=groups, max_k=16)
+    qs = na.Queries.generate(tt, 1, 96 if lanes == 64 else 8)
+    say(f"workload1 point lanes={lanes} n={len(qs)}")
+    compare(tt, eng, qs)
+
+
+def test_gpu_backward_point_queries_workload1():
+    tt, img = image_for("w1", lambda: na.Timetable.generate(1))
+    eng = na.Engine(img, lanes=64, groups=1, max_k=16)
+    qs = na.Queries.generate(tt, 1, 64)
+    for i in range(64):
+        qs[i].dir = na.DIR_BWD
+    say("workload1 backward n=64")
+    compare(tt, eng, qs)
+
+
+def test_gpu_range_and_point_mix_workload3():
+    tt, img = image_for("w3s", lambda: na.Timetable.generate(3, scale=0.03))
+    eng = na.Engine(img, lanes=64, groups=2, max_k=10)
+    qs = na.Queries.generate(tt, 3, 128)
+    say("workload3 mix n=128")
+    compare(tt, eng, qs)
+
+
+def test_gpu_intermodal_workload4():
+    tt, img = image_for("w3s", lambda: na.Timetable.generate(3, scale=0.03))
+    eng = na.Engine(img, lanes=64, groups=1, max_k=10)
+    qs = na.Queries.generate(tt, 4, 64)
+    say("workload4 intermodal n=64")
+    compare(tt, eng, qs)
+
+
+def test_gpu_workload2_plain_and_batched_agree():
+    tt, img = image_for("w2s", lambda: na.Timetable.generate(2, scale=0.1))
+    qs = na.Queries.generate(tt, 2, 128)
+    say("workload2 batched n=128")
+    eng = na.Engine(img, lanes=64, groups=2, max_k=16)
+    compare(tt, eng, qs, check_state=False)
+    say("workload2 plain n=16")
+    plain = na.Engine(img, lanes=1, groups=1, max_k=16)
+    compare(tt, plain, qs.subset(list(range(16))), check_state=False)
+
+
+def test_gpu_more_queries_than_slots_reuses_slots():
+    tt, img = image_for("w1", lambda: na.Timetable.generate(1))
+    eng = na.Engine(img, lanes=64, groups=1, max_k=9)
+    qs = na.Queries.generate(tt, 1, 300)
+    say("workload1 300 queries through 64 slots")
+    compare(tt, eng, qs, check_state=False)
+
+
+def test_gpu_served_results_equal_alone_and_in_mix():
+    tt, img = image_for("w3s", lambda: na.Timetable.generate(3, scale=0.03))
+    qs = na.Queries.generate(tt, 5, 96, seed=0x7172 + 3)
+    say("serve mix n=96")
+    eng = na.Engine(img, lanes=64, groups=1, max_k=10)
+    arrivals = np.cumsum(np.random.default_rng(1).exponential(2.0, len(qs)))
+    served, wall = eng.serve(qs, arrivals)
+    alone = na.Engine(img, lanes=64, groups=1, max_k=10)
+    for i in range(0, 96, 12):
+        one = alone.search(qs.subset([i]))[0]
+        assert served[i]["journeys"] == one["journeys"], i
+        assert served[i]["journeys"] == oracle.search(tt, qs[i])["journeys"], i
+        assert served[i]["latency_ms"] >= 0.0

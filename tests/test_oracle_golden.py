@@ -1,0 +1,76 @@
+"""Pins the CPU oracle with golden journeys of nigiri's own tests."""
+import pytest
+
+import nigiri_amd as na
+import oracle
+from golden import raptor_abc as G
+from ttbuilder import Builder, day_number
+
+
+def hm(s):
+    h, m = s.split(":")
+    return int(h) * 60 + int(m)
+
+
+def abc_timetable():
+    b = Builder(day_number(*G.RANGE_FROM), G.RANGE_DAYS, 3)
+    loc = {"A": 9, "B": 10, "C": 11}
+    day = day_number(*G.DATE)
+    for name, (fr, to, first, travel, n, hw) in G.TRAINS.items():
+        trips = []
+        for i in range(n):
+            dep = first + i * hw
+            trips.append(([dep, dep + travel], [day]))
+        b.add_line([loc[fr], loc[to]], trips)
+    return b.build(), loc, day * 1440
+
+
+def run(tt, q):
+    return oracle.search(tt, q)["journeys"]
+
+
+@pytest.mark.parametrize("direction", ["fwd", "bwd"])
+def test_raptor_abc_range(direction):
+    tt, loc, day0 = abc_timetable()
+    iv = (day0 + hm(G.INTERVAL[0]), day0 + hm(G.INTERVAL[1]))
+    if direction == "fwd":
+        spec = dict(start=[(loc["A"], 0)], dest=[(loc["C"], 0)], interval=iv, dir=na.DIR_FWD)
+        expected = G.FWD_EXPECTED
+    else:
+        spec = dict(start=[(loc["C"], 0)], dest=[(loc["A"], 0)], interval=iv, dir=na.DIR_BWD)
+        expected = G.BWD_EXPECTED
+    qs = na.Queries.build([spec])
+    js = run(tt, qs[0])
+    got = []
+    for start, dest, _, transfers in js:
+        dep, arr = min(start, dest), max(start, dest)
+        got.append(((dep - day0) // 60, (dep - day0) % 60, (arr - day0) // 60, (arr - day0) % 60,
+                    transfers))
+    want = [(hm(d) // 60, hm(d) % 60, hm(a) // 60, hm(a) % 60, t) for d, a, t in expected]
+    assert got == want
+
+
+def test_raptor_abc_point_query_closed_form():
+    # ontrip at 05:10: next 1337 at 05:30 -> B 06:30 (+2 transfer) -> 7331 06:45 -> C 07:45
+    tt, loc, day0 = abc_timetable()
+    qs = na.Queries.build([dict(start=[(loc["A"], 0)], dest=[(loc["C"], 0)], time=day0 + hm("05:10"))])
+    js = run(tt, qs[0])
+    assert js == [(day0 + hm("05:10"), day0 + hm("07:45"), loc["C"], 1)]
+
+
+def test_transfer_time_is_respected():
+    # arrival at B 06:00; with a 16 minute transfer time the 06:15 connection is missed
+    b = Builder(day_number(*G.RANGE_FROM), G.RANGE_DAYS, 3)
+    day = day_number(*G.DATE)
+    b.transfer[10] = 16
+    b.add_line([9, 10], [([300, 360], [day])])
+    b.add_line([10, 11], [([375, 435], [day]), ([390, 450], [day])])
+    tt = b.build()
+    qs = na.Queries.build([dict(start=[(9, 0)], dest=[(11, 0)], time=day * 1440 + 290)])
+    assert run(tt, qs[0]) == [(day * 1440 + 290, day * 1440 + 450, 11, 1)]
+
+
+def test_start_dest_overlap_is_empty():
+    tt, loc, day0 = abc_timetable()
+    qs = na.Queries.build([dict(start=[(loc["A"], 0)], dest=[(loc["A"], 0)], time=day0 + 300)])
+    assert run(tt, qs[0]) == []

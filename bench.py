@@ -1,0 +1,176 @@
+"""Benchmark: earliest-arrival queries/s with p50/p99 latency on one MI355X,
+batched (64 queries per wavefront) against the plain one-query-per-launch
+GPU path, with the CPU oracle timed on the host cores as a baseline.
+
+A step is one batch of `--batch` queries of the workload, run through the
+serving loop with every query arriving at the start of the step (host
+preparation overlaps device rounds).  Inputs (timetable image, query state)
+are resident on the GPU before the timed region starts.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    1: "2k-stop grid timetable (45x45), 200 lines, 20 trips/line, 7 days; point EA, max 7 transfers",
+    2: "50k-stop random-geometric timetable, 5k lines x 100 trips, 365-day bitfields, "
+       "footpath degree 4; point EA queries, max 14 transfers",
+    3: "200k-stop timetable, 20k lines, 800k trips, degree 6; 70% point / 30% range (1-4 h), "
+       "max 8 transfers",
+    4: "200k-stop timetable; intermodal point queries, 1-50 start/dest offsets, max 8 transfers",
+    5: "500k-stop timetable, 60k lines, 2.5M trips, degree 6; 60% point / 25% range / "
+       "15% intermodal, max 8 transfers",
+}
+
+
+def sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def route_scan_bytes(work):
+    """Algorithmic bytes of the route-scan kernel (byte model of the design notes)."""
+    return (8 * work["route_stops"] + 2 * work["events"] + 8 * work["route_day_tests"]
+            + 12 * work["transport_day_tests"] + 2 * work["tmp_writes"])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", type=int, default=2)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--batch", type=int, default=2048)
+    ap.add_argument("--queries", type=int, default=10000)
+    ap.add_argument("--plain-sample", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(queries shard across GPUs without communication; see the design notes)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import nigiri_amd as na
+    cfg = args.workload
+    t0 = time.time()
+    tt = na.Timetable.generate(cfg, scale=args.scale)
+    qs = na.Queries.generate(tt, cfg, args.queries)
+    print(f"[bench] workload {cfg}: {tt.view.n_locations} locations, {tt.view.n_routes} routes, "
+          f"{tt.view.n_transports} transports, {len(qs)} queries ({time.time() - t0:.1f}s)",
+          file=sys.stderr, flush=True)
+    img = na.Image(tt)
+    max_k = 16 if cfg in (2,) else (9 if cfg == 1 else 10)
+    groups = max(1, args.batch // 64)
+    eng = na.Engine(img, lanes=64, groups=groups, max_k=max_k)
+    eng.profile(True)
+    B = groups * 64
+    n_q = len(qs)
+
+    def batch(i):
+        idx = [(i * B + j) % n_q for j in range(B)]
+        return qs.subset(idx)
+
+    zeros = np.zeros(B)
+    for w in range(args.warmup):
+        eng.serve(batch(w), zeros)
+        print(f"[bench] warmup step {w} done", file=sys.stderr, flush=True)
+    eng.stats(reset=True)
+    sync()
+    lat = []
+    n_journeys = 0
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        res, wall = eng.serve(batch(args.warmup + s), zeros)
+        lat += [r["latency_ms"] for r in res]
+        n_journeys += sum(len(r["journeys"]) for r in res)
+        print(f"[bench] step {s}: {wall:.1f} ms", file=sys.stderr, flush=True)
+    sync()
+    t_total = time.perf_counter() - t_start
+    st = eng.stats()
+    qps = args.steps * B / t_total
+    lat = np.array(lat)
+
+    # plain one-query-per-launch GPU path on a sample of the same queries
+    plain = na.Engine(img, lanes=1, groups=1, max_k=max_k)
+    ps = qs.subset(list(range(min(args.plain_sample, n_q))))
+    plain.search(ps.subset([0]))  # warm-up
+    sync()
+    tp = time.perf_counter()
+    plain.search(ps)
+    sync()
+    plain_qps = len(ps) / (time.perf_counter() - tp)
+    del plain
+
+    # roofline of the dominant kernel (route scan), live from HIP events
+    launches = max(1, st["route_scan_launches"])
+    alg_bytes = route_scan_bytes(st["work"])
+    avg_ms = st["route_scan_ms"] / launches
+    achieved = (alg_bytes / launches) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        probe = qs.subset(list(range(min(16, n_q))))
+        r = oracle.search_many(tt, probe, threads=1)
+        per_q = max(1e-4, r["wall_ms"] / 1e3 / len(probe))
+        n_cpu = int(min(n_q, max(threads * 4, args.cpu_seconds * threads / per_q)))
+        sample = qs.subset(list(range(n_cpu)))
+        r = oracle.search_many(tt, sample, threads=threads)
+        cpu = {"value": round(n_cpu / (r["wall_ms"] / 1e3), 2), "unit": "queries/s",
+               "cores": threads, "kind": "port",
+               "sample": f"first {n_cpu} queries of the workload, CPU oracle (C++ restatement "
+                         f"of nigiri's raptor), {threads} threads, "
+                         f"p50 {np.percentile(r['latency_ms'], 50):.2f} ms, "
+                         f"p99 {np.percentile(r['latency_ms'], 99):.2f} ms per query"}
+
+    line = {
+        "metric": "earliest-arrival queries/s + p50/p99 latency, batched vs 1-query GPU, 1 MI355X",
+        "value": round(qps, 2),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_total * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int16",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": f"config {cfg}: " + WORKLOADS[cfg], "scale": args.scale,
+                   "batch": B, "lanes": 64, "groups": groups, "max_k": max_k,
+                   "queries": n_q},
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "plain_1query_qps": round(plain_qps, 2),
+        "speedup_vs_plain": round(qps / plain_qps, 2) if plain_qps > 0 else None,
+        "journeys": n_journeys,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "k_route_scan",
+                     "alg_bytes_per_launch": int(alg_bytes / launches),
+                     "avg_launch_ms": round(avg_ms, 4), "launches": launches},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

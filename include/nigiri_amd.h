@@ -261,6 +261,25 @@ int nigiri_amd_algo_download(nigiri_amd_engine* e, uint32_t slot,
                              int16_t* round_times, int16_t* best,
                              int16_t* tmp, int16_t* time_at_dest);
 
+/* ---- instrumentation (no counterpart in the reference) -------------------
+ * Work counters of the algorithmic byte model, accumulated on the device
+ * (units: update_route stop iterations, events examined, traffic-day tests,
+ * ... as the oracle counts them), and, when profiling is enabled, HIP-event
+ * timings of the route-scan and transfer kernels.                          */
+typedef struct nigiri_engine_stats {
+  uint64_t epochs;
+  uint64_t rounds;
+  uint64_t route_scan_launches;
+  double route_scan_ms;
+  uint64_t transfer_launches;
+  double transfer_ms;
+  uint64_t work[16];
+} nigiri_engine_stats;
+
+int nigiri_amd_engine_profile(nigiri_amd_engine* e, int enable);
+int nigiri_amd_engine_stats(nigiri_amd_engine* e, nigiri_engine_stats* out,
+                            int reset);
+
 /* ---- serving loop ------------------------------------------------------ */
 /* Runs a continuous-batching server over the engine: query i becomes
  * visible at arrival_ms[i] (relative to the start), is admitted when a slot

@@ -91,6 +91,17 @@ class AlgoInit(C.Structure):
                 ("tts_additional_time", C.c_int16), ("tts_factor", C.c_float)]
 
 
+class EngineStats(C.Structure):
+    _fields_ = [("epochs", C.c_uint64), ("rounds", C.c_uint64),
+                ("route_scan_launches", C.c_uint64), ("route_scan_ms", C.c_double),
+                ("transfer_launches", C.c_uint64), ("transfer_ms", C.c_double),
+                ("work", C.c_uint64 * 16)]
+
+
+WORK_NAMES = ["route_stops", "events", "route_day_tests", "transport_day_tests", "tmp_writes",
+              "transfer_stations", "footpaths", "marked_stations", "location_routes"]
+
+
 # every entry point of include/nigiri_amd.h: name -> (restype, argtypes)
 _P = C.c_void_p
 SIGNATURES = {
@@ -120,6 +131,8 @@ SIGNATURES = {
     "nigiri_amd_algo_download": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_int16),
                                            C.POINTER(C.c_int16), C.POINTER(C.c_int16),
                                            C.POINTER(C.c_int16)]),
+    "nigiri_amd_engine_profile": (C.c_int, [_P, C.c_int]),
+    "nigiri_amd_engine_stats": (C.c_int, [_P, C.POINTER(EngineStats), C.c_int]),
     "nigiri_amd_serve": (C.c_int, [_P, C.POINTER(Query), C.POINTER(C.c_double), C.c_uint32,
                                    C.POINTER(Result), C.POINTER(Journey), C.c_uint32,
                                    C.POINTER(C.c_double)]),
@@ -381,6 +394,16 @@ class Engine:
         b = C.c_uint64()
         _check(lib().nigiri_amd_engine_state_bytes(self.handle, C.byref(b)))
         return b.value
+
+    def profile(self, enable=True):
+        _check(lib().nigiri_amd_engine_profile(self.handle, int(enable)))
+
+    def stats(self, reset=False):
+        st = EngineStats()
+        _check(lib().nigiri_amd_engine_stats(self.handle, C.byref(st), int(reset)))
+        d = {n: getattr(st, n) for n, _ in EngineStats._fields_ if n != "work"}
+        d["work"] = {n: st.work[i] for i, n in enumerate(WORK_NAMES)}
+        return d
 
     def search(self, queries, journey_capacity=None):
         n = len(queries)

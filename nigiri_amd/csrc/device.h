@@ -17,6 +17,20 @@ constexpr std::uint32_t kPosIntermodal = 1U << 30;
 constexpr std::uint32_t kPosFootpath = 2U << 30;
 constexpr std::uint16_t kUnreachableLb = 0xFFFFU;
 
+// work counters: units of the algorithmic byte model
+enum : int {
+  kWorkRouteStops = 0,     // update_route stop iterations (per query)
+  kWorkEvents = 1,         // events examined by get_earliest_transport
+  kWorkRouteDayTests = 2,  // route traffic-day tests
+  kWorkTransportTests = 3, // transport traffic-day tests
+  kWorkTmpWrites = 4,      // tmp improvements in the route phase
+  kWorkTransferStations = 5,
+  kWorkFootpaths = 6,
+  kWorkMarkedStations = 7,
+  kWorkLocationRoutes = 8,
+  kWorkCount = 16
+};
+
 // Read-only timetable image in HBM (one allocation per array).
 struct dev_image {
   std::uint32_t L, R, T, NB;
@@ -77,8 +91,11 @@ struct dev_state {
   std::uint64_t* list_t;  // stations marked in mark_t
   std::uint64_t* list_r;  // stations marked in mark_r
   std::uint64_t* list_route;
-  std::uint64_t* list_hz;  // hazards: (slot << 32 | location)
+  std::uint64_t* list_hz;  // stations with hazard lanes (g << 32 | location)
+  std::uint64_t* hz_mask;  // hazard lanes per station (same layout as marks)
   std::uint32_t* counts;   // [0] list_t, [1] list_r, [2] list_route, [3] hz, [4] overflow flags
+  // algorithmic work counters (see kWork* indices)
+  unsigned long long* work;
   std::size_t cap_t, cap_r, cap_route, cap_hz;
   // per slot
   slot_param* param;

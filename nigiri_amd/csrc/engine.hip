@@ -139,8 +139,13 @@ struct nigiri_amd_engine {
   std::vector<std::uint8_t> algo_new;    // reset_arrivals requested
   std::vector<std::uint8_t> algo_fresh;  // next_start_time requested
   std::vector<std::uint8_t> algo_dir;
+  // instrumentation
+  bool profile = false;
+  std::vector<hipEvent_t> events;  // 4 per round of one epoch
+  nigiri_engine_stats stats{};
 
   ~nigiri_amd_engine() {
+    for (auto ev : events) (void)hipEventDestroy(ev);
     for (auto* p : grow_ptrs) (void)hipFree(p);
     if (stream != nullptr) (void)hipStreamDestroy(stream);
   }
@@ -255,7 +260,25 @@ struct runner {
     a.st = e.st;
     a.k = k;
     a.invalid = fwd ? SHRT_MAX : SHRT_MIN;
+    if (e.profile && static_cast<std::size_t>(4 * k + 3) < e.events.size()) {
+      for (int i = 0; i != 4; ++i) a.ev[i] = e.events[static_cast<std::size_t>(4 * k + i)];
+    }
     return a;
+  }
+
+  void collect_timings(std::uint32_t n_rounds_run) {
+    if (!e.profile) return;
+    for (std::uint32_t k = 1U; k <= n_rounds_run; ++k) {
+      float ms = 0.0F;
+      if (hipEventElapsedTime(&ms, e.events[4U * k], e.events[4U * k + 1U]) == hipSuccess) {
+        e.stats.route_scan_ms += ms;
+        ++e.stats.route_scan_launches;
+      }
+      if (hipEventElapsedTime(&ms, e.events[4U * k + 2U], e.events[4U * k + 3U]) == hipSuccess) {
+        e.stats.transfer_ms += ms;
+        ++e.stats.transfer_launches;
+      }
+    }
   }
 
   // one epoch step over all slots; returns false when nothing ran
@@ -421,11 +444,14 @@ struct runner {
       HIP_CHECK(hipMemcpyAsync(e.d_sweep_groups, sweep_groups.data(), 4U * sweep_groups.size(), hipMemcpyHostToDevice, stream));
     }
     int const grid_waves = 2048;
+    std::uint32_t n_run = 0U;
     for (std::uint32_t k = 1U; k < max_end_k; ++k) {
       launch_round(fwd, W, rargs(static_cast<std::int32_t>(k)), e.d_sweep_groups,
                    static_cast<std::uint32_t>(sweep_groups.size()), S, grid_waves, stream);
       HIP_CHECK(hipGetLastError());
       ++rounds;
+      ++e.stats.rounds;
+      n_run = k;
       HIP_CHECK(hipMemcpyAsync(e.h_active.p, tr.grp_active, 8U * G, hipMemcpyDeviceToHost, stream));
       HIP_CHECK(hipStreamSynchronize(stream));
       bool alive = false;
@@ -434,18 +460,19 @@ struct runner {
     }
     launch_epoch_end(W, tr, S, stream);
     HIP_CHECK(hipGetLastError());
+    ++e.stats.epochs;
     auto const rows = static_cast<std::size_t>(S) * kMaxDestBits * K;
     HIP_CHECK(hipMemcpyAsync(e.h_rows.p, tr.dest_rows, 2U * rows, hipMemcpyDeviceToHost, stream));
     HIP_CHECK(hipMemcpyAsync(e.h_counts.p, tr.counts, 4U * 8U, hipMemcpyDeviceToHost, stream));
     HIP_CHECK(hipStreamSynchronize(stream));
     if (e.h_counts.p[4] != 0U) {
+      auto const flags = e.h_counts.p[4];
+      std::uint32_t zero = 0U;
+      HIP_CHECK(hipMemcpy(tr.counts + 4, &zero, 4U, hipMemcpyHostToDevice));
       throw status_error{NIGIRI_AMD_ECAPACITY,
-                         "device work list overflow (flags " + std::to_string(e.h_counts.p[4]) + ")"};
+                         "device work list overflow (flags " + std::to_string(flags) + ")"};
     }
-    // reset the overflow word for the next epoch
-    std::uint32_t zero = 0U;
-    HIP_CHECK(hipMemcpyAsync(tr.counts + 4, &zero, 4U, hipMemcpyHostToDevice, stream));
-    HIP_CHECK(hipStreamSynchronize(stream));
+    collect_timings(n_run);
 
     for (std::uint32_t s = 0; s != S; ++s) {
       auto& si = slots[s];
@@ -661,6 +688,7 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     st.mark_t = m.alloc<std::uint64_t>(lwords);
     st.mark_r = m.alloc<std::uint64_t>(lwords);
     st.dest_mask = m.alloc<std::uint64_t>(lwords);
+    st.hz_mask = m.alloc<std::uint64_t>(lwords);
     st.route_mark = m.alloc<std::uint64_t>(rwords);
     st.grp_active = m.alloc<std::uint64_t>(G);
     st.grp_any_route = m.alloc<std::uint64_t>(G);
@@ -668,12 +696,13 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     st.cap_t = static_cast<std::size_t>(G) * L;
     st.cap_r = static_cast<std::size_t>(G) * L;
     st.cap_route = static_cast<std::size_t>(G) * R;
-    st.cap_hz = std::max<std::size_t>(1U << 16, static_cast<std::size_t>(S) * 64U);
+    st.cap_hz = static_cast<std::size_t>(G) * L;
     st.list_t = m.alloc<std::uint64_t>(st.cap_t);
     st.list_r = m.alloc<std::uint64_t>(st.cap_r);
     st.list_route = m.alloc<std::uint64_t>(st.cap_route);
     st.list_hz = m.alloc<std::uint64_t>(st.cap_hz);
     st.counts = m.alloc<std::uint32_t>(8U);
+    st.work = m.alloc<unsigned long long>(kWorkCount);
     st.param = m.alloc<slot_param>(S);
     st.tad = m.alloc<std::int16_t>(static_cast<std::size_t>(S) * kRowsMax);
     st.tad0 = m.alloc<std::int16_t>(S);
@@ -688,11 +717,13 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     HIP_CHECK(hipMemsetAsync(st.mark_t, 0, 8U * lwords, e->stream));
     HIP_CHECK(hipMemsetAsync(st.mark_r, 0, 8U * lwords, e->stream));
     HIP_CHECK(hipMemsetAsync(st.dest_mask, 0, 8U * lwords, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.hz_mask, 0, 8U * lwords, e->stream));
     HIP_CHECK(hipMemsetAsync(st.route_mark, 0, 8U * rwords, e->stream));
     HIP_CHECK(hipMemsetAsync(st.grp_active, 0, 8U * G, e->stream));
     HIP_CHECK(hipMemsetAsync(st.grp_any_route, 0, 8U * G, e->stream));
     HIP_CHECK(hipMemsetAsync(st.grp_any_tmp, 0, 8U * G, e->stream));
     HIP_CHECK(hipMemsetAsync(st.counts, 0, 32U, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.work, 0, 8U * kWorkCount, e->stream));
     HIP_CHECK(hipMemsetAsync(st.n_chg, 0, 4U * S, e->stream));
     e->h_param.assign(S, slot_param{});
     e->h_dest.assign(static_cast<std::size_t>(S) * kMaxDestBits, 0U);
@@ -731,6 +762,36 @@ int nigiri_amd_engine_state_bytes(const nigiri_amd_engine* e, uint64_t* b) {
 }
 
 void nigiri_amd_engine_destroy(nigiri_amd_engine* e) { delete e; }
+
+int nigiri_amd_engine_profile(nigiri_amd_engine* e, int enable) {
+  if (e == nullptr) return NIGIRI_AMD_EINVAL;
+  return guarded([&]() {
+    HIP_CHECK(hipSetDevice(e->img->device));
+    if (enable != 0 && e->events.empty()) {
+      e->events.resize(4U * (kRowsMax + 1U));
+      for (auto& ev : e->events) HIP_CHECK(hipEventCreate(&ev));
+    }
+    e->profile = enable != 0;
+    return NIGIRI_AMD_OK;
+  });
+}
+
+int nigiri_amd_engine_stats(nigiri_amd_engine* e, nigiri_engine_stats* out, int reset) {
+  if (e == nullptr || out == nullptr) return NIGIRI_AMD_EINVAL;
+  return guarded([&]() {
+    HIP_CHECK(hipSetDevice(e->img->device));
+    HIP_CHECK(hipStreamSynchronize(e->stream));
+    unsigned long long w[kWorkCount];
+    HIP_CHECK(hipMemcpy(w, e->st.work, sizeof(w), hipMemcpyDeviceToHost));
+    *out = e->stats;
+    for (int i = 0; i != kWorkCount; ++i) out->work[i] = w[i];
+    if (reset != 0) {
+      e->stats = nigiri_engine_stats{};
+      HIP_CHECK(hipMemset(e->st.work, 0, sizeof(w)));
+    }
+    return NIGIRI_AMD_OK;
+  });
+}
 
 int nigiri_amd_search(nigiri_amd_engine* e, const nigiri_query* qs, uint32_t n, nigiri_result* res,
                       nigiri_journey* journeys, uint32_t journey_capacity) {

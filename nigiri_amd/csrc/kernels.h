@@ -39,6 +39,8 @@ struct round_args {
   dev_state st;
   std::int32_t k;
   std::int16_t invalid;
+  // optional timing events around the route scan / transfer kernels
+  hipEvent_t ev[4];
 };
 
 // Launch helpers (defined in kernels.hip), all asynchronous on `stream`.

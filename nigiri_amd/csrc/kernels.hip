@@ -63,6 +63,16 @@ __device__ inline void list_push(std::uint64_t* list, std::uint32_t* count, std:
   }
 }
 
+// adds a per-lane count to a global counter with one atomic per wavefront
+__device__ inline void wave_count(unsigned long long* counter, unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    v += __shfl_xor(v, off, 64);
+  }
+  if ((threadIdx.x & 63U) == 0U && v != 0ULL) {
+    atomicAdd(counter, v);
+  }
+}
+
 // ---- state indexing ---------------------------------------------------
 template <int W>
 struct lay {
@@ -214,6 +224,11 @@ __global__ void k_mark_routes(round_args a) {
       auto const word = s.mark_t[lay<W>::lword(s, g, l)] & s.grp_active[g];
       if (word == 0ULL) continue;
       auto const r0 = im.location_route_offsets[l], r1 = im.location_route_offsets[l + 1U];
+      if (lane == 0U) {
+        auto const nq = static_cast<unsigned long long>(__popcll(word));
+        atomicAdd(&s.work[kWorkMarkedStations], nq);
+        atomicAdd(&s.work[kWorkLocationRoutes], nq * (r1 - r0));
+      }
       for (auto j = r0 + lane; j < r1; j += 64U) {
         auto const r = im.location_routes[j];
         auto const old = atomicOr(
@@ -236,6 +251,8 @@ __global__ void k_mark_routes(round_args a) {
       auto const g = static_cast<std::uint32_t>(e >> 32), l = static_cast<std::uint32_t>(e);
       if (!lay<W>::test(s.mark_t, s, g, l, 0U) || (s.grp_active[g] & 1ULL) == 0ULL) continue;
       auto const r0 = im.location_route_offsets[l], r1 = im.location_route_offsets[l + 1U];
+      atomicAdd(&s.work[kWorkMarkedStations], 1ULL);
+      atomicAdd(&s.work[kWorkLocationRoutes], static_cast<unsigned long long>(r1 - r0));
       for (auto j = r0; j < r1; ++j) {
         auto const r = im.location_routes[j];
         auto const bit = lay<W>::rbit(r, 0U);
@@ -268,26 +285,32 @@ __device__ inline std::int16_t to_delta(std::uint16_t day, int mam, std::uint16_
 }
 
 // get_earliest_transport (raptor.h:1351-1442)
+struct et_counters {
+  unsigned long long events = 0, route_days = 0, transport_tests = 0;
+};
+
 template <bool Fwd>
 __device__ inline et_t<Fwd> earliest_transport(dev_image const& im, std::uint32_t r,
                                                std::uint32_t t_from, std::uint32_t n_t,
                                                std::uint32_t ev_base, std::uint16_t day_at_stop,
                                                int mam_at_stop, int lb_l, int tad_k,
-                                               std::uint16_t base) {
+                                               std::uint16_t base, et_counters& c) {
   using D = dirop<Fwd>;
   auto const rbf = im.route_traffic_days[r];
   for (std::uint32_t i = 0U; i != 6U; ++i) {  // kMaxTravelTime / 1 day + 1
     auto const day = static_cast<std::uint16_t>(Fwd ? day_at_stop + i : day_at_stop - i);
+    ++c.route_days;
     if (!bit_test(im.bitfields, rbf, day)) continue;
     std::uint32_t pos = 0U;
-    if (i == 0U) {
-      while (pos != n_t) {
+    if (i == 0U) {  // linear lower bound on the minute of day (linear_lower_bound.h:7-15)
+      for (; pos != n_t; ++pos) {
+        ++c.events;
         auto const t_off = Fwd ? pos : n_t - 1U - pos;
         if (!D::better(im.event_times[ev_base + t_off] >> 5, mam_at_stop)) break;
-        ++pos;
       }
     }
     for (; pos < n_t; ++pos) {
+      ++c.events;
       auto const t_off = Fwd ? pos : n_t - 1U - pos;
       auto const ev = im.event_times[ev_base + t_off];
       int const ev_mam = ev >> 5;
@@ -296,6 +319,7 @@ __device__ inline et_t<Fwd> earliest_transport(dev_image const& im, std::uint32_
       }
       if (i == 0U && !D::better_eq(mam_at_stop, ev_mam)) continue;
       auto const start_day = static_cast<std::uint16_t>(static_cast<int>(day) - (ev & 31));
+      ++c.transport_tests;
       if (!bit_test(im.bitfields, im.transport_traffic_days[t_from + t_off], start_day)) continue;
       return {t_off, start_day};
     }
@@ -328,6 +352,8 @@ __global__ void k_route_scan(round_args a) {
   std::uint32_t const step = W == 64 ? gridDim.x * kWavesPerBlock : gridDim.x * blockDim.x;
   std::uint32_t item = W == 64 ? (blockIdx.x * blockDim.x + threadIdx.x) / 64U
                                : blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long c_stops = 0ULL, c_tmp = 0ULL;
+  et_counters c_et;
   for (; item < n_items; item += step) {
     auto const e = s.list_route[item];
     auto const g = static_cast<std::uint32_t>(e >> 32), r = static_cast<std::uint32_t>(e);
@@ -366,6 +392,7 @@ __global__ void k_route_scan(round_args a) {
       bool const can_finish = Fwd ? out_ok : in_ok;
       bool const can_start = Fwd ? in_ok : out_ok;
       auto const c = lay<W>::col(s, g, l, lane_id);
+      if (!done) ++c_stops;
 
       bool upd = false;
       bool ride = false;
@@ -376,6 +403,7 @@ __global__ void k_route_scan(round_args a) {
         if (D::better_eq(by, tad_k) && lbv != kUnreachableLb && D::better_eq(by + D::dir(lbv), tad_k)) {
           atomic_best16<Fwd>(&s.tmp[c], by);
           upd = true;
+          ++c_tmp;
         }
       }
       if constexpr (W == 64) {
@@ -407,7 +435,7 @@ __global__ void k_route_scan(round_args a) {
         int const mam = static_cast<std::int16_t>(minutes % 1440);
         auto const ev_base = stf + n_t * (stop_idx * 2U - (Fwd ? 0U : 1U));
         auto const new_et = earliest_transport<Fwd>(im, r, t_from, n_t, ev_base, day, mam,
-                                                    s.lb[c], tad_k, base);
+                                                    s.lb[c], tad_k, base, c_et);
         // current_best is valid whenever a ride was evaluated at this stop
         // (it then includes the valid round_times[k-1] label) or best / tmp
         // hold a label
@@ -429,14 +457,21 @@ __global__ void k_route_scan(round_args a) {
       if (any) atomicOr(reinterpret_cast<unsigned long long*>(&s.grp_any_tmp[g]), 1ULL);
     }
   }
+  wave_count(&s.work[kWorkRouteStops], c_stops);
+  wave_count(&s.work[kWorkTmpWrites], c_tmp);
+  wave_count(&s.work[kWorkEvents], c_et.events);
+  wave_count(&s.work[kWorkRouteDayTests], c_et.route_days);
+  wave_count(&s.work[kWorkTransportTests], c_et.transport_tests);
 }
 
 // clear the marks listed in `list` (all lanes)
 template <int W>
 __global__ void k_clear_marks(dev_state s, int which) {
-  auto const n = which == 0 ? clamp_count(s.counts[0], s.cap_t) : clamp_count(s.counts[1], s.cap_r);
-  auto* marks = which == 0 ? s.mark_t : s.mark_r;
-  auto const* list = which == 0 ? s.list_t : s.list_r;
+  auto const n = which == 0   ? clamp_count(s.counts[0], s.cap_t)
+                 : which == 1 ? clamp_count(s.counts[1], s.cap_r)
+                              : clamp_count(s.counts[3], s.cap_hz);
+  auto* marks = which == 0 ? s.mark_t : (which == 1 ? s.mark_r : s.hz_mask);
+  auto const* list = which == 0 ? s.list_t : (which == 1 ? s.list_r : s.list_hz);
   for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     auto const e = list[i];
     marks[lay<W>::lword(s, static_cast<std::uint32_t>(e >> 32), static_cast<std::uint32_t>(e))] = 0ULL;
@@ -581,30 +616,31 @@ __global__ void k_dest_pass(round_args a, std::uint32_t n_slots) {
 }
 
 // One non-destination relaxation (update_transfers raptor.h:699-740 /
-// update_footpaths raptor.h:785-833).  Returns true when it improved best
-// (the station becomes marked).
+// update_footpaths raptor.h:785-833).  Returns kRelaxImproved when it
+// improved best (the station becomes marked), kRelaxHazard when it was a
+// pruned pre-write after a time_at_dest change (exact replay needed if the
+// station also has a passing relaxation), else 0.
+constexpr int kRelaxImproved = 1;
+constexpr int kRelaxHazard = 2;
+
 template <bool Fwd, int W>
-__device__ inline bool relax(dev_state const& s, std::uint32_t g, std::uint32_t lane,
-                             std::uint32_t slot, std::uint32_t k, std::uint32_t t, int x,
-                             std::uint32_t pos, std::int16_t tad0) {
+__device__ inline int relax(dev_state const& s, std::uint32_t g, std::uint32_t lane,
+                            std::uint32_t slot, std::uint32_t k, std::uint32_t t, int x,
+                            std::uint32_t pos, std::int16_t tad0) {
   using D = dirop<Fwd>;
   auto const c = lay<W>::col(s, g, t, lane);
   auto const xv = clamp16(x);
-  if (!D::better(xv, s.best[c])) return false;
+  if (!D::better(xv, s.best[c])) return 0;
   auto const tad_e = tad_at(s, slot, pos);
   int const lbv = s.lb[c];
   bool const elig = D::better_eq(xv, tad_e) && lbv != kUnreachableLb &&
                     D::better_eq(xv + D::dir(lbv), tad_e);
   if (elig) {
     auto const old = atomic_best16<Fwd>(&s.best[c], xv);
-    return D::better(xv, old);
+    return D::better(xv, old) ? kRelaxImproved : 0;
   }
   atomic_best16<Fwd>(&s.round_times[lay<W>::rt(s, g, k, t, lane)], xv);  // pre-write
-  if (tad_e != tad0) {
-    list_push(s.list_hz, &s.counts[3], s.cap_hz, (static_cast<std::uint64_t>(slot) << 32) | t,
-              &s.counts[4]);
-  }
-  return false;
+  return tad_e != tad0 ? kRelaxHazard : 0;
 }
 
 template <bool Fwd, int W>
@@ -621,6 +657,7 @@ __global__ void k_transfer_fp(round_args a) {
                                : blockIdx.x * blockDim.x + threadIdx.x;
   auto const* off = Fwd ? im.fp_out_offsets : im.fp_in_offsets;
   auto const* fps = Fwd ? im.fp_out : im.fp_in;
+  unsigned long long c_st = 0ULL, c_fp = 0ULL;
   for (; item < n_items; item += step) {
     auto const e = s.list_r[item];
     auto const g = static_cast<std::uint32_t>(e >> 32), i = static_cast<std::uint32_t>(e);
@@ -632,45 +669,61 @@ __global__ void k_transfer_fp(round_args a) {
     auto const tmp_i = s.tmp[lay<W>::col(s, g, i, lane_id)];
     bool const valid = on && tmp_i != D::kInvalid;
     auto const tad0 = s.tad0[slot];
+    if (on) {
+      ++c_st;
+      c_fp += off[i + 1U] - off[i];
+    }
 
-    auto const emit = [&](std::uint32_t t, bool improved) {
+    auto const emit = [&](std::uint32_t t, int code) {
       if constexpr (W == 64) {
-        auto const m = __ballot(improved);
+        auto const m = __ballot(code == kRelaxImproved);
+        auto const h = __ballot(code == kRelaxHazard);
+        auto const w = lay<W>::lword(s, g, t);
         if (m != 0ULL && lane_id == static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1)) {
-          auto const old = atomicOr(reinterpret_cast<unsigned long long*>(&s.mark_t[lay<W>::lword(s, g, t)]),
+          auto const old = atomicOr(reinterpret_cast<unsigned long long*>(&s.mark_t[w]),
                                     static_cast<unsigned long long>(m));
           if (old == 0ULL) {
             list_push(s.list_t, &s.counts[0], s.cap_t, (static_cast<std::uint64_t>(g) << 32) | t, overflow);
           }
         }
+        if (h != 0ULL && lane_id == static_cast<std::uint32_t>(__ffsll(static_cast<long long>(h)) - 1)) {
+          auto const old = atomicOr(reinterpret_cast<unsigned long long*>(&s.hz_mask[w]),
+                                    static_cast<unsigned long long>(h));
+          if (old == 0ULL) {
+            list_push(s.list_hz, &s.counts[3], s.cap_hz, (static_cast<std::uint64_t>(g) << 32) | t, overflow);
+          }
+        }
       } else {
-        if (improved) mark_one<W>(s.mark_t, s.list_t, &s.counts[0], s.cap_t, overflow, s, g, t, 0U);
+        if (code == kRelaxImproved) mark_one<W>(s.mark_t, s.list_t, &s.counts[0], s.cap_t, overflow, s, g, t, 0U);
+        if (code == kRelaxHazard) mark_one<W>(s.hz_mask, s.list_hz, &s.counts[3], s.cap_hz, overflow, s, g, t, 0U);
       }
     };
 
     // transfer at i (destination stations are handled by k_dest_pass)
     {
       bool const is_dest = lay<W>::test(s.dest_mask, s, g, i, lane_id);
-      bool improved = false;
+      int code = 0;
       if (valid && !is_dest) {
         auto const tt = adjusted<std::uint8_t>(p, im.transfer_time[i]);
-        improved = relax<Fwd, W>(s, g, lane_id, slot, k, i, tmp_i + D::dir(tt), kPosTransfer | i, tad0);
+        code = relax<Fwd, W>(s, g, lane_id, slot, k, i, tmp_i + D::dir(tt), kPosTransfer | i, tad0);
       }
-      emit(i, improved);
+      emit(i, code);
     }
     // footpaths of i in list order
     for (auto fe = off[i]; fe != off[i + 1U]; ++fe) {
       auto const fp = fps[fe];
       auto const t = fp & ((1U << 23) - 1U);
-      bool improved = false;
+      int code = 0;
       if (valid && !lay<W>::test(s.dest_mask, s, g, t, lane_id)) {
         auto const dur = static_cast<std::int16_t>(fp >> 23);
-        improved = relax<Fwd, W>(s, g, lane_id, slot, k, t, tmp_i + D::dir(adjusted<std::int16_t>(p, dur)),
-                                 kPosFootpath | fe, tad0);
+        code = relax<Fwd, W>(s, g, lane_id, slot, k, t, tmp_i + D::dir(adjusted<std::int16_t>(p, dur)),
+                             kPosFootpath | fe, tad0);
       }
-      emit(t, improved);
+      emit(t, code);
     }
   }
+  wave_count(&s.work[kWorkTransferStations], c_st);
+  wave_count(&s.work[kWorkFootpaths], c_fp);
 }
 
 // round_times[k] = best for every station improved in the transfer phase
@@ -701,10 +754,12 @@ __global__ void k_hazards(round_args a) {
   auto const& s = a.st;
   auto const k = static_cast<std::uint32_t>(a.k);
   auto const n = clamp_count(s.counts[3], s.cap_hz);
-  for (std::uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < n; h += gridDim.x * blockDim.x) {
-    auto const e = s.list_hz[h];
-    auto const slot = static_cast<std::uint32_t>(e >> 32), t = static_cast<std::uint32_t>(e);
-    auto const g = slot / W, lane = slot % W;
+  for (std::uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < n * W; h += gridDim.x * blockDim.x) {
+    auto const e = s.list_hz[h / W];
+    auto const g = static_cast<std::uint32_t>(e >> 32), t = static_cast<std::uint32_t>(e);
+    auto const lane = h % W, slot = g * W + lane;
+    if (!lay<W>::test(s.hz_mask, s, g, t, lane)) continue;
+    if ((s.grp_active[g] & (1ULL << lane)) == 0ULL) continue;
     if (!lay<W>::test(s.mark_t, s, g, t, lane)) continue;  // no passing relaxation
     if (lay<W>::test(s.dest_mask, s, g, t, lane)) continue;
     auto const& p = s.param[slot];
@@ -914,15 +969,20 @@ void round_impl(round_args const& a, std::uint32_t const* sweep_groups, std::uin
   }
   hipLaunchKernelGGL((k_round_begin<Fwd, W>), dim3(slot_blocks), dim3(kBlock), 0, st, a, n_slots);
   hipLaunchKernelGGL((k_mark_routes<W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
+  if (a.ev[0] != nullptr) (void)hipEventRecord(a.ev[0], st);
   hipLaunchKernelGGL((k_route_scan<Fwd, W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
+  if (a.ev[1] != nullptr) (void)hipEventRecord(a.ev[1], st);
   hipLaunchKernelGGL((k_clear_marks<W>), dim3(1024), dim3(kBlock), 0, st, a.st, 0);
   hipLaunchKernelGGL((k_group_update), dim3(grid_for(a.st.G, kBlock, 1 << 20)), dim3(kBlock), 0, st, a.st);
   hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 1U | 4U);  // list_t, routes
   hipLaunchKernelGGL((k_dest_pass<Fwd, W>), dim3(grid_for(n_slots, 64, 1 << 20)), dim3(64), 0, st, a, n_slots);
+  if (a.ev[2] != nullptr) (void)hipEventRecord(a.ev[2], st);
   hipLaunchKernelGGL((k_transfer_fp<Fwd, W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
+  if (a.ev[3] != nullptr) (void)hipEventRecord(a.ev[3], st);
   hipLaunchKernelGGL((k_rt_fixup<Fwd, W>), dim3(1024), dim3(kBlock), 0, st, a);
-  hipLaunchKernelGGL((k_hazards<Fwd, W>), dim3(64), dim3(kBlock), 0, st, a);
+  hipLaunchKernelGGL((k_hazards<Fwd, W>), dim3(256), dim3(kBlock), 0, st, a);
   hipLaunchKernelGGL((k_clear_marks<W>), dim3(1024), dim3(kBlock), 0, st, a.st, 1);
+  hipLaunchKernelGGL((k_clear_marks<W>), dim3(256), dim3(kBlock), 0, st, a.st, 2);
   hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 2U | 8U);  // list_r, hazards
 }
 

@@ -134,3 +134,28 @@ def test_gpu_served_results_equal_alone_and_in_mix():
         assert served[i]["journeys"] == one["journeys"], i
         assert served[i]["journeys"] == oracle.search(tt, qs[i])["journeys"], i
         assert served[i]["latency_ms"] >= 0.0
+
+
+def test_gpu_work_counters_equal_oracle_counters():
+    # the algorithmic byte model counts the same units on both sides
+    tt, img = image_for("w3s", lambda: na.Timetable.generate(3, scale=0.03))
+    eng = na.Engine(img, lanes=64, groups=1, max_k=10)
+    qs = na.Queries.generate(tt, 3, 64)
+    say("work counters n=64")
+    eng.stats(reset=True)
+    eng.search(qs)
+    w = eng.stats()["work"]
+    ref = {"route_stops": 0, "events": 0, "route_day_tests": 0, "transport_day_tests": 0,
+           "marked_stations": 0, "location_routes": 0, "footpaths": 0, "transfer_stations": 0}
+    for i in range(len(qs)):
+        s = oracle.search(tt, qs[i])["stats"]
+        ref["route_stops"] += s["n_route_stops_visited"]
+        ref["events"] += s["n_events_scanned"]
+        ref["route_day_tests"] += s["n_route_day_tests"]
+        ref["transport_day_tests"] += s["n_transport_day_tests"]
+        ref["marked_stations"] += s["n_marked_stations"]
+        ref["location_routes"] += s["n_location_routes_read"]
+        ref["footpaths"] += s["n_footpaths_visited"]
+        ref["transfer_stations"] += s["n_transfer_stations"]
+    for k, v in ref.items():
+        assert w[k] == v, (k, w[k], v)

@@ -59,7 +59,19 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--admission-wait-ms", type=float, default=2000.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of 512 MiB (scales rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(512 << 20, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 512 << 20}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(queries shard across GPUs without communication; see the design notes)",
@@ -79,6 +91,9 @@ def main():
     groups = max(1, args.batch // 64)
     eng = na.Engine(img, lanes=64, groups=groups, max_k=max_k)
     eng.profile(True)
+    # a step submits a whole batch at once: let preparation fill every slot
+    # before the first epoch starts
+    eng.configure(admission_wait_ms=args.admission_wait_ms)
     B = groups * 64
     n_q = len(qs)
 
@@ -162,6 +177,8 @@ def main():
         "plain_1query_qps": round(plain_qps, 2),
         "speedup_vs_plain": round(qps / plain_qps, 2) if plain_qps > 0 else None,
         "journeys": n_journeys,
+        "lane_utilization": round(st["work"]["route_stops"] / max(1, 64 * st["work"]["wave_stops"]), 4),
+        "rounds": st["rounds"], "epochs": st["epochs"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "kernel": "k_route_scan",

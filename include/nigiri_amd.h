@@ -150,6 +150,10 @@ typedef struct nigiri_result {
   uint32_t journey_offset; /* index of the first journey in the out array   */
   uint32_t n_epochs;       /* start-time groups executed                    */
   uint32_t n_rounds;       /* RAPTOR rounds executed, summed over epochs    */
+  int32_t slot;            /* engine slot that ran the query (-1: none);
+                              its state stays readable with
+                              nigiri_amd_algo_download until the slot is
+                              reused                                        */
   int32_t interval_from;   /* search interval actually searched             */
   int32_t interval_to;
   int16_t fastest_direct;
@@ -279,6 +283,20 @@ typedef struct nigiri_engine_stats {
 int nigiri_amd_engine_profile(nigiri_amd_engine* e, int enable);
 int nigiri_amd_engine_stats(nigiri_amd_engine* e, nigiri_engine_stats* out,
                             int reset);
+
+/* ---- engine configuration (no counterpart in the reference) -------------
+ * admission_wait_ms: serving loop; at an epoch boundary, wait up to this
+ *   long for queries that have arrived but are still being prepared, so
+ *   that free slots start the epoch filled (default 2 ms).
+ * grid_waves: wavefronts per persistent-kernel launch (0 = 32 per CU).   */
+typedef struct nigiri_engine_config {
+  double admission_wait_ms;
+  uint32_t grid_waves;
+  uint32_t reserved[7];
+} nigiri_engine_config;
+
+int nigiri_amd_engine_configure(nigiri_amd_engine* e,
+                                const nigiri_engine_config* c);
 
 /* ---- serving loop ------------------------------------------------------ */
 /* Runs a continuous-batching server over the engine: query i becomes

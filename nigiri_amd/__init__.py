@@ -77,7 +77,7 @@ class Journey(C.Structure):
 
 class Result(C.Structure):
     _fields_ = [("status", C.c_int32), ("n_journeys", C.c_uint32), ("journey_offset", C.c_uint32),
-                ("n_epochs", C.c_uint32), ("n_rounds", C.c_uint32),
+                ("n_epochs", C.c_uint32), ("n_rounds", C.c_uint32), ("slot", C.c_int32),
                 ("interval_from", C.c_int32), ("interval_to", C.c_int32),
                 ("fastest_direct", C.c_int16), ("base_day", C.c_int16), ("latency_ms", C.c_double)]
 
@@ -98,8 +98,14 @@ class EngineStats(C.Structure):
                 ("work", C.c_uint64 * 16)]
 
 
+class EngineConfig(C.Structure):
+    _fields_ = [("admission_wait_ms", C.c_double), ("grid_waves", C.c_uint32),
+                ("reserved", C.c_uint32 * 7)]
+
+
 WORK_NAMES = ["route_stops", "events", "route_day_tests", "transport_day_tests", "tmp_writes",
-              "transfer_stations", "footpaths", "marked_stations", "location_routes"]
+              "transfer_stations", "footpaths", "marked_stations", "location_routes",
+              "wave_stops"]
 
 
 # every entry point of include/nigiri_amd.h: name -> (restype, argtypes)
@@ -131,6 +137,7 @@ SIGNATURES = {
     "nigiri_amd_algo_download": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_int16),
                                            C.POINTER(C.c_int16), C.POINTER(C.c_int16),
                                            C.POINTER(C.c_int16)]),
+    "nigiri_amd_engine_configure": (C.c_int, [_P, C.POINTER(EngineConfig)]),
     "nigiri_amd_engine_profile": (C.c_int, [_P, C.c_int]),
     "nigiri_amd_engine_stats": (C.c_int, [_P, C.POINTER(EngineStats), C.c_int]),
     "nigiri_amd_serve": (C.c_int, [_P, C.POINTER(Query), C.POINTER(C.c_double), C.c_uint32,
@@ -371,7 +378,7 @@ def _collect(res, jr, n):
                jr[r.journey_offset + j].dest, jr[r.journey_offset + j].transfers)
               for j in range(r.n_journeys)]
         out.append({"status": r.status, "journeys": js, "n_epochs": r.n_epochs,
-                    "base_day": r.base_day, "fastest_direct": r.fastest_direct,
+                    "base_day": r.base_day, "fastest_direct": r.fastest_direct, "slot": r.slot,
                     "latency_ms": r.latency_ms})
     return out
 
@@ -394,6 +401,12 @@ class Engine:
         b = C.c_uint64()
         _check(lib().nigiri_amd_engine_state_bytes(self.handle, C.byref(b)))
         return b.value
+
+    def configure(self, admission_wait_ms=2.0, grid_waves=0):
+        c = EngineConfig()
+        c.admission_wait_ms = admission_wait_ms
+        c.grid_waves = grid_waves
+        _check(lib().nigiri_amd_engine_configure(self.handle, C.byref(c)))
 
     def profile(self, enable=True):
         _check(lib().nigiri_amd_engine_profile(self.handle, int(enable)))

@@ -28,6 +28,7 @@ enum : int {
   kWorkFootpaths = 6,
   kWorkMarkedStations = 7,
   kWorkLocationRoutes = 8,
+  kWorkWaveStops = 9,      // wavefront stop iterations of the batched route scan
   kWorkCount = 16
 };
 
@@ -78,7 +79,8 @@ struct dev_state {
   std::uint32_t LW, RW;  // words per group for W == 1 bitsets
   std::int16_t* round_times;
   std::int16_t* best;
-  std::int16_t* tmp;
+  std::int32_t* tmp;       // int32 so that route scans update it with
+                           // non-returning atomicMin / atomicMax
   std::uint16_t* lb;
   std::uint64_t* mark_t;   // marks of the transfer phase / starts
   std::uint64_t* mark_r;   // marks of the route phase
@@ -93,6 +95,12 @@ struct dev_state {
   std::uint64_t* list_route;
   std::uint64_t* list_hz;  // stations with hazard lanes (g << 32 | location)
   std::uint64_t* hz_mask;  // hazard lanes per station (same layout as marks)
+  // batched engine (W == 64): one bit per location / route and group,
+  // set next to the lane-mask words and compacted into the work lists
+  std::uint64_t* sum_t;
+  std::uint64_t* sum_r;
+  std::uint64_t* sum_route;
+  std::uint64_t* sum_hz;
   std::uint32_t* counts;   // [0] list_t, [1] list_r, [2] list_route, [3] hz, [4] overflow flags
   // algorithmic work counters (see kWork* indices)
   unsigned long long* work;

@@ -139,6 +139,9 @@ struct nigiri_amd_engine {
   std::vector<std::uint8_t> algo_new;    // reset_arrivals requested
   std::vector<std::uint8_t> algo_fresh;  // next_start_time requested
   std::vector<std::uint8_t> algo_dir;
+  // configuration
+  double admission_wait_ms = 2.0;
+  std::uint32_t grid_waves = 8192U;
   // instrumentation
   bool profile = false;
   std::vector<hipEvent_t> events;  // 4 per round of one epoch
@@ -295,6 +298,10 @@ struct runner {
     bool any = false;
     std::uint32_t max_end_k = 0U;
     std::vector<std::uint8_t> sweep(G, 0U);
+    // continuing range queries keep their slots; free slots take queued
+    // queries, ordered by locality so that neighbouring lanes of a
+    // wavefront start close to each other
+    std::vector<std::uint32_t> free_list;
     for (std::uint32_t s = 0; s != S; ++s) {
       auto& si = slots[s];
       si.is_new = false;
@@ -303,22 +310,34 @@ struct runner {
         ++si.epoch;
       } else {
         si.q = -1;
-        while (true) {
-          auto const q = next();
-          if (q < 0) break;
-          auto& p = pq[static_cast<std::size_t>(q)];
-          if (p.epochs_.empty()) {  // overlap or no start labels
-            finish(p);
-            on_done(q);
-            continue;
-          }
-          si.q = q;
-          si.epoch = 0U;
-          si.is_new = true;
-          break;
-        }
-        if (si.q < 0) continue;
+        free_list.push_back(s);
       }
+    }
+    std::vector<std::int32_t> admitted;
+    while (admitted.size() < free_list.size()) {
+      auto const q = next();
+      if (q < 0) break;
+      auto& p = pq[static_cast<std::size_t>(q)];
+      if (p.epochs_.empty()) {  // overlap or no start labels
+        finish(p);
+        on_done(q);
+        continue;
+      }
+      admitted.push_back(q);
+    }
+    std::stable_sort(admitted.begin(), admitted.end(), [&](std::int32_t a, std::int32_t b) {
+      return pq[static_cast<std::size_t>(a)].locality_key_ < pq[static_cast<std::size_t>(b)].locality_key_;
+    });
+    for (std::size_t j = 0; j != admitted.size(); ++j) {
+      auto& si = slots[free_list[j]];
+      si.q = admitted[j];
+      pq[static_cast<std::size_t>(admitted[j])].slot_ = static_cast<std::int32_t>(free_list[j]);
+      si.epoch = 0U;
+      si.is_new = true;
+    }
+    for (std::uint32_t s = 0; s != S; ++s) {
+      auto& si = slots[s];
+      if (si.q < 0) continue;
       si.running = true;
       any = true;
       auto& p = pq[static_cast<std::size_t>(si.q)];
@@ -443,7 +462,7 @@ struct runner {
     if (!sweep_groups.empty()) {
       HIP_CHECK(hipMemcpyAsync(e.d_sweep_groups, sweep_groups.data(), 4U * sweep_groups.size(), hipMemcpyHostToDevice, stream));
     }
-    int const grid_waves = 2048;
+    auto const grid_waves = static_cast<int>(e.grid_waves);
     std::uint32_t n_run = 0U;
     for (std::uint32_t k = 1U; k < max_end_k; ++k) {
       launch_round(fwd, W, rargs(static_cast<std::int32_t>(k)), e.d_sweep_groups,
@@ -485,6 +504,15 @@ struct runner {
       }
     }
     return true;
+  }
+
+  // slots that take a new query at the next epoch boundary
+  std::uint32_t free_slots() const {
+    std::uint32_t n = 0U;
+    for (auto const& si : slots) {
+      if (si.q < 0 || si.epoch + 1U >= pq[static_cast<std::size_t>(si.q)].epochs_.size()) ++n;
+    }
+    return n;
   }
 
   void upload_lbs(std::vector<std::uint32_t> const& new_slots, epoch_args& a) {
@@ -567,6 +595,7 @@ std::uint32_t write_results(std::vector<prepared_query> const& pq, std::vector<i
     r = nigiri_result{};
     r.status = status[i];
     r.journey_offset = pos;
+    r.slot = -1;
     if (status[i] != NIGIRI_AMD_OK) continue;
     auto const& p = pq[i];
     r.fastest_direct = p.fastest_direct_;
@@ -574,6 +603,7 @@ std::uint32_t write_results(std::vector<prepared_query> const& pq, std::vector<i
     r.interval_from = p.iv_from_;
     r.interval_to = p.iv_to_;
     r.n_epochs = static_cast<std::uint32_t>(p.epochs_.size());
+    r.slot = p.slot_;
     if (epochs != nullptr) r.n_rounds = (*epochs)[i];
     if (latency != nullptr) r.latency_ms = (*latency)[i];
     for (auto const& j : p.results_) {
@@ -668,6 +698,11 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     e->K = max_k;
     e->S = lanes * n_groups;
     HIP_CHECK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    {
+      int cus = 0;
+      HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, img->device));
+      e->grid_waves = static_cast<std::uint32_t>(std::max(1, cus) * 32);
+    }
     auto& st = e->st;
     auto& m = e->mem;
     auto const L = img->d.L, R = img->d.R, G = e->G, W = e->W, K = e->K, S = e->S;
@@ -681,7 +716,7 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     auto const cols = static_cast<std::size_t>(G) * L * W;
     st.round_times = m.alloc<std::int16_t>(cols * K);
     st.best = m.alloc<std::int16_t>(cols);
-    st.tmp = m.alloc<std::int16_t>(cols);
+    st.tmp = m.alloc<std::int32_t>(cols);
     st.lb = m.alloc<std::uint16_t>(cols);
     auto const lwords = W == 64U ? static_cast<std::size_t>(G) * L : static_cast<std::size_t>(G) * st.LW;
     auto const rwords = W == 64U ? static_cast<std::size_t>(G) * R : static_cast<std::size_t>(G) * st.RW;
@@ -689,6 +724,11 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     st.mark_r = m.alloc<std::uint64_t>(lwords);
     st.dest_mask = m.alloc<std::uint64_t>(lwords);
     st.hz_mask = m.alloc<std::uint64_t>(lwords);
+    auto const lsum = static_cast<std::size_t>(G) * st.LW, rsum = static_cast<std::size_t>(G) * st.RW;
+    st.sum_t = m.alloc<std::uint64_t>(lsum);
+    st.sum_r = m.alloc<std::uint64_t>(lsum);
+    st.sum_hz = m.alloc<std::uint64_t>(lsum);
+    st.sum_route = m.alloc<std::uint64_t>(rsum);
     st.route_mark = m.alloc<std::uint64_t>(rwords);
     st.grp_active = m.alloc<std::uint64_t>(G);
     st.grp_any_route = m.alloc<std::uint64_t>(G);
@@ -718,6 +758,10 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     HIP_CHECK(hipMemsetAsync(st.mark_r, 0, 8U * lwords, e->stream));
     HIP_CHECK(hipMemsetAsync(st.dest_mask, 0, 8U * lwords, e->stream));
     HIP_CHECK(hipMemsetAsync(st.hz_mask, 0, 8U * lwords, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.sum_t, 0, 8U * lsum, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.sum_r, 0, 8U * lsum, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.sum_hz, 0, 8U * lsum, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.sum_route, 0, 8U * rsum, e->stream));
     HIP_CHECK(hipMemsetAsync(st.route_mark, 0, 8U * rwords, e->stream));
     HIP_CHECK(hipMemsetAsync(st.grp_active, 0, 8U * G, e->stream));
     HIP_CHECK(hipMemsetAsync(st.grp_any_route, 0, 8U * G, e->stream));
@@ -762,6 +806,13 @@ int nigiri_amd_engine_state_bytes(const nigiri_amd_engine* e, uint64_t* b) {
 }
 
 void nigiri_amd_engine_destroy(nigiri_amd_engine* e) { delete e; }
+
+int nigiri_amd_engine_configure(nigiri_amd_engine* e, const nigiri_engine_config* c) {
+  if (e == nullptr || c == nullptr || c->admission_wait_ms < 0.0) return NIGIRI_AMD_EINVAL;
+  e->admission_wait_ms = c->admission_wait_ms;
+  if (c->grid_waves != 0U) e->grid_waves = c->grid_waves;
+  return NIGIRI_AMD_OK;
+}
 
 int nigiri_amd_engine_profile(nigiri_amd_engine* e, int enable) {
   if (e == nullptr) return NIGIRI_AMD_EINVAL;
@@ -866,6 +917,7 @@ int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs, const double*
     std::deque<std::int32_t> ready;
     std::atomic<std::uint32_t> next_prep{0U};
     std::atomic<std::uint32_t> n_done{0U};
+    std::atomic<std::uint32_t> n_prepared_ctr{0U};
     auto const t0 = clock::now();
     auto const ms_since = [&](clock::time_point t) {
       return std::chrono::duration<double, std::milli>(t - t0).count();
@@ -892,10 +944,12 @@ int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs, const double*
           if (status[i] != NIGIRI_AMD_OK) {
             latency[i] = ms_since(clock::now()) - arrival_ms[i];
             n_done.fetch_add(1U);
+            n_prepared_ctr.fetch_add(1U);
             continue;
           }
           std::lock_guard<std::mutex> lock{mu};
           ready.push_back(static_cast<std::int32_t>(i));
+          n_prepared_ctr.fetch_add(1U);
         }
       });
     }
@@ -911,8 +965,26 @@ int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs, const double*
       latency[static_cast<std::size_t>(q)] = ms_since(clock::now()) - arrival_ms[q];
       n_done.fetch_add(1U);
     };
+    std::atomic<std::uint32_t>& n_prepared = n_prepared_ctr;
     try {
       while (n_done.load() < n) {
+        // admission window: let arrived queries finish preparation so that
+        // free slots start the next epoch filled
+        auto const free = r.free_slots();
+        auto const deadline = clock::now() + std::chrono::duration_cast<clock::duration>(
+                                                  std::chrono::duration<double, std::milli>(e->admission_wait_ms));
+        while (clock::now() < deadline) {
+          std::size_t n_ready = 0U;
+          {
+            std::lock_guard<std::mutex> lock{mu};
+            n_ready = ready.size();
+          }
+          auto const now_ms = ms_since(clock::now());
+          auto const arrived = static_cast<std::uint32_t>(
+              std::upper_bound(arrival_ms, arrival_ms + n, now_ms) - arrival_ms);
+          if (n_ready >= free || n_prepared.load() >= arrived) break;
+          std::this_thread::sleep_for(std::chrono::microseconds(20));
+        }
         if (!r.step()) {
           std::this_thread::sleep_for(std::chrono::microseconds(50));
         }

@@ -123,6 +123,18 @@ __device__ inline void mark_one(std::uint64_t* marks, std::uint64_t* list, std::
   }
 }
 
+// Marks of the batched engine: the lane mask word and a one-bit-per-location
+// summary word, both set with non-returning atomics; a compaction kernel
+// turns the summary into the next phase's work list.
+__device__ inline void mark_wave64(std::uint64_t* marks, std::uint64_t* sum, dev_state const& s,
+                                   std::uint32_t g, std::uint32_t l, std::uint64_t lanes,
+                                   std::uint32_t words_per_group, std::uint32_t n_per_group) {
+  atomicOr(reinterpret_cast<unsigned long long*>(&marks[static_cast<std::size_t>(g) * n_per_group + l]),
+           static_cast<unsigned long long>(lanes));
+  atomicOr(reinterpret_cast<unsigned long long*>(&sum[static_cast<std::size_t>(g) * words_per_group + l / 64U]),
+           1ULL << (l % 64U));
+}
+
 // ---- transfer-time settings (transfer_time_settings.h:21-31) ------------
 template <typename T>
 __device__ inline T adjusted(slot_param const& p, T const d) {
@@ -230,14 +242,7 @@ __global__ void k_mark_routes(round_args a) {
         atomicAdd(&s.work[kWorkLocationRoutes], nq * (r1 - r0));
       }
       for (auto j = r0 + lane; j < r1; j += 64U) {
-        auto const r = im.location_routes[j];
-        auto const old = atomicOr(
-            reinterpret_cast<unsigned long long*>(&s.route_mark[lay<W>::rword(s, g, r)]),
-            static_cast<unsigned long long>(word));
-        if (old == 0ULL) {
-          list_push(s.list_route, &s.counts[2], s.cap_route,
-                    (static_cast<std::uint64_t>(g) << 32) | r, overflow);
-        }
+        mark_wave64(s.route_mark, s.sum_route, s, g, im.location_routes[j], word, s.RW, s.R);
       }
       if (lane == 0U && r1 != r0) {
         atomicOr(reinterpret_cast<unsigned long long*>(&s.grp_any_route[g]),
@@ -284,43 +289,73 @@ __device__ inline std::int16_t to_delta(std::uint16_t day, int mam, std::uint16_
   return clamp16((static_cast<int>(day) - static_cast<int>(base)) * 1440 + mam);
 }
 
-// get_earliest_transport (raptor.h:1351-1442)
 struct et_counters {
   unsigned long long events = 0, route_days = 0, transport_tests = 0;
 };
 
+// Traffic-day bitfields of one route scan: the route's 8 words and either
+// the staged 8 words per transport (LDS) or the transports' bitfield
+// indices into the global table.
+struct route_bits {
+  std::uint64_t const* route;   // 8 words
+  std::uint64_t const* staged;  // n_t x 8 words (valid when use_staged)
+  std::uint32_t const* tbf;     // bitfield index per transport
+  std::uint64_t const* table;   // global bitfield table
+  bool use_staged;
+  __device__ bool route_active(std::uint32_t day) const {
+    return day < 512U && ((route[day / 64U] >> (day % 64U)) & 1ULL) != 0ULL;
+  }
+  __device__ bool transport_active(std::uint32_t t_off, std::uint32_t day) const {
+    if (day >= 512U) return false;  // out-of-range day index: not active
+    auto const w = use_staged ? staged[t_off * 8U + day / 64U]
+                              : table[static_cast<std::size_t>(tbf[t_off]) * 8U + day / 64U];
+    return ((w >> (day % 64U)) & 1ULL) != 0ULL;
+  }
+};
+
+// get_earliest_transport (raptor.h:1351-1442).  `ev` points at the event
+// block of the boarding stop (one delta per transport, in LDS or global).
+// The first-day seek (linear_lb, linear_lower_bound.h:7-15) is a binary
+// search: the minutes of day of one stop block are non-decreasing because
+// routes keep FIFO order modulo 1440 (checked when a timetable is built);
+// the event counter still counts what the linear scan examines.
 template <bool Fwd>
-__device__ inline et_t<Fwd> earliest_transport(dev_image const& im, std::uint32_t r,
-                                               std::uint32_t t_from, std::uint32_t n_t,
-                                               std::uint32_t ev_base, std::uint16_t day_at_stop,
+__device__ inline et_t<Fwd> earliest_transport(route_bits const& bits, std::uint16_t const* ev,
+                                               std::uint32_t n_t, std::uint16_t day_at_stop,
                                                int mam_at_stop, int lb_l, int tad_k,
                                                std::uint16_t base, et_counters& c) {
   using D = dirop<Fwd>;
-  auto const rbf = im.route_traffic_days[r];
   for (std::uint32_t i = 0U; i != 6U; ++i) {  // kMaxTravelTime / 1 day + 1
     auto const day = static_cast<std::uint16_t>(Fwd ? day_at_stop + i : day_at_stop - i);
     ++c.route_days;
-    if (!bit_test(im.bitfields, rbf, day)) continue;
+    if (!bits.route_active(day)) continue;
     std::uint32_t pos = 0U;
-    if (i == 0U) {  // linear lower bound on the minute of day (linear_lower_bound.h:7-15)
-      for (; pos != n_t; ++pos) {
-        ++c.events;
-        auto const t_off = Fwd ? pos : n_t - 1U - pos;
-        if (!D::better(im.event_times[ev_base + t_off] >> 5, mam_at_stop)) break;
+    if (i == 0U) {
+      std::uint32_t lo = 0U, hi = n_t;
+      while (lo < hi) {
+        auto const mid = (lo + hi) / 2U;
+        auto const t_off = Fwd ? mid : n_t - 1U - mid;
+        if (D::better(ev[t_off] >> 5, mam_at_stop)) {
+          lo = mid + 1U;
+        } else {
+          hi = mid;
+        }
       }
+      pos = lo;
+      c.events += pos < n_t ? pos + 1U : n_t;
     }
     for (; pos < n_t; ++pos) {
       ++c.events;
       auto const t_off = Fwd ? pos : n_t - 1U - pos;
-      auto const ev = im.event_times[ev_base + t_off];
-      int const ev_mam = ev >> 5;
+      auto const e = ev[t_off];
+      int const ev_mam = e >> 5;
       if (!D::better_eq(to_delta<Fwd>(day, ev_mam, base) + D::dir(lb_l), tad_k)) {
         return {~0U, 0xFFFFU};
       }
       if (i == 0U && !D::better_eq(mam_at_stop, ev_mam)) continue;
-      auto const start_day = static_cast<std::uint16_t>(static_cast<int>(day) - (ev & 31));
+      auto const start_day = static_cast<std::uint16_t>(static_cast<int>(day) - (e & 31));
       ++c.transport_tests;
-      if (!bit_test(im.bitfields, im.transport_traffic_days[t_from + t_off], start_day)) continue;
+      if (!bits.transport_active(t_off, start_day)) continue;
       return {t_off, start_day};
     }
   }
@@ -328,140 +363,406 @@ __device__ inline et_t<Fwd> earliest_transport(dev_image const& im, std::uint32_
 }
 
 template <bool Fwd>
-__device__ inline std::int16_t time_at_stop(dev_image const& im, std::uint32_t stf,
-                                            std::uint32_t n_t, et_t<Fwd> et,
-                                            std::uint32_t stop_idx, bool arr,
+__device__ inline std::int16_t time_at_stop(std::uint16_t const* ev, std::uint32_t n_t,
+                                            et_t<Fwd> et, std::uint32_t stop_idx, bool arr,
                                             std::uint16_t base) {
-  auto const ev = im.event_times[stf + n_t * (stop_idx * 2U - (arr ? 1U : 0U)) + et.t_off];
-  return to_delta<Fwd>(et.day, (ev & 31) * 1440 + (ev >> 5), base);
+  auto const e = ev[n_t * (stop_idx * 2U - (arr ? 1U : 0U)) + et.t_off];
+  return to_delta<Fwd>(et.day, (e & 31) * 1440 + (e >> 5), base);
 }
 
-// route_scan: loop_routes + update_route (raptor.h:496-575, 1181-1349).
-// The route phase reads only round_times[k-1], best, lb, time_at_dest and
-// the previous marks, and writes tmp by minimum and marks by OR: route
-// order does not matter, so routes run in parallel.
+// Per-route data of one route scan: stop words, transport bitfield indices
+// and the route's event block, either in LDS or in the global image.
+struct route_view {
+  std::uint32_t const* stops;
+  std::uint16_t const* ev;
+  route_bits bits;
+  std::uint32_t n_stops, n_t;
+};
+
+__device__ inline void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// update_route (raptor.h:1181-1349), plain engine: one route per lane, all
+// data read from the global image and state.
 template <bool Fwd, int W>
-__global__ void k_route_scan(round_args a) {
+__device__ inline void scan_route(route_view const& rv, dev_image const& im, dev_state const& s,
+                                  std::uint32_t g, std::uint32_t r, std::uint32_t k,
+                                  std::uint32_t lane_id, bool on, std::uint16_t base, int tad_k,
+                                  unsigned long long& c_stops, unsigned long long& c_tmp,
+                                  et_counters& c_et, bool& any) {
   using D = dirop<Fwd>;
-  auto const& im = a.im;
-  auto const& s = a.st;
-  auto const k = static_cast<std::uint32_t>(a.k);
   auto* overflow = &s.counts[4];
-  auto const n_items = clamp_count(s.counts[2], s.cap_route);
-  std::uint32_t const lane_id = W == 64 ? (threadIdx.x & 63U) : 0U;
-  std::uint32_t const step = W == 64 ? gridDim.x * kWavesPerBlock : gridDim.x * blockDim.x;
-  std::uint32_t item = W == 64 ? (blockIdx.x * blockDim.x + threadIdx.x) / 64U
-                               : blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long c_stops = 0ULL, c_tmp = 0ULL;
-  et_counters c_et;
-  for (; item < n_items; item += step) {
-    auto const e = s.list_route[item];
-    auto const g = static_cast<std::uint32_t>(e >> 32), r = static_cast<std::uint32_t>(e);
-    auto const rw = lay<W>::rword(s, g, r);
-    auto const mword = s.route_mark[rw];
-    std::uint64_t lanes;
-    if constexpr (W == 64) {
-      lanes = mword & s.grp_active[g] & s.grp_any_route[g];
-      if (lane_id == 0U) s.route_mark[rw] = 0ULL;
-    } else {
-      lanes = ((mword >> (r % 64U)) & 1ULL) & s.grp_active[g] & s.grp_any_route[g];
-      // W == 1: several routes share a word; clear only this route's bit
-      atomicAnd(reinterpret_cast<unsigned long long*>(&s.route_mark[rw]),
-                ~static_cast<unsigned long long>(lay<W>::rbit(r, 0U)));
+  auto const n_stops = rv.n_stops;
+  auto const n_t = rv.n_t;
+  et_t<Fwd> et{~0U, 0xFFFFU};
+  bool done = !on;
+  for (std::uint32_t i = 0U; i != n_stops; ++i) {
+    auto const stop_idx = Fwd ? i : n_stops - i - 1U;
+    auto const stp = rv.stops[stop_idx];
+    auto const l = stp & ((1U << 28) - 1U);
+    auto const mw = s.mark_t[lay<W>::lword(s, g, l)];
+    auto const c = lay<W>::col(s, g, l, lane_id);
+    int const lbv = done ? 0 : static_cast<int>(s.lb[c]);
+    std::int16_t const prev = (!done && (mw & lay<W>::lbit(l, lane_id)) != 0ULL)
+                                  ? s.round_times[lay<W>::rt(s, g, k - 1U, l, lane_id)]
+                                  : D::kInvalid;
+    auto const in_ok = ((stp >> 28) & 1U) != 0U, out_ok = ((stp >> 29) & 1U) != 0U;
+    bool const can_finish = Fwd ? out_ok : in_ok;
+    bool const can_start = Fwd ? in_ok : out_ok;
+    if (!done) ++c_stops;
+    bool ride = false;
+    if (!done && et.valid() && can_finish) {
+      ride = true;
+      auto const by = time_at_stop<Fwd>(rv.ev, n_t, et, stop_idx, Fwd, base);
+      if (D::better_eq(by, tad_k) && lbv != kUnreachableLb && D::better_eq(by + D::dir(lbv), tad_k)) {
+        if constexpr (Fwd) {
+          atomicMin(&s.tmp[c], static_cast<int>(by));
+        } else {
+          atomicMax(&s.tmp[c], static_cast<int>(by));
+        }
+        ++c_tmp;
+        any = true;
+        mark_one<W>(s.mark_r, s.list_r, &s.counts[1], s.cap_r, overflow, s, g, l, 0U);
+      }
     }
-    if (lanes == 0ULL) continue;
-    bool const on = ((lanes >> lane_id) & 1ULL) != 0ULL;
-    auto const slot = g * W + lane_id;
-    auto const base = s.param[slot].base;
-    int const tad_k = s.tad[static_cast<std::size_t>(slot) * kRowsMax + k];
+    bool const is_last = i == n_stops - 1U;
+    if (done || is_last || !can_start || (mw & lay<W>::lbit(l, lane_id)) == 0ULL) continue;
+    if (lbv == kUnreachableLb) {
+      done = true;  // break (raptor.h:1309-1311)
+      continue;
+    }
+    auto const et_dep = et.valid() ? time_at_stop<Fwd>(rv.ev, n_t, et, stop_idx, !Fwd, base)
+                                   : D::kInvalid;
+    if (prev != D::kInvalid && D::better_eq(prev, et_dep)) {
+      int const minutes = static_cast<int>(base) * 1440 + prev;
+      auto const day = static_cast<std::uint16_t>(minutes / 1440);
+      int const mam = static_cast<std::int16_t>(minutes % 1440);
+      auto const* ev_stop = rv.ev + n_t * (stop_idx * 2U - (Fwd ? 0U : 1U));
+      auto const new_et = earliest_transport<Fwd>(rv.bits, ev_stop, n_t, day, mam, lbv, tad_k,
+                                                  base, c_et);
+      if (new_et.valid()) {
+        bool const cb_valid = ride || s.best[c] != D::kInvalid || s.tmp[c] != D::kInvalid;
+        if (!cb_valid ||
+            D::better_eq(time_at_stop<Fwd>(rv.ev, n_t, new_et, stop_idx, !Fwd, base), et_dep)) {
+          et = new_et;
+        }
+      }
+    }
+  }
+}
 
-    auto const s0 = im.route_stop_offsets[r];
-    auto const n_stops = im.route_stop_offsets[r + 1U] - s0;
-    auto const t_from = im.route_transport_from[r];
-    auto const n_t = im.route_transport_from[r + 1U] - t_from;
-    auto const stf = im.route_stop_time_from[r];
+// ---- batched route scan (one wavefront per (64-query group, route)) -------
+//
+// LDS per wavefront: the route's traffic-day bitfield and, for routes with at
+// most kStagedTransports transports, the transports' bitfields (staged once
+// per route); then, chunk by chunk, the stop words, station-mark words, the
+// lower-bound and round_times[k-1] rows of the 64 lanes and the event blocks
+// (arrival, departure) of the chunk's stops.  All staging loads of a chunk
+// are issued together; the scan of the chunk then reads LDS only, except for
+// transport bitfields of large routes and the rare current_best check.
+constexpr std::uint32_t kChunkMax = 8U;
+constexpr std::uint32_t kStagedTransports = 16U;
+constexpr std::uint32_t kChunkEvBytes = 4096U;
+constexpr std::uint32_t kMaxRouteTransports = kChunkEvBytes / 4U;  // chunk of >= 1 stop
 
-    et_t<Fwd> et{~0U, 0xFFFFU};
-    bool done = !on;
-    bool any = false;
-    for (std::uint32_t i = 0U; i != n_stops; ++i) {
+struct wave_lds {
+  std::uint64_t rbits[8];
+  std::uint64_t tbits[kStagedTransports * 8U];
+  std::uint64_t mark[kChunkMax];
+  std::uint32_t stop[kChunkMax];
+  std::uint16_t lb[kChunkMax][64];
+  std::int16_t rt[kChunkMax][64];
+  std::uint16_t ev[kChunkEvBytes / 2U];  // [chunk stop][arr, dep][n_t]
+};
+constexpr std::uint32_t kWaveLdsBytes = sizeof(wave_lds);
+
+template <bool Fwd>
+__device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state const& s,
+                                    std::uint32_t g, std::uint32_t r, std::uint32_t k,
+                                    std::uint32_t lane, bool on, std::uint16_t base, int tad_k,
+                                    std::uint32_t& c_stops, std::uint32_t& c_tmp,
+                                    std::uint32_t& c_ev, std::uint32_t& c_rd,
+                                    std::uint32_t& c_tt, bool& any) {
+  using D = dirop<Fwd>;
+  auto const L = s.L;
+  auto const s0 = im.route_stop_offsets[r];
+  auto const n_stops = im.route_stop_offsets[r + 1U] - s0;
+  auto const t_from = im.route_transport_from[r];
+  auto const n_t = im.route_transport_from[r + 1U] - t_from;
+  auto const* stops_g = im.route_stops + s0;
+  auto const* ev_g = im.event_times + im.route_stop_time_from[r];
+  auto const* tbf_g = im.transport_traffic_days + t_from;
+  // per-group state bases: uniform, indexed with 32-bit lane offsets
+  auto const* lb_g = s.lb + static_cast<std::size_t>(g) * L * 64U;
+  auto const* rt_g = s.round_times + (static_cast<std::size_t>(g) * s.K + (k - 1U)) * L * 64U;
+  auto* tmp_g = s.tmp + static_cast<std::size_t>(g) * L * 64U;
+  auto const* best_g = s.best + static_cast<std::size_t>(g) * L * 64U;
+  auto const* mark_t_g = s.mark_t + static_cast<std::size_t>(g) * L;
+  auto* mark_r_g = s.mark_r + static_cast<std::size_t>(g) * L;
+  auto* sum_r_g = s.sum_r + static_cast<std::size_t>(g) * s.LW;
+
+  // route-level staging: traffic-day bitfields
+  auto const rbf = im.route_traffic_days[r];
+  bool const staged_t = n_t <= kStagedTransports;
+  if (lane < 8U) x.rbits[lane] = im.bitfields[static_cast<std::size_t>(rbf) * 8U + lane];
+  if (staged_t) {
+    for (auto j = lane; j < 8U * n_t; j += 64U) {
+      x.tbits[j] = im.bitfields[static_cast<std::size_t>(tbf_g[j / 8U]) * 8U + j % 8U];
+    }
+  }
+  route_bits const bits{x.rbits, x.tbits, tbf_g, im.bitfields, staged_t};
+  auto const chunk = min(kChunkMax, kChunkEvBytes / (4U * n_t));
+
+  et_t<Fwd> et{~0U, 0xFFFFU};
+  bool done = !on;
+  std::uint64_t const lbit = 1ULL << lane;
+  for (std::uint32_t c0 = 0U; c0 < n_stops; c0 += chunk) {
+    auto const nc = min(chunk, n_stops - c0);
+    wave_sync_lds();  // readers of the previous chunk are done
+    // stop words and station-mark words (lane j: stop c0 + j)
+    if (lane < nc) {
+      auto const si = c0 + lane;
+      auto const stp = stops_g[Fwd ? si : n_stops - si - 1U];
+      x.stop[lane] = stp;
+      x.mark[lane] = mark_t_g[stp & ((1U << 28) - 1U)];
+    }
+    // event blocks: for chunk stop j, [arr at stop_idx | dep at stop_idx]
+    for (auto e = lane; e < 2U * nc * n_t; e += 64U) {
+      auto const j = e / (2U * n_t), rem = e % (2U * n_t);
+      auto const which = rem / n_t, t = rem % n_t;  // 0 arrival, 1 departure
+      auto const si = c0 + j;
+      auto const stop_idx = Fwd ? si : n_stops - si - 1U;
+      bool const exists = which == 0U ? stop_idx != 0U : stop_idx + 1U != n_stops;
+      x.ev[e] = exists ? ev_g[n_t * (2U * stop_idx - (which == 0U ? 1U : 0U)) + t] : 0U;
+    }
+    wave_sync_lds();
+    // lower-bound and round_times[k-1] rows, all loads in flight together
+    std::uint16_t lbr[kChunkMax];
+    std::int16_t rtr[kChunkMax];
+#pragma unroll
+    for (std::uint32_t j = 0U; j < kChunkMax; ++j) {
+      lbr[j] = 0U;
+      rtr[j] = D::kInvalid;
+      if (j < nc) {
+        auto const l = x.stop[j] & ((1U << 28) - 1U);
+        auto const off = l * 64U + lane;
+        if (on) lbr[j] = lb_g[off];
+        if (on && (x.mark[j] & lbit) != 0ULL) rtr[j] = rt_g[off];
+      }
+    }
+#pragma unroll
+    for (std::uint32_t j = 0U; j < kChunkMax; ++j) {
+      if (j < nc) {
+        x.lb[j][lane] = lbr[j];
+        x.rt[j][lane] = rtr[j];
+      }
+    }
+    wave_sync_lds();
+
+    for (std::uint32_t j = 0U; j < nc; ++j) {
+      auto const i = c0 + j;
       auto const stop_idx = Fwd ? i : n_stops - i - 1U;
-      auto const stp = im.route_stops[s0 + stop_idx];
+      auto const stp = x.stop[j];
       auto const l = stp & ((1U << 28) - 1U);
+      auto const mw = x.mark[j];
+      int const lbv = x.lb[j][lane];
+      std::int16_t const prev = x.rt[j][lane];
+      auto const* ev_arr = x.ev + (2U * j) * n_t;
+      auto const* ev_dep = ev_arr + n_t;
       auto const in_ok = ((stp >> 28) & 1U) != 0U, out_ok = ((stp >> 29) & 1U) != 0U;
       bool const can_finish = Fwd ? out_ok : in_ok;
       bool const can_start = Fwd ? in_ok : out_ok;
-      auto const c = lay<W>::col(s, g, l, lane_id);
-      if (!done) ++c_stops;
+      auto const off = l * 64U + lane;
+      c_stops += done ? 0U : 1U;
 
       bool upd = false;
       bool ride = false;
       if (!done && et.valid() && can_finish) {
         ride = true;
-        auto const by = time_at_stop<Fwd>(im, stf, n_t, et, stop_idx, Fwd, base);
-        int const lbv = s.lb[c];
+        auto const e = (Fwd ? ev_arr : ev_dep)[et.t_off];
+        auto const by = to_delta<Fwd>(et.day, (e & 31) * 1440 + (e >> 5), base);
         if (D::better_eq(by, tad_k) && lbv != kUnreachableLb && D::better_eq(by + D::dir(lbv), tad_k)) {
-          atomic_best16<Fwd>(&s.tmp[c], by);
+          if constexpr (Fwd) {
+            atomicMin(&tmp_g[off], static_cast<int>(by));
+          } else {
+            atomicMax(&tmp_g[off], static_cast<int>(by));
+          }
           upd = true;
           ++c_tmp;
         }
       }
-      if constexpr (W == 64) {
-        auto const m = __ballot(upd);
-        if (m != 0ULL && lane_id == static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1)) {
-          auto const old = atomicOr(reinterpret_cast<unsigned long long*>(&s.mark_r[lay<W>::lword(s, g, l)]),
-                                    static_cast<unsigned long long>(m));
-          if (old == 0ULL) {
-            list_push(s.list_r, &s.counts[1], s.cap_r, (static_cast<std::uint64_t>(g) << 32) | l, overflow);
-          }
-        }
-      } else {
-        if (upd) mark_one<W>(s.mark_r, s.list_r, &s.counts[1], s.cap_r, overflow, s, g, l, 0U);
+      auto const m = __ballot(upd);
+      if (m != 0ULL && lane == static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1)) {
+        atomicOr(reinterpret_cast<unsigned long long*>(&mark_r_g[l]), static_cast<unsigned long long>(m));
+        atomicOr(reinterpret_cast<unsigned long long*>(&sum_r_g[l / 64U]), 1ULL << (l % 64U));
       }
       any |= upd;
 
       bool const is_last = i == n_stops - 1U;
-      if (done || is_last || !can_start || !lay<W>::test(s.mark_t, s, g, l, lane_id)) continue;
-      if (s.lb[c] == kUnreachableLb) {
+      if (done || is_last || !can_start || (mw & lbit) == 0ULL) continue;
+      if (lbv == kUnreachableLb) {
         done = true;  // break (raptor.h:1309-1311)
         continue;
       }
-      auto const et_dep = et.valid() ? time_at_stop<Fwd>(im, stf, n_t, et, stop_idx, !Fwd, base)
-                                     : D::kInvalid;
-      auto const prev = s.round_times[lay<W>::rt(s, g, k - 1U, l, lane_id)];
+      auto const* ev_board = Fwd ? ev_dep : ev_arr;
+      std::int16_t et_dep = D::kInvalid;
+      if (et.valid()) {
+        auto const e = ev_board[et.t_off];
+        et_dep = to_delta<Fwd>(et.day, (e & 31) * 1440 + (e >> 5), base);
+      }
       if (prev != D::kInvalid && D::better_eq(prev, et_dep)) {
         int const minutes = static_cast<int>(base) * 1440 + prev;
         auto const day = static_cast<std::uint16_t>(minutes / 1440);
         int const mam = static_cast<std::int16_t>(minutes % 1440);
-        auto const ev_base = stf + n_t * (stop_idx * 2U - (Fwd ? 0U : 1U));
-        auto const new_et = earliest_transport<Fwd>(im, r, t_from, n_t, ev_base, day, mam,
-                                                    s.lb[c], tad_k, base, c_et);
-        // current_best is valid whenever a ride was evaluated at this stop
-        // (it then includes the valid round_times[k-1] label) or best / tmp
-        // hold a label
-        bool const cb_valid = ride || s.best[c] != D::kInvalid || s.tmp[c] != D::kInvalid;
-        if (new_et.valid() &&
-            (!cb_valid ||
-             D::better_eq(time_at_stop<Fwd>(im, stf, n_t, new_et, stop_idx, !Fwd, base), et_dep))) {
-          et = new_et;
+        et_counters ce;
+        auto const new_et = earliest_transport<Fwd>(bits, ev_board, n_t, day, mam, lbv, tad_k,
+                                                    base, ce);
+        c_ev += static_cast<std::uint32_t>(ce.events);
+        c_rd += static_cast<std::uint32_t>(ce.route_days);
+        c_tt += static_cast<std::uint32_t>(ce.transport_tests);
+        if (new_et.valid()) {
+          // current_best (raptor.h:1328-1329) is valid whenever a ride was
+          // evaluated at this stop (it then includes the valid round_times
+          // [k-1] label) or best / tmp hold a label
+          bool const cb_valid = ride || best_g[off] != D::kInvalid || tmp_g[off] != D::kInvalid;
+          auto const e = ev_board[new_et.t_off];
+          auto const new_dep = to_delta<Fwd>(new_et.day, (e & 31) * 1440 + (e >> 5), base);
+          if (!cb_valid || D::better_eq(new_dep, et_dep)) {
+            et = new_et;
+          }
         }
       }
     }
-    if constexpr (W == 64) {
+  }
+}
+
+// route_scan: loop_routes + update_route (raptor.h:496-575, 1181-1349).
+// The route phase reads only round_times[k-1], best, lb, time_at_dest and
+// the previous marks, and writes tmp by minimum and marks by OR: route
+// order does not matter, so routes run in parallel.  Batched engine: one
+// wavefront per (64-query group, route), staged in LDS; plain engine: one
+// route per lane.
+template <bool Fwd, int W>
+__global__ void __launch_bounds__(256, 5) k_route_scan(round_args a) {
+  auto const& im = a.im;
+  auto const& s = a.st;
+  auto const k = static_cast<std::uint32_t>(a.k);
+  auto const n_items = clamp_count(s.counts[2], s.cap_route);
+  if constexpr (W == 64) {
+    __shared__ wave_lds lds_all[kWavesPerBlock];
+    auto& x = lds_all[threadIdx.x / 64U];
+    std::uint32_t const lane = threadIdx.x & 63U;
+    std::uint32_t c_stops = 0U, c_tmp = 0U, c_ev = 0U, c_rd = 0U, c_tt = 0U, c_ws = 0U;
+    for (std::uint32_t item = (blockIdx.x * blockDim.x + threadIdx.x) / 64U; item < n_items;
+         item += gridDim.x * kWavesPerBlock) {
+      auto const e = s.list_route[item];
+      auto const g = static_cast<std::uint32_t>(e >> 32), r = static_cast<std::uint32_t>(e);
+      auto const rw = static_cast<std::size_t>(g) * s.R + r;
+      auto const lanes = s.route_mark[rw] & s.grp_active[g] & s.grp_any_route[g];
+      if (lane == 0U) s.route_mark[rw] = 0ULL;
+      if (lanes == 0ULL) continue;
+      bool const on = ((lanes >> lane) & 1ULL) != 0ULL;
+      auto const slot = g * 64U + lane;
+      auto const base = s.param[slot].base;
+      int const tad_k = s.tad[static_cast<std::size_t>(slot) * kRowsMax + k];
+      c_ws += im.route_stop_offsets[r + 1U] - im.route_stop_offsets[r];
+      bool any = false;
+      wave_sync_lds();  // the previous item's readers are done
+      scan_route64<Fwd>(x, im, s, g, r, k, lane, on, base, tad_k, c_stops, c_tmp, c_ev, c_rd, c_tt,
+                        any);
       auto const m = __ballot(any);
-      if (m != 0ULL && lane_id == static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1)) {
+      if (m != 0ULL && lane == static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1)) {
         atomicOr(reinterpret_cast<unsigned long long*>(&s.grp_any_tmp[g]),
                  static_cast<unsigned long long>(m));
       }
-    } else {
+    }
+    wave_count(&s.work[kWorkRouteStops], c_stops);
+    wave_count(&s.work[kWorkTmpWrites], c_tmp);
+    wave_count(&s.work[kWorkEvents], c_ev);
+    wave_count(&s.work[kWorkRouteDayTests], c_rd);
+    wave_count(&s.work[kWorkTransportTests], c_tt);
+    if (lane == 0U && c_ws != 0U) atomicAdd(&s.work[kWorkWaveStops], static_cast<unsigned long long>(c_ws));
+  } else {
+    unsigned long long c_stops = 0ULL, c_tmp = 0ULL;
+    et_counters c_et;
+    for (std::uint32_t item = blockIdx.x * blockDim.x + threadIdx.x; item < n_items;
+         item += gridDim.x * blockDim.x) {
+      auto const e = s.list_route[item];
+      auto const g = static_cast<std::uint32_t>(e >> 32), r = static_cast<std::uint32_t>(e);
+      auto const rw = lay<W>::rword(s, g, r);
+      auto const mword = s.route_mark[rw];
+      auto const lanes = ((mword >> (r % 64U)) & 1ULL) & s.grp_active[g] & s.grp_any_route[g];
+      // several routes share a word; clear only this route's bit
+      atomicAnd(reinterpret_cast<unsigned long long*>(&s.route_mark[rw]),
+                ~static_cast<unsigned long long>(lay<W>::rbit(r, 0U)));
+      if (lanes == 0ULL) continue;
+      auto const slot = g;
+      auto const base = s.param[slot].base;
+      int const tad_k = s.tad[static_cast<std::size_t>(slot) * kRowsMax + k];
+      auto const s0 = im.route_stop_offsets[r];
+      auto const t_from = im.route_transport_from[r];
+      auto const rbf = im.route_traffic_days[r];
+      route_view const rv{im.route_stops + s0, im.event_times + im.route_stop_time_from[r],
+                          route_bits{im.bitfields + static_cast<std::size_t>(rbf) * 8U, nullptr,
+                                     im.transport_traffic_days + t_from, im.bitfields, false},
+                          im.route_stop_offsets[r + 1U] - s0, im.route_transport_from[r + 1U] - t_from};
+      bool any = false;
+      scan_route<Fwd, W>(rv, im, s, g, r, k, 0U, true, base, tad_k, c_stops, c_tmp, c_et, any);
       if (any) atomicOr(reinterpret_cast<unsigned long long*>(&s.grp_any_tmp[g]), 1ULL);
     }
+    wave_count(&s.work[kWorkRouteStops], c_stops);
+    wave_count(&s.work[kWorkTmpWrites], c_tmp);
+    wave_count(&s.work[kWorkEvents], c_et.events);
+    wave_count(&s.work[kWorkRouteDayTests], c_et.route_days);
+    wave_count(&s.work[kWorkTransportTests], c_et.transport_tests);
   }
-  wave_count(&s.work[kWorkRouteStops], c_stops);
-  wave_count(&s.work[kWorkTmpWrites], c_tmp);
-  wave_count(&s.work[kWorkEvents], c_et.events);
-  wave_count(&s.work[kWorkRouteDayTests], c_et.route_days);
-  wave_count(&s.work[kWorkTransportTests], c_et.transport_tests);
+}
+
+// Compaction (batched engine): summary words -> work list entries
+// (g << 32 | index), in ascending order per group; clears the summary.
+__global__ void k_compact(std::uint64_t* sum, std::uint32_t words_per_group, std::uint32_t n_groups,
+                          std::uint64_t* list, std::uint32_t* count, std::size_t cap,
+                          std::uint32_t* overflow) {
+  auto const total = static_cast<std::size_t>(words_per_group) * n_groups;
+  auto const lane = threadIdx.x & 63U;
+  for (std::size_t base_i = (blockIdx.x * static_cast<std::size_t>(blockDim.x)) + (threadIdx.x & ~63U);
+       base_i < total; base_i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    auto const i = base_i + lane;
+    std::uint64_t v = 0ULL;
+    if (i < total) {
+      v = sum[i];
+      if (v != 0ULL) sum[i] = 0ULL;
+    }
+    unsigned n = static_cast<unsigned>(__popcll(v));
+    // inclusive scan of n over the wavefront
+    unsigned incl = n;
+    for (int off = 1; off < 64; off <<= 1) {
+      auto const t = __shfl_up(incl, off, 64);
+      if (static_cast<int>(lane) >= off) incl += t;
+    }
+    auto const total_n = __shfl(incl, 63, 64);
+    unsigned wave_base = 0U;
+    if (lane == 63U && total_n != 0U) wave_base = atomicAdd(count, total_n);
+    wave_base = __shfl(wave_base, 63, 64);
+    if (v != 0ULL) {
+      auto pos = wave_base + incl - n;
+      auto const g = static_cast<std::uint32_t>(i / words_per_group);
+      auto const w = static_cast<std::uint32_t>(i % words_per_group);
+      while (v != 0ULL) {
+        auto const b = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(v)) - 1);
+        v &= v - 1ULL;
+        if (pos < cap) {
+          list[pos] = (static_cast<std::uint64_t>(g) << 32) | (w * 64U + b);
+        } else {
+          atomicOr(overflow, 1U);
+        }
+        ++pos;
+      }
+    }
+  }
 }
 
 // clear the marks listed in `list` (all lanes)
@@ -598,7 +899,11 @@ __global__ void k_dest_pass(round_args a, std::uint32_t n_slots) {
         if (lbv == kUnreachableLb || !D::better_eq(e.x + D::dir(lbv), tad[k])) continue;
         s.round_times[rti] = e.x;
         s.best[c] = e.x;
-        mark_one<W>(s.mark_t, s.list_t, &s.counts[0], s.cap_t, &s.counts[4], s, g, e.target, lane);
+        if constexpr (W == 64) {
+          mark_wave64(s.mark_t, s.sum_t, s, g, e.target, 1ULL << lane, s.LW, s.L);
+        } else {
+          mark_one<W>(s.mark_t, s.list_t, &s.counts[0], s.cap_t, &s.counts[4], s, g, e.target, lane);
+        }
         update_tad<Fwd>(tad, k, e.x);
       }
     }
@@ -678,20 +983,11 @@ __global__ void k_transfer_fp(round_args a) {
       if constexpr (W == 64) {
         auto const m = __ballot(code == kRelaxImproved);
         auto const h = __ballot(code == kRelaxHazard);
-        auto const w = lay<W>::lword(s, g, t);
         if (m != 0ULL && lane_id == static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1)) {
-          auto const old = atomicOr(reinterpret_cast<unsigned long long*>(&s.mark_t[w]),
-                                    static_cast<unsigned long long>(m));
-          if (old == 0ULL) {
-            list_push(s.list_t, &s.counts[0], s.cap_t, (static_cast<std::uint64_t>(g) << 32) | t, overflow);
-          }
+          mark_wave64(s.mark_t, s.sum_t, s, g, t, m, s.LW, s.L);
         }
         if (h != 0ULL && lane_id == static_cast<std::uint32_t>(__ffsll(static_cast<long long>(h)) - 1)) {
-          auto const old = atomicOr(reinterpret_cast<unsigned long long*>(&s.hz_mask[w]),
-                                    static_cast<unsigned long long>(h));
-          if (old == 0ULL) {
-            list_push(s.list_hz, &s.counts[3], s.cap_hz, (static_cast<std::uint64_t>(g) << 32) | t, overflow);
-          }
+          mark_wave64(s.hz_mask, s.sum_hz, s, g, t, h, s.LW, s.L);
         }
       } else {
         if (code == kRelaxImproved) mark_one<W>(s.mark_t, s.list_t, &s.counts[0], s.cap_t, overflow, s, g, t, 0U);
@@ -912,7 +1208,11 @@ __global__ void k_add_starts(epoch_args a) {
     auto const g = slot / W, lane = slot % W;
     atomic_best16<Fwd>(&s.best[lay<W>::col(s, g, l, lane)], d);
     atomic_best16<Fwd>(&s.round_times[lay<W>::rt(s, g, 0U, l, lane)], d);
-    mark_one<W>(s.mark_t, s.list_t, &s.counts[0], s.cap_t, &s.counts[4], s, g, l, lane);
+    if constexpr (W == 64) {
+      mark_wave64(s.mark_t, s.sum_t, s, g, l, 1ULL << lane, s.LW, s.L);
+    } else {
+      mark_one<W>(s.mark_t, s.list_t, &s.counts[0], s.cap_t, &s.counts[4], s, g, l, lane);
+    }
   }
 }
 
@@ -947,7 +1247,7 @@ __global__ void k_gather_slot(dev_state s, std::uint32_t slot, std::int16_t* rt,
     } else if (row == s.K) {
       best[l] = s.best[lay<W>::col(s, g, l, lane)];
     } else {
-      tmp[l] = s.tmp[lay<W>::col(s, g, l, lane)];
+      tmp[l] = static_cast<std::int16_t>(s.tmp[lay<W>::col(s, g, l, lane)]);
     }
   }
 }
@@ -955,6 +1255,14 @@ __global__ void k_gather_slot(dev_state s, std::uint32_t slot, std::int16_t* rt,
 int grid_for(std::size_t work, int per_block, int cap) {
   auto const b = (work + static_cast<std::size_t>(per_block) - 1U) / static_cast<std::size_t>(per_block);
   return static_cast<int>(b < 1U ? 1U : (b > static_cast<std::size_t>(cap) ? static_cast<std::size_t>(cap) : b));
+}
+
+void compact(std::uint64_t* sum, std::uint32_t words_per_group, std::uint32_t n_groups,
+             std::uint64_t* list, std::uint32_t* count, std::size_t cap, dev_state const& s,
+             hipStream_t st) {
+  auto const total = static_cast<std::size_t>(words_per_group) * n_groups;
+  hipLaunchKernelGGL(k_compact, dim3(grid_for(total, kBlock, 8192)), dim3(kBlock), 0, st, sum,
+                     words_per_group, n_groups, list, count, cap, &s.counts[4]);
 }
 
 template <bool Fwd, int W>
@@ -969,16 +1277,26 @@ void round_impl(round_args const& a, std::uint32_t const* sweep_groups, std::uin
   }
   hipLaunchKernelGGL((k_round_begin<Fwd, W>), dim3(slot_blocks), dim3(kBlock), 0, st, a, n_slots);
   hipLaunchKernelGGL((k_mark_routes<W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
+  if constexpr (W == 64) {
+    compact(a.st.sum_route, a.st.RW, a.st.G, a.st.list_route, &a.st.counts[2], a.st.cap_route, a.st, st);
+  }
   if (a.ev[0] != nullptr) (void)hipEventRecord(a.ev[0], st);
   hipLaunchKernelGGL((k_route_scan<Fwd, W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
   if (a.ev[1] != nullptr) (void)hipEventRecord(a.ev[1], st);
   hipLaunchKernelGGL((k_clear_marks<W>), dim3(1024), dim3(kBlock), 0, st, a.st, 0);
   hipLaunchKernelGGL((k_group_update), dim3(grid_for(a.st.G, kBlock, 1 << 20)), dim3(kBlock), 0, st, a.st);
   hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 1U | 4U);  // list_t, routes
+  if constexpr (W == 64) {
+    compact(a.st.sum_r, a.st.LW, a.st.G, a.st.list_r, &a.st.counts[1], a.st.cap_r, a.st, st);
+  }
   hipLaunchKernelGGL((k_dest_pass<Fwd, W>), dim3(grid_for(n_slots, 64, 1 << 20)), dim3(64), 0, st, a, n_slots);
   if (a.ev[2] != nullptr) (void)hipEventRecord(a.ev[2], st);
   hipLaunchKernelGGL((k_transfer_fp<Fwd, W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
   if (a.ev[3] != nullptr) (void)hipEventRecord(a.ev[3], st);
+  if constexpr (W == 64) {
+    compact(a.st.sum_t, a.st.LW, a.st.G, a.st.list_t, &a.st.counts[0], a.st.cap_t, a.st, st);
+    compact(a.st.sum_hz, a.st.LW, a.st.G, a.st.list_hz, &a.st.counts[3], a.st.cap_hz, a.st, st);
+  }
   hipLaunchKernelGGL((k_rt_fixup<Fwd, W>), dim3(1024), dim3(kBlock), 0, st, a);
   hipLaunchKernelGGL((k_hazards<Fwd, W>), dim3(256), dim3(kBlock), 0, st, a);
   hipLaunchKernelGGL((k_clear_marks<W>), dim3(1024), dim3(kBlock), 0, st, a.st, 1);
@@ -997,6 +1315,9 @@ void epoch_begin_impl(epoch_args const& a, hipStream_t st) {
   hipLaunchKernelGGL((k_slot_init<Fwd, W>), dim3(grid_for(a.n_slots, kBlock, 1 << 20)), dim3(kBlock), 0, st, a);
   if (a.n_starts != 0U) {
     hipLaunchKernelGGL((k_add_starts<Fwd, W>), dim3(grid_for(a.n_starts, kBlock, 4096)), dim3(kBlock), 0, st, a);
+    if constexpr (W == 64) {
+      compact(a.st.sum_t, a.st.LW, a.st.G, a.st.list_t, &a.st.counts[0], a.st.cap_t, a.st, st);
+    }
   }
 }
 

@@ -251,6 +251,23 @@ void prepare(timetable const& tt, nigiri_query const& q, prepared_query& p,
   }
   lower_bounds(tt, fwd, dest_min, p.lb_, s);
 
+  // scheduling hint: queries that start close to each other mark the same
+  // routes in their first rounds, so the batch scheduler groups them
+  if (q.n_start != 0U) {
+    auto const l = q.start_loc[0];
+    std::uint32_t cx = l, cy = 0U;
+    if (!tt.coord_x.empty()) {
+      cx = static_cast<std::uint32_t>(std::max(0.0F, tt.coord_x[l]) / 256.0F) & 0xFFFFU;
+      cy = static_cast<std::uint32_t>(std::max(0.0F, tt.coord_y[l]) / 256.0F) & 0xFFFFU;
+    }
+    std::uint64_t key = 0U;
+    for (int b = 0; b != 16; ++b) {
+      key |= static_cast<std::uint64_t>((cx >> b) & 1U) << (2 * b);
+      key |= static_cast<std::uint64_t>((cy >> b) & 1U) << (2 * b + 1);
+    }
+    p.locality_key_ = key;
+  }
+
   // base day: round<days>(interval midpoint) - internal interval start
   auto const mid = static_cast<std::int64_t>(p.iv_from_) + (p.iv_to_ - p.iv_from_) / 2;
   p.base_ = static_cast<std::uint16_t>((round_to_days(mid) * 1440 - tt.internal_from) / 1440);

@@ -62,6 +62,8 @@ struct prepared_query {
   std::vector<std::uint16_t> dist_min_;
   std::vector<std::uint16_t> lb_;  // [n_locations]
   std::vector<epoch> epochs_;
+  std::uint64_t locality_key_{0U};  // Morton code of the first start stop
+  std::int32_t slot_{-1};           // engine slot that ran the query
   std::vector<host_journey> results_;  // running Pareto set
 
   std::int32_t base_minutes(timetable const& tt) const {

@@ -42,8 +42,8 @@ def compare(tt, eng, qs, idx=None, check_state=True):
         assert got["status"] == 0
         assert got["journeys"] == ref["journeys"], (i, got["journeys"], ref["journeys"])
         assert got["base_day"] == ref["base_day"]
-        if check_state and i < eng.slots:
-            st = eng.download(i)
+        if check_state and len(qs) <= eng.slots:
+            st = eng.download(got["slot"])
             K = eng.max_k
             for key in STATE_KEYS:
                 a = st[key]

@@ -74,6 +74,10 @@ struct slot_param {
 //   station marks / dest masks: W == 64 -> one uint64 lane mask per (g, l);
 //                               W == 1  -> bit l % 64 of word g*LW + l/64
 //   route marks likewise over R.
+// 64-bit words covering n flag bytes (flag arrays are allocated and read in
+// whole words)
+inline constexpr std::size_t flag_words(std::size_t n) { return (n + 7U) / 8U; }
+
 struct dev_state {
   std::uint32_t W, G, K, L, R;
   std::uint32_t LW, RW;  // words per group for W == 1 bitsets
@@ -84,23 +88,24 @@ struct dev_state {
   std::uint16_t* lb;
   std::uint64_t* mark_t;   // marks of the transfer phase / starts
   std::uint64_t* mark_r;   // marks of the route phase
-  std::uint64_t* route_mark;
+  std::uint64_t* route_mark;  // plain engine only
   std::uint64_t* dest_mask;
   std::uint64_t* grp_active;
-  std::uint64_t* grp_any_route;
-  std::uint64_t* grp_any_tmp;
+  std::uint8_t* any_route;  // per slot: a marked station had routes
+  std::uint8_t* any_tmp;    // per slot: the route phase improved tmp
   // work lists: packed (g << 32 | index)
   std::uint64_t* list_t;  // stations marked in mark_t
   std::uint64_t* list_r;  // stations marked in mark_r
   std::uint64_t* list_route;
   std::uint64_t* list_hz;  // stations with hazard lanes (g << 32 | location)
   std::uint64_t* hz_mask;  // hazard lanes per station (same layout as marks)
-  // batched engine (W == 64): one bit per location / route and group,
-  // set next to the lane-mask words and compacted into the work lists
-  std::uint64_t* sum_t;
-  std::uint64_t* sum_r;
-  std::uint64_t* sum_route;
-  std::uint64_t* sum_hz;
+  // batched engine (W == 64): one flag byte per (group, location / route),
+  // stored (no atomic) next to the lane-mask words and compacted into the
+  // work lists
+  std::uint8_t* flag_t;
+  std::uint8_t* flag_r;
+  std::uint8_t* flag_route;
+  std::uint8_t* flag_hz;
   std::uint32_t* counts;   // [0] list_t, [1] list_r, [2] list_route, [3] hz, [4] overflow flags
   // algorithmic work counters (see kWork* indices)
   unsigned long long* work;

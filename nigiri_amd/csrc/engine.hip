@@ -724,15 +724,16 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     st.mark_r = m.alloc<std::uint64_t>(lwords);
     st.dest_mask = m.alloc<std::uint64_t>(lwords);
     st.hz_mask = m.alloc<std::uint64_t>(lwords);
-    auto const lsum = static_cast<std::size_t>(G) * st.LW, rsum = static_cast<std::size_t>(G) * st.RW;
-    st.sum_t = m.alloc<std::uint64_t>(lsum);
-    st.sum_r = m.alloc<std::uint64_t>(lsum);
-    st.sum_hz = m.alloc<std::uint64_t>(lsum);
-    st.sum_route = m.alloc<std::uint64_t>(rsum);
-    st.route_mark = m.alloc<std::uint64_t>(rwords);
+    auto const lflag = 8U * flag_words(static_cast<std::size_t>(G) * L);
+    auto const rflag = 8U * flag_words(static_cast<std::size_t>(G) * R);
+    st.flag_t = m.alloc<std::uint8_t>(lflag);
+    st.flag_r = m.alloc<std::uint8_t>(lflag);
+    st.flag_hz = m.alloc<std::uint8_t>(lflag);
+    st.flag_route = m.alloc<std::uint8_t>(rflag);
+    st.route_mark = W == 64U ? nullptr : m.alloc<std::uint64_t>(rwords);
     st.grp_active = m.alloc<std::uint64_t>(G);
-    st.grp_any_route = m.alloc<std::uint64_t>(G);
-    st.grp_any_tmp = m.alloc<std::uint64_t>(G);
+    st.any_route = m.alloc<std::uint8_t>(static_cast<std::size_t>(G) * W);
+    st.any_tmp = m.alloc<std::uint8_t>(static_cast<std::size_t>(G) * W);
     st.cap_t = static_cast<std::size_t>(G) * L;
     st.cap_r = static_cast<std::size_t>(G) * L;
     st.cap_route = static_cast<std::size_t>(G) * R;
@@ -758,14 +759,14 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     HIP_CHECK(hipMemsetAsync(st.mark_r, 0, 8U * lwords, e->stream));
     HIP_CHECK(hipMemsetAsync(st.dest_mask, 0, 8U * lwords, e->stream));
     HIP_CHECK(hipMemsetAsync(st.hz_mask, 0, 8U * lwords, e->stream));
-    HIP_CHECK(hipMemsetAsync(st.sum_t, 0, 8U * lsum, e->stream));
-    HIP_CHECK(hipMemsetAsync(st.sum_r, 0, 8U * lsum, e->stream));
-    HIP_CHECK(hipMemsetAsync(st.sum_hz, 0, 8U * lsum, e->stream));
-    HIP_CHECK(hipMemsetAsync(st.sum_route, 0, 8U * rsum, e->stream));
-    HIP_CHECK(hipMemsetAsync(st.route_mark, 0, 8U * rwords, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.flag_t, 0, lflag, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.flag_r, 0, lflag, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.flag_hz, 0, lflag, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.flag_route, 0, rflag, e->stream));
+    if (st.route_mark != nullptr) HIP_CHECK(hipMemsetAsync(st.route_mark, 0, 8U * rwords, e->stream));
     HIP_CHECK(hipMemsetAsync(st.grp_active, 0, 8U * G, e->stream));
-    HIP_CHECK(hipMemsetAsync(st.grp_any_route, 0, 8U * G, e->stream));
-    HIP_CHECK(hipMemsetAsync(st.grp_any_tmp, 0, 8U * G, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.any_route, 0, static_cast<std::size_t>(G) * W, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.any_tmp, 0, static_cast<std::size_t>(G) * W, e->stream));
     HIP_CHECK(hipMemsetAsync(st.counts, 0, 32U, e->stream));
     HIP_CHECK(hipMemsetAsync(st.work, 0, 8U * kWorkCount, e->stream));
     HIP_CHECK(hipMemsetAsync(st.n_chg, 0, 4U * S, e->stream));

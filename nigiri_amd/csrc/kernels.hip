@@ -126,13 +126,11 @@ __device__ inline void mark_one(std::uint64_t* marks, std::uint64_t* list, std::
 // Marks of the batched engine: the lane mask word and a one-bit-per-location
 // summary word, both set with non-returning atomics; a compaction kernel
 // turns the summary into the next phase's work list.
-__device__ inline void mark_wave64(std::uint64_t* marks, std::uint64_t* sum, dev_state const& s,
-                                   std::uint32_t g, std::uint32_t l, std::uint64_t lanes,
-                                   std::uint32_t words_per_group, std::uint32_t n_per_group) {
-  atomicOr(reinterpret_cast<unsigned long long*>(&marks[static_cast<std::size_t>(g) * n_per_group + l]),
-           static_cast<unsigned long long>(lanes));
-  atomicOr(reinterpret_cast<unsigned long long*>(&sum[static_cast<std::size_t>(g) * words_per_group + l / 64U]),
-           1ULL << (l % 64U));
+__device__ inline void mark_wave64(std::uint64_t* marks, std::uint8_t* flags, std::uint32_t g,
+                                   std::uint32_t l, std::uint64_t lanes, std::uint32_t n_per_group) {
+  auto const i = static_cast<std::size_t>(g) * n_per_group + l;
+  atomicOr(reinterpret_cast<unsigned long long*>(&marks[i]), static_cast<unsigned long long>(lanes));
+  flags[i] = 1U;
 }
 
 // ---- transfer-time settings (transfer_time_settings.h:21-31) ------------
@@ -228,27 +226,28 @@ __global__ void k_mark_routes(round_args a) {
   auto* overflow = &s.counts[4];
   auto const n_items = clamp_count(s.counts[0], s.cap_t);
   if constexpr (W == 64) {
+    // flag_route[g][r] = 1 for every route of a marked station (plain
+    // stores); the route scan rebuilds the lane mask from the station marks
     auto const lane = threadIdx.x & 63U;
+    std::uint32_t c_st = 0U, c_lr = 0U;
     for (std::uint32_t item = (blockIdx.x * blockDim.x + threadIdx.x) / 64U; item < n_items;
          item += gridDim.x * kWavesPerBlock) {
       auto const e = s.list_t[item];
       auto const g = static_cast<std::uint32_t>(e >> 32), l = static_cast<std::uint32_t>(e);
-      auto const word = s.mark_t[lay<W>::lword(s, g, l)] & s.grp_active[g];
+      auto const word = s.mark_t[static_cast<std::size_t>(g) * s.L + l] & s.grp_active[g];
       if (word == 0ULL) continue;
       auto const r0 = im.location_route_offsets[l], r1 = im.location_route_offsets[l + 1U];
       if (lane == 0U) {
-        auto const nq = static_cast<unsigned long long>(__popcll(word));
-        atomicAdd(&s.work[kWorkMarkedStations], nq);
-        atomicAdd(&s.work[kWorkLocationRoutes], nq * (r1 - r0));
+        auto const nq = static_cast<std::uint32_t>(__popcll(word));
+        c_st += nq;
+        c_lr += nq * (r1 - r0);
       }
-      for (auto j = r0 + lane; j < r1; j += 64U) {
-        mark_wave64(s.route_mark, s.sum_route, s, g, im.location_routes[j], word, s.RW, s.R);
-      }
-      if (lane == 0U && r1 != r0) {
-        atomicOr(reinterpret_cast<unsigned long long*>(&s.grp_any_route[g]),
-                 static_cast<unsigned long long>(word));
-      }
+      auto* fr = s.flag_route + static_cast<std::size_t>(g) * s.R;
+      for (auto j = r0 + lane; j < r1; j += 64U) fr[im.location_routes[j]] = 1U;
+      if (r1 != r0 && ((word >> lane) & 1ULL) != 0ULL) s.any_route[g * 64U + lane] = 1U;
     }
+    wave_count(&s.work[kWorkMarkedStations], c_st);
+    wave_count(&s.work[kWorkLocationRoutes], c_lr);
   } else {
     for (std::uint32_t item = blockIdx.x * blockDim.x + threadIdx.x; item < n_items;
          item += gridDim.x * blockDim.x) {
@@ -269,9 +268,7 @@ __global__ void k_mark_routes(round_args a) {
                     (static_cast<std::uint64_t>(g) << 32) | r, overflow);
         }
       }
-      if (r1 != r0) {
-        atomicOr(reinterpret_cast<unsigned long long*>(&s.grp_any_route[g]), 1ULL);
-      }
+      if (r1 != r0) s.any_route[g] = 1U;
     }
   }
 }
@@ -485,7 +482,7 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
                                     std::uint32_t lane, bool on, std::uint16_t base, int tad_k,
                                     std::uint32_t& c_stops, std::uint32_t& c_tmp,
                                     std::uint32_t& c_ev, std::uint32_t& c_rd,
-                                    std::uint32_t& c_tt, bool& any) {
+                                    std::uint32_t& c_tt, bool& any, bool& started) {
   using D = dirop<Fwd>;
   auto const L = s.L;
   auto const s0 = im.route_stop_offsets[r];
@@ -502,7 +499,7 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
   auto const* best_g = s.best + static_cast<std::size_t>(g) * L * 64U;
   auto const* mark_t_g = s.mark_t + static_cast<std::size_t>(g) * L;
   auto* mark_r_g = s.mark_r + static_cast<std::size_t>(g) * L;
-  auto* sum_r_g = s.sum_r + static_cast<std::size_t>(g) * s.LW;
+  auto* flag_r_g = s.flag_r + static_cast<std::size_t>(g) * L;
 
   // route-level staging: traffic-day bitfields
   auto const rbf = im.route_traffic_days[r];
@@ -517,7 +514,13 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
   auto const chunk = min(kChunkMax, kChunkEvBytes / (4U * n_t));
 
   et_t<Fwd> et{~0U, 0xFFFFU};
+  // A lane takes part in the route when it is active and one of the route's
+  // stops is marked for it; `started` turns on at its first marked stop.
+  // Before that the lane has no trip (boarding needs a mark), so it reads
+  // nothing; its visited stops count only if the route was marked for it.
   bool done = !on;
+  started = false;
+  std::uint32_t my_stops = 0U;
   std::uint64_t const lbit = 1ULL << lane;
   for (std::uint32_t c0 = 0U; c0 < n_stops; c0 += chunk) {
     auto const nc = min(chunk, n_stops - c0);
@@ -539,9 +542,11 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
       x.ev[e] = exists ? ev_g[n_t * (2U * stop_idx - (which == 0U ? 1U : 0U)) + t] : 0U;
     }
     wave_sync_lds();
-    // lower-bound and round_times[k-1] rows, all loads in flight together
+    // lower-bound and round_times[k-1] rows of the started lanes, all
+    // loads in flight together
     std::uint16_t lbr[kChunkMax];
     std::int16_t rtr[kChunkMax];
+    bool const live = !done;
 #pragma unroll
     for (std::uint32_t j = 0U; j < kChunkMax; ++j) {
       lbr[j] = 0U;
@@ -549,8 +554,10 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
       if (j < nc) {
         auto const l = x.stop[j] & ((1U << 28) - 1U);
         auto const off = l * 64U + lane;
-        if (on) lbr[j] = lb_g[off];
-        if (on && (x.mark[j] & lbit) != 0ULL) rtr[j] = rt_g[off];
+        bool const marked = (x.mark[j] & lbit) != 0ULL;
+        started = started || marked;
+        if (live && started) lbr[j] = lb_g[off];
+        if (live && marked) rtr[j] = rt_g[off];
       }
     }
 #pragma unroll
@@ -576,7 +583,7 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
       bool const can_finish = Fwd ? out_ok : in_ok;
       bool const can_start = Fwd ? in_ok : out_ok;
       auto const off = l * 64U + lane;
-      c_stops += done ? 0U : 1U;
+      my_stops += done ? 0U : 1U;
 
       bool upd = false;
       bool ride = false;
@@ -597,7 +604,7 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
       auto const m = __ballot(upd);
       if (m != 0ULL && lane == static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1)) {
         atomicOr(reinterpret_cast<unsigned long long*>(&mark_r_g[l]), static_cast<unsigned long long>(m));
-        atomicOr(reinterpret_cast<unsigned long long*>(&sum_r_g[l / 64U]), 1ULL << (l % 64U));
+        flag_r_g[l] = 1U;
       }
       any |= upd;
 
@@ -637,6 +644,8 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
       }
     }
   }
+  started = started && on;
+  if (started) c_stops += my_stops;
 }
 
 // route_scan: loop_routes + update_route (raptor.h:496-575, 1181-1349).
@@ -660,24 +669,18 @@ __global__ void __launch_bounds__(256, 5) k_route_scan(round_args a) {
          item += gridDim.x * kWavesPerBlock) {
       auto const e = s.list_route[item];
       auto const g = static_cast<std::uint32_t>(e >> 32), r = static_cast<std::uint32_t>(e);
-      auto const rw = static_cast<std::size_t>(g) * s.R + r;
-      auto const lanes = s.route_mark[rw] & s.grp_active[g] & s.grp_any_route[g];
-      if (lane == 0U) s.route_mark[rw] = 0ULL;
-      if (lanes == 0ULL) continue;
-      bool const on = ((lanes >> lane) & 1ULL) != 0ULL;
+      bool const on = ((s.grp_active[g] >> lane) & 1ULL) != 0ULL;
       auto const slot = g * 64U + lane;
       auto const base = s.param[slot].base;
       int const tad_k = s.tad[static_cast<std::size_t>(slot) * kRowsMax + k];
-      c_ws += im.route_stop_offsets[r + 1U] - im.route_stop_offsets[r];
-      bool any = false;
+      bool any = false, started = false;
       wave_sync_lds();  // the previous item's readers are done
       scan_route64<Fwd>(x, im, s, g, r, k, lane, on, base, tad_k, c_stops, c_tmp, c_ev, c_rd, c_tt,
-                        any);
-      auto const m = __ballot(any);
-      if (m != 0ULL && lane == static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1)) {
-        atomicOr(reinterpret_cast<unsigned long long*>(&s.grp_any_tmp[g]),
-                 static_cast<unsigned long long>(m));
+                        any, started);
+      if (__ballot(started) != 0ULL) {
+        c_ws += im.route_stop_offsets[r + 1U] - im.route_stop_offsets[r];
       }
+      if (any) s.any_tmp[slot] = 1U;
     }
     wave_count(&s.work[kWorkRouteStops], c_stops);
     wave_count(&s.work[kWorkTmpWrites], c_tmp);
@@ -694,7 +697,8 @@ __global__ void __launch_bounds__(256, 5) k_route_scan(round_args a) {
       auto const g = static_cast<std::uint32_t>(e >> 32), r = static_cast<std::uint32_t>(e);
       auto const rw = lay<W>::rword(s, g, r);
       auto const mword = s.route_mark[rw];
-      auto const lanes = ((mword >> (r % 64U)) & 1ULL) & s.grp_active[g] & s.grp_any_route[g];
+      auto const lanes = ((mword >> (r % 64U)) & 1ULL) & s.grp_active[g] &
+                         (s.any_route[g] != 0U ? 1ULL : 0ULL);
       // several routes share a word; clear only this route's bit
       atomicAnd(reinterpret_cast<unsigned long long*>(&s.route_mark[rw]),
                 ~static_cast<unsigned long long>(lay<W>::rbit(r, 0U)));
@@ -711,7 +715,7 @@ __global__ void __launch_bounds__(256, 5) k_route_scan(round_args a) {
                           im.route_stop_offsets[r + 1U] - s0, im.route_transport_from[r + 1U] - t_from};
       bool any = false;
       scan_route<Fwd, W>(rv, im, s, g, r, k, 0U, true, base, tad_k, c_stops, c_tmp, c_et, any);
-      if (any) atomicOr(reinterpret_cast<unsigned long long*>(&s.grp_any_tmp[g]), 1ULL);
+      if (any) s.any_tmp[g] = 1U;
     }
     wave_count(&s.work[kWorkRouteStops], c_stops);
     wave_count(&s.work[kWorkTmpWrites], c_tmp);
@@ -721,23 +725,24 @@ __global__ void __launch_bounds__(256, 5) k_route_scan(round_args a) {
   }
 }
 
-// Compaction (batched engine): summary words -> work list entries
-// (g << 32 | index), in ascending order per group; clears the summary.
-__global__ void k_compact(std::uint64_t* sum, std::uint32_t words_per_group, std::uint32_t n_groups,
+// Compaction (batched engine): flag bytes -> work list entries
+// (g << 32 | index); clears the flags.  Each lane tests 8 flags with one
+// 64-bit load; a wavefront scan places the entries.
+__global__ void k_compact(std::uint8_t* flags, std::uint32_t n_per_group, std::size_t n_words,
                           std::uint64_t* list, std::uint32_t* count, std::size_t cap,
                           std::uint32_t* overflow) {
-  auto const total = static_cast<std::size_t>(words_per_group) * n_groups;
+  auto* words = reinterpret_cast<std::uint64_t*>(flags);
   auto const lane = threadIdx.x & 63U;
   for (std::size_t base_i = (blockIdx.x * static_cast<std::size_t>(blockDim.x)) + (threadIdx.x & ~63U);
-       base_i < total; base_i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+       base_i < n_words; base_i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
     auto const i = base_i + lane;
     std::uint64_t v = 0ULL;
-    if (i < total) {
-      v = sum[i];
-      if (v != 0ULL) sum[i] = 0ULL;
+    if (i < n_words) {
+      v = words[i];
+      if (v != 0ULL) words[i] = 0ULL;
     }
-    unsigned n = static_cast<unsigned>(__popcll(v));
-    // inclusive scan of n over the wavefront
+    if (__ballot(v != 0ULL) == 0ULL) continue;
+    unsigned n = static_cast<unsigned>(__popcll(v));  // flags are 0 or 1
     unsigned incl = n;
     for (int off = 1; off < 64; off <<= 1) {
       auto const t = __shfl_up(incl, off, 64);
@@ -745,22 +750,21 @@ __global__ void k_compact(std::uint64_t* sum, std::uint32_t words_per_group, std
     }
     auto const total_n = __shfl(incl, 63, 64);
     unsigned wave_base = 0U;
-    if (lane == 63U && total_n != 0U) wave_base = atomicAdd(count, total_n);
+    if (lane == 63U) wave_base = atomicAdd(count, total_n);
     wave_base = __shfl(wave_base, 63, 64);
-    if (v != 0ULL) {
-      auto pos = wave_base + incl - n;
-      auto const g = static_cast<std::uint32_t>(i / words_per_group);
-      auto const w = static_cast<std::uint32_t>(i % words_per_group);
-      while (v != 0ULL) {
-        auto const b = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(v)) - 1);
-        v &= v - 1ULL;
-        if (pos < cap) {
-          list[pos] = (static_cast<std::uint64_t>(g) << 32) | (w * 64U + b);
-        } else {
-          atomicOr(overflow, 1U);
-        }
-        ++pos;
+    auto pos = wave_base + incl - n;
+    while (v != 0ULL) {
+      auto const b = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(v)) - 1);
+      v &= v - 1ULL;
+      auto const idx = i * 8U + b / 8U;
+      if (pos < cap) {
+        auto const g = static_cast<std::uint32_t>(idx / n_per_group);
+        auto const l = static_cast<std::uint32_t>(idx % n_per_group);
+        list[pos] = (static_cast<std::uint64_t>(g) << 32) | l;
+      } else {
+        atomicOr(overflow, 1U);
       }
+      ++pos;
     }
   }
 }
@@ -780,20 +784,32 @@ __global__ void k_clear_marks(dev_state s, int which) {
 }
 
 // group_update: slots without marked routes or without route improvements
-// end their epoch (raptor.h:243-246, 370-373)
+// end their epoch (raptor.h:243-246, 370-373).  One thread per slot; the
+// batched engine gathers a group's 64 slots with one ballot.
 __global__ void k_group_update(dev_state s) {
-  auto const g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= s.G) return;
-  s.grp_active[g] &= s.grp_any_route[g] & s.grp_any_tmp[g];
-  s.grp_any_route[g] = 0ULL;
-  s.grp_any_tmp[g] = 0ULL;
+  auto const slot = blockIdx.x * blockDim.x + threadIdx.x;
+  auto const n_slots = s.G * s.W;
+  bool const keep = slot < n_slots && s.any_route[slot] != 0U && s.any_tmp[slot] != 0U;
+  if (s.W == 64U) {
+    auto const m = __ballot(keep);
+    if (slot < n_slots && (threadIdx.x & 63U) == 0U) s.grp_active[slot / 64U] &= m;
+  } else if (slot < n_slots && !keep) {
+    s.grp_active[slot] = 0ULL;
+  }
+  if (slot < n_slots) {
+    s.any_route[slot] = 0U;
+    s.any_tmp[slot] = 0U;
+  }
 }
 
 __global__ void k_deactivate_all(dev_state s) {
   for (std::uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < s.G; g += gridDim.x * blockDim.x) {
     s.grp_active[g] = 0ULL;
-    s.grp_any_route[g] = 0ULL;
-    s.grp_any_tmp[g] = 0ULL;
+  }
+  for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < s.G * s.W;
+       i += gridDim.x * blockDim.x) {
+    s.any_route[i] = 0U;
+    s.any_tmp[i] = 0U;
   }
 }
 
@@ -900,7 +916,7 @@ __global__ void k_dest_pass(round_args a, std::uint32_t n_slots) {
         s.round_times[rti] = e.x;
         s.best[c] = e.x;
         if constexpr (W == 64) {
-          mark_wave64(s.mark_t, s.sum_t, s, g, e.target, 1ULL << lane, s.LW, s.L);
+          mark_wave64(s.mark_t, s.flag_t, g, e.target, 1ULL << lane, s.L);
         } else {
           mark_one<W>(s.mark_t, s.list_t, &s.counts[0], s.cap_t, &s.counts[4], s, g, e.target, lane);
         }
@@ -984,10 +1000,10 @@ __global__ void k_transfer_fp(round_args a) {
         auto const m = __ballot(code == kRelaxImproved);
         auto const h = __ballot(code == kRelaxHazard);
         if (m != 0ULL && lane_id == static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1)) {
-          mark_wave64(s.mark_t, s.sum_t, s, g, t, m, s.LW, s.L);
+          mark_wave64(s.mark_t, s.flag_t, g, t, m, s.L);
         }
         if (h != 0ULL && lane_id == static_cast<std::uint32_t>(__ffsll(static_cast<long long>(h)) - 1)) {
-          mark_wave64(s.hz_mask, s.sum_hz, s, g, t, h, s.LW, s.L);
+          mark_wave64(s.hz_mask, s.flag_hz, g, t, h, s.L);
         }
       } else {
         if (code == kRelaxImproved) mark_one<W>(s.mark_t, s.list_t, &s.counts[0], s.cap_t, overflow, s, g, t, 0U);
@@ -1209,7 +1225,7 @@ __global__ void k_add_starts(epoch_args a) {
     atomic_best16<Fwd>(&s.best[lay<W>::col(s, g, l, lane)], d);
     atomic_best16<Fwd>(&s.round_times[lay<W>::rt(s, g, 0U, l, lane)], d);
     if constexpr (W == 64) {
-      mark_wave64(s.mark_t, s.sum_t, s, g, l, 1ULL << lane, s.LW, s.L);
+      mark_wave64(s.mark_t, s.flag_t, g, l, 1ULL << lane, s.L);
     } else {
       mark_one<W>(s.mark_t, s.list_t, &s.counts[0], s.cap_t, &s.counts[4], s, g, l, lane);
     }
@@ -1257,12 +1273,11 @@ int grid_for(std::size_t work, int per_block, int cap) {
   return static_cast<int>(b < 1U ? 1U : (b > static_cast<std::size_t>(cap) ? static_cast<std::size_t>(cap) : b));
 }
 
-void compact(std::uint64_t* sum, std::uint32_t words_per_group, std::uint32_t n_groups,
-             std::uint64_t* list, std::uint32_t* count, std::size_t cap, dev_state const& s,
-             hipStream_t st) {
-  auto const total = static_cast<std::size_t>(words_per_group) * n_groups;
-  hipLaunchKernelGGL(k_compact, dim3(grid_for(total, kBlock, 8192)), dim3(kBlock), 0, st, sum,
-                     words_per_group, n_groups, list, count, cap, &s.counts[4]);
+void compact(std::uint8_t* flags, std::uint32_t n_per_group, std::uint64_t* list,
+             std::uint32_t* count, std::size_t cap, dev_state const& s, hipStream_t st) {
+  auto const n_words = flag_words(static_cast<std::size_t>(n_per_group) * s.G);
+  hipLaunchKernelGGL(k_compact, dim3(grid_for(n_words, kBlock, 8192)), dim3(kBlock), 0, st, flags,
+                     n_per_group, n_words, list, count, cap, &s.counts[4]);
 }
 
 template <bool Fwd, int W>
@@ -1278,27 +1293,27 @@ void round_impl(round_args const& a, std::uint32_t const* sweep_groups, std::uin
   hipLaunchKernelGGL((k_round_begin<Fwd, W>), dim3(slot_blocks), dim3(kBlock), 0, st, a, n_slots);
   hipLaunchKernelGGL((k_mark_routes<W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
   if constexpr (W == 64) {
-    compact(a.st.sum_route, a.st.RW, a.st.G, a.st.list_route, &a.st.counts[2], a.st.cap_route, a.st, st);
+    compact(a.st.flag_route, a.st.R, a.st.list_route, &a.st.counts[2], a.st.cap_route, a.st, st);
   }
   if (a.ev[0] != nullptr) (void)hipEventRecord(a.ev[0], st);
   hipLaunchKernelGGL((k_route_scan<Fwd, W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
   if (a.ev[1] != nullptr) (void)hipEventRecord(a.ev[1], st);
   hipLaunchKernelGGL((k_clear_marks<W>), dim3(1024), dim3(kBlock), 0, st, a.st, 0);
-  hipLaunchKernelGGL((k_group_update), dim3(grid_for(a.st.G, kBlock, 1 << 20)), dim3(kBlock), 0, st, a.st);
+  hipLaunchKernelGGL((k_group_update), dim3(grid_for(n_slots, kBlock, 1 << 20)), dim3(kBlock), 0, st, a.st);
   hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 1U | 4U);  // list_t, routes
   if constexpr (W == 64) {
-    compact(a.st.sum_r, a.st.LW, a.st.G, a.st.list_r, &a.st.counts[1], a.st.cap_r, a.st, st);
+    compact(a.st.flag_r, a.st.L, a.st.list_r, &a.st.counts[1], a.st.cap_r, a.st, st);
   }
   hipLaunchKernelGGL((k_dest_pass<Fwd, W>), dim3(grid_for(n_slots, 64, 1 << 20)), dim3(64), 0, st, a, n_slots);
   if (a.ev[2] != nullptr) (void)hipEventRecord(a.ev[2], st);
   hipLaunchKernelGGL((k_transfer_fp<Fwd, W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
   if (a.ev[3] != nullptr) (void)hipEventRecord(a.ev[3], st);
   if constexpr (W == 64) {
-    compact(a.st.sum_t, a.st.LW, a.st.G, a.st.list_t, &a.st.counts[0], a.st.cap_t, a.st, st);
-    compact(a.st.sum_hz, a.st.LW, a.st.G, a.st.list_hz, &a.st.counts[3], a.st.cap_hz, a.st, st);
+    compact(a.st.flag_t, a.st.L, a.st.list_t, &a.st.counts[0], a.st.cap_t, a.st, st);
+    compact(a.st.flag_hz, a.st.L, a.st.list_hz, &a.st.counts[3], a.st.cap_hz, a.st, st);
   }
   hipLaunchKernelGGL((k_rt_fixup<Fwd, W>), dim3(1024), dim3(kBlock), 0, st, a);
-  hipLaunchKernelGGL((k_hazards<Fwd, W>), dim3(256), dim3(kBlock), 0, st, a);
+  hipLaunchKernelGGL((k_hazards<Fwd, W>), dim3(1024), dim3(kBlock), 0, st, a);
   hipLaunchKernelGGL((k_clear_marks<W>), dim3(1024), dim3(kBlock), 0, st, a.st, 1);
   hipLaunchKernelGGL((k_clear_marks<W>), dim3(256), dim3(kBlock), 0, st, a.st, 2);
   hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 2U | 8U);  // list_r, hazards
@@ -1316,7 +1331,7 @@ void epoch_begin_impl(epoch_args const& a, hipStream_t st) {
   if (a.n_starts != 0U) {
     hipLaunchKernelGGL((k_add_starts<Fwd, W>), dim3(grid_for(a.n_starts, kBlock, 4096)), dim3(kBlock), 0, st, a);
     if constexpr (W == 64) {
-      compact(a.st.sum_t, a.st.LW, a.st.G, a.st.list_t, &a.st.counts[0], a.st.cap_t, a.st, st);
+      compact(a.st.flag_t, a.st.L, a.st.list_t, &a.st.counts[0], a.st.cap_t, a.st, st);
     }
   }
 }

@@ -179,6 +179,7 @@ def main():
         "journeys": n_journeys,
         "lane_utilization": round(st["work"]["route_stops"] / max(1, 64 * st["work"]["wave_stops"]), 4),
         "rounds": st["rounds"], "epochs": st["epochs"],
+        "work": {k: int(v) for k, v in st["work"].items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "kernel": "k_route_scan",

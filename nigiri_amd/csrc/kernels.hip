@@ -526,11 +526,21 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
     auto const nc = min(chunk, n_stops - c0);
     wave_sync_lds();  // readers of the previous chunk are done
     // stop words and station-mark words (lane j: stop c0 + j)
+    std::uint64_t mreg = 0ULL;
     if (lane < nc) {
       auto const si = c0 + lane;
       auto const stp = stops_g[Fwd ? si : n_stops - si - 1U];
+      mreg = mark_t_g[stp & ((1U << 28) - 1U)];
       x.stop[lane] = stp;
-      x.mark[lane] = mark_t_g[stp & ((1U << 28) - 1U)];
+      x.mark[lane] = mreg;
+    }
+    // A chunk in which no live lane rides and no live lane is marked changes
+    // nothing: rides need a trip, boarding and the lower-bound break need a
+    // mark.  Such a chunk only counts its stops.
+    auto const live = __ballot(!done);
+    if (__ballot(!done && et.valid()) == 0ULL && __ballot((mreg & live) != 0ULL) == 0ULL) {
+      my_stops += done ? 0U : nc;
+      continue;
     }
     // event blocks: for chunk stop j, [arr at stop_idx | dep at stop_idx]
     for (auto e = lane; e < 2U * nc * n_t; e += 64U) {
@@ -546,7 +556,6 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
     // loads in flight together
     std::uint16_t lbr[kChunkMax];
     std::int16_t rtr[kChunkMax];
-    bool const live = !done;
 #pragma unroll
     for (std::uint32_t j = 0U; j < kChunkMax; ++j) {
       lbr[j] = 0U;
@@ -556,8 +565,8 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
         auto const off = l * 64U + lane;
         bool const marked = (x.mark[j] & lbit) != 0ULL;
         started = started || marked;
-        if (live && started) lbr[j] = lb_g[off];
-        if (live && marked) rtr[j] = rt_g[off];
+        if (!done && started) lbr[j] = lb_g[off];
+        if (!done && marked) rtr[j] = rt_g[off];
       }
     }
 #pragma unroll

@@ -46,6 +46,20 @@ def route_scan_bytes(work):
             + 12 * work["transport_day_tests"] + 2 * work["tmp_writes"])
 
 
+def committed_traffic(workload, scale, batch):
+    """HBM bytes per route-scan launch from the committed rocprof PMC summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by a device copy of known size).
+    PMC counters need their own profiler runs, so the bench line carries the
+    figure of the last profiled build for the default workload only."""
+    path = os.path.join(ROOT, "profiles", "k_route_scan_workload2_pmc.md")
+    if (workload, scale, batch) != (2, 1.0, 2048) or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,7 +196,7 @@ def main():
         "work": {k: int(v) for k, v in st["work"].items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_route_scan",
+                     "traffic": committed_traffic(cfg, args.scale, B), "kernel": "k_route_scan",
                      "alg_bytes_per_launch": int(alg_bytes / launches),
                      "avg_launch_ms": round(avg_ms, 4), "launches": launches},
         "cpu_baseline": cpu,

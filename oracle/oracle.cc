@@ -20,14 +20,16 @@
 //
 // How each part is pinned (tests/test_oracle_*.py):
 //   raptor execute / search driver: golden journeys of the reference tests
-//     (test/routing/raptor_test.cc:18-112, transcribed into our builder),
-//     closed forms on periodic line networks, and brute force (time-expanded
-//     exhaustive search) on tiny random networks.
-//   dijkstra lb: brute-force Bellman-Ford on random graphs; the reference
-//     value of test/routing/dijkstra_test.cc:97-135 (lb = 60).
+//     (test/routing/raptor_test.cc:18-112 and the forward case of
+//     test/routing/raptor_intermodal_test.cc:16-83, transcribed into our
+//     builder), closed forms on periodic line networks, and brute force
+//     (time-expanded exhaustive search) on tiny random networks.
+//   dijkstra lb: brute-force Bellman-Ford on random graphs, and lb <= the
+//     travel time of every journey.  (test/routing/dijkstra_test.cc needs
+//     parent-station equivalence, which this timetable model does not have.)
 //   get_fastest_direct: brute force over footpath chains.
-//   get_starts: golden start-label lists (test/routing/start_times_test.cc)
-//     transcribed where the fixture is expressible, plus invariants.
+//   get_starts: the printed start-label list of
+//     test/routing/start_times_test.cc:26-308 (files_simple timetable).
 //   Parity unpinned (documented): the exact pruned contents of round_times /
 //     best / tmp beyond what journeys expose, cista::bitset::test for indices
 //     >= 512 (treated as "not set"), and the iteration order of the start

@@ -3,10 +3,11 @@ backward :68-112) on the HRD timetable files_abc()
 (test/loader/hrd/hrd_timetable.h:164-178), transcribed as data.
 
 Timetable (HRD times are local, +02:00 from 2020-03-29 02:00, so UTC is
-local - 2 h on 2020-03-30; bitfield 000005 = 30.03 only):
-  RE 1337  A -> B : 48 trips every 30 min, first A dep 02:30 local = 00:30 UTC,
+local - 2 h on 2020-03-30; bitfield 000005 = 30.03 only; the HRD count is
+the number of repetitions after the first trip):
+  RE 1337  A -> B : 1 + 48 trips every 30 min, first A dep 02:30 local = 00:30 UTC,
                     B arr 03:30 local = 01:30 UTC
-  RE 7331  B -> C : 92 trips every 15 min, first B dep 02:30 local = 00:30 UTC,
+  RE 7331  B -> C : 1 + 92 trips every 15 min, first B dep 02:30 local = 00:30 UTC,
                     C arr 03:30 local = 01:30 UTC
   transfer time 2 min (printed as "FOOTPATH (duration=2)" at B); no footpaths
   between A, B and C.
@@ -24,9 +25,17 @@ RANGE_FROM = (2020, 3, 28)
 RANGE_DAYS = 3
 TRAINS = {
     # name: (from, to, first dep UTC minutes, travel minutes, n trips, headway)
-    "1337": ("A", "B", 30, 60, 48, 30),
-    "7331": ("B", "C", 30, 60, 92, 15),
+    "1337": ("A", "B", 30, 60, 49, 30),
+    "7331": ("B", "C", 30, 60, 93, 15),
 }
 INTERVAL = ("05:00", "06:00")
 FWD_EXPECTED = [("05:00", "07:15", 1), ("05:30", "07:45", 1)]
 BWD_EXPECTED = [("03:00", "05:15", 1), ("03:30", "05:45", 1)]
+
+# test/routing/raptor_intermodal_test.cc:16-83 (raptor_intermodal_forward) on
+# the same timetable: start offset {A, 10 min}, destination offset {C, 15 min},
+# interval [05:00, 06:00) UTC, forward.  Journeys START -> END as
+# (departure, arrival, transfers).
+INTERMODAL_START = ("A", 10)
+INTERMODAL_DEST = ("C", 15)
+INTERMODAL_FWD_EXPECTED = [("05:20", "08:00", 1), ("05:50", "08:30", 1)]

@@ -66,6 +66,21 @@ def test_gpu_golden_abc_range_fwd_bwd():
         compare(tt, eng, qs)
 
 
+def test_gpu_golden_abc_intermodal_fwd():
+    tt, loc, day0 = abc_timetable()
+    img = na.Image(tt)
+    eng = na.Engine(img, lanes=64, groups=1, max_k=16)
+    iv = (day0 + hm(G.INTERVAL[0]), day0 + hm(G.INTERVAL[1]))
+    (s, sd), (d, dd) = G.INTERMODAL_START, G.INTERMODAL_DEST
+    say("golden abc intermodal fwd")
+    qs = na.Queries.build([dict(start=[(loc[s], sd)], dest=[(loc[d], dd)], interval=iv,
+                                start_match_mode=na.MATCH_INTERMODAL,
+                                dest_match_mode=na.MATCH_INTERMODAL)])
+    res = compare(tt, eng, qs)
+    want = [(day0 + hm(a), day0 + hm(b), 1, k) for a, b, k in G.INTERMODAL_FWD_EXPECTED]
+    assert sorted(res[0]["journeys"]) == sorted(want)
+
+
 @pytest.mark.parametrize("lanes", [64, 1])
 def test_gpu_point_queries_workload1(lanes):
     tt, img = image_for("w1", lambda: na.Timetable.generate(1))

@@ -4,6 +4,7 @@ import pytest
 import nigiri_amd as na
 import oracle
 from golden import raptor_abc as G
+from golden import start_times_simple as ST
 from ttbuilder import Builder, day_number
 
 
@@ -74,3 +75,44 @@ def test_start_dest_overlap_is_empty():
     tt, loc, day0 = abc_timetable()
     qs = na.Queries.build([dict(start=[(loc["A"], 0)], dest=[(loc["A"], 0)], time=day0 + 300)])
     assert run(tt, qs[0]) == []
+
+
+def line_timetable(spec):
+    b = Builder(day_number(*spec.RANGE_FROM), spec.RANGE_DAYS, 3)
+    names = sorted({n for fr, to, *_ in spec.TRAINS.values() for n in (fr, to)})
+    loc = {n: 9 + i for i, n in enumerate(names)}
+    day = day_number(*spec.DATE)
+    for name, (fr, to, first, travel, n, hw) in spec.TRAINS.items():
+        b.add_line([loc[fr], loc[to]], [([first + i * hw, first + i * hw + travel], [day])
+                                        for i in range(n)])
+    return b.build(), loc, day * 1440
+
+
+def test_start_times_golden_labels():
+    # get_starts (start_times.cc) against start_times_test.cc's printed list
+    tt, loc, day0 = line_timetable(ST)
+    mar = day0 - 30 * 1440  # minutes of 2020-03-00 00:00 UTC
+
+    def t(s):
+        d, hhmm = s.split(" ")
+        return mar + int(d) * 1440 + hm(hhmm)
+
+    qs = na.Queries.build([dict(start=[(loc[s], d) for s, d in ST.OFFSETS], dest=[(loc["A"], 0)],
+                                interval=(t(ST.INTERVAL[0]), t(ST.INTERVAL[1])))])
+    got = oracle.starts(tt, qs[0])
+    uniq = [x for i, x in enumerate(got) if i == 0 or got[i - 1] != x]
+    name = {v: k for k, v in loc.items()}
+    assert [(a, b, name[s]) for a, b, s in uniq] == [(t(a), t(b), s) for a, b, s in ST.EXPECTED]
+
+
+def test_raptor_intermodal_forward_golden():
+    tt, loc, day0 = abc_timetable()
+    iv = (day0 + hm(G.INTERVAL[0]), day0 + hm(G.INTERVAL[1]))
+    s, sd = G.INTERMODAL_START
+    d, dd = G.INTERMODAL_DEST
+    qs = na.Queries.build([dict(start=[(loc[s], sd)], dest=[(loc[d], dd)], interval=iv,
+                                start_match_mode=na.MATCH_INTERMODAL,
+                                dest_match_mode=na.MATCH_INTERMODAL)])
+    js = run(tt, qs[0])
+    want = [(day0 + hm(a), day0 + hm(b), 1, k) for a, b, k in G.INTERMODAL_FWD_EXPECTED]
+    assert sorted(js) == sorted(want)

@@ -106,7 +106,14 @@ struct dev_state {
   std::uint8_t* flag_r;
   std::uint8_t* flag_route;
   std::uint8_t* flag_hz;
-  std::uint32_t* counts;   // [0] list_t, [1] list_r, [2] list_route, [3] hz, [4] overflow flags
+  // batched engine: route-phase marks, one byte per (g, l, lane) [G][L][64]
+  std::uint8_t* mark_rb;
+  // batched engine: packed route wavefronts (route << 32 | n_lanes) and
+  // their lanes (g << 6 | lane, 64 per wavefront)
+  std::uint64_t* packed_info;
+  std::uint16_t* packed_lanes;
+  std::uint32_t* counts;   // [0] list_t, [1] list_r, [2] list_route, [3] hz, [4] overflow flags,
+                           // [5] packed route wavefronts
   // algorithmic work counters (see kWork* indices)
   unsigned long long* work;
   std::size_t cap_t, cap_r, cap_route, cap_hz;

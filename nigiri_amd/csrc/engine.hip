@@ -685,8 +685,14 @@ void nigiri_amd_image_destroy(nigiri_amd_image* img) { delete img; }
 int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_groups,
                              uint32_t max_k, nigiri_amd_engine** out) {
   if (img == nullptr || out == nullptr || (lanes != 64U && lanes != 1U) || n_groups == 0U ||
+      (lanes == 64U && n_groups > 1024U) ||
       max_k < 2U || max_k > 16U) {
-    set_error("engine_create: lanes must be 1 or 64, n_groups > 0, 2 <= max_k <= 16");
+    set_error("engine_create: lanes must be 1 or 64, 0 < n_groups (<= 1024 for 64 lanes), "
+              "2 <= max_k <= 16");
+    return NIGIRI_AMD_EINVAL;
+  }
+  if (lanes == 64U && static_cast<std::uint64_t>(n_groups) * img->d.L * 64U >= (1ULL << 32)) {
+    set_error("engine_create: n_groups * n_locations * 64 must stay below 2^32");
     return NIGIRI_AMD_EINVAL;
   }
   return guarded([&]() {
@@ -731,6 +737,7 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     st.flag_hz = m.alloc<std::uint8_t>(lflag);
     st.flag_route = m.alloc<std::uint8_t>(rflag);
     st.route_mark = W == 64U ? nullptr : m.alloc<std::uint64_t>(rwords);
+    st.mark_rb = W == 64U ? m.alloc<std::uint8_t>(cols) : nullptr;
     st.grp_active = m.alloc<std::uint64_t>(G);
     st.any_route = m.alloc<std::uint8_t>(static_cast<std::size_t>(G) * W);
     st.any_tmp = m.alloc<std::uint8_t>(static_cast<std::size_t>(G) * W);
@@ -741,6 +748,8 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     st.list_t = m.alloc<std::uint64_t>(st.cap_t);
     st.list_r = m.alloc<std::uint64_t>(st.cap_r);
     st.list_route = m.alloc<std::uint64_t>(st.cap_route);
+    st.packed_info = W == 64U ? m.alloc<std::uint64_t>(st.cap_route) : nullptr;
+    st.packed_lanes = W == 64U ? m.alloc<std::uint16_t>(st.cap_route * 64U) : nullptr;
     st.list_hz = m.alloc<std::uint64_t>(st.cap_hz);
     st.counts = m.alloc<std::uint32_t>(8U);
     st.work = m.alloc<unsigned long long>(kWorkCount);
@@ -764,6 +773,7 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     HIP_CHECK(hipMemsetAsync(st.flag_hz, 0, lflag, e->stream));
     HIP_CHECK(hipMemsetAsync(st.flag_route, 0, rflag, e->stream));
     if (st.route_mark != nullptr) HIP_CHECK(hipMemsetAsync(st.route_mark, 0, 8U * rwords, e->stream));
+    if (st.mark_rb != nullptr) HIP_CHECK(hipMemsetAsync(st.mark_rb, 0, cols, e->stream));
     HIP_CHECK(hipMemsetAsync(st.grp_active, 0, 8U * G, e->stream));
     HIP_CHECK(hipMemsetAsync(st.any_route, 0, static_cast<std::size_t>(G) * W, e->stream));
     HIP_CHECK(hipMemsetAsync(st.any_tmp, 0, static_cast<std::size_t>(G) * W, e->stream));

@@ -133,6 +133,17 @@ __device__ inline void mark_wave64(std::uint64_t* marks, std::uint8_t* flags, st
   flags[i] = 1U;
 }
 
+// Route-phase mark of (g, l, lane): a byte per lane for the batched engine
+// (set with plain stores by the route scan), a bitset for the plain engine.
+template <int W>
+__device__ inline bool marked_r(dev_state const& s, std::uint32_t g, std::uint32_t l, std::uint32_t lane) {
+  if constexpr (W == 64) {
+    return s.mark_rb[(static_cast<std::size_t>(g) * s.L + l) * 64U + lane] != 0U;
+  } else {
+    return lay<W>::test(s.mark_r, s, g, l, lane);
+  }
+}
+
 // ---- transfer-time settings (transfer_time_settings.h:21-31) ------------
 template <typename T>
 __device__ inline T adjusted(slot_param const& p, T const d) {
@@ -226,8 +237,8 @@ __global__ void k_mark_routes(round_args a) {
   auto* overflow = &s.counts[4];
   auto const n_items = clamp_count(s.counts[0], s.cap_t);
   if constexpr (W == 64) {
-    // flag_route[g][r] = 1 for every route of a marked station (plain
-    // stores); the route scan rebuilds the lane mask from the station marks
+    // flag_route[r][g] = 1 for every route of a marked station (plain
+    // stores); route_pack rebuilds the lane masks from the station marks
     auto const lane = threadIdx.x & 63U;
     std::uint32_t c_st = 0U, c_lr = 0U;
     for (std::uint32_t item = (blockIdx.x * blockDim.x + threadIdx.x) / 64U; item < n_items;
@@ -242,8 +253,10 @@ __global__ void k_mark_routes(round_args a) {
         c_st += nq;
         c_lr += nq * (r1 - r0);
       }
-      auto* fr = s.flag_route + static_cast<std::size_t>(g) * s.R;
-      for (auto j = r0 + lane; j < r1; j += 64U) fr[im.location_routes[j]] = 1U;
+      // route-major flags: [route][group]
+      for (auto j = r0 + lane; j < r1; j += 64U) {
+        s.flag_route[static_cast<std::size_t>(im.location_routes[j]) * s.G + g] = 1U;
+      }
       if (r1 != r0 && ((word >> lane) & 1ULL) != 0ULL) s.any_route[g * 64U + lane] = 1U;
     }
     wave_count(&s.work[kWorkMarkedStations], c_st);
@@ -468,18 +481,27 @@ constexpr std::uint32_t kMaxRouteTransports = kChunkEvBytes / 4U;  // chunk of >
 struct wave_lds {
   std::uint64_t rbits[8];
   std::uint64_t tbits[kStagedTransports * 8U];
-  std::uint64_t mark[kChunkMax];
   std::uint32_t stop[kChunkMax];
   std::uint16_t lb[kChunkMax][64];
   std::int16_t rt[kChunkMax][64];
   std::uint16_t ev[kChunkEvBytes / 2U];  // [chunk stop][arr, dep][n_t]
 };
 constexpr std::uint32_t kWaveLdsBytes = sizeof(wave_lds);
+constexpr std::uint32_t kLocMask = (1U << 28) - 1U;
 
+// One query of a packed wavefront: group g, lane ln of that group.
+struct lane_query {
+  std::uint32_t g, ln;
+  bool on;
+};
+
+// update_route (raptor.h:1181-1349) for one route and up to 64 queries of
+// any groups (one per lane).  Route data (stops, events, bitfields) is
+// uniform and staged in LDS; each lane reads its own query's state.
 template <bool Fwd>
 __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state const& s,
-                                    std::uint32_t g, std::uint32_t r, std::uint32_t k,
-                                    std::uint32_t lane, bool on, std::uint16_t base, int tad_k,
+                                    std::uint32_t r, std::uint32_t k, std::uint32_t lane,
+                                    lane_query const q, std::uint16_t base, int tad_k,
                                     std::uint32_t& c_stops, std::uint32_t& c_tmp,
                                     std::uint32_t& c_ev, std::uint32_t& c_rd,
                                     std::uint32_t& c_tt, bool& any, bool& started) {
@@ -492,14 +514,14 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
   auto const* stops_g = im.route_stops + s0;
   auto const* ev_g = im.event_times + im.route_stop_time_from[r];
   auto const* tbf_g = im.transport_traffic_days + t_from;
-  // per-group state bases: uniform, indexed with 32-bit lane offsets
-  auto const* lb_g = s.lb + static_cast<std::size_t>(g) * L * 64U;
-  auto const* rt_g = s.round_times + (static_cast<std::size_t>(g) * s.K + (k - 1U)) * L * 64U;
-  auto* tmp_g = s.tmp + static_cast<std::size_t>(g) * L * 64U;
-  auto const* best_g = s.best + static_cast<std::size_t>(g) * L * 64U;
-  auto const* mark_t_g = s.mark_t + static_cast<std::size_t>(g) * L;
-  auto* mark_r_g = s.mark_r + static_cast<std::size_t>(g) * L;
-  auto* flag_r_g = s.flag_r + static_cast<std::size_t>(g) * L;
+  // this lane's query: element l of a [G][L][64] array is at col0 + 64 l
+  // (G * L * 64 < 2^32 is checked when the engine is created); the half of
+  // the station-mark word (g, l) that holds its bit is u32 word mrow + 2 l
+  std::uint32_t const col0 = q.g * L * 64U + q.ln;
+  std::uint32_t const mrow = 2U * q.g * L + (q.ln >> 5);
+  std::uint32_t const mbit = 1U << (q.ln & 31U);
+  auto const* rt_q = s.round_times + (static_cast<std::size_t>(q.g) * s.K + (k - 1U)) * L * 64U + q.ln;
+  auto const* mark_t32 = reinterpret_cast<std::uint32_t const*>(s.mark_t);
 
   // route-level staging: traffic-day bitfields
   auto const rbf = im.route_traffic_days[r];
@@ -514,33 +536,37 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
   auto const chunk = min(kChunkMax, kChunkEvBytes / (4U * n_t));
 
   et_t<Fwd> et{~0U, 0xFFFFU};
-  // A lane takes part in the route when it is active and one of the route's
-  // stops is marked for it; `started` turns on at its first marked stop.
-  // Before that the lane has no trip (boarding needs a mark), so it reads
-  // nothing; its visited stops count only if the route was marked for it.
-  bool done = !on;
+  // `started`: one of the route's stops is marked for this query.  Before
+  // its first marked stop a query has no trip (boarding needs a mark), so
+  // nothing happens; its visited stops count only if the route was marked
+  // for it (the packing admits only such queries, so on lanes start).
+  bool done = !q.on;
   started = false;
   std::uint32_t my_stops = 0U;
-  std::uint64_t const lbit = 1ULL << lane;
   for (std::uint32_t c0 = 0U; c0 < n_stops; c0 += chunk) {
     auto const nc = min(chunk, n_stops - c0);
     wave_sync_lds();  // readers of the previous chunk are done
-    // stop words and station-mark words (lane j: stop c0 + j)
-    std::uint64_t mreg = 0ULL;
     if (lane < nc) {
       auto const si = c0 + lane;
-      auto const stp = stops_g[Fwd ? si : n_stops - si - 1U];
-      mreg = mark_t_g[stp & ((1U << 28) - 1U)];
-      x.stop[lane] = stp;
-      x.mark[lane] = mreg;
+      x.stop[lane] = stops_g[Fwd ? si : n_stops - si - 1U];
     }
-    // A chunk in which no live lane rides and no live lane is marked changes
-    // nothing: rides need a trip, boarding and the lower-bound break need a
-    // mark.  Such a chunk only counts its stops.
-    auto const live = __ballot(!done);
-    if (__ballot(!done && et.valid()) == 0ULL && __ballot((mreg & live) != 0ULL) == 0ULL) {
-      my_stops += done ? 0U : nc;
-      continue;
+    wave_sync_lds();
+    // this lane's station marks, lower bounds and round_times[k-1] at the
+    // chunk's stops, and the chunk's event blocks, all in flight together
+    std::uint16_t lbr[kChunkMax];
+    std::int16_t rtr[kChunkMax];
+    std::uint32_t mkr[kChunkMax];
+#pragma unroll
+    for (std::uint32_t j = 0U; j < kChunkMax; ++j) {
+      lbr[j] = 0U;
+      rtr[j] = D::kInvalid;
+      mkr[j] = 0U;
+      if (j < nc && !done) {
+        auto const l = x.stop[j] & kLocMask;
+        mkr[j] = mark_t32[mrow + 2U * l];
+        lbr[j] = s.lb[col0 + l * 64U];
+        rtr[j] = rt_q[static_cast<std::size_t>(l) * 64U];
+      }
     }
     // event blocks: for chunk stop j, [arr at stop_idx | dep at stop_idx]
     for (auto e = lane; e < 2U * nc * n_t; e += 64U) {
@@ -551,23 +577,18 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
       bool const exists = which == 0U ? stop_idx != 0U : stop_idx + 1U != n_stops;
       x.ev[e] = exists ? ev_g[n_t * (2U * stop_idx - (which == 0U ? 1U : 0U)) + t] : 0U;
     }
-    wave_sync_lds();
-    // lower-bound and round_times[k-1] rows of the started lanes, all
-    // loads in flight together
-    std::uint16_t lbr[kChunkMax];
-    std::int16_t rtr[kChunkMax];
+    std::uint32_t mbits = 0U;  // bit j: this query is marked at chunk stop j
 #pragma unroll
     for (std::uint32_t j = 0U; j < kChunkMax; ++j) {
-      lbr[j] = 0U;
-      rtr[j] = D::kInvalid;
-      if (j < nc) {
-        auto const l = x.stop[j] & ((1U << 28) - 1U);
-        auto const off = l * 64U + lane;
-        bool const marked = (x.mark[j] & lbit) != 0ULL;
-        started = started || marked;
-        if (!done && started) lbr[j] = lb_g[off];
-        if (!done && marked) rtr[j] = rt_g[off];
-      }
+      mbits |= ((mkr[j] & mbit) != 0U ? 1U : 0U) << j;
+    }
+    started = started || mbits != 0U;
+    // A chunk in which no live lane rides and no live lane is marked changes
+    // nothing: rides need a trip, boarding and the lower-bound break need a
+    // mark.  Such a chunk only counts its stops.
+    if (__ballot(!done && (et.valid() || mbits != 0U)) == 0ULL) {
+      my_stops += done ? 0U : nc;
+      continue;
     }
 #pragma unroll
     for (std::uint32_t j = 0U; j < kChunkMax; ++j) {
@@ -580,10 +601,9 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
 
     for (std::uint32_t j = 0U; j < nc; ++j) {
       auto const i = c0 + j;
-      auto const stop_idx = Fwd ? i : n_stops - i - 1U;
       auto const stp = x.stop[j];
-      auto const l = stp & ((1U << 28) - 1U);
-      auto const mw = x.mark[j];
+      auto const l = stp & kLocMask;
+      bool const marked = ((mbits >> j) & 1U) != 0U;
       int const lbv = x.lb[j][lane];
       std::int16_t const prev = x.rt[j][lane];
       auto const* ev_arr = x.ev + (2U * j) * n_t;
@@ -591,10 +611,9 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
       auto const in_ok = ((stp >> 28) & 1U) != 0U, out_ok = ((stp >> 29) & 1U) != 0U;
       bool const can_finish = Fwd ? out_ok : in_ok;
       bool const can_start = Fwd ? in_ok : out_ok;
-      auto const off = l * 64U + lane;
+      auto const off = col0 + l * 64U;
       my_stops += done ? 0U : 1U;
 
-      bool upd = false;
       bool ride = false;
       if (!done && et.valid() && can_finish) {
         ride = true;
@@ -602,23 +621,19 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
         auto const by = to_delta<Fwd>(et.day, (e & 31) * 1440 + (e >> 5), base);
         if (D::better_eq(by, tad_k) && lbv != kUnreachableLb && D::better_eq(by + D::dir(lbv), tad_k)) {
           if constexpr (Fwd) {
-            atomicMin(&tmp_g[off], static_cast<int>(by));
+            atomicMin(&s.tmp[off], static_cast<int>(by));
           } else {
-            atomicMax(&tmp_g[off], static_cast<int>(by));
+            atomicMax(&s.tmp[off], static_cast<int>(by));
           }
-          upd = true;
+          s.mark_rb[off] = 1U;
+          s.flag_r[q.g * L + l] = 1U;
+          any = true;
           ++c_tmp;
         }
       }
-      auto const m = __ballot(upd);
-      if (m != 0ULL && lane == static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1)) {
-        atomicOr(reinterpret_cast<unsigned long long*>(&mark_r_g[l]), static_cast<unsigned long long>(m));
-        flag_r_g[l] = 1U;
-      }
-      any |= upd;
 
       bool const is_last = i == n_stops - 1U;
-      if (done || is_last || !can_start || (mw & lbit) == 0ULL) continue;
+      if (done || is_last || !can_start || !marked) continue;
       if (lbv == kUnreachableLb) {
         done = true;  // break (raptor.h:1309-1311)
         continue;
@@ -643,7 +658,7 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
           // current_best (raptor.h:1328-1329) is valid whenever a ride was
           // evaluated at this stop (it then includes the valid round_times
           // [k-1] label) or best / tmp hold a label
-          bool const cb_valid = ride || best_g[off] != D::kInvalid || tmp_g[off] != D::kInvalid;
+          bool const cb_valid = ride || s.best[off] != D::kInvalid || s.tmp[off] != D::kInvalid;
           auto const e = ev_board[new_et.t_off];
           auto const new_dep = to_delta<Fwd>(new_et.day, (e & 31) * 1440 + (e >> 5), base);
           if (!cb_valid || D::better_eq(new_dep, et_dep)) {
@@ -653,38 +668,125 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
       }
     }
   }
-  started = started && on;
+  started = started && q.on;
   if (started) c_stops += my_stops;
+}
+
+// route_pack (batched engine): the flagged (route, group) pairs, compacted
+// in route-major order, become packed wavefronts that each hold up to 64
+// queries (group, lane) of one route.  A query takes part in a route when
+// it is active and one of the route's stops is marked for it: those are the
+// queries for which mark_routes (raptor.h:228-241) marked the route.
+// Pairs are packed greedily in list order; one wavefront packs a window of
+// 64 list entries.
+__global__ void k_route_pack(round_args a) {
+  auto const& im = a.im;
+  auto const& s = a.st;
+  __shared__ std::uint32_t fills_all[kWavesPerBlock][64];
+  auto* fills = fills_all[threadIdx.x / 64U];
+  auto const lane = threadIdx.x & 63U;
+  auto const n_items = clamp_count(s.counts[2], s.cap_route);
+  auto const wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64U;
+  for (std::uint32_t w0 = wave * 64U; w0 < n_items; w0 += gridDim.x * kWavesPerBlock * 64U) {
+    auto const item = w0 + lane;
+    std::uint32_t r = ~0U, g = 0U;
+    std::uint64_t mask = 0ULL;
+    if (item < n_items) {
+      auto const e = s.list_route[item];
+      r = static_cast<std::uint32_t>(e >> 32);
+      g = static_cast<std::uint32_t>(e);
+      auto const act = s.grp_active[g];
+      if (act != 0ULL) {
+        auto const* mt = s.mark_t + static_cast<std::size_t>(g) * s.L;
+        auto const i1 = im.route_stop_offsets[r + 1U];
+        auto i = im.route_stop_offsets[r];
+        for (; i + 4U <= i1; i += 4U) {
+          auto const a0 = im.route_stops[i] & kLocMask, a1 = im.route_stops[i + 1U] & kLocMask;
+          auto const a2 = im.route_stops[i + 2U] & kLocMask, a3 = im.route_stops[i + 3U] & kLocMask;
+          mask |= mt[a0] | mt[a1] | mt[a2] | mt[a3];
+        }
+        for (; i < i1; ++i) mask |= mt[im.route_stops[i] & kLocMask];
+        mask &= act;
+      }
+    }
+    auto const c = static_cast<std::uint32_t>(__popcll(mask));
+    // greedy assignment in item order (uniform loop over the 64 entries)
+    std::uint32_t my_w = 0U, my_o = 0U, n_w = 0U, fill = 0U, cur_r = ~0U;
+    for (std::uint32_t t = 0U; t < 64U; ++t) {
+      auto const ct = static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(c), static_cast<int>(t)));
+      if (ct == 0U) continue;
+      auto const rt = static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(r), static_cast<int>(t)));
+      if (n_w == 0U || rt != cur_r || fill + ct > 64U) {
+        if (n_w != 0U && lane == 0U) fills[n_w - 1U] = fill;
+        ++n_w;
+        fill = 0U;
+        cur_r = rt;
+      }
+      if (lane == t) {
+        my_w = n_w - 1U;
+        my_o = fill;
+      }
+      fill += ct;
+    }
+    if (n_w != 0U && lane == 0U) fills[n_w - 1U] = fill;
+    wave_sync_lds();
+    std::uint32_t base = 0U;
+    if (lane == 0U && n_w != 0U) base = atomicAdd(&s.counts[5], n_w);
+    base = static_cast<std::uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(base)));
+    if (c != 0U) {
+      auto const pw = static_cast<std::size_t>(base) + my_w;
+      if (pw < s.cap_route) {
+        if (my_o == 0U) s.packed_info[pw] = (static_cast<std::uint64_t>(r) << 32) | fills[my_w];
+        auto* out = s.packed_lanes + pw * 64U + my_o;
+        auto m = mask;
+        for (std::uint32_t j = 0U; m != 0ULL; ++j) {
+          auto const b = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1);
+          m &= m - 1ULL;
+          out[j] = static_cast<std::uint16_t>((g << 6) | b);
+        }
+      } else {
+        atomicOr(&s.counts[4], 1U);
+      }
+    }
+    wave_sync_lds();  // fills is reused by the next window
+  }
 }
 
 // route_scan: loop_routes + update_route (raptor.h:496-575, 1181-1349).
 // The route phase reads only round_times[k-1], best, lb, time_at_dest and
 // the previous marks, and writes tmp by minimum and marks by OR: route
 // order does not matter, so routes run in parallel.  Batched engine: one
-// wavefront per (64-query group, route), staged in LDS; plain engine: one
-// route per lane.
+// packed wavefront per route (up to 64 queries of any groups, route data
+// staged in LDS once); plain engine: one route per lane.
 template <bool Fwd, int W>
 __global__ void __launch_bounds__(256, 5) k_route_scan(round_args a) {
   auto const& im = a.im;
   auto const& s = a.st;
   auto const k = static_cast<std::uint32_t>(a.k);
-  auto const n_items = clamp_count(s.counts[2], s.cap_route);
   if constexpr (W == 64) {
     __shared__ wave_lds lds_all[kWavesPerBlock];
     auto& x = lds_all[threadIdx.x / 64U];
     std::uint32_t const lane = threadIdx.x & 63U;
+    auto const n_items = clamp_count(s.counts[5], s.cap_route);
     std::uint32_t c_stops = 0U, c_tmp = 0U, c_ev = 0U, c_rd = 0U, c_tt = 0U, c_ws = 0U;
     for (std::uint32_t item = (blockIdx.x * blockDim.x + threadIdx.x) / 64U; item < n_items;
          item += gridDim.x * kWavesPerBlock) {
-      auto const e = s.list_route[item];
-      auto const g = static_cast<std::uint32_t>(e >> 32), r = static_cast<std::uint32_t>(e);
-      bool const on = ((s.grp_active[g] >> lane) & 1ULL) != 0ULL;
-      auto const slot = g * 64U + lane;
+      auto const info = s.packed_info[item];
+      auto const r = static_cast<std::uint32_t>(info >> 32);
+      auto const n_lanes = static_cast<std::uint32_t>(info);
+      lane_query q{0U, 0U, false};
+      if (lane < n_lanes) {
+        auto const d = s.packed_lanes[static_cast<std::size_t>(item) * 64U + lane];
+        q.g = static_cast<std::uint32_t>(d) >> 6;
+        q.ln = static_cast<std::uint32_t>(d) & 63U;
+        q.on = ((s.grp_active[q.g] >> q.ln) & 1ULL) != 0ULL;
+      }
+      auto const slot = q.g * 64U + q.ln;
       auto const base = s.param[slot].base;
       int const tad_k = s.tad[static_cast<std::size_t>(slot) * kRowsMax + k];
       bool any = false, started = false;
       wave_sync_lds();  // the previous item's readers are done
-      scan_route64<Fwd>(x, im, s, g, r, k, lane, on, base, tad_k, c_stops, c_tmp, c_ev, c_rd, c_tt,
+      scan_route64<Fwd>(x, im, s, r, k, lane, q, base, tad_k, c_stops, c_tmp, c_ev, c_rd, c_tt,
                         any, started);
       if (__ballot(started) != 0ULL) {
         c_ws += im.route_stop_offsets[r + 1U] - im.route_stop_offsets[r];
@@ -698,6 +800,7 @@ __global__ void __launch_bounds__(256, 5) k_route_scan(round_args a) {
     wave_count(&s.work[kWorkTransportTests], c_tt);
     if (lane == 0U && c_ws != 0U) atomicAdd(&s.work[kWorkWaveStops], static_cast<unsigned long long>(c_ws));
   } else {
+    auto const n_items = clamp_count(s.counts[2], s.cap_route);
     unsigned long long c_stops = 0ULL, c_tmp = 0ULL;
     et_counters c_et;
     for (std::uint32_t item = blockIdx.x * blockDim.x + threadIdx.x; item < n_items;
@@ -734,9 +837,10 @@ __global__ void __launch_bounds__(256, 5) k_route_scan(round_args a) {
   }
 }
 
-// Compaction (batched engine): flag bytes -> work list entries
-// (g << 32 | index); clears the flags.  Each lane tests 8 flags with one
-// 64-bit load; a wavefront scan places the entries.
+// Compaction (batched engine): flag bytes of a [major][minor] array -> work
+// list entries (major << 32 | minor); clears the flags.  Each lane tests 8
+// flags with one 64-bit load; a wavefront scan places the entries, so the
+// entries of one wavefront's 512 flags are in ascending order.
 __global__ void k_compact(std::uint8_t* flags, std::uint32_t n_per_group, std::size_t n_words,
                           std::uint64_t* list, std::uint32_t* count, std::size_t cap,
                           std::uint32_t* overflow) {
@@ -786,6 +890,16 @@ __global__ void k_clear_marks(dev_state s, int which) {
                               : clamp_count(s.counts[3], s.cap_hz);
   auto* marks = which == 0 ? s.mark_t : (which == 1 ? s.mark_r : s.hz_mask);
   auto const* list = which == 0 ? s.list_t : (which == 1 ? s.list_r : s.list_hz);
+  if (W == 64 && which == 1) {
+    // batched route-phase marks: 64 bytes per (g, l), cleared as 8 words
+    auto* words = reinterpret_cast<std::uint64_t*>(s.mark_rb);
+    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n * 8U; i += gridDim.x * blockDim.x) {
+      auto const e = list[i / 8U];
+      auto const gl = static_cast<std::size_t>(e >> 32) * s.L + static_cast<std::uint32_t>(e);
+      words[gl * 8U + i % 8U] = 0ULL;
+    }
+    return;
+  }
   for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     auto const e = list[i];
     marks[lay<W>::lword(s, static_cast<std::uint32_t>(e >> 32), static_cast<std::uint32_t>(e))] = 0ULL;
@@ -822,10 +936,12 @@ __global__ void k_deactivate_all(dev_state s) {
   }
 }
 
+// clears the work-list counts selected by `mask` (bit i: counts[i]; bit 4,
+// the overflow flags, is never cleared here)
 __global__ void k_reset_counts(dev_state s, std::uint32_t mask) {
   if (threadIdx.x == 0U && blockIdx.x == 0U) {
-    for (int i = 0; i < 4; ++i) {
-      if ((mask >> i) & 1U) s.counts[i] = 0U;
+    for (int i = 0; i < 8; ++i) {
+      if (i != 4 && ((mask >> i) & 1U) != 0U) s.counts[i] = 0U;
     }
   }
 }
@@ -865,7 +981,7 @@ __global__ void k_dest_pass(round_args a, std::uint32_t n_slots) {
   if (p.intermodal != 0U) {
     for (std::uint32_t j = 0; j < p.n_dist; ++j) {
       auto const i = s.dist_loc[slot * kMaxDist + j];
-      if (!lay<W>::test(s.mark_r, s, g, i, lane)) continue;
+      if (!marked_r<W>(s, g, i, lane)) continue;
       auto const t = s.tmp[lay<W>::col(s, g, i, lane)];
       if (t == D::kInvalid) continue;
       add(kPosIntermodal | i, 1U, t + D::dir(s.dist_min[slot * kMaxDist + j]), 1U);
@@ -873,7 +989,7 @@ __global__ void k_dest_pass(round_args a, std::uint32_t n_slots) {
   } else {
     for (std::uint32_t j = 0; j < p.n_dest; ++j) {
       auto const d = s.dest[slot * kMaxDestBits + j];
-      if (lay<W>::test(s.mark_r, s, g, d, lane)) {
+      if (marked_r<W>(s, g, d, lane)) {
         auto const t = s.tmp[lay<W>::col(s, g, d, lane)];
         if (t != D::kInvalid) add(kPosTransfer | d, d, t, 0U);  // transfer time 0 at dest
       }
@@ -883,7 +999,7 @@ __global__ void k_dest_pass(round_args a, std::uint32_t n_slots) {
       auto const* map = Fwd ? im.fp_in_to_out : im.fp_out_to_in;
       for (auto e = off[d]; e != off[d + 1U]; ++e) {
         auto const i = fps[e] & ((1U << 23) - 1U);
-        if (!lay<W>::test(s.mark_r, s, g, i, lane)) continue;
+        if (!marked_r<W>(s, g, i, lane)) continue;
         auto const t = s.tmp[lay<W>::col(s, g, i, lane)];
         if (t == D::kInvalid) continue;
         auto const dur = static_cast<std::int16_t>(fps[e] >> 23);
@@ -991,9 +1107,14 @@ __global__ void k_transfer_fp(round_args a) {
   for (; item < n_items; item += step) {
     auto const e = s.list_r[item];
     auto const g = static_cast<std::uint32_t>(e >> 32), i = static_cast<std::uint32_t>(e);
-    auto const lanes = lay<W>::lanes_at(s.mark_r[lay<W>::lword(s, g, i)], i) & s.grp_active[g];
-    if (lanes == 0ULL) continue;
-    bool const on = ((lanes >> lane_id) & 1ULL) != 0ULL;
+    bool on;
+    if constexpr (W == 64) {
+      on = ((s.grp_active[g] >> lane_id) & 1ULL) != 0ULL && marked_r<W>(s, g, i, lane_id);
+      if (__ballot(on) == 0ULL) continue;
+    } else {
+      on = (lay<W>::lanes_at(s.mark_r[lay<W>::lword(s, g, i)], i) & s.grp_active[g]) != 0ULL;
+      if (!on) continue;
+    }
     auto const slot = g * W + lane_id;
     auto const& p = s.param[slot];
     auto const tmp_i = s.tmp[lay<W>::col(s, g, i, lane_id)];
@@ -1096,7 +1217,7 @@ __global__ void k_hazards(round_args a) {
     auto const* fps = Fwd ? im.fp_in : im.fp_out;
     auto const* map = Fwd ? im.fp_in_to_out : im.fp_out_to_in;
     auto const for_each_event = [&](auto&& fn) {
-      if (lay<W>::test(s.mark_r, s, g, t, lane)) {
+      if (marked_r<W>(s, g, t, lane)) {
         auto const tmp_t = s.tmp[c];
         if (tmp_t != D::kInvalid) {
           fn(kPosTransfer | t, clamp16(tmp_t + D::dir(adjusted<std::uint8_t>(p, im.transfer_time[t]))));
@@ -1104,7 +1225,7 @@ __global__ void k_hazards(round_args a) {
       }
       for (auto fe = off[t]; fe != off[t + 1U]; ++fe) {
         auto const src = fps[fe] & ((1U << 23) - 1U);
-        if (!lay<W>::test(s.mark_r, s, g, src, lane)) continue;
+        if (!marked_r<W>(s, g, src, lane)) continue;
         auto const tmp_s = s.tmp[lay<W>::col(s, g, src, lane)];
         if (tmp_s == D::kInvalid) continue;
         auto const dur = static_cast<std::int16_t>(fps[fe] >> 23);
@@ -1282,9 +1403,10 @@ int grid_for(std::size_t work, int per_block, int cap) {
   return static_cast<int>(b < 1U ? 1U : (b > static_cast<std::size_t>(cap) ? static_cast<std::size_t>(cap) : b));
 }
 
-void compact(std::uint8_t* flags, std::uint32_t n_per_group, std::uint64_t* list,
+void compact(std::uint8_t* flags, std::uint32_t n_minor, std::uint32_t n_major, std::uint64_t* list,
              std::uint32_t* count, std::size_t cap, dev_state const& s, hipStream_t st) {
-  auto const n_words = flag_words(static_cast<std::size_t>(n_per_group) * s.G);
+  auto const n_per_group = n_minor;
+  auto const n_words = flag_words(static_cast<std::size_t>(n_minor) * n_major);
   hipLaunchKernelGGL(k_compact, dim3(grid_for(n_words, kBlock, 8192)), dim3(kBlock), 0, st, flags,
                      n_per_group, n_words, list, count, cap, &s.counts[4]);
 }
@@ -1302,24 +1424,25 @@ void round_impl(round_args const& a, std::uint32_t const* sweep_groups, std::uin
   hipLaunchKernelGGL((k_round_begin<Fwd, W>), dim3(slot_blocks), dim3(kBlock), 0, st, a, n_slots);
   hipLaunchKernelGGL((k_mark_routes<W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
   if constexpr (W == 64) {
-    compact(a.st.flag_route, a.st.R, a.st.list_route, &a.st.counts[2], a.st.cap_route, a.st, st);
+    compact(a.st.flag_route, a.st.G, a.st.R, a.st.list_route, &a.st.counts[2], a.st.cap_route, a.st, st);
+    hipLaunchKernelGGL(k_route_pack, dim3(item_blocks), dim3(kBlock), 0, st, a);
   }
   if (a.ev[0] != nullptr) (void)hipEventRecord(a.ev[0], st);
   hipLaunchKernelGGL((k_route_scan<Fwd, W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
   if (a.ev[1] != nullptr) (void)hipEventRecord(a.ev[1], st);
   hipLaunchKernelGGL((k_clear_marks<W>), dim3(1024), dim3(kBlock), 0, st, a.st, 0);
   hipLaunchKernelGGL((k_group_update), dim3(grid_for(n_slots, kBlock, 1 << 20)), dim3(kBlock), 0, st, a.st);
-  hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 1U | 4U);  // list_t, routes
+  hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 1U | 4U | 32U);  // list_t, routes, packed
   if constexpr (W == 64) {
-    compact(a.st.flag_r, a.st.L, a.st.list_r, &a.st.counts[1], a.st.cap_r, a.st, st);
+    compact(a.st.flag_r, a.st.L, a.st.G, a.st.list_r, &a.st.counts[1], a.st.cap_r, a.st, st);
   }
   hipLaunchKernelGGL((k_dest_pass<Fwd, W>), dim3(grid_for(n_slots, 64, 1 << 20)), dim3(64), 0, st, a, n_slots);
   if (a.ev[2] != nullptr) (void)hipEventRecord(a.ev[2], st);
   hipLaunchKernelGGL((k_transfer_fp<Fwd, W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
   if (a.ev[3] != nullptr) (void)hipEventRecord(a.ev[3], st);
   if constexpr (W == 64) {
-    compact(a.st.flag_t, a.st.L, a.st.list_t, &a.st.counts[0], a.st.cap_t, a.st, st);
-    compact(a.st.flag_hz, a.st.L, a.st.list_hz, &a.st.counts[3], a.st.cap_hz, a.st, st);
+    compact(a.st.flag_t, a.st.L, a.st.G, a.st.list_t, &a.st.counts[0], a.st.cap_t, a.st, st);
+    compact(a.st.flag_hz, a.st.L, a.st.G, a.st.list_hz, &a.st.counts[3], a.st.cap_hz, a.st, st);
   }
   hipLaunchKernelGGL((k_rt_fixup<Fwd, W>), dim3(1024), dim3(kBlock), 0, st, a);
   hipLaunchKernelGGL((k_hazards<Fwd, W>), dim3(1024), dim3(kBlock), 0, st, a);
@@ -1340,7 +1463,7 @@ void epoch_begin_impl(epoch_args const& a, hipStream_t st) {
   if (a.n_starts != 0U) {
     hipLaunchKernelGGL((k_add_starts<Fwd, W>), dim3(grid_for(a.n_starts, kBlock, 4096)), dim3(kBlock), 0, st, a);
     if constexpr (W == 64) {
-      compact(a.st.flag_t, a.st.L, a.st.list_t, &a.st.counts[0], a.st.cap_t, a.st, st);
+      compact(a.st.flag_t, a.st.L, a.st.G, a.st.list_t, &a.st.counts[0], a.st.cap_t, a.st, st);
     }
   }
 }
@@ -1396,7 +1519,7 @@ void launch_epoch_end(std::uint32_t W, dev_state const& st, std::uint32_t n_slot
     hipLaunchKernelGGL((k_clear_marks<1>), dim3(1024), dim3(kBlock), 0, stream, st, 0);
   }
   hipLaunchKernelGGL((k_deactivate_all), dim3(grid_for(st.G, kBlock, 1024)), dim3(kBlock), 0, stream, st);
-  hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, stream, st, 0xFU);
+  hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, stream, st, 0x2FU);
 }
 
 }  // namespace nigiri_amd

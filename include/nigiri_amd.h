@@ -260,10 +260,12 @@ int nigiri_amd_algo_execute(nigiri_amd_engine* e, uint32_t slot,
                             uint32_t* n_rounds);
 /* Copies the slot's state to host arrays (any pointer may be NULL):
  * round_times[max_k][n_locations], best[n_locations], tmp[n_locations],
- * time_at_dest[max_k]. */
+ * time_at_dest[max_k], lb[n_locations] (the lower bounds the search used;
+ * the raptor reads them from the caller's lb vector, raptor.h:430-446). */
 int nigiri_amd_algo_download(nigiri_amd_engine* e, uint32_t slot,
                              int16_t* round_times, int16_t* best,
-                             int16_t* tmp, int16_t* time_at_dest);
+                             int16_t* tmp, int16_t* time_at_dest,
+                             uint16_t* lb);
 
 /* ---- instrumentation (no counterpart in the reference) -------------------
  * Work counters of the algorithmic byte model, accumulated on the device

@@ -136,7 +136,7 @@ SIGNATURES = {
                                           C.POINTER(C.c_int16), C.POINTER(C.c_uint32)]),
     "nigiri_amd_algo_download": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_int16),
                                            C.POINTER(C.c_int16), C.POINTER(C.c_int16),
-                                           C.POINTER(C.c_int16)]),
+                                           C.POINTER(C.c_int16), C.POINTER(C.c_uint16)]),
     "nigiri_amd_engine_configure": (C.c_int, [_P, C.POINTER(EngineConfig)]),
     "nigiri_amd_engine_profile": (C.c_int, [_P, C.c_int]),
     "nigiri_amd_engine_stats": (C.c_int, [_P, C.POINTER(EngineStats), C.c_int]),
@@ -443,10 +443,11 @@ class Engine:
         best = np.zeros(L, np.int16)
         tmp = np.zeros(L, np.int16)
         tad = np.zeros(16, np.int16)
+        lb = np.zeros(L, np.uint16)
         _check(lib().nigiri_amd_algo_download(self.handle, slot, ptr(rt, C.c_int16),
                                               ptr(best, C.c_int16), ptr(tmp, C.c_int16),
-                                              ptr(tad, C.c_int16)))
-        return {"round_times": rt, "best": best, "tmp": tmp, "time_at_dest": tad}
+                                              ptr(tad, C.c_int16), ptr(lb, C.c_uint16)))
+        return {"round_times": rt, "best": best, "tmp": tmp, "time_at_dest": tad, "lb": lb}
 
     def close(self):
         if self.handle:

@@ -52,6 +52,11 @@ struct dev_image {
   std::uint32_t const* fp_in;
   std::uint32_t const* fp_in_to_out;  // in-edge index -> out-edge index
   std::uint32_t const* fp_out_to_in;  // out-edge index -> in-edge index
+  // lower-bound graphs (loader/build_lb_graph.h), packed like footpaths
+  std::uint32_t const* lb_fwd_offsets;
+  std::uint32_t const* lb_fwd;
+  std::uint32_t const* lb_bwd_offsets;
+  std::uint32_t const* lb_bwd;
 };
 
 // Per-slot parameters written by the host at admission.

@@ -120,6 +120,21 @@ struct nigiri_amd_engine {
   std::int32_t* d_lb_row = nullptr;
   std::uint32_t* d_lb_groups = nullptr;
   std::uint32_t lb_rows_cap = 64U;
+  // device lower bounds (batched engine): up to lbq_chunk queries per
+  // launch, one workgroup each, with [lbq_chunk][L] scratch arrays
+  std::uint32_t lbq_chunk = 0U;
+  std::uint32_t* d_lbq_dist = nullptr;
+  std::uint32_t* d_lbq_stamp_near = nullptr;
+  std::uint32_t* d_lbq_stamp_far = nullptr;
+  std::uint32_t* d_lbq_near0 = nullptr;
+  std::uint32_t* d_lbq_near1 = nullptr;
+  std::uint32_t* d_lbq_far0 = nullptr;
+  std::uint32_t* d_lbq_far1 = nullptr;
+  std::uint16_t* d_lbq_rows = nullptr;
+  std::uint32_t* d_seed_off = nullptr;
+  std::uint32_t* d_seed_loc = nullptr;
+  std::uint16_t* d_seed_dur = nullptr;
+  std::size_t seed_cap = 0U;
   // growable device buffers for starts / dest ops
   std::uint32_t* d_start_slot = nullptr;
   std::uint32_t* d_start_loc = nullptr;
@@ -228,6 +243,10 @@ void build_image(nigiri_amd_image& img, timetable const& tt, hipStream_t s) {
   }
   d.fp_in_to_out = m.upload(in_to_out, s);
   d.fp_out_to_in = m.upload(out_to_in, s);
+  d.lb_fwd_offsets = m.upload(tt.lb_fwd_offsets, s);
+  d.lb_fwd = m.upload(tt.lb_fwd, s);
+  d.lb_bwd_offsets = m.upload(tt.lb_bwd_offsets, s);
+  d.lb_bwd = m.upload(tt.lb_bwd, s);
 }
 
 std::int16_t clamp16(long long v) {
@@ -449,8 +468,13 @@ struct runner {
     a.slot_worst = e.d_worst;
     a.n_slots = S;
 
-    // lower bounds of new slots, in chunks of lb_rows_cap staging rows
-    upload_lbs(new_slots, a);
+    // lower bounds of new slots: computed on the device by the batched
+    // engine, uploaded from the host preparation by the plain engine
+    if (W == 64U) {
+      device_lbs(new_slots);
+    } else {
+      upload_lbs(new_slots, a);
+    }
 
     launch_epoch_begin(fwd, W, a, stream);
     HIP_CHECK(hipGetLastError());
@@ -515,6 +539,76 @@ struct runner {
     return n;
   }
 
+  // Lower bounds of the new slots on the device, lbq_chunk queries at a
+  // time: one workgroup per query writes a row, the rows are transposed
+  // into the lane-interleaved lb columns.
+  void device_lbs(std::vector<std::uint32_t> const& new_slots) {
+    if (new_slots.empty()) return;
+    auto const stream = e.stream;
+    auto const L = e.st.L;
+    std::vector<std::int32_t> row_of(e.S, -1);
+    for (std::size_t from = 0; from < new_slots.size(); from += e.lbq_chunk) {
+      auto const to = std::min<std::size_t>(new_slots.size(), from + e.lbq_chunk);
+      std::vector<std::uint32_t> chunk(new_slots.begin() + static_cast<std::ptrdiff_t>(from),
+                                       new_slots.begin() + static_cast<std::ptrdiff_t>(to));
+      std::vector<std::uint32_t> seed_off{0U}, seed_loc, groups;
+      std::vector<std::uint16_t> seed_dur;
+      std::fill(row_of.begin(), row_of.end(), -1);
+      for (std::size_t j = 0; j != chunk.size(); ++j) {
+        auto const s = chunk[j];
+        for (auto const& [l, d] : pq[static_cast<std::size_t>(slots[s].q)].lb_seeds_) {
+          seed_loc.push_back(l);
+          seed_dur.push_back(d);
+        }
+        seed_off.push_back(static_cast<std::uint32_t>(seed_loc.size()));
+        row_of[s] = static_cast<std::int32_t>(j);
+        auto const g = s / e.W;
+        if (groups.empty() || groups.back() != g) groups.push_back(g);
+      }
+      if (seed_loc.size() > e.seed_cap) {
+        e.seed_cap = seed_loc.size() * 2U;
+        e.d_seed_loc = e.grow(e.d_seed_loc, e.seed_cap);
+        e.d_seed_dur = e.grow(e.d_seed_dur, e.seed_cap);
+      }
+      HIP_CHECK(hipMemcpyAsync(e.d_seed_off, seed_off.data(), 4U * seed_off.size(), hipMemcpyHostToDevice, stream));
+      if (!seed_loc.empty()) {
+        HIP_CHECK(hipMemcpyAsync(e.d_seed_loc, seed_loc.data(), 4U * seed_loc.size(), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(e.d_seed_dur, seed_dur.data(), 2U * seed_dur.size(), hipMemcpyHostToDevice, stream));
+      }
+      lb_args q{};
+      q.im = e.img->d;
+      q.L = L;
+      q.n_q = static_cast<std::uint32_t>(chunk.size());
+      q.seed_off = e.d_seed_off;
+      q.seed_loc = e.d_seed_loc;
+      q.seed_dur = e.d_seed_dur;
+      q.dist = e.d_lbq_dist;
+      q.stamp_near = e.d_lbq_stamp_near;
+      q.stamp_far = e.d_lbq_stamp_far;
+      q.near0 = e.d_lbq_near0;
+      q.near1 = e.d_lbq_near1;
+      q.far0 = e.d_lbq_far0;
+      q.far1 = e.d_lbq_far1;
+      q.rows = e.d_lbq_rows;
+      launch_lb_queries(fwd, q, stream);
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipMemcpyAsync(e.d_lb_slot, chunk.data(), 4U * chunk.size(), hipMemcpyHostToDevice, stream));
+      HIP_CHECK(hipMemcpyAsync(e.d_lb_row, row_of.data(), 4U * e.S, hipMemcpyHostToDevice, stream));
+      HIP_CHECK(hipMemcpyAsync(e.d_lb_groups, groups.data(), 4U * groups.size(), hipMemcpyHostToDevice, stream));
+      epoch_args a{};
+      a.st = e.st;
+      a.lb_staging = e.d_lbq_rows;
+      a.lb_slot = e.d_lb_slot;
+      a.n_lb = static_cast<std::uint32_t>(chunk.size());
+      a.lb_row_of_slot = e.d_lb_row;
+      a.lb_groups = e.d_lb_groups;
+      a.n_lb_groups = static_cast<std::uint32_t>(groups.size());
+      launch_lb_scatter(e.W, a, stream);
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipStreamSynchronize(stream));  // host vectors go out of scope
+    }
+  }
+
   void upload_lbs(std::vector<std::uint32_t> const& new_slots, epoch_args& a) {
     if (new_slots.empty()) return;
     auto const L = e.st.L;
@@ -555,7 +649,7 @@ struct runner {
 // prepare queries on host threads
 void prepare_all(timetable const& tt, nigiri_query const* qs, std::uint32_t n,
                  std::vector<prepared_query>& pq, std::vector<int>& status,
-                 std::vector<std::string>& errors) {
+                 std::vector<std::string>& errors, bool host_lb) {
   pq.resize(n);
   status.assign(n, NIGIRI_AMD_OK);
   errors.assign(n, {});
@@ -570,7 +664,7 @@ void prepare_all(timetable const& tt, nigiri_query const* qs, std::uint32_t n,
         auto const i = next.fetch_add(1U);
         if (i >= n) break;
         try {
-          prepare(tt, qs[i], pq[i], scratch);
+          prepare(tt, qs[i], pq[i], scratch, host_lb);
         } catch (status_error const& e) {
           status[i] = e.code_;
           errors[i] = e.what();
@@ -636,7 +730,7 @@ void engine_run_dir(nigiri_amd_engine& e, bool fwd, std::vector<prepared_query>&
 void engine_prepare_all(timetable const& tt, nigiri_query const* qs, std::uint32_t n,
                         std::vector<prepared_query>& pq, std::vector<int>& status,
                         std::vector<std::string>& errors) {
-  prepare_all(tt, qs, n, pq, status, errors);
+  prepare_all(tt, qs, n, pq, status, errors, true);
 }
 
 std::uint32_t engine_write_results(std::vector<prepared_query> const& pq, std::vector<int> const& status,
@@ -796,6 +890,24 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     e->d_lb_slot = m.alloc<std::uint32_t>(e->lb_rows_cap);
     e->d_lb_row = m.alloc<std::int32_t>(S);
     e->d_lb_groups = m.alloc<std::uint32_t>(G);
+    if (W == 64U) {
+      // device lower-bound scratch: 30 bytes per location and query, at
+      // most about 4 GiB, at least one query per launch
+      auto const per_q = 30ULL * L;
+      e->lbq_chunk = static_cast<std::uint32_t>(
+          std::max<std::uint64_t>(1U, std::min<std::uint64_t>(S, (4ULL << 30) / per_q)));
+      auto const n = static_cast<std::size_t>(e->lbq_chunk) * L;
+      e->d_lbq_dist = m.alloc<std::uint32_t>(n);
+      e->d_lbq_stamp_near = m.alloc<std::uint32_t>(n);
+      e->d_lbq_stamp_far = m.alloc<std::uint32_t>(n);
+      e->d_lbq_near0 = m.alloc<std::uint32_t>(n);
+      e->d_lbq_near1 = m.alloc<std::uint32_t>(n);
+      e->d_lbq_far0 = m.alloc<std::uint32_t>(n);
+      e->d_lbq_far1 = m.alloc<std::uint32_t>(n);
+      e->d_lbq_rows = m.alloc<std::uint16_t>(n);
+      e->d_seed_off = m.alloc<std::uint32_t>(static_cast<std::size_t>(e->lbq_chunk) + 1U);
+      e->d_lb_slot = m.alloc<std::uint32_t>(e->lbq_chunk);
+    }
     e->h_lb.reserve(static_cast<std::size_t>(e->lb_rows_cap) * L);
     e->h_active.reserve(G);
     e->h_rows.reserve(static_cast<std::size_t>(S) * kMaxDestBits * K);
@@ -867,7 +979,8 @@ int nigiri_amd_search(nigiri_amd_engine* e, const nigiri_query* qs, uint32_t n, 
     std::vector<prepared_query> pq;
     std::vector<int> status;
     std::vector<std::string> errors;
-    prepare_all(tt, qs, n, pq, status, errors);
+    // the batched engine computes lower bounds on the device
+    prepare_all(tt, qs, n, pq, status, errors, e->W != 64U);
     for (int dir = 0; dir != 2; ++dir) {
       std::vector<std::int32_t> queue;
       for (std::uint32_t i = 0; i != n; ++i) {
@@ -946,7 +1059,7 @@ int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs, const double*
                                     std::chrono::duration<double, std::milli>(arrival_ms[i]));
           std::this_thread::sleep_until(due);
           try {
-            prepare(tt, qs[i], pq[i], scratch);
+            prepare(tt, qs[i], pq[i], scratch, e->W != 64U);
           } catch (status_error const& ex) {
             status[i] = ex.code_;
           } catch (std::exception const&) {
@@ -1228,17 +1341,20 @@ int nigiri_amd_algo_execute(nigiri_amd_engine* e, uint32_t slot, int32_t start_t
 }
 
 int nigiri_amd_algo_download(nigiri_amd_engine* e, uint32_t slot, int16_t* round_times, int16_t* best,
-                             int16_t* tmp, int16_t* time_at_dest) {
+                             int16_t* tmp, int16_t* time_at_dest, uint16_t* lb) {
   if (e == nullptr || slot >= e->S) return NIGIRI_AMD_EINVAL;
   return guarded([&]() {
     HIP_CHECK(hipSetDevice(e->img->device));
     auto const L = e->st.L, K = e->K;
     std::int16_t* d = nullptr;
-    HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&d), 2U * static_cast<std::size_t>(K + 2U) * L));
-    std::vector<std::int16_t> h(static_cast<std::size_t>(K + 2U) * L);
+    auto const rows = static_cast<std::size_t>(K + 3U);
+    HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&d), 2U * rows * L));
+    std::vector<std::int16_t> h(rows * L);
     try {
       launch_gather_slot(e->W, e->st, slot, d, d + static_cast<std::size_t>(K) * L,
-                         d + static_cast<std::size_t>(K + 1U) * L, e->stream);
+                         d + static_cast<std::size_t>(K + 1U) * L,
+                         reinterpret_cast<std::uint16_t*>(d + static_cast<std::size_t>(K + 2U) * L),
+                         e->stream);
       HIP_CHECK(hipGetLastError());
       HIP_CHECK(hipMemcpyAsync(h.data(), d, 2U * h.size(), hipMemcpyDeviceToHost, e->stream));
       HIP_CHECK(hipStreamSynchronize(e->stream));
@@ -1250,6 +1366,7 @@ int nigiri_amd_algo_download(nigiri_amd_engine* e, uint32_t slot, int16_t* round
     if (round_times != nullptr) std::memcpy(round_times, h.data(), 2U * static_cast<std::size_t>(K) * L);
     if (best != nullptr) std::memcpy(best, h.data() + static_cast<std::size_t>(K) * L, 2U * L);
     if (tmp != nullptr) std::memcpy(tmp, h.data() + static_cast<std::size_t>(K + 1U) * L, 2U * L);
+    if (lb != nullptr) std::memcpy(lb, h.data() + static_cast<std::size_t>(K + 2U) * L, 2U * L);
     if (time_at_dest != nullptr) {
       HIP_CHECK(hipMemcpy(time_at_dest, e->st.tad + static_cast<std::size_t>(slot) * kRowsMax,
                           2U * kRowsMax, hipMemcpyDeviceToHost));

@@ -43,6 +43,29 @@ struct round_args {
   hipEvent_t ev[4];
 };
 
+// Device lower bounds: exact shortest travel times to the destination
+// offsets on the lower-bound graph with the 7201-minute cut-off of nigiri's
+// dial-bucket Dijkstra (dijkstra.cc:18-67), one workgroup per query.  Query
+// q's seeds are seed_loc/seed_dur[seed_off[q] .. seed_off[q+1]); its result
+// row is rows[q][L] (0xFFFF = unreachable).  Scratch arrays are [n_q][L].
+struct lb_args {
+  dev_image im;
+  std::uint32_t L;
+  std::uint32_t n_q;
+  std::uint32_t const* seed_off;
+  std::uint32_t const* seed_loc;
+  std::uint16_t const* seed_dur;
+  std::uint32_t* dist;
+  std::uint32_t* stamp_near;
+  std::uint32_t* stamp_far;
+  std::uint32_t* near0;
+  std::uint32_t* near1;
+  std::uint32_t* far0;
+  std::uint32_t* far1;
+  std::uint16_t* rows;
+};
+void launch_lb_queries(bool fwd, lb_args const& a, hipStream_t stream);
+
 // Launch helpers (defined in kernels.hip), all asynchronous on `stream`.
 void launch_round(bool fwd, std::uint32_t W, round_args const& a,
                   std::uint32_t const* sweep_groups, std::uint32_t n_sweep_groups,
@@ -86,7 +109,7 @@ void launch_epoch_begin(bool fwd, std::uint32_t W, epoch_args const& a,
 void launch_lb_scatter(std::uint32_t W, epoch_args const& a, hipStream_t stream);
 // state download: round_times [K][L], best [L], tmp [L] of one slot
 void launch_gather_slot(std::uint32_t W, dev_state const& st, std::uint32_t slot,
-                        std::int16_t* rt, std::int16_t* best, std::int16_t* tmp,
+                        std::int16_t* rt, std::int16_t* best, std::int16_t* tmp, std::uint16_t* lb,
                         hipStream_t stream);
 void launch_epoch_end(std::uint32_t W, dev_state const& st,
                       std::uint32_t n_slots, hipStream_t stream);

@@ -1272,6 +1272,146 @@ __global__ void k_lane_reset(epoch_args a) {
   }
 }
 
+// ---- device lower bounds -------------------------------------------------
+// One workgroup per query: near-far label-correcting shortest paths on the
+// lower-bound graph from the query's destination offsets.  Locations whose
+// tentative distance is at most the threshold T are relaxed in sweeps (near
+// list); improvements above T wait in the far list until T advances to the
+// smallest waiting distance + delta - 1.  Distances are only accepted below
+// the cut-off of nigiri's dial (7201 = kMaxTravelTime + 1 buckets), so the
+// result is the exact shortest distance when it is below the cut-off and
+// unreachable (0xFFFF) otherwise: the values the bucket Dijkstra of
+// dijkstra.cc:18-67 produces.
+constexpr std::uint32_t kLbCut = 7201U;
+constexpr std::uint32_t kLbInf = 0xFFFFFFFFU;
+constexpr std::uint32_t kLbDelta = 8U;
+constexpr int kLbBlock = 256;
+
+template <bool Fwd>
+__global__ void __launch_bounds__(kLbBlock) k_lb_query(lb_args a) {
+  auto const q = blockIdx.x;
+  if (q >= a.n_q) return;
+  auto const L = a.L;
+  auto const tid = threadIdx.x;
+  auto const* off = Fwd ? a.im.lb_fwd_offsets : a.im.lb_bwd_offsets;
+  auto const* edges = Fwd ? a.im.lb_fwd : a.im.lb_bwd;
+  auto const base = static_cast<std::size_t>(q) * L;
+  auto* dist = a.dist + base;
+  auto* stamp_near = a.stamp_near + base;
+  auto* stamp_far = a.stamp_far + base;
+  std::uint32_t* near = a.near0 + base;
+  std::uint32_t* near_next = a.near1 + base;
+  std::uint32_t* far = a.far0 + base;
+  std::uint32_t* far_next = a.far1 + base;
+  __shared__ std::uint32_t s_near, s_far, s_min;
+
+  for (auto l = tid; l < L; l += kLbBlock) {
+    dist[l] = kLbInf;
+    stamp_near[l] = 0U;
+    stamp_far[l] = 0U;
+  }
+  if (tid == 0U) s_far = 0U;
+  __syncthreads();
+  auto const s0 = a.seed_off[q], s1 = a.seed_off[q + 1U];
+  for (auto i = s0 + tid; i < s1; i += kLbBlock) {
+    atomicMin(&dist[a.seed_loc[i]], static_cast<std::uint32_t>(a.seed_dur[i]));
+  }
+  __syncthreads();
+  // every seed starts in the far list of phase 1
+  for (auto i = s0 + tid; i < s1; i += kLbBlock) {
+    auto const l = a.seed_loc[i];
+    if (atomicExch(&stamp_far[l], 1U) != 1U) far[atomicAdd(&s_far, 1U)] = l;
+  }
+  __syncthreads();
+  std::uint32_t n_far = s_far, n_near = 0U;
+  std::uint32_t phase = 1U, sweep = 1U;
+  std::int64_t T = -1;  // every distance <= T has been relaxed
+  while (true) {
+    if (n_near == 0U) {
+      if (n_far == 0U) break;
+      // advance T past the smallest waiting distance, then split the far
+      // list (entries whose distance fell to <= T were relaxed already)
+      if (tid == 0U) s_min = kLbInf;
+      __syncthreads();
+      for (auto i = tid; i < n_far; i += kLbBlock) {
+        auto const d = dist[far[i]];
+        if (static_cast<std::int64_t>(d) > T) atomicMin(&s_min, d);
+      }
+      __syncthreads();
+      auto const mn = s_min;
+      if (mn == kLbInf) break;
+      auto const t_old = T;
+      T = static_cast<std::int64_t>(mn) + kLbDelta - 1;
+      ++phase;
+      ++sweep;
+      if (tid == 0U) {
+        s_near = 0U;
+        s_far = 0U;
+      }
+      __syncthreads();
+      for (auto i = tid; i < n_far; i += kLbBlock) {
+        auto const l = far[i];
+        auto const d = static_cast<std::int64_t>(dist[l]);
+        if (d <= t_old) continue;
+        if (d <= T) {
+          if (atomicExch(&stamp_near[l], sweep) != sweep) near_next[atomicAdd(&s_near, 1U)] = l;
+        } else if (atomicExch(&stamp_far[l], phase) != phase) {
+          far_next[atomicAdd(&s_far, 1U)] = l;
+        }
+      }
+      __syncthreads();
+      n_near = s_near;
+      n_far = s_far;
+      auto* tn = near;
+      near = near_next;
+      near_next = tn;
+      auto* tf = far;
+      far = far_next;
+      far_next = tf;
+      __syncthreads();
+      continue;
+    }
+    // sweep: relax every near location; improvements <= T join the next
+    // near list, larger ones the far list of this phase
+    ++sweep;
+    if (tid == 0U) {
+      s_near = 0U;
+      s_far = n_far;
+    }
+    __syncthreads();
+    for (auto i = tid; i < n_near; i += kLbBlock) {
+      auto const u = near[i];
+      auto const du = dist[u];
+      for (auto ei = off[u]; ei != off[u + 1U]; ++ei) {
+        auto const ed = edges[ei];
+        auto const t = ed & ((1U << 23) - 1U);
+        auto const nd = du + (ed >> 23);
+        if (nd < kLbCut && nd < dist[t]) {
+          auto const old = atomicMin(&dist[t], nd);
+          if (nd < old) {
+            if (static_cast<std::int64_t>(nd) <= T) {
+              if (atomicExch(&stamp_near[t], sweep) != sweep) near_next[atomicAdd(&s_near, 1U)] = t;
+            } else if (atomicExch(&stamp_far[t], phase) != phase) {
+              far[atomicAdd(&s_far, 1U)] = t;
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    n_near = s_near;
+    n_far = s_far;
+    auto* tn = near;
+    near = near_next;
+    near_next = tn;
+    __syncthreads();
+  }
+  for (auto l = tid; l < L; l += kLbBlock) {
+    auto const d = dist[l];
+    a.rows[base + l] = d < kLbCut ? static_cast<std::uint16_t>(d) : kUnreachableLb;
+  }
+}
+
 // lower bounds: staging rows -> interleaved columns.  W == 64: LDS-tiled
 // transpose, one block per (group, 64-location tile), both sides coalesced.
 template <int W>
@@ -1382,9 +1522,9 @@ __global__ void k_extract(dev_state s, std::uint32_t n_slots) {
 // copies one slot's state into contiguous rows (state download)
 template <int W>
 __global__ void k_gather_slot(dev_state s, std::uint32_t slot, std::int16_t* rt, std::int16_t* best,
-                              std::int16_t* tmp) {
+                              std::int16_t* tmp, std::uint16_t* lb) {
   auto const g = slot / W, lane = slot % W;
-  auto const total = static_cast<std::size_t>(s.K + 2U) * s.L;
+  auto const total = static_cast<std::size_t>(s.K + 3U) * s.L;
   for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < total;
        i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
     auto const row = static_cast<std::uint32_t>(i / s.L), l = static_cast<std::uint32_t>(i % s.L);
@@ -1392,8 +1532,10 @@ __global__ void k_gather_slot(dev_state s, std::uint32_t slot, std::int16_t* rt,
       rt[i] = s.round_times[lay<W>::rt(s, g, row, l, lane)];
     } else if (row == s.K) {
       best[l] = s.best[lay<W>::col(s, g, l, lane)];
-    } else {
+    } else if (row == s.K + 1U) {
       tmp[l] = static_cast<std::int16_t>(s.tmp[lay<W>::col(s, g, l, lane)]);
+    } else {
+      lb[l] = s.lb[lay<W>::col(s, g, l, lane)];
     }
   }
 }
@@ -1482,6 +1624,15 @@ void launch_round(bool fwd, std::uint32_t W, round_args const& a, std::uint32_t 
   }
 }
 
+void launch_lb_queries(bool fwd, lb_args const& a, hipStream_t stream) {
+  if (a.n_q == 0U) return;
+  if (fwd) {
+    hipLaunchKernelGGL((k_lb_query<true>), dim3(a.n_q), dim3(kLbBlock), 0, stream, a);
+  } else {
+    hipLaunchKernelGGL((k_lb_query<false>), dim3(a.n_q), dim3(kLbBlock), 0, stream, a);
+  }
+}
+
 void launch_lb_scatter(std::uint32_t W, epoch_args const& a, hipStream_t stream) {
   if (a.n_lb == 0U) return;
   if (W == 64U) {
@@ -1500,11 +1651,11 @@ void launch_epoch_begin(bool fwd, std::uint32_t W, epoch_args const& a, hipStrea
 }
 
 void launch_gather_slot(std::uint32_t W, dev_state const& st, std::uint32_t slot, std::int16_t* rt,
-                        std::int16_t* best, std::int16_t* tmp, hipStream_t stream) {
+                        std::int16_t* best, std::int16_t* tmp, std::uint16_t* lb, hipStream_t stream) {
   if (W == 64U) {
-    hipLaunchKernelGGL((k_gather_slot<64>), dim3(2048), dim3(kBlock), 0, stream, st, slot, rt, best, tmp);
+    hipLaunchKernelGGL((k_gather_slot<64>), dim3(2048), dim3(kBlock), 0, stream, st, slot, rt, best, tmp, lb);
   } else {
-    hipLaunchKernelGGL((k_gather_slot<1>), dim3(2048), dim3(kBlock), 0, stream, st, slot, rt, best, tmp);
+    hipLaunchKernelGGL((k_gather_slot<1>), dim3(2048), dim3(kBlock), 0, stream, st, slot, rt, best, tmp, lb);
   }
 }
 

@@ -182,7 +182,7 @@ void starts_at_stop(timetable const& tt, bool fwd, std::uint32_t r, std::uint32_
 }  // namespace
 
 void prepare(timetable const& tt, nigiri_query const& q, prepared_query& p,
-             host_scratch& s) {
+             host_scratch& s, bool host_lb) {
   if (q.min_connection_count != 0U || q.prf_idx != 0) {
     throw status_error{NIGIRI_AMD_EUNSUPPORTED,
                        "min_connection_count > 0 and profiles != 0 are not supported"};
@@ -249,7 +249,12 @@ void prepare(timetable const& tt, nigiri_query const& q, prepared_query& p,
       p.dests_.push_back(l);
     }
   }
-  lower_bounds(tt, fwd, dest_min, p.lb_, s);
+  p.lb_seeds_ = dest_min;
+  if (host_lb) {
+    lower_bounds(tt, fwd, dest_min, p.lb_, s);
+  } else {
+    p.lb_.clear();
+  }
 
   // scheduling hint: queries that start close to each other mark the same
   // routes in their first rounds, so the batch scheduler groups them

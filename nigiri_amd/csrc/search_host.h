@@ -6,6 +6,7 @@
 #pragma once
 
 #include <cstdint>
+#include <utility>
 #include <vector>
 
 #include "timetable.h"
@@ -60,7 +61,9 @@ struct prepared_query {
   std::vector<std::uint32_t> dests_;  // ascending (non-intermodal)
   std::vector<std::uint32_t> dist_loc_;  // ascending (intermodal)
   std::vector<std::uint16_t> dist_min_;
-  std::vector<std::uint16_t> lb_;  // [n_locations]
+  std::vector<std::uint16_t> lb_;  // [n_locations] (host lower bounds only)
+  // shortest-path sources of the lower bounds: (location, offset minutes)
+  std::vector<std::pair<std::uint32_t, std::uint16_t>> lb_seeds_;
   std::vector<epoch> epochs_;
   std::uint64_t locality_key_{0U};  // Morton code of the first start stop
   std::int32_t slot_{-1};           // engine slot that ran the query
@@ -83,9 +86,11 @@ struct host_scratch {
   std::vector<std::vector<std::uint32_t>> buckets_;
 };
 
-// Throws status_error for unsupported or malformed queries.
+// Throws status_error for unsupported or malformed queries.  With
+// host_lb == false the lower bounds are left to the device (lb_seeds_ holds
+// their sources either way).
 void prepare(timetable const& tt, nigiri_query const& q, prepared_query& p,
-             host_scratch& s);
+             host_scratch& s, bool host_lb = true);
 
 // Journey extraction of one epoch (raptor.h:393-413) followed by the
 // reconstruction gate of search_interval() (search.h:457-475).

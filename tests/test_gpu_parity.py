@@ -1,7 +1,7 @@
 """GPU parity: the HIP engine (through the C ABI) against the CPU oracle.
 
 Bar: bit-exact journeys (start, dest_time, dest, transfers) and bit-exact
-final state (round_times rows, best, tmp, time_at_dest) per query.
+final state (round_times rows, best, tmp, time_at_dest, lower bounds) per query.
 """
 import sys
 
@@ -15,7 +15,7 @@ from test_oracle_golden import abc_timetable, hm
 
 pytestmark = pytest.mark.gpu
 
-STATE_KEYS = ("round_times", "best", "tmp", "time_at_dest")
+STATE_KEYS = ("round_times", "best", "tmp", "time_at_dest", "lb")
 
 
 def say(msg):

@@ -325,51 +325,74 @@ struct route_bits {
 
 // get_earliest_transport (raptor.h:1351-1442).  `ev` points at the event
 // block of the boarding stop (one delta per transport, in LDS or global).
-// The first-day seek (linear_lb, linear_lower_bound.h:7-15) is a binary
-// search: the minutes of day of one stop block are non-decreasing because
-// routes keep FIFO order modulo 1440 (checked when a timetable is built);
-// the event counter still counts what the linear scan examines.
+// The first-day seek (linear_lb, linear_lower_bound.h:7-15) is a branch-free
+// binary search with `top` = the largest power of two <= n_t steps: the
+// minutes of day of one stop block are non-decreasing because routes keep
+// FIFO order modulo 1440 (checked when a timetable is built), which also
+// makes the first day's "departure not before the arrival" test always
+// pass after the seek.  The six-day loop and the event scan are one flat
+// loop (one event or one day per iteration), so that lanes leave it
+// independently.  The counters count what the reference's linear scan
+// examines.
 template <bool Fwd>
 __device__ inline et_t<Fwd> earliest_transport(route_bits const& bits, std::uint16_t const* ev,
-                                               std::uint32_t n_t, std::uint16_t day_at_stop,
-                                               int mam_at_stop, int lb_l, int tad_k,
-                                               std::uint16_t base, et_counters& c) {
+                                               std::uint32_t n_t, std::uint32_t top,
+                                               std::uint16_t day_at_stop, int mam_at_stop,
+                                               int lb_l, int tad_k, std::uint16_t base,
+                                               et_counters& c) {
   using D = dirop<Fwd>;
-  for (std::uint32_t i = 0U; i != 6U; ++i) {  // kMaxTravelTime / 1 day + 1
-    auto const day = static_cast<std::uint16_t>(Fwd ? day_at_stop + i : day_at_stop - i);
-    ++c.route_days;
-    if (!bits.route_active(day)) continue;
-    std::uint32_t pos = 0U;
-    if (i == 0U) {
-      std::uint32_t lo = 0U, hi = n_t;
-      while (lo < hi) {
-        auto const mid = (lo + hi) / 2U;
-        auto const t_off = Fwd ? mid : n_t - 1U - mid;
-        if (D::better(ev[t_off] >> 5, mam_at_stop)) {
-          lo = mid + 1U;
-        } else {
-          hi = mid;
-        }
+  constexpr std::uint32_t kDays = 6U;  // kMaxTravelTime / 1 day + 1
+  auto day = day_at_stop;
+  std::uint32_t i = 0U;
+  std::uint32_t pos = n_t;
+  ++c.route_days;
+  if (bits.route_active(day)) {
+    std::uint32_t p = 0U;
+    for (auto step = top; step != 0U; step >>= 1U) {
+      auto const np = p + step;
+      if (np <= n_t) {
+        auto const t_off = Fwd ? np - 1U : n_t - np;
+        if (D::better(ev[t_off] >> 5, mam_at_stop)) p = np;
       }
-      pos = lo;
-      c.events += pos < n_t ? pos + 1U : n_t;
     }
-    for (; pos < n_t; ++pos) {
-      ++c.events;
+    pos = p;
+    c.events += p < n_t ? p + 1U : n_t;
+  }
+  et_t<Fwd> res{~0U, 0xFFFFU};
+  std::uint32_t n_ev = 0U, n_days = 0U, n_tt = 0U;
+  bool go = true;
+  while (go) {
+    if (pos >= n_t) {
+      ++i;
+      if (i == kDays) {
+        go = false;
+      } else {
+        day = static_cast<std::uint16_t>(Fwd ? day + 1U : day - 1U);
+        ++n_days;
+        pos = bits.route_active(day) ? 0U : n_t;
+      }
+    } else {
+      ++n_ev;
       auto const t_off = Fwd ? pos : n_t - 1U - pos;
       auto const e = ev[t_off];
-      int const ev_mam = e >> 5;
-      if (!D::better_eq(to_delta<Fwd>(day, ev_mam, base) + D::dir(lb_l), tad_k)) {
-        return {~0U, 0xFFFFU};
+      if (!D::better_eq(to_delta<Fwd>(day, e >> 5, base) + D::dir(lb_l), tad_k)) {
+        go = false;
+      } else {
+        auto const start_day = static_cast<std::uint16_t>(static_cast<int>(day) - (e & 31));
+        ++n_tt;
+        if (bits.transport_active(t_off, start_day)) {
+          res = {t_off, start_day};
+          go = false;
+        } else {
+          ++pos;
+        }
       }
-      if (i == 0U && !D::better_eq(mam_at_stop, ev_mam)) continue;
-      auto const start_day = static_cast<std::uint16_t>(static_cast<int>(day) - (e & 31));
-      ++c.transport_tests;
-      if (!bits.transport_active(t_off, start_day)) continue;
-      return {t_off, start_day};
     }
   }
-  return {~0U, 0xFFFFU};
+  c.events += n_ev;
+  c.route_days += n_days;
+  c.transport_tests += n_tt;
+  return res;
 }
 
 template <bool Fwd>
@@ -451,8 +474,8 @@ __device__ inline void scan_route(route_view const& rv, dev_image const& im, dev
       auto const day = static_cast<std::uint16_t>(minutes / 1440);
       int const mam = static_cast<std::int16_t>(minutes % 1440);
       auto const* ev_stop = rv.ev + n_t * (stop_idx * 2U - (Fwd ? 0U : 1U));
-      auto const new_et = earliest_transport<Fwd>(rv.bits, ev_stop, n_t, day, mam, lbv, tad_k,
-                                                  base, c_et);
+      auto const new_et = earliest_transport<Fwd>(rv.bits, ev_stop, n_t, 1U << (31 - __clz(n_t)), day,
+                                                  mam, lbv, tad_k, base, c_et);
       if (new_et.valid()) {
         bool const cb_valid = ride || s.best[c] != D::kInvalid || s.tmp[c] != D::kInvalid;
         if (!cb_valid ||
@@ -534,6 +557,7 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
   }
   route_bits const bits{x.rbits, x.tbits, tbf_g, im.bitfields, staged_t};
   auto const chunk = min(kChunkMax, kChunkEvBytes / (4U * n_t));
+  auto const top = 1U << (31 - __clz(n_t));  // largest power of two <= n_t
 
   et_t<Fwd> et{~0U, 0xFFFFU};
   // `started`: one of the route's stops is marked for this query.  Before
@@ -649,8 +673,8 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
         auto const day = static_cast<std::uint16_t>(minutes / 1440);
         int const mam = static_cast<std::int16_t>(minutes % 1440);
         et_counters ce;
-        auto const new_et = earliest_transport<Fwd>(bits, ev_board, n_t, day, mam, lbv, tad_k,
-                                                    base, ce);
+        auto const new_et = earliest_transport<Fwd>(bits, ev_board, n_t, top, day, mam, lbv,
+                                                    tad_k, base, ce);
         c_ev += static_cast<std::uint32_t>(ce.events);
         c_rd += static_cast<std::uint32_t>(ce.route_days);
         c_tt += static_cast<std::uint32_t>(ce.transport_tests);

@@ -66,6 +66,10 @@ struct lb_args {
 };
 void launch_lb_queries(bool fwd, lb_args const& a, hipStream_t stream);
 
+// The batched route scan stages the events of at least one stop in LDS: it
+// takes routes of at most this many transports.
+constexpr std::uint32_t kBatchedMaxRouteTransports = 256U;
+
 // Launch helpers (defined in kernels.hip), all asynchronous on `stream`.
 void launch_round(bool fwd, std::uint32_t W, round_args const& a,
                   std::uint32_t const* sweep_groups, std::uint32_t n_sweep_groups,

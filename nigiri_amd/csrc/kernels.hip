@@ -21,6 +21,12 @@ struct dirop {
   __device__ static int dir(int a) { return Fwd ? a : -a; }
 };
 
+// A value that is the same in every active lane, moved to a scalar register
+// (lets the compiler use scalar loads and scalar branches on it).
+__device__ inline std::uint32_t uni(std::uint32_t v) {
+  return static_cast<std::uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+}
+
 __device__ inline std::int16_t clamp16(int v) {
   return static_cast<std::int16_t>(v < -32768 ? -32768 : (v > 32767 ? 32767 : v));
 }
@@ -315,11 +321,16 @@ struct route_bits {
   __device__ bool route_active(std::uint32_t day) const {
     return day < 512U && ((route[day / 64U] >> (day % 64U)) & 1ULL) != 0ULL;
   }
+  // Staged: the transports' words are in LDS (route scans of small routes)
+  template <bool Staged>
   __device__ bool transport_active(std::uint32_t t_off, std::uint32_t day) const {
     if (day >= 512U) return false;  // out-of-range day index: not active
-    auto const w = use_staged ? staged[t_off * 8U + day / 64U]
-                              : table[static_cast<std::size_t>(tbf[t_off]) * 8U + day / 64U];
+    auto const w = Staged ? staged[t_off * 8U + day / 64U]
+                          : table[static_cast<std::size_t>(tbf[t_off]) * 8U + day / 64U];
     return ((w >> (day % 64U)) & 1ULL) != 0ULL;
+  }
+  __device__ bool transport_active(std::uint32_t t_off, std::uint32_t day) const {
+    return use_staged ? transport_active<true>(t_off, day) : transport_active<false>(t_off, day);
   }
 };
 
@@ -334,7 +345,7 @@ struct route_bits {
 // loop (one event or one day per iteration), so that lanes leave it
 // independently.  The counters count what the reference's linear scan
 // examines.
-template <bool Fwd>
+template <bool Fwd, bool Staged>
 __device__ inline et_t<Fwd> earliest_transport(route_bits const& bits, std::uint16_t const* ev,
                                                std::uint32_t n_t, std::uint32_t top,
                                                std::uint16_t day_at_stop, int mam_at_stop,
@@ -380,7 +391,7 @@ __device__ inline et_t<Fwd> earliest_transport(route_bits const& bits, std::uint
       } else {
         auto const start_day = static_cast<std::uint16_t>(static_cast<int>(day) - (e & 31));
         ++n_tt;
-        if (bits.transport_active(t_off, start_day)) {
+        if (bits.transport_active<Staged>(t_off, start_day)) {
           res = {t_off, start_day};
           go = false;
         } else {
@@ -474,12 +485,13 @@ __device__ inline void scan_route(route_view const& rv, dev_image const& im, dev
       auto const day = static_cast<std::uint16_t>(minutes / 1440);
       int const mam = static_cast<std::int16_t>(minutes % 1440);
       auto const* ev_stop = rv.ev + n_t * (stop_idx * 2U - (Fwd ? 0U : 1U));
-      auto const new_et = earliest_transport<Fwd>(rv.bits, ev_stop, n_t, 1U << (31 - __clz(n_t)), day,
-                                                  mam, lbv, tad_k, base, c_et);
+      auto const new_et = earliest_transport<Fwd, false>(rv.bits, ev_stop, n_t, 1U << (31 - __clz(n_t)),
+                                                         day, mam, lbv, tad_k, base, c_et);
       if (new_et.valid()) {
-        bool const cb_valid = ride || s.best[c] != D::kInvalid || s.tmp[c] != D::kInvalid;
-        if (!cb_valid ||
-            D::better_eq(time_at_stop<Fwd>(rv.ev, n_t, new_et, stop_idx, !Fwd, base), et_dep)) {
+        // current_best is only read when the new trip departs later (see
+        // scan_route64)
+        if (D::better_eq(time_at_stop<Fwd>(rv.ev, n_t, new_et, stop_idx, !Fwd, base), et_dep) ||
+            !(ride || s.best[c] != D::kInvalid || s.tmp[c] != D::kInvalid)) {
           et = new_et;
         }
       }
@@ -496,10 +508,10 @@ __device__ inline void scan_route(route_view const& rv, dev_image const& im, dev
 // (arrival, departure) of the chunk's stops.  All staging loads of a chunk
 // are issued together; the scan of the chunk then reads LDS only, except for
 // transport bitfields of large routes and the rare current_best check.
-constexpr std::uint32_t kChunkMax = 8U;
+constexpr std::uint32_t kChunkMax = 4U;
 constexpr std::uint32_t kStagedTransports = 16U;
-constexpr std::uint32_t kChunkEvBytes = 4096U;
-constexpr std::uint32_t kMaxRouteTransports = kChunkEvBytes / 4U;  // chunk of >= 1 stop
+constexpr std::uint32_t kChunkEvBytes = 1024U;
+static_assert(kChunkEvBytes / 4U >= kBatchedMaxRouteTransports, "a chunk holds at least one stop");
 
 struct wave_lds {
   std::uint64_t rbits[8];
@@ -530,12 +542,12 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
                                     std::uint32_t& c_tt, bool& any, bool& started) {
   using D = dirop<Fwd>;
   auto const L = s.L;
-  auto const s0 = im.route_stop_offsets[r];
-  auto const n_stops = im.route_stop_offsets[r + 1U] - s0;
-  auto const t_from = im.route_transport_from[r];
-  auto const n_t = im.route_transport_from[r + 1U] - t_from;
+  auto const s0 = uni(im.route_stop_offsets[r]);
+  auto const n_stops = uni(im.route_stop_offsets[r + 1U]) - s0;
+  auto const t_from = uni(im.route_transport_from[r]);
+  auto const n_t = uni(im.route_transport_from[r + 1U]) - t_from;
   auto const* stops_g = im.route_stops + s0;
-  auto const* ev_g = im.event_times + im.route_stop_time_from[r];
+  auto const* ev_g = im.event_times + uni(im.route_stop_time_from[r]);
   auto const* tbf_g = im.transport_traffic_days + t_from;
   // this lane's query: element l of a [G][L][64] array is at col0 + 64 l
   // (G * L * 64 < 2^32 is checked when the engine is created); the half of
@@ -547,7 +559,7 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
   auto const* mark_t32 = reinterpret_cast<std::uint32_t const*>(s.mark_t);
 
   // route-level staging: traffic-day bitfields
-  auto const rbf = im.route_traffic_days[r];
+  auto const rbf = uni(im.route_traffic_days[r]);
   bool const staged_t = n_t <= kStagedTransports;
   if (lane < 8U) x.rbits[lane] = im.bitfields[static_cast<std::size_t>(rbf) * 8U + lane];
   if (staged_t) {
@@ -625,7 +637,7 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
 
     for (std::uint32_t j = 0U; j < nc; ++j) {
       auto const i = c0 + j;
-      auto const stp = x.stop[j];
+      auto const stp = uni(x.stop[j]);
       auto const l = stp & kLocMask;
       bool const marked = ((mbits >> j) & 1U) != 0U;
       int const lbv = x.lb[j][lane];
@@ -673,19 +685,24 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
         auto const day = static_cast<std::uint16_t>(minutes / 1440);
         int const mam = static_cast<std::int16_t>(minutes % 1440);
         et_counters ce;
-        auto const new_et = earliest_transport<Fwd>(bits, ev_board, n_t, top, day, mam, lbv,
-                                                    tad_k, base, ce);
+        auto const new_et =
+            staged_t ? earliest_transport<Fwd, true>(bits, ev_board, n_t, top, day, mam, lbv, tad_k, base, ce)
+                     : earliest_transport<Fwd, false>(bits, ev_board, n_t, top, day, mam, lbv, tad_k, base, ce);
         c_ev += static_cast<std::uint32_t>(ce.events);
         c_rd += static_cast<std::uint32_t>(ce.route_days);
         c_tt += static_cast<std::uint32_t>(ce.transport_tests);
         if (new_et.valid()) {
-          // current_best (raptor.h:1328-1329) is valid whenever a ride was
-          // evaluated at this stop (it then includes the valid round_times
-          // [k-1] label) or best / tmp hold a label
-          bool const cb_valid = ride || s.best[off] != D::kInvalid || s.tmp[off] != D::kInvalid;
+          // et = new_et if current_best (raptor.h:1328-1329) is invalid or
+          // the new trip departs no later.  The search scans departures in
+          // time order from prev <= et_dep, so a found trip departs no
+          // later than et: the state loads of current_best are only made
+          // when that test fails.  current_best is valid whenever a ride
+          // was evaluated at this stop (it then includes the valid
+          // round_times[k-1] label) or best / tmp hold a label.
           auto const e = ev_board[new_et.t_off];
           auto const new_dep = to_delta<Fwd>(new_et.day, (e & 31) * 1440 + (e >> 5), base);
-          if (!cb_valid || D::better_eq(new_dep, et_dep)) {
+          if (D::better_eq(new_dep, et_dep) ||
+              !(ride || s.best[off] != D::kInvalid || s.tmp[off] != D::kInvalid)) {
             et = new_et;
           }
         }
@@ -783,7 +800,7 @@ __global__ void k_route_pack(round_args a) {
 // packed wavefront per route (up to 64 queries of any groups, route data
 // staged in LDS once); plain engine: one route per lane.
 template <bool Fwd, int W>
-__global__ void __launch_bounds__(256, 5) k_route_scan(round_args a) {
+__global__ void __launch_bounds__(256, 8) k_route_scan(round_args a) {
   auto const& im = a.im;
   auto const& s = a.st;
   auto const k = static_cast<std::uint32_t>(a.k);
@@ -796,8 +813,8 @@ __global__ void __launch_bounds__(256, 5) k_route_scan(round_args a) {
     for (std::uint32_t item = (blockIdx.x * blockDim.x + threadIdx.x) / 64U; item < n_items;
          item += gridDim.x * kWavesPerBlock) {
       auto const info = s.packed_info[item];
-      auto const r = static_cast<std::uint32_t>(info >> 32);
-      auto const n_lanes = static_cast<std::uint32_t>(info);
+      auto const r = uni(static_cast<std::uint32_t>(info >> 32));
+      auto const n_lanes = uni(static_cast<std::uint32_t>(info));
       lane_query q{0U, 0U, false};
       if (lane < n_lanes) {
         auto const d = s.packed_lanes[static_cast<std::size_t>(item) * 64U + lane];

@@ -174,3 +174,21 @@ def test_gpu_work_counters_equal_oracle_counters():
         ref["transfer_stations"] += s["n_transfer_stations"]
     for k, v in ref.items():
         assert w[k] == v, (k, w[k], v)
+
+
+def test_gpu_batched_engine_rejects_routes_beyond_its_lds_staging():
+    # a route of 300 transports: the plain engine runs it, the batched
+    # engine refuses it when it is created (no kernel is launched)
+    from ttbuilder import Builder
+    b = Builder(20000, 3, 3)
+    day = 20001
+    b.add_line([9, 10, 11], [([i * 4, i * 4 + 5, i * 4 + 6, i * 4 + 12], [day]) for i in range(300)])
+    tt = b.build()
+    img = na.Image(tt)
+    say("batched engine on a 300-transport route")
+    with pytest.raises(na.NigiriError) as ex:
+        na.Engine(img, lanes=64, groups=1, max_k=8)
+    assert ex.value.code == na.EUNSUPPORTED
+    plain = na.Engine(img, lanes=1, groups=1, max_k=8)
+    qs = na.Queries.build([dict(start=[(9, 0)], dest=[(11, 0)], time=day * 1440 + 100)])
+    compare(tt, plain, qs)

@@ -15,6 +15,8 @@
 #include <atomic>
 #include <chrono>
 #include <climits>
+#include <condition_variable>
+#include <exception>
 #include <cstring>
 #include <deque>
 #include <functional>
@@ -120,8 +122,12 @@ struct nigiri_amd_engine {
   std::int32_t* d_lb_row = nullptr;
   std::uint32_t* d_lb_groups = nullptr;
   std::uint32_t lb_rows_cap = 64U;
-  // device lower bounds (batched engine): up to lbq_chunk queries per
+  // device lower bounds (batched engine): a second stream computes them
+  // ahead of admission into a pool of S rows; up to lbq_chunk queries per
   // launch, one workgroup each, with [lbq_chunk][L] scratch arrays
+  hipStream_t lb_stream = nullptr;
+  std::uint16_t* d_lb_pool = nullptr;
+  std::uint32_t* d_lbq_out_row = nullptr;
   std::uint32_t lbq_chunk = 0U;
   std::uint32_t* d_lbq_dist = nullptr;
   std::uint32_t* d_lbq_stamp_near = nullptr;
@@ -130,7 +136,6 @@ struct nigiri_amd_engine {
   std::uint32_t* d_lbq_near1 = nullptr;
   std::uint32_t* d_lbq_far0 = nullptr;
   std::uint32_t* d_lbq_far1 = nullptr;
-  std::uint16_t* d_lbq_rows = nullptr;
   std::uint32_t* d_seed_off = nullptr;
   std::uint32_t* d_seed_loc = nullptr;
   std::uint16_t* d_seed_dur = nullptr;
@@ -166,6 +171,7 @@ struct nigiri_amd_engine {
     for (auto ev : events) (void)hipEventDestroy(ev);
     for (auto* p : grow_ptrs) (void)hipFree(p);
     if (stream != nullptr) (void)hipStreamDestroy(stream);
+    if (lb_stream != nullptr) (void)hipStreamDestroy(lb_stream);
   }
 
   template <typename T>
@@ -263,6 +269,165 @@ struct slot_info {
   bool is_new = false;
 };
 
+// Lower bounds of queued queries (batched engine), computed ahead of
+// admission by a worker thread on the engine's second stream: queries are
+// submitted once prepared, become ready when their row of the pool holds
+// their bounds, and give the row back after admission has transposed it.
+static_assert(kMaxQueryDests == kMaxDestBits && kMaxQueryEgress == kMaxDist,
+              "query limits of the host preparation and the device state agree");
+
+struct lb_pipeline {
+  nigiri_amd_engine& e;
+  bool fwd;
+  std::vector<prepared_query>& pq;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<std::int32_t> waiting, ready;
+  std::vector<std::uint32_t> free_rows;
+  std::size_t in_flight = 0U;
+  std::uint32_t n_finished = 0U;
+  bool stop = false;
+  std::exception_ptr err;
+  std::thread worker;
+
+  lb_pipeline(nigiri_amd_engine& eng, bool f, std::vector<prepared_query>& p) : e{eng}, fwd{f}, pq{p} {
+    for (std::uint32_t r = 0; r != e.S; ++r) free_rows.push_back(e.S - 1U - r);
+    worker = std::thread{[this]() { run(); }};
+  }
+  ~lb_pipeline() {
+    {
+      std::lock_guard<std::mutex> lock{mu};
+      stop = true;
+    }
+    cv.notify_all();
+    worker.join();
+  }
+  void submit(std::int32_t q) {
+    {
+      std::lock_guard<std::mutex> lock{mu};
+      waiting.push_back(q);
+    }
+    cv.notify_all();
+  }
+  void check() {
+    if (err) std::rethrow_exception(err);
+  }
+  std::int32_t pop_ready() {
+    std::lock_guard<std::mutex> lock{mu};
+    check();
+    if (ready.empty()) return -1;
+    auto const q = ready.front();
+    ready.pop_front();
+    return q;
+  }
+  std::size_t n_ready() {
+    std::lock_guard<std::mutex> lock{mu};
+    check();
+    return ready.size();
+  }
+  std::uint32_t finished() {
+    std::lock_guard<std::mutex> lock{mu};
+    check();
+    return n_finished;
+  }
+  // blocks until `want` queries are ready or nothing more is coming
+  void wait_ready(std::size_t want) {
+    std::unique_lock<std::mutex> lock{mu};
+    cv.wait(lock, [&]() {
+      return err || ready.size() >= want || (waiting.empty() && in_flight == 0U);
+    });
+    check();
+  }
+  void release(std::uint32_t row) {
+    {
+      std::lock_guard<std::mutex> lock{mu};
+      free_rows.push_back(row);
+    }
+    cv.notify_all();
+  }
+
+  void run() {
+    try {
+      HIP_CHECK(hipSetDevice(e.img->device));
+      while (true) {
+        std::vector<std::int32_t> batch;
+        std::vector<std::uint32_t> rows;
+        {
+          std::unique_lock<std::mutex> lock{mu};
+          cv.wait(lock, [&]() { return stop || (!waiting.empty() && !free_rows.empty()); });
+          if (stop) return;
+          while (!waiting.empty() && !free_rows.empty() && batch.size() < e.lbq_chunk) {
+            auto const q = waiting.front();
+            waiting.pop_front();
+            auto const row = free_rows.back();
+            free_rows.pop_back();
+            pq[static_cast<std::size_t>(q)].lb_row_ = static_cast<std::int32_t>(row);
+            batch.push_back(q);
+            rows.push_back(row);
+          }
+          in_flight = batch.size();
+        }
+        compute(batch, rows);
+        {
+          std::lock_guard<std::mutex> lock{mu};
+          for (auto const q : batch) ready.push_back(q);
+          n_finished += static_cast<std::uint32_t>(batch.size());
+          in_flight = 0U;
+        }
+        cv.notify_all();
+      }
+    } catch (...) {
+      {
+        std::lock_guard<std::mutex> lock{mu};
+        err = std::current_exception();
+        in_flight = 0U;
+      }
+      cv.notify_all();
+    }
+  }
+
+  void compute(std::vector<std::int32_t> const& batch, std::vector<std::uint32_t> const& rows) {
+    auto const stream = e.lb_stream;
+    std::vector<std::uint32_t> seed_off{0U}, seed_loc;
+    std::vector<std::uint16_t> seed_dur;
+    for (auto const q : batch) {
+      for (auto const& [l, d] : pq[static_cast<std::size_t>(q)].lb_seeds_) {
+        seed_loc.push_back(l);
+        seed_dur.push_back(d);
+      }
+      seed_off.push_back(static_cast<std::uint32_t>(seed_loc.size()));
+    }
+    if (seed_loc.size() > e.seed_cap) {
+      throw status_error{NIGIRI_AMD_EUNSUPPORTED, "too many destination offsets"};
+    }
+    HIP_CHECK(hipMemcpyAsync(e.d_seed_off, seed_off.data(), 4U * seed_off.size(), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(e.d_lbq_out_row, rows.data(), 4U * rows.size(), hipMemcpyHostToDevice, stream));
+    if (!seed_loc.empty()) {
+      HIP_CHECK(hipMemcpyAsync(e.d_seed_loc, seed_loc.data(), 4U * seed_loc.size(), hipMemcpyHostToDevice, stream));
+      HIP_CHECK(hipMemcpyAsync(e.d_seed_dur, seed_dur.data(), 2U * seed_dur.size(), hipMemcpyHostToDevice, stream));
+    }
+    lb_args q{};
+    q.im = e.img->d;
+    q.L = e.st.L;
+    q.n_q = static_cast<std::uint32_t>(batch.size());
+    q.seed_off = e.d_seed_off;
+    q.seed_loc = e.d_seed_loc;
+    q.seed_dur = e.d_seed_dur;
+    q.dist = e.d_lbq_dist;
+    q.stamp_near = e.d_lbq_stamp_near;
+    q.stamp_far = e.d_lbq_stamp_far;
+    q.near0 = e.d_lbq_near0;
+    q.near1 = e.d_lbq_near1;
+    q.far0 = e.d_lbq_far0;
+    q.far1 = e.d_lbq_far1;
+    q.out_row = e.d_lbq_out_row;
+    q.rows = e.d_lb_pool;
+    launch_lb_queries(fwd, q, stream);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(stream));
+  }
+};
+
 struct runner {
   nigiri_amd_engine& e;
   timetable const& tt;
@@ -270,6 +435,7 @@ struct runner {
   std::vector<prepared_query>& pq;
   std::function<std::int32_t()> next;            // next query id or -1
   std::function<void(std::int32_t)> on_done;      // query finished
+  std::function<void(std::uint32_t)> release_row; // lower-bound row consumed
   std::vector<slot_info> slots;
   std::uint64_t rounds = 0U, epochs = 0U;
 
@@ -338,6 +504,10 @@ struct runner {
       if (q < 0) break;
       auto& p = pq[static_cast<std::size_t>(q)];
       if (p.epochs_.empty()) {  // overlap or no start labels
+        if (p.lb_row_ >= 0 && release_row) {
+          release_row(static_cast<std::uint32_t>(p.lb_row_));
+          p.lb_row_ = -1;
+        }
         finish(p);
         on_done(q);
         continue;
@@ -471,7 +641,7 @@ struct runner {
     // lower bounds of new slots: computed on the device by the batched
     // engine, uploaded from the host preparation by the plain engine
     if (W == 64U) {
-      device_lbs(new_slots);
+      scatter_pool_rows(new_slots);
     } else {
       upload_lbs(new_slots, a);
     }
@@ -539,74 +709,36 @@ struct runner {
     return n;
   }
 
-  // Lower bounds of the new slots on the device, lbq_chunk queries at a
-  // time: one workgroup per query writes a row, the rows are transposed
-  // into the lane-interleaved lb columns.
-  void device_lbs(std::vector<std::uint32_t> const& new_slots) {
+  // Lower bounds of the new slots (batched engine): their rows were
+  // computed ahead of admission (lb_pipeline); transpose them into the
+  // lane-interleaved lb columns, then hand the rows back.
+  void scatter_pool_rows(std::vector<std::uint32_t> const& new_slots) {
     if (new_slots.empty()) return;
     auto const stream = e.stream;
-    auto const L = e.st.L;
     std::vector<std::int32_t> row_of(e.S, -1);
-    for (std::size_t from = 0; from < new_slots.size(); from += e.lbq_chunk) {
-      auto const to = std::min<std::size_t>(new_slots.size(), from + e.lbq_chunk);
-      std::vector<std::uint32_t> chunk(new_slots.begin() + static_cast<std::ptrdiff_t>(from),
-                                       new_slots.begin() + static_cast<std::ptrdiff_t>(to));
-      std::vector<std::uint32_t> seed_off{0U}, seed_loc, groups;
-      std::vector<std::uint16_t> seed_dur;
-      std::fill(row_of.begin(), row_of.end(), -1);
-      for (std::size_t j = 0; j != chunk.size(); ++j) {
-        auto const s = chunk[j];
-        for (auto const& [l, d] : pq[static_cast<std::size_t>(slots[s].q)].lb_seeds_) {
-          seed_loc.push_back(l);
-          seed_dur.push_back(d);
-        }
-        seed_off.push_back(static_cast<std::uint32_t>(seed_loc.size()));
-        row_of[s] = static_cast<std::int32_t>(j);
-        auto const g = s / e.W;
-        if (groups.empty() || groups.back() != g) groups.push_back(g);
-      }
-      if (seed_loc.size() > e.seed_cap) {
-        e.seed_cap = seed_loc.size() * 2U;
-        e.d_seed_loc = e.grow(e.d_seed_loc, e.seed_cap);
-        e.d_seed_dur = e.grow(e.d_seed_dur, e.seed_cap);
-      }
-      HIP_CHECK(hipMemcpyAsync(e.d_seed_off, seed_off.data(), 4U * seed_off.size(), hipMemcpyHostToDevice, stream));
-      if (!seed_loc.empty()) {
-        HIP_CHECK(hipMemcpyAsync(e.d_seed_loc, seed_loc.data(), 4U * seed_loc.size(), hipMemcpyHostToDevice, stream));
-        HIP_CHECK(hipMemcpyAsync(e.d_seed_dur, seed_dur.data(), 2U * seed_dur.size(), hipMemcpyHostToDevice, stream));
-      }
-      lb_args q{};
-      q.im = e.img->d;
-      q.L = L;
-      q.n_q = static_cast<std::uint32_t>(chunk.size());
-      q.seed_off = e.d_seed_off;
-      q.seed_loc = e.d_seed_loc;
-      q.seed_dur = e.d_seed_dur;
-      q.dist = e.d_lbq_dist;
-      q.stamp_near = e.d_lbq_stamp_near;
-      q.stamp_far = e.d_lbq_stamp_far;
-      q.near0 = e.d_lbq_near0;
-      q.near1 = e.d_lbq_near1;
-      q.far0 = e.d_lbq_far0;
-      q.far1 = e.d_lbq_far1;
-      q.rows = e.d_lbq_rows;
-      launch_lb_queries(fwd, q, stream);
-      HIP_CHECK(hipGetLastError());
-      HIP_CHECK(hipMemcpyAsync(e.d_lb_slot, chunk.data(), 4U * chunk.size(), hipMemcpyHostToDevice, stream));
-      HIP_CHECK(hipMemcpyAsync(e.d_lb_row, row_of.data(), 4U * e.S, hipMemcpyHostToDevice, stream));
-      HIP_CHECK(hipMemcpyAsync(e.d_lb_groups, groups.data(), 4U * groups.size(), hipMemcpyHostToDevice, stream));
-      epoch_args a{};
-      a.st = e.st;
-      a.lb_staging = e.d_lbq_rows;
-      a.lb_slot = e.d_lb_slot;
-      a.n_lb = static_cast<std::uint32_t>(chunk.size());
-      a.lb_row_of_slot = e.d_lb_row;
-      a.lb_groups = e.d_lb_groups;
-      a.n_lb_groups = static_cast<std::uint32_t>(groups.size());
-      launch_lb_scatter(e.W, a, stream);
-      HIP_CHECK(hipGetLastError());
-      HIP_CHECK(hipStreamSynchronize(stream));  // host vectors go out of scope
+    std::vector<std::uint32_t> groups, rows;
+    for (auto const s : new_slots) {
+      auto const row = pq[static_cast<std::size_t>(slots[s].q)].lb_row_;
+      if (row < 0) throw status_error{NIGIRI_AMD_EDEVICE, "query admitted without lower bounds"};
+      row_of[s] = row;
+      rows.push_back(static_cast<std::uint32_t>(row));
+      auto const g = s / e.W;
+      if (groups.empty() || groups.back() != g) groups.push_back(g);
     }
+    HIP_CHECK(hipMemcpyAsync(e.d_lb_row, row_of.data(), 4U * e.S, hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(e.d_lb_groups, groups.data(), 4U * groups.size(), hipMemcpyHostToDevice, stream));
+    epoch_args a{};
+    a.st = e.st;
+    a.lb_staging = e.d_lb_pool;
+    a.n_lb = static_cast<std::uint32_t>(new_slots.size());
+    a.lb_row_of_slot = e.d_lb_row;
+    a.lb_groups = e.d_lb_groups;
+    a.n_lb_groups = static_cast<std::uint32_t>(groups.size());
+    launch_lb_scatter(e.W, a, stream);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(stream));  // rows are free again, host vectors go out of scope
+    for (auto const s : new_slots) pq[static_cast<std::size_t>(slots[s].q)].lb_row_ = -1;
+    for (auto const row : rows) release_row(row);
   }
 
   void upload_lbs(std::vector<std::uint32_t> const& new_slots, epoch_args& a) {
@@ -724,6 +856,30 @@ void engine_run_dir(nigiri_amd_engine& e, bool fwd, std::vector<prepared_query>&
   r.on_done = std::move(done);
   while (r.step()) {
     if (after_step) after_step();
+  }
+}
+
+// Batched engine, all queries known: lower bounds are computed ahead of
+// admission; each epoch starts once the free slots can be filled (or no
+// more queries are coming).
+void run_batched(nigiri_amd_engine& e, bool fwd, std::vector<prepared_query>& pq,
+                 std::vector<std::int32_t> const& queue) {
+  lb_pipeline pipe{e, fwd, pq};
+  for (auto const q : queue) pipe.submit(q);
+  runner r{e, *e.img->tt, fwd, pq};
+  std::size_t popped = 0U, done = 0U;
+  r.next = [&]() {
+    auto const q = pipe.pop_ready();
+    if (q >= 0) ++popped;
+    return q;
+  };
+  r.on_done = [&](std::int32_t) { ++done; };
+  r.release_row = [&](std::uint32_t row) { pipe.release(row); };
+  while (done < queue.size()) {
+    pipe.wait_ready(std::min<std::size_t>(r.free_slots(), queue.size() - popped));
+    if (!r.step() && done < queue.size()) {
+      throw status_error{NIGIRI_AMD_EDEVICE, "batched search made no progress"};
+    }
   }
 }
 
@@ -914,9 +1070,13 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
       e->d_lbq_near1 = m.alloc<std::uint32_t>(n);
       e->d_lbq_far0 = m.alloc<std::uint32_t>(n);
       e->d_lbq_far1 = m.alloc<std::uint32_t>(n);
-      e->d_lbq_rows = m.alloc<std::uint16_t>(n);
+      e->d_lb_pool = m.alloc<std::uint16_t>(static_cast<std::size_t>(S) * L);
+      e->d_lbq_out_row = m.alloc<std::uint32_t>(e->lbq_chunk);
       e->d_seed_off = m.alloc<std::uint32_t>(static_cast<std::size_t>(e->lbq_chunk) + 1U);
-      e->d_lb_slot = m.alloc<std::uint32_t>(e->lbq_chunk);
+      e->seed_cap = static_cast<std::size_t>(e->lbq_chunk) * (kMaxDist + kMaxDestBits);
+      e->d_seed_loc = m.alloc<std::uint32_t>(e->seed_cap);
+      e->d_seed_dur = m.alloc<std::uint16_t>(e->seed_cap);
+      HIP_CHECK(hipStreamCreateWithFlags(&e->lb_stream, hipStreamNonBlocking));
     }
     e->h_lb.reserve(static_cast<std::size_t>(e->lb_rows_cap) * L);
     e->h_active.reserve(G);
@@ -999,6 +1159,10 @@ int nigiri_amd_search(nigiri_amd_engine* e, const nigiri_query* qs, uint32_t n, 
         }
       }
       if (queue.empty()) continue;
+      if (e->W == 64U) {
+        run_batched(*e, dir == 0, pq, queue);
+        continue;
+      }
       std::size_t head = 0U;
       engine_run_dir(
           *e, dir == 0, pq,
@@ -1052,6 +1216,11 @@ int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs, const double*
     std::atomic<std::uint32_t> next_prep{0U};
     std::atomic<std::uint32_t> n_done{0U};
     std::atomic<std::uint32_t> n_prepared_ctr{0U};
+    std::atomic<std::uint32_t> n_failed{0U};
+    bool const fwd = qs[0].dir == NIGIRI_AMD_DIR_FWD;
+    // batched engine: lower bounds are computed ahead of admission
+    std::unique_ptr<lb_pipeline> pipe;
+    if (e->W == 64U) pipe = std::make_unique<lb_pipeline>(*e, fwd, pq);
     auto const t0 = clock::now();
     auto const ms_since = [&](clock::time_point t) {
       return std::chrono::duration<double, std::milli>(t - t0).count();
@@ -1078,22 +1247,31 @@ int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs, const double*
           if (status[i] != NIGIRI_AMD_OK) {
             latency[i] = ms_since(clock::now()) - arrival_ms[i];
             n_done.fetch_add(1U);
+            n_failed.fetch_add(1U);
             n_prepared_ctr.fetch_add(1U);
             continue;
           }
-          std::lock_guard<std::mutex> lock{mu};
-          ready.push_back(static_cast<std::int32_t>(i));
+          if (pipe) {
+            pipe->submit(static_cast<std::int32_t>(i));
+          } else {
+            std::lock_guard<std::mutex> lock{mu};
+            ready.push_back(static_cast<std::int32_t>(i));
+          }
           n_prepared_ctr.fetch_add(1U);
         }
       });
     }
-    runner r{*e, tt, qs[0].dir == NIGIRI_AMD_DIR_FWD, pq};
+    runner r{*e, tt, fwd, pq};
     r.next = [&]() -> std::int32_t {
+      if (pipe) return pipe->pop_ready();
       std::lock_guard<std::mutex> lock{mu};
       if (ready.empty()) return -1;
       auto const q = ready.front();
       ready.pop_front();
       return q;
+    };
+    r.release_row = [&](std::uint32_t row) {
+      if (pipe) pipe->release(row);
     };
     r.on_done = [&](std::int32_t q) {
       latency[static_cast<std::size_t>(q)] = ms_since(clock::now()) - arrival_ms[q];
@@ -1109,14 +1287,19 @@ int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs, const double*
                                                   std::chrono::duration<double, std::milli>(e->admission_wait_ms));
         while (clock::now() < deadline) {
           std::size_t n_ready = 0U;
-          {
+          std::uint32_t n_settled = 0U;  // arrived queries that cannot get more ready
+          if (pipe) {
+            n_ready = pipe->n_ready();
+            n_settled = n_failed.load() + pipe->finished();
+          } else {
             std::lock_guard<std::mutex> lock{mu};
             n_ready = ready.size();
+            n_settled = n_prepared.load();
           }
           auto const now_ms = ms_since(clock::now());
           auto const arrived = static_cast<std::uint32_t>(
               std::upper_bound(arrival_ms, arrival_ms + n, now_ms) - arrival_ms);
-          if (n_ready >= free || n_prepared.load() >= arrived) break;
+          if (n_ready >= free || n_settled >= arrived) break;
           std::this_thread::sleep_for(std::chrono::microseconds(20));
         }
         if (!r.step()) {

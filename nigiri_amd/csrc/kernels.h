@@ -47,7 +47,8 @@ struct round_args {
 // offsets on the lower-bound graph with the 7201-minute cut-off of nigiri's
 // dial-bucket Dijkstra (dijkstra.cc:18-67), one workgroup per query.  Query
 // q's seeds are seed_loc/seed_dur[seed_off[q] .. seed_off[q+1]); its result
-// row is rows[q][L] (0xFFFF = unreachable).  Scratch arrays are [n_q][L].
+// row is rows[out_row[q]][L] (0xFFFF = unreachable).  Scratch arrays are
+// [n_q][L].
 struct lb_args {
   dev_image im;
   std::uint32_t L;
@@ -62,8 +63,11 @@ struct lb_args {
   std::uint32_t* near1;
   std::uint32_t* far0;
   std::uint32_t* far1;
+  std::uint32_t const* out_row;
   std::uint16_t* rows;
 };
+
+// One workgroup per query, all sweeps in one launch.
 void launch_lb_queries(bool fwd, lb_args const& a, hipStream_t stream);
 
 // The batched route scan stages the events of at least one stop in LDS: it

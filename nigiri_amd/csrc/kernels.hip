@@ -1,6 +1,8 @@
 // gfx950 round kernels (see kernels.h for the round structure).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace nigiri_amd {
@@ -1326,7 +1328,34 @@ __global__ void k_lane_reset(epoch_args a) {
 constexpr std::uint32_t kLbCut = 7201U;
 constexpr std::uint32_t kLbInf = 0xFFFFFFFFU;
 constexpr std::uint32_t kLbDelta = 8U;
-constexpr int kLbBlock = 256;
+constexpr int kLbBlock = 1024;
+
+// Relaxes the lower-bound edges of u (tentative distance du < kLbCut): the
+// edge words and the targets' distances are loaded 8 at a time so that
+// their latencies overlap; push(t, nd) is called for every improvement.
+template <typename Push>
+__device__ inline void lb_relax_node(std::uint32_t u, std::uint32_t du, std::uint32_t const* off,
+                                     std::uint32_t const* edges, std::uint32_t* dist, Push&& push) {
+  auto const e0 = off[u], e1 = off[u + 1U];
+  for (auto b = e0; b < e1; b += 8U) {
+    std::uint32_t ed[8], dt[8];
+#pragma unroll
+    for (std::uint32_t j = 0U; j < 8U; ++j) ed[j] = b + j < e1 ? edges[b + j] : 0U;
+#pragma unroll
+    for (std::uint32_t j = 0U; j < 8U; ++j) dt[j] = b + j < e1 ? dist[ed[j] & ((1U << 23) - 1U)] : 0U;
+#pragma unroll
+    for (std::uint32_t j = 0U; j < 8U; ++j) {
+      if (b + j < e1) {
+        auto const t = ed[j] & ((1U << 23) - 1U);
+        auto const nd = du + (ed[j] >> 23);
+        if (nd < kLbCut && nd < dt[j]) {
+          auto const old = atomicMin(&dist[t], nd);
+          if (nd < old) push(t, nd);
+        }
+      }
+    }
+  }
+}
 
 template <bool Fwd>
 __global__ void __launch_bounds__(kLbBlock) k_lb_query(lb_args a) {
@@ -1422,22 +1451,13 @@ __global__ void __launch_bounds__(kLbBlock) k_lb_query(lb_args a) {
     __syncthreads();
     for (auto i = tid; i < n_near; i += kLbBlock) {
       auto const u = near[i];
-      auto const du = dist[u];
-      for (auto ei = off[u]; ei != off[u + 1U]; ++ei) {
-        auto const ed = edges[ei];
-        auto const t = ed & ((1U << 23) - 1U);
-        auto const nd = du + (ed >> 23);
-        if (nd < kLbCut && nd < dist[t]) {
-          auto const old = atomicMin(&dist[t], nd);
-          if (nd < old) {
-            if (static_cast<std::int64_t>(nd) <= T) {
-              if (atomicExch(&stamp_near[t], sweep) != sweep) near_next[atomicAdd(&s_near, 1U)] = t;
-            } else if (atomicExch(&stamp_far[t], phase) != phase) {
-              far[atomicAdd(&s_far, 1U)] = t;
-            }
-          }
+      lb_relax_node(u, dist[u], off, edges, dist, [&](std::uint32_t t, std::uint32_t nd) {
+        if (static_cast<std::int64_t>(nd) <= T) {
+          if (atomicExch(&stamp_near[t], sweep) != sweep) near_next[atomicAdd(&s_near, 1U)] = t;
+        } else if (atomicExch(&stamp_far[t], phase) != phase) {
+          far[atomicAdd(&s_far, 1U)] = t;
         }
-      }
+      });
     }
     __syncthreads();
     n_near = s_near;
@@ -1447,9 +1467,10 @@ __global__ void __launch_bounds__(kLbBlock) k_lb_query(lb_args a) {
     near_next = tn;
     __syncthreads();
   }
+  auto* row = a.rows + static_cast<std::size_t>(a.out_row[q]) * L;
   for (auto l = tid; l < L; l += kLbBlock) {
     auto const d = dist[l];
-    a.rows[base + l] = d < kLbCut ? static_cast<std::uint16_t>(d) : kUnreachableLb;
+    row[l] = d < kLbCut ? static_cast<std::uint16_t>(d) : kUnreachableLb;
   }
 }
 

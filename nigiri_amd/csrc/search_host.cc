@@ -249,6 +249,10 @@ void prepare(timetable const& tt, nigiri_query const& q, prepared_query& p,
       p.dests_.push_back(l);
     }
   }
+  if ((dest_im && p.dist_loc_.size() > kMaxQueryEgress) ||
+      (!dest_im && p.dests_.size() > kMaxQueryDests)) {
+    throw status_error{NIGIRI_AMD_EUNSUPPORTED, "too many destination offsets"};
+  }
   p.lb_seeds_ = dest_min;
   if (host_lb) {
     lower_bounds(tt, fwd, dest_min, p.lb_, s);

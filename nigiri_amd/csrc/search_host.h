@@ -13,6 +13,11 @@
 
 namespace nigiri_amd {
 
+// Per-query limits of the device engines: destination stations of a
+// non-intermodal query, egress offsets of an intermodal one.
+constexpr std::size_t kMaxQueryDests = 16U;
+constexpr std::size_t kMaxQueryEgress = 64U;
+
 struct host_tts {
   bool default_{true};
   std::int16_t min_transfer_time_{0}, additional_time_{0};
@@ -67,6 +72,7 @@ struct prepared_query {
   std::vector<epoch> epochs_;
   std::uint64_t locality_key_{0U};  // Morton code of the first start stop
   std::int32_t slot_{-1};           // engine slot that ran the query
+  std::int32_t lb_row_{-1};         // device lower-bound row (batched engine)
   std::vector<host_journey> results_;  // running Pareto set
 
   std::int32_t base_minutes(timetable const& tt) const {

@@ -91,6 +91,16 @@ def test_gpu_point_queries_workload1(lanes):
     compare(tt, eng, qs)
 
 
+def test_gpu_large_batch_workload1():
+    # a batch larger than the multi-workgroup lower-bound sweep takes: its
+    # lower bounds come from the one-workgroup-per-query kernel
+    tt, img = image_for("w1", lambda: na.Timetable.generate(1))
+    eng = na.Engine(img, lanes=64, groups=4, max_k=16)
+    qs = na.Queries.generate(tt, 1, 256, seed=0x7172 + 11)
+    say("workload1 large batch n=256")
+    compare(tt, eng, qs)
+
+
 def test_gpu_backward_point_queries_workload1():
     tt, img = image_for("w1", lambda: na.Timetable.generate(1))
     eng = na.Engine(img, lanes=64, groups=1, max_k=16)

@@ -60,6 +60,58 @@ def committed_traffic(workload, scale, batch):
     return None
 
 
+def serve_main(args):
+    """Serving: queries of the workload's mix arrive as a Poisson process;
+    reports latency percentiles (arrival -> completion) per query kind and
+    the single-query latency of the plain and the batched engine."""
+    import nigiri_amd as na
+    cfg = args.workload
+    tt = na.Timetable.generate(cfg, scale=args.scale)
+    qs = na.Queries.generate(tt, cfg, args.queries)
+    n = len(qs)
+    img = na.Image(tt)
+    max_k = 16 if cfg in (1, 2) else 10
+    eng = na.Engine(img, lanes=64, groups=args.groups, max_k=max_k)
+    eng.configure(admission_wait_ms=1.0)
+    kinds = np.array(["intermodal" if qs[i].dest_match_mode == na.MATCH_INTERMODAL
+                      else ("range" if qs[i].is_interval else "point") for i in range(n)])
+    eng.serve(qs.subset(list(range(min(64, n)))), np.zeros(min(64, n)))  # warm-up
+    print("[bench] serving warm-up done", file=sys.stderr, flush=True)
+    arrivals = np.cumsum(np.random.default_rng(7).exponential(1000.0 / args.serve, n))
+    eng.stats(reset=True)
+    res, wall = eng.serve(qs, arrivals)
+    lat = np.array([r["latency_ms"] for r in res])
+    st = eng.stats()
+
+    def pct(a, p):
+        return round(float(np.percentile(a, p)), 2) if len(a) else None
+
+    kinds_out = {}
+    for k in ("point", "range", "intermodal"):
+        m = kinds == k
+        kinds_out[k] = {"n": int(m.sum()), "p50_ms": pct(lat[m], 50), "p99_ms": pct(lat[m], 99)}
+    plain = na.Engine(img, lanes=1, groups=1, max_k=max_k)
+    pts = [i for i in range(n) if kinds[i] == "point"][:6]
+    single_plain, single_batched = [], []
+    for i in pts:
+        t = time.perf_counter()
+        plain.search(qs.subset([i]))
+        single_plain.append((time.perf_counter() - t) * 1e3)
+        t = time.perf_counter()
+        eng.search(qs.subset([i]))
+        single_batched.append((time.perf_counter() - t) * 1e3)
+    print(json.dumps({
+        "metric": "serving latency (arrival -> completion), Poisson arrivals, 1 MI355X",
+        "workload": f"config {cfg}: " + WORKLOADS[cfg], "scale": args.scale,
+        "offered_qps": args.serve, "queries": n, "slots": 64 * args.groups,
+        "wall_ms": round(wall, 1), "completed_qps": round(n / (wall / 1e3), 1),
+        "p50_ms": pct(lat, 50), "p90_ms": pct(lat, 90), "p99_ms": pct(lat, 99),
+        "by_kind": kinds_out, "epochs": st["epochs"], "rounds": st["rounds"],
+        "single_point_ms": {"plain": round(float(np.median(single_plain)), 1),
+                            "batched": round(float(np.median(single_batched)), 1)},
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +128,10 @@ def main():
     ap.add_argument("--admission-wait-ms", type=float, default=2000.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 512 MiB (scales rocprof FETCH/WRITE_SIZE)")
+    ap.add_argument("--serve", type=float, default=0.0, metavar="RATE",
+                    help="serving measurement instead: Poisson arrivals at RATE queries/s "
+                         "of the workload's query mix through the continuous-batching loop")
+    ap.add_argument("--groups", type=int, default=16, help="64-query groups (serving mode)")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -91,6 +147,10 @@ def main():
               "(queries shard across GPUs without communication; see the design notes)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.serve > 0.0:
+        serve_main(args)
+        return
 
     import nigiri_amd as na
     cfg = args.workload

@@ -124,6 +124,7 @@ struct dev_state {
   std::size_t cap_t, cap_r, cap_route, cap_hz;
   // per slot
   slot_param* param;
+  std::uint8_t* kslot;     // [S] round of the slot's running epoch
   std::int16_t* tad;       // [S][16]
   std::int16_t* tad0;      // [S]
   std::uint32_t* n_chg;    // [S]

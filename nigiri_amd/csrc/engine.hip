@@ -1,14 +1,14 @@
 // Engine host code: device timetable image, device query state, the
-// epoch-synchronous batch scheduler and the C ABI entry points that use the
-// GPU.
+// round-step scheduler and the C ABI entry points that use the GPU.
 //
-// Scheduling model: all running slots start a start-time epoch together
-// (next_start_time + add_start, raptor.h:179-203), then run rounds
-// k = 1, 2, ... in lock step (each slot stops at its own break or end_k),
-// then the host extracts the destination round times of every slot and
-// updates its Pareto set.  At every epoch boundary a slot either continues
-// its range query with the next start time or takes the next queued query
-// (iteration-level continuous batching).
+// Scheduling model: a step runs one RAPTOR round (raptor.h:218-387) for
+// every running slot, each at its own round index k.  A slot whose
+// start-time epoch ends (its break or end_k) hands its destination round
+// times to the host, which updates the query's Pareto set; at the next
+// step the slot starts its next epoch (next_start_time + add_start,
+// raptor.h:179-203) or, when the query is done, takes the next queued
+// query (iteration-level continuous batching: slots never wait for other
+// slots' epochs).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -117,6 +117,7 @@ struct nigiri_amd_engine {
   std::uint8_t* d_flags = nullptr;
   std::int16_t* d_worst = nullptr;
   std::uint32_t* d_sweep_groups = nullptr;
+  std::uint32_t* d_slot_list = nullptr;  // [S] slots whose epoch ended
   std::uint16_t* d_lb_staging = nullptr;
   std::uint32_t* d_lb_slot = nullptr;
   std::int32_t* d_lb_row = nullptr;
@@ -442,93 +443,43 @@ struct runner {
   runner(nigiri_amd_engine& eng, timetable const& t, bool f, std::vector<prepared_query>& p)
       : e{eng}, tt{t}, fwd{f}, pq{p}, slots(eng.S) {}
 
-  round_args rargs(std::int32_t k) const {
+  round_args rargs() const {
     round_args a{};
     a.im = e.img->d;
     a.st = e.st;
-    a.k = k;
     a.invalid = fwd ? SHRT_MAX : SHRT_MIN;
-    if (e.profile && static_cast<std::size_t>(4 * k + 3) < e.events.size()) {
-      for (int i = 0; i != 4; ++i) a.ev[i] = e.events[static_cast<std::size_t>(4 * k + i)];
+    if (e.profile && e.events.size() >= 4U) {
+      for (int i = 0; i != 4; ++i) a.ev[i] = e.events[static_cast<std::size_t>(i)];
     }
     return a;
   }
 
-  void collect_timings(std::uint32_t n_rounds_run) {
-    if (!e.profile) return;
-    for (std::uint32_t k = 1U; k <= n_rounds_run; ++k) {
-      float ms = 0.0F;
-      if (hipEventElapsedTime(&ms, e.events[4U * k], e.events[4U * k + 1U]) == hipSuccess) {
-        e.stats.route_scan_ms += ms;
-        ++e.stats.route_scan_launches;
-      }
-      if (hipEventElapsedTime(&ms, e.events[4U * k + 2U], e.events[4U * k + 3U]) == hipSuccess) {
-        e.stats.transfer_ms += ms;
-        ++e.stats.transfer_launches;
-      }
+  void collect_timings() {
+    if (!e.profile || e.events.size() < 4U) return;
+    float ms = 0.0F;
+    if (hipEventElapsedTime(&ms, e.events[0], e.events[1]) == hipSuccess) {
+      e.stats.route_scan_ms += ms;
+      ++e.stats.route_scan_launches;
+    }
+    if (hipEventElapsedTime(&ms, e.events[2], e.events[3]) == hipSuccess) {
+      e.stats.transfer_ms += ms;
+      ++e.stats.transfer_launches;
     }
   }
 
-  // one epoch step over all slots; returns false when nothing ran
-  bool step() {
+  // Epoch begin (raptor.h:174-203) for the listed slots only: lanes of
+  // other slots keep running.
+  void begin_epochs(std::vector<std::uint32_t> const& starting) {
     auto const S = e.S, W = e.W, G = e.G, K = e.K;
     std::vector<std::uint64_t> rt_mask(G, 0U), bt_mask(G, 0U);
     std::vector<std::uint8_t> flags(S, 0U);
     std::vector<std::int16_t> worst(S, 0);
-    std::vector<std::uint32_t> st_slot, st_loc;
+    std::vector<std::uint32_t> st_slot, st_loc, new_slots;
     std::vector<std::int16_t> st_delta;
     std::vector<std::uint64_t> dest_ops;
     std::vector<std::uint8_t> dest_set;
-    std::vector<std::uint32_t> new_slots;
-    bool any = false;
-    std::uint32_t max_end_k = 0U;
-    std::vector<std::uint8_t> sweep(G, 0U);
-    // continuing range queries keep their slots; free slots take queued
-    // queries, ordered by locality so that neighbouring lanes of a
-    // wavefront start close to each other
-    std::vector<std::uint32_t> free_list;
-    for (std::uint32_t s = 0; s != S; ++s) {
+    for (auto const s : starting) {
       auto& si = slots[s];
-      si.is_new = false;
-      si.running = false;
-      if (si.q >= 0 && si.epoch + 1U < pq[static_cast<std::size_t>(si.q)].epochs_.size()) {
-        ++si.epoch;
-      } else {
-        si.q = -1;
-        free_list.push_back(s);
-      }
-    }
-    std::vector<std::int32_t> admitted;
-    while (admitted.size() < free_list.size()) {
-      auto const q = next();
-      if (q < 0) break;
-      auto& p = pq[static_cast<std::size_t>(q)];
-      if (p.epochs_.empty()) {  // overlap or no start labels
-        if (p.lb_row_ >= 0 && release_row) {
-          release_row(static_cast<std::uint32_t>(p.lb_row_));
-          p.lb_row_ = -1;
-        }
-        finish(p);
-        on_done(q);
-        continue;
-      }
-      admitted.push_back(q);
-    }
-    std::stable_sort(admitted.begin(), admitted.end(), [&](std::int32_t a, std::int32_t b) {
-      return pq[static_cast<std::size_t>(a)].locality_key_ < pq[static_cast<std::size_t>(b)].locality_key_;
-    });
-    for (std::size_t j = 0; j != admitted.size(); ++j) {
-      auto& si = slots[free_list[j]];
-      si.q = admitted[j];
-      pq[static_cast<std::size_t>(admitted[j])].slot_ = static_cast<std::int32_t>(free_list[j]);
-      si.epoch = 0U;
-      si.is_new = true;
-    }
-    for (std::uint32_t s = 0; s != S; ++s) {
-      auto& si = slots[s];
-      if (si.q < 0) continue;
-      si.running = true;
-      any = true;
       auto& p = pq[static_cast<std::size_t>(si.q)];
       auto const g = s / W, lane = s % W;
       bt_mask[g] |= 1ULL << lane;
@@ -566,8 +517,6 @@ struct runner {
             e.slot_dest_bits[s].push_back(l);
           }
         }
-      } else {
-        sweep[g] = 1U;  // later epoch: round_times rows hold earlier labels
       }
       auto const& ep = p.epochs_[si.epoch];
       auto const base = p.base_minutes(tt);
@@ -578,19 +527,15 @@ struct runner {
         st_loc.push_back(ep.loc_[j]);
         st_delta.push_back(clamp16(static_cast<long long>(ep.time_[j]) - base));
       }
-      max_end_k = std::max<std::uint32_t>(max_end_k, std::min<std::uint32_t>(p.end_k_, K));
+      si.running = true;
+      ++e.stats.epochs;
     }
-    if (!any) return false;
-    ++epochs;
     auto const stream = e.stream;
     auto const& tr = e.st;
-
-    // parameters
     HIP_CHECK(hipMemcpyAsync(tr.param, e.h_param.data(), sizeof(slot_param) * S, hipMemcpyHostToDevice, stream));
     HIP_CHECK(hipMemcpyAsync(tr.dest, e.h_dest.data(), sizeof(std::uint32_t) * e.h_dest.size(), hipMemcpyHostToDevice, stream));
     HIP_CHECK(hipMemcpyAsync(tr.dist_loc, e.h_dist_loc.data(), sizeof(std::uint32_t) * e.h_dist_loc.size(), hipMemcpyHostToDevice, stream));
     HIP_CHECK(hipMemcpyAsync(tr.dist_min, e.h_dist_min.data(), sizeof(std::uint16_t) * e.h_dist_min.size(), hipMemcpyHostToDevice, stream));
-
     epoch_args a{};
     a.st = e.st;
     a.invalid = fwd ? SHRT_MAX : SHRT_MIN;
@@ -637,7 +582,6 @@ struct runner {
     a.slot_flags = e.d_flags;
     a.slot_worst = e.d_worst;
     a.n_slots = S;
-
     // lower bounds of new slots: computed on the device by the batched
     // engine, uploaded from the host preparation by the plain engine
     if (W == 64U) {
@@ -645,10 +589,75 @@ struct runner {
     } else {
       upload_lbs(new_slots, a);
     }
-
     launch_epoch_begin(fwd, W, a, stream);
     HIP_CHECK(hipGetLastError());
+    // the host vectors above go out of scope: wait for the copies
+    HIP_CHECK(hipStreamSynchronize(stream));
+  }
 
+  // One step: idle slots start the next epoch of their range query or take
+  // a new query, then every running slot advances one round (each at its
+  // own round index); slots whose epoch ended hand their destination rows
+  // to the host.  Returns false when no slot runs.
+  bool step() {
+    auto const S = e.S, W = e.W, G = e.G, K = e.K;
+    // continuing range queries start their next epoch; free slots take
+    // queued queries, ordered by locality so that neighbouring lanes start
+    // close to each other
+    std::vector<std::uint32_t> free_list, starting;
+    for (std::uint32_t s = 0; s != S; ++s) {
+      auto& si = slots[s];
+      si.is_new = false;
+      if (si.running) continue;
+      if (si.q >= 0 && si.epoch + 1U < pq[static_cast<std::size_t>(si.q)].epochs_.size()) {
+        ++si.epoch;
+        starting.push_back(s);
+      } else {
+        si.q = -1;
+        free_list.push_back(s);
+      }
+    }
+    std::vector<std::int32_t> admitted;
+    while (admitted.size() < free_list.size()) {
+      auto const q = next();
+      if (q < 0) break;
+      auto& p = pq[static_cast<std::size_t>(q)];
+      if (p.epochs_.empty()) {  // overlap or no start labels
+        if (p.lb_row_ >= 0 && release_row) {
+          release_row(static_cast<std::uint32_t>(p.lb_row_));
+          p.lb_row_ = -1;
+        }
+        finish(p);
+        on_done(q);
+        continue;
+      }
+      admitted.push_back(q);
+    }
+    std::stable_sort(admitted.begin(), admitted.end(), [&](std::int32_t a, std::int32_t b) {
+      return pq[static_cast<std::size_t>(a)].locality_key_ < pq[static_cast<std::size_t>(b)].locality_key_;
+    });
+    for (std::size_t j = 0; j != admitted.size(); ++j) {
+      auto const s = free_list[j];
+      auto& si = slots[s];
+      si.q = admitted[j];
+      pq[static_cast<std::size_t>(admitted[j])].slot_ = static_cast<std::int32_t>(s);
+      si.epoch = 0U;
+      si.is_new = true;
+      starting.push_back(s);
+    }
+    if (!starting.empty()) begin_epochs(starting);
+
+    bool any = false;
+    std::vector<std::uint8_t> sweep(G, 0U);
+    for (std::uint32_t s = 0; s != S; ++s) {
+      if (!slots[s].running) continue;
+      any = true;
+      // after its first epoch a slot sees earlier rows in round_times
+      if (slots[s].epoch != 0U) sweep[s / W] = 1U;
+    }
+    if (!any) return false;
+    auto const stream = e.stream;
+    auto const& tr = e.st;
     std::vector<std::uint32_t> sweep_groups;
     for (std::uint32_t g = 0; g != G; ++g) {
       if (sweep[g] != 0U) sweep_groups.push_back(g);
@@ -656,26 +665,12 @@ struct runner {
     if (!sweep_groups.empty()) {
       HIP_CHECK(hipMemcpyAsync(e.d_sweep_groups, sweep_groups.data(), 4U * sweep_groups.size(), hipMemcpyHostToDevice, stream));
     }
-    auto const grid_waves = static_cast<int>(e.grid_waves);
-    std::uint32_t n_run = 0U;
-    for (std::uint32_t k = 1U; k < max_end_k; ++k) {
-      launch_round(fwd, W, rargs(static_cast<std::int32_t>(k)), e.d_sweep_groups,
-                   static_cast<std::uint32_t>(sweep_groups.size()), S, grid_waves, stream);
-      HIP_CHECK(hipGetLastError());
-      ++rounds;
-      ++e.stats.rounds;
-      n_run = k;
-      HIP_CHECK(hipMemcpyAsync(e.h_active.p, tr.grp_active, 8U * G, hipMemcpyDeviceToHost, stream));
-      HIP_CHECK(hipStreamSynchronize(stream));
-      bool alive = false;
-      for (std::uint32_t g = 0; g != G; ++g) alive |= e.h_active.p[g] != 0U;
-      if (!alive) break;
-    }
-    launch_epoch_end(W, tr, S, stream);
+    launch_round(fwd, W, rargs(), e.d_sweep_groups, static_cast<std::uint32_t>(sweep_groups.size()), S,
+                 static_cast<int>(e.grid_waves), stream);
     HIP_CHECK(hipGetLastError());
-    ++e.stats.epochs;
-    auto const rows = static_cast<std::size_t>(S) * kMaxDestBits * K;
-    HIP_CHECK(hipMemcpyAsync(e.h_rows.p, tr.dest_rows, 2U * rows, hipMemcpyDeviceToHost, stream));
+    ++rounds;
+    ++e.stats.rounds;
+    HIP_CHECK(hipMemcpyAsync(e.h_active.p, tr.grp_active, 8U * G, hipMemcpyDeviceToHost, stream));
     HIP_CHECK(hipMemcpyAsync(e.h_counts.p, tr.counts, 4U * 8U, hipMemcpyDeviceToHost, stream));
     HIP_CHECK(hipStreamSynchronize(stream));
     if (e.h_counts.p[4] != 0U) {
@@ -685,13 +680,25 @@ struct runner {
       throw status_error{NIGIRI_AMD_ECAPACITY,
                          "device work list overflow (flags " + std::to_string(flags) + ")"};
     }
-    collect_timings(n_run);
+    collect_timings();
 
+    // slots whose epoch ended in this round
+    std::vector<std::uint32_t> finished;
     for (std::uint32_t s = 0; s != S; ++s) {
-      auto& si = slots[s];
-      if (!si.running) continue;
+      if (slots[s].running && ((e.h_active.p[s / W] >> (s % W)) & 1ULL) == 0ULL) finished.push_back(s);
+    }
+    if (finished.empty()) return true;
+    HIP_CHECK(hipMemcpyAsync(e.d_slot_list, finished.data(), 4U * finished.size(), hipMemcpyHostToDevice, stream));
+    launch_extract(W, tr, e.d_slot_list, static_cast<std::uint32_t>(finished.size()), stream);
+    HIP_CHECK(hipGetLastError());
+    auto const per_slot = static_cast<std::size_t>(kMaxDestBits) * K;
+    HIP_CHECK(hipMemcpyAsync(e.h_rows.p, tr.dest_rows, 2U * per_slot * finished.size(), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    for (std::size_t j = 0; j != finished.size(); ++j) {
+      auto& si = slots[finished[j]];
+      si.running = false;
       auto& p = pq[static_cast<std::size_t>(si.q)];
-      add_epoch_results(tt, p, si.epoch, e.h_rows.p + static_cast<std::size_t>(s) * kMaxDestBits * K, K);
+      add_epoch_results(tt, p, si.epoch, e.h_rows.p + j * per_slot, K);
       if (si.epoch + 1U == p.epochs_.size()) {
         finish(p);
         on_done(si.q);
@@ -700,11 +707,15 @@ struct runner {
     return true;
   }
 
-  // slots that take a new query at the next epoch boundary
+  bool any_running() const {
+    return std::any_of(slots.begin(), slots.end(), [](slot_info const& si) { return si.running; });
+  }
+
+  // idle slots that take a new query at the next step
   std::uint32_t free_slots() const {
     std::uint32_t n = 0U;
     for (auto const& si : slots) {
-      if (si.q < 0 || si.epoch + 1U >= pq[static_cast<std::size_t>(si.q)].epochs_.size()) ++n;
+      if (!si.running && (si.q < 0 || si.epoch + 1U >= pq[static_cast<std::size_t>(si.q)].epochs_.size())) ++n;
     }
     return n;
   }
@@ -860,8 +871,7 @@ void engine_run_dir(nigiri_amd_engine& e, bool fwd, std::vector<prepared_query>&
 }
 
 // Batched engine, all queries known: lower bounds are computed ahead of
-// admission; each epoch starts once the free slots can be filled (or no
-// more queries are coming).
+// admission; queries are admitted as slots free up.
 void run_batched(nigiri_amd_engine& e, bool fwd, std::vector<prepared_query>& pq,
                  std::vector<std::int32_t> const& queue) {
   lb_pipeline pipe{e, fwd, pq};
@@ -876,9 +886,12 @@ void run_batched(nigiri_amd_engine& e, bool fwd, std::vector<prepared_query>& pq
   r.on_done = [&](std::int32_t) { ++done; };
   r.release_row = [&](std::uint32_t row) { pipe.release(row); };
   while (done < queue.size()) {
-    pipe.wait_ready(std::min<std::size_t>(r.free_slots(), queue.size() - popped));
+    // with nothing running, wait until the free slots can be filled (or
+    // no more queries are coming); otherwise admit what is ready
+    if (!r.any_running()) pipe.wait_ready(std::min<std::size_t>(r.free_slots(), queue.size() - popped));
     if (!r.step() && done < queue.size()) {
-      throw status_error{NIGIRI_AMD_EDEVICE, "batched search made no progress"};
+      if (popped == queue.size()) throw status_error{NIGIRI_AMD_EDEVICE, "batched search made no progress"};
+      pipe.wait_ready(1U);
     }
   }
 }
@@ -1014,6 +1027,7 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     st.counts = m.alloc<std::uint32_t>(8U);
     st.work = m.alloc<unsigned long long>(kWorkCount);
     st.param = m.alloc<slot_param>(S);
+    st.kslot = m.alloc<std::uint8_t>(S);
     st.tad = m.alloc<std::int16_t>(static_cast<std::size_t>(S) * kRowsMax);
     st.tad0 = m.alloc<std::int16_t>(S);
     st.n_chg = m.alloc<std::uint32_t>(S);
@@ -1040,6 +1054,7 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     HIP_CHECK(hipMemsetAsync(st.counts, 0, 32U, e->stream));
     HIP_CHECK(hipMemsetAsync(st.work, 0, 8U * kWorkCount, e->stream));
     HIP_CHECK(hipMemsetAsync(st.n_chg, 0, 4U * S, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.kslot, 0, S, e->stream));
     e->h_param.assign(S, slot_param{});
     e->h_dest.assign(static_cast<std::size_t>(S) * kMaxDestBits, 0U);
     e->h_dist_loc.assign(static_cast<std::size_t>(S) * kMaxDist, 0U);
@@ -1051,6 +1066,7 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     e->d_flags = m.alloc<std::uint8_t>(S);
     e->d_worst = m.alloc<std::int16_t>(S);
     e->d_sweep_groups = m.alloc<std::uint32_t>(G);
+    e->d_slot_list = m.alloc<std::uint32_t>(S);
     e->lb_rows_cap = std::min<std::uint32_t>(S, 64U);
     e->d_lb_staging = m.alloc<std::uint16_t>(static_cast<std::size_t>(e->lb_rows_cap) * L);
     e->d_lb_slot = m.alloc<std::uint32_t>(e->lb_rows_cap);
@@ -1495,24 +1511,27 @@ int nigiri_amd_algo_execute(nigiri_amd_engine* e, uint32_t slot, int32_t start_t
     a.n_slots = S;
     launch_epoch_begin(fwd, e->W, a, stream);
     HIP_CHECK(hipGetLastError());
+    // rounds until the slot's epoch ends (at most end_k - 1)
     std::uint32_t rounds = 0U;
-    for (std::uint32_t k = 1U; k < end_k; ++k) {
-      round_args ra{};
-      ra.im = e->img->d;
-      ra.st = tr;
-      ra.k = static_cast<std::int32_t>(k);
-      ra.invalid = a.invalid;
-      // a slot after its first epoch sees earlier rows in round_times
-      std::uint32_t const n_sweep = e->algo_new[slot] != 0U ? 0U : 1U;
+    round_args ra{};
+    ra.im = e->img->d;
+    ra.st = tr;
+    ra.invalid = a.invalid;
+    // a slot after its first epoch sees earlier rows in round_times
+    std::uint32_t const n_sweep = e->algo_new[slot] != 0U ? 0U : 1U;
+    while (true) {
       launch_round(fwd, e->W, ra, e->d_reset_groups, n_sweep, S, 2048, stream);
       HIP_CHECK(hipGetLastError());
       ++rounds;
       HIP_CHECK(hipMemcpyAsync(e->h_active.p, tr.grp_active, 8U * G, hipMemcpyDeviceToHost, stream));
       HIP_CHECK(hipStreamSynchronize(stream));
       if ((e->h_active.p[g] & (1ULL << lane)) == 0U) break;
+      if (rounds > e->K) throw status_error{NIGIRI_AMD_EDEVICE, "epoch did not end"};
     }
-    launch_epoch_end(e->W, tr, S, stream);
-    auto const rows = static_cast<std::size_t>(S) * kMaxDestBits * e->K;
+    std::uint32_t const one = slot;
+    HIP_CHECK(hipMemcpyAsync(e->d_slot_list, &one, 4U, hipMemcpyHostToDevice, stream));
+    launch_extract(e->W, tr, e->d_slot_list, 1U, stream);
+    auto const rows = static_cast<std::size_t>(kMaxDestBits) * e->K;
     HIP_CHECK(hipMemcpyAsync(e->h_rows.p, tr.dest_rows, 2U * rows, hipMemcpyDeviceToHost, stream));
     HIP_CHECK(hipMemcpyAsync(e->h_counts.p, tr.counts, 32U, hipMemcpyDeviceToHost, stream));
     HIP_CHECK(hipStreamSynchronize(stream));
@@ -1521,10 +1540,7 @@ int nigiri_amd_algo_execute(nigiri_amd_engine* e, uint32_t slot, int32_t start_t
       HIP_CHECK(hipMemcpy(tr.counts + 4, &zero, 4U, hipMemcpyHostToDevice));
       throw status_error{NIGIRI_AMD_ECAPACITY, "device work list overflow"};
     }
-    if (dest_times != nullptr) {
-      std::memcpy(dest_times, e->h_rows.p + static_cast<std::size_t>(slot) * kMaxDestBits * e->K,
-                  2U * kMaxDestBits * e->K);
-    }
+    if (dest_times != nullptr) std::memcpy(dest_times, e->h_rows.p, 2U * rows);
     if (n_rounds != nullptr) *n_rounds = rounds;
     e->algo_new[slot] = 0U;
     e->algo_fresh[slot] = 0U;

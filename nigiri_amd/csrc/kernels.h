@@ -37,7 +37,6 @@ namespace nigiri_amd {
 struct round_args {
   dev_image im;
   dev_state st;
-  std::int32_t k;
   std::int16_t invalid;
   // optional timing events around the route scan / transfer kernels
   hipEvent_t ev[4];
@@ -119,7 +118,9 @@ void launch_lb_scatter(std::uint32_t W, epoch_args const& a, hipStream_t stream)
 void launch_gather_slot(std::uint32_t W, dev_state const& st, std::uint32_t slot,
                         std::int16_t* rt, std::int16_t* best, std::int16_t* tmp, std::uint16_t* lb,
                         hipStream_t stream);
-void launch_epoch_end(std::uint32_t W, dev_state const& st,
-                      std::uint32_t n_slots, hipStream_t stream);
+// round_times[k][dest bit] of the listed slots -> st.dest_rows (journey
+// extraction input, raptor.h:393-413)
+void launch_extract(std::uint32_t W, dev_state const& st, std::uint32_t const* slots,
+                    std::uint32_t n_slots, hipStream_t stream);
 
 }  // namespace nigiri_amd

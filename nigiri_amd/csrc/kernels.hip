@@ -193,7 +193,24 @@ __global__ void k_round_end_check(round_args a, std::uint32_t n_slots) {
   auto const g = slot / W, lane = slot % W;
   auto const lbit = 1ULL << lane;
   if ((s.grp_active[g] & lbit) == 0ULL) return;
-  if (a.k >= s.param[slot].end_k) {
+  auto const k = static_cast<std::uint32_t>(s.kslot[slot]) + 1U;
+  s.kslot[slot] = static_cast<std::uint8_t>(k);
+  if (k >= s.param[slot].end_k) {  // (a slot is retired after its last round)
+    atomicAnd(reinterpret_cast<unsigned long long*>(&s.grp_active[g]), ~lbit);
+  }
+}
+
+// round_finish: a slot whose next round would reach end_k ends its epoch
+// now (raptor.h:218: rounds k = 1 .. end_k - 1)
+template <int W>
+__global__ void k_round_finish(round_args a, std::uint32_t n_slots) {
+  auto const& s = a.st;
+  auto const slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= n_slots) return;
+  auto const g = slot / W, lane = slot % W;
+  auto const lbit = 1ULL << lane;
+  if ((s.grp_active[g] & lbit) == 0ULL) return;
+  if (static_cast<std::uint32_t>(s.kslot[slot]) + 1U >= s.param[slot].end_k) {
     atomicAnd(reinterpret_cast<unsigned long long*>(&s.grp_active[g]), ~lbit);
   }
 }
@@ -208,12 +225,13 @@ __global__ void k_round_begin(round_args a, std::uint32_t n_slots) {
   if ((s.grp_active[g] & (1ULL << lane)) == 0ULL) return;
   auto const& p = s.param[slot];
   auto* tad = s.tad + static_cast<std::size_t>(slot) * kRowsMax;
+  auto const k = static_cast<std::int32_t>(s.kslot[slot]);
   if (p.intermodal != 0U) {
-    update_tad<Fwd>(tad, a.k, s.best[lay<W>::col(s, g, 1U, lane)]);
+    update_tad<Fwd>(tad, k, s.best[lay<W>::col(s, g, 1U, lane)]);
   } else {
     for (std::uint32_t j = 0; j < p.n_dest; ++j) {
       auto const d = s.dest[slot * kMaxDestBits + j];
-      update_tad<Fwd>(tad, a.k, s.best[lay<W>::col(s, g, d, lane)]);
+      update_tad<Fwd>(tad, k, s.best[lay<W>::col(s, g, d, lane)]);
     }
   }
 }
@@ -232,7 +250,7 @@ __global__ void k_best_sweep(round_args a, std::uint32_t const* groups, std::uin
     auto const l = static_cast<std::uint32_t>(rem / W), lane = static_cast<std::uint32_t>(rem % W);
     if ((s.grp_active[g] & (1ULL << lane)) == 0ULL) continue;
     auto const c = lay<W>::col(s, g, l, lane);
-    auto const v = s.round_times[lay<W>::rt(s, g, a.k, l, lane)];
+    auto const v = s.round_times[lay<W>::rt(s, g, s.kslot[g * W + lane], l, lane)];
     s.best[c] = dirop<Fwd>::best_of(v, s.best[c]);
   }
 }
@@ -805,7 +823,6 @@ template <bool Fwd, int W>
 __global__ void __launch_bounds__(256, 8) k_route_scan(round_args a) {
   auto const& im = a.im;
   auto const& s = a.st;
-  auto const k = static_cast<std::uint32_t>(a.k);
   if constexpr (W == 64) {
     __shared__ wave_lds lds_all[kWavesPerBlock];
     auto& x = lds_all[threadIdx.x / 64U];
@@ -826,6 +843,7 @@ __global__ void __launch_bounds__(256, 8) k_route_scan(round_args a) {
       }
       auto const slot = q.g * 64U + q.ln;
       auto const base = s.param[slot].base;
+      auto const k = static_cast<std::uint32_t>(s.kslot[slot]);  // this query's round
       int const tad_k = s.tad[static_cast<std::size_t>(slot) * kRowsMax + k];
       bool any = false, started = false;
       wave_sync_lds();  // the previous item's readers are done
@@ -860,6 +878,7 @@ __global__ void __launch_bounds__(256, 8) k_route_scan(round_args a) {
       if (lanes == 0ULL) continue;
       auto const slot = g;
       auto const base = s.param[slot].base;
+      auto const k = static_cast<std::uint32_t>(s.kslot[slot]);
       int const tad_k = s.tad[static_cast<std::size_t>(slot) * kRowsMax + k];
       auto const s0 = im.route_stop_offsets[r];
       auto const t_from = im.route_transport_from[r];
@@ -1002,9 +1021,9 @@ __global__ void k_dest_pass(round_args a, std::uint32_t n_slots) {
   using D = dirop<Fwd>;
   auto const& im = a.im;
   auto const& s = a.st;
-  auto const k = static_cast<std::uint32_t>(a.k);
   auto const slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= n_slots) return;
+  auto const k = static_cast<std::uint32_t>(s.kslot[slot]);
   auto const g = slot / W, lane = slot % W;
   auto* tad = s.tad + static_cast<std::size_t>(slot) * kRowsMax;
   s.n_chg[slot] = 0U;
@@ -1137,7 +1156,6 @@ __global__ void k_transfer_fp(round_args a) {
   using D = dirop<Fwd>;
   auto const& im = a.im;
   auto const& s = a.st;
-  auto const k = static_cast<std::uint32_t>(a.k);
   auto* overflow = &s.counts[4];
   auto const n_items = clamp_count(s.counts[1], s.cap_r);
   std::uint32_t const lane_id = W == 64 ? (threadIdx.x & 63U) : 0U;
@@ -1160,6 +1178,7 @@ __global__ void k_transfer_fp(round_args a) {
     }
     auto const slot = g * W + lane_id;
     auto const& p = s.param[slot];
+    auto const k = static_cast<std::uint32_t>(s.kslot[slot]);
     auto const tmp_i = s.tmp[lay<W>::col(s, g, i, lane_id)];
     bool const valid = on && tmp_i != D::kInvalid;
     auto const tad0 = s.tad0[slot];
@@ -1216,7 +1235,6 @@ __global__ void k_transfer_fp(round_args a) {
 template <bool Fwd, int W>
 __global__ void k_rt_fixup(round_args a) {
   auto const& s = a.st;
-  auto const k = static_cast<std::uint32_t>(a.k);
   auto const n = clamp_count(s.counts[0], s.cap_t);
   for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n * W;
        i += gridDim.x * blockDim.x) {
@@ -1226,6 +1244,7 @@ __global__ void k_rt_fixup(round_args a) {
     if (!lay<W>::test(s.mark_t, s, g, t, lane)) continue;
     if ((s.grp_active[g] & (1ULL << lane)) == 0ULL) continue;
     if (lay<W>::test(s.dest_mask, s, g, t, lane)) continue;
+    auto const k = static_cast<std::uint32_t>(s.kslot[g * W + lane]);
     s.round_times[lay<W>::rt(s, g, k, t, lane)] = s.best[lay<W>::col(s, g, t, lane)];
   }
 }
@@ -1237,7 +1256,6 @@ __global__ void k_hazards(round_args a) {
   using D = dirop<Fwd>;
   auto const& im = a.im;
   auto const& s = a.st;
-  auto const k = static_cast<std::uint32_t>(a.k);
   auto const n = clamp_count(s.counts[3], s.cap_hz);
   for (std::uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < n * W; h += gridDim.x * blockDim.x) {
     auto const e = s.list_hz[h / W];
@@ -1248,6 +1266,7 @@ __global__ void k_hazards(round_args a) {
     if (!lay<W>::test(s.mark_t, s, g, t, lane)) continue;  // no passing relaxation
     if (lay<W>::test(s.dest_mask, s, g, t, lane)) continue;
     auto const& p = s.param[slot];
+    auto const k = static_cast<std::uint32_t>(s.kslot[slot]);
     auto const c = lay<W>::col(s, g, t, lane);
     auto const bt = s.best[c];
     int const lbv = s.lb[c];
@@ -1294,6 +1313,16 @@ __global__ void k_hazards(round_args a) {
 template <int W>
 __global__ void k_lane_reset(epoch_args a) {
   auto const& s = a.st;
+  // station marks left by an epoch that ended at its last round are cleared
+  // for the restarting lanes (the words may hold running lanes' marks)
+  auto const words = static_cast<std::size_t>(W == 64 ? s.L : s.LW);
+  for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x;
+       i < words * a.n_reset_groups; i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    auto const g = a.reset_groups[i / words];
+    auto const m = W == 64 ? a.reset_bt_mask[g] : (a.reset_bt_mask[g] != 0ULL ? ~0ULL : 0ULL);
+    auto& w = s.mark_t[static_cast<std::size_t>(g) * words + i % words];
+    if ((w & m) != 0ULL) w &= ~m;
+  }
   auto const per_group = static_cast<std::size_t>(s.L) * W;
   auto const total = per_group * a.n_reset_groups;
   for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < total;
@@ -1542,6 +1571,7 @@ __global__ void k_slot_init(epoch_args a) {
   }
   auto const w = a.slot_worst[slot];
   for (int i = 0; i < kRowsMax; ++i) tad[i] = dirop<Fwd>::best_of(w, tad[i]);
+  s.kslot[slot] = 0U;
   auto const g = slot / W, lane = slot % W;
   atomicOr(reinterpret_cast<unsigned long long*>(&s.grp_active[g]), 1ULL << lane);
 }
@@ -1556,22 +1586,21 @@ __global__ void k_add_starts(epoch_args a) {
     auto const g = slot / W, lane = slot % W;
     atomic_best16<Fwd>(&s.best[lay<W>::col(s, g, l, lane)], d);
     atomic_best16<Fwd>(&s.round_times[lay<W>::rt(s, g, 0U, l, lane)], d);
-    if constexpr (W == 64) {
-      mark_wave64(s.mark_t, s.flag_t, g, l, 1ULL << lane, s.L);
-    } else {
-      mark_one<W>(s.mark_t, s.list_t, &s.counts[0], s.cap_t, &s.counts[4], s, g, l, lane);
-    }
+    // appended to the station work list of the next round unless a lane
+    // already marked the word there (other slots may be mid-epoch)
+    mark_one<W>(s.mark_t, s.list_t, &s.counts[0], s.cap_t, &s.counts[4], s, g, l, lane);
   }
 }
 
-// journey extraction input: round_times[k][dest bit] for every running slot
+// journey extraction input: round_times[k][dest bit] of the listed slots
 template <int W>
-__global__ void k_extract(dev_state s, std::uint32_t n_slots) {
+__global__ void k_extract(dev_state s, std::uint32_t const* slots, std::uint32_t n_slots) {
   auto const total = static_cast<std::size_t>(n_slots) * kMaxDestBits * s.K;
-  for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < total;
-       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
-    auto const slot = static_cast<std::uint32_t>(i / (kMaxDestBits * s.K));
-    auto const rem = static_cast<std::uint32_t>(i % (kMaxDestBits * s.K));
+  for (std::size_t ii = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; ii < total;
+       ii += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    auto const slot = slots[ii / (kMaxDestBits * s.K)];
+    auto const rem = static_cast<std::uint32_t>(ii % (kMaxDestBits * s.K));
+    auto const i = ii;  // rows of the listed slots, in list order
     auto const j = rem / s.K, k = rem % s.K;
     auto const& p = s.param[slot];
     auto const nbits = p.intermodal != 0U ? 1U : p.n_dest;
@@ -1653,6 +1682,7 @@ void round_impl(round_args const& a, std::uint32_t const* sweep_groups, std::uin
   hipLaunchKernelGGL((k_clear_marks<W>), dim3(1024), dim3(kBlock), 0, st, a.st, 1);
   hipLaunchKernelGGL((k_clear_marks<W>), dim3(256), dim3(kBlock), 0, st, a.st, 2);
   hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 2U | 8U);  // list_r, hazards
+  hipLaunchKernelGGL((k_round_finish<W>), dim3(slot_blocks), dim3(kBlock), 0, st, a, n_slots);
 }
 
 template <bool Fwd, int W>
@@ -1666,9 +1696,6 @@ void epoch_begin_impl(epoch_args const& a, hipStream_t st) {
   hipLaunchKernelGGL((k_slot_init<Fwd, W>), dim3(grid_for(a.n_slots, kBlock, 1 << 20)), dim3(kBlock), 0, st, a);
   if (a.n_starts != 0U) {
     hipLaunchKernelGGL((k_add_starts<Fwd, W>), dim3(grid_for(a.n_starts, kBlock, 4096)), dim3(kBlock), 0, st, a);
-    if constexpr (W == 64) {
-      compact(a.st.flag_t, a.st.L, a.st.G, a.st.list_t, &a.st.counts[0], a.st.cap_t, a.st, st);
-    }
   }
 }
 
@@ -1721,18 +1748,15 @@ void launch_gather_slot(std::uint32_t W, dev_state const& st, std::uint32_t slot
   }
 }
 
-void launch_epoch_end(std::uint32_t W, dev_state const& st, std::uint32_t n_slots,
-                      hipStream_t stream) {
+void launch_extract(std::uint32_t W, dev_state const& st, std::uint32_t const* slots,
+                    std::uint32_t n_slots, hipStream_t stream) {
+  if (n_slots == 0U) return;
   auto const work = static_cast<std::size_t>(n_slots) * kMaxDestBits * st.K;
   if (W == 64U) {
-    hipLaunchKernelGGL((k_extract<64>), dim3(grid_for(work, kBlock, 4096)), dim3(kBlock), 0, stream, st, n_slots);
-    hipLaunchKernelGGL((k_clear_marks<64>), dim3(1024), dim3(kBlock), 0, stream, st, 0);
+    hipLaunchKernelGGL((k_extract<64>), dim3(grid_for(work, kBlock, 4096)), dim3(kBlock), 0, stream, st, slots, n_slots);
   } else {
-    hipLaunchKernelGGL((k_extract<1>), dim3(grid_for(work, kBlock, 4096)), dim3(kBlock), 0, stream, st, n_slots);
-    hipLaunchKernelGGL((k_clear_marks<1>), dim3(1024), dim3(kBlock), 0, stream, st, 0);
+    hipLaunchKernelGGL((k_extract<1>), dim3(grid_for(work, kBlock, 4096)), dim3(kBlock), 0, stream, st, slots, n_slots);
   }
-  hipLaunchKernelGGL((k_deactivate_all), dim3(grid_for(st.G, kBlock, 1024)), dim3(kBlock), 0, stream, st);
-  hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, stream, st, 0x2FU);
 }
 
 }  // namespace nigiri_amd

@@ -86,10 +86,12 @@ def serve_main(args):
     def pct(a, p):
         return round(float(np.percentile(a, p)), 2) if len(a) else None
 
+    queue = np.array([r["queue_ms"] for r in res])
     kinds_out = {}
     for k in ("point", "range", "intermodal"):
         m = kinds == k
-        kinds_out[k] = {"n": int(m.sum()), "p50_ms": pct(lat[m], 50), "p99_ms": pct(lat[m], 99)}
+        kinds_out[k] = {"n": int(m.sum()), "p50_ms": pct(lat[m], 50), "p99_ms": pct(lat[m], 99),
+                        "queue_p50_ms": pct(queue[m], 50), "queue_p99_ms": pct(queue[m], 99)}
     plain = na.Engine(img, lanes=1, groups=1, max_k=max_k)
     pts = [i for i in range(n) if kinds[i] == "point"][:6]
     single_plain, single_batched = [], []

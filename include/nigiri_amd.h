@@ -158,7 +158,9 @@ typedef struct nigiri_result {
   int32_t interval_to;
   int16_t fastest_direct;
   int16_t base_day;
-  double latency_ms;       /* admission -> completion (served path)         */
+  double latency_ms;       /* arrival -> completion (served path)           */
+  double queue_ms;         /* arrival -> first round on the device (served
+                              path: preparation, lower bounds, free slot)   */
 } nigiri_result;
 
 typedef struct nigiri_amd_tt nigiri_amd_tt;         /* host timetable       */

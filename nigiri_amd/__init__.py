@@ -79,7 +79,8 @@ class Result(C.Structure):
     _fields_ = [("status", C.c_int32), ("n_journeys", C.c_uint32), ("journey_offset", C.c_uint32),
                 ("n_epochs", C.c_uint32), ("n_rounds", C.c_uint32), ("slot", C.c_int32),
                 ("interval_from", C.c_int32), ("interval_to", C.c_int32),
-                ("fastest_direct", C.c_int16), ("base_day", C.c_int16), ("latency_ms", C.c_double)]
+                ("fastest_direct", C.c_int16), ("base_day", C.c_int16), ("latency_ms", C.c_double),
+                ("queue_ms", C.c_double)]
 
 
 class AlgoInit(C.Structure):
@@ -379,7 +380,7 @@ def _collect(res, jr, n):
               for j in range(r.n_journeys)]
         out.append({"status": r.status, "journeys": js, "n_epochs": r.n_epochs,
                     "base_day": r.base_day, "fastest_direct": r.fastest_direct, "slot": r.slot,
-                    "latency_ms": r.latency_ms})
+                    "latency_ms": r.latency_ms, "queue_ms": r.queue_ms})
     return out
 
 

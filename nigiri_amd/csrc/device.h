@@ -118,7 +118,8 @@ struct dev_state {
   std::uint64_t* packed_info;
   std::uint16_t* packed_lanes;
   std::uint32_t* counts;   // [0] list_t, [1] list_r, [2] list_route, [3] hz, [4] overflow flags,
-                           // [5] packed route wavefronts
+                           // [5] packed route wavefronts, [6] / [7] work cursors of
+                           // the route scan / transfer kernels (batched engine)
   // algorithmic work counters (see kWork* indices)
   unsigned long long* work;
   std::size_t cap_t, cap_r, cap_route, cap_hz;

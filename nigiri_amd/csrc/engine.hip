@@ -123,24 +123,27 @@ struct nigiri_amd_engine {
   std::int32_t* d_lb_row = nullptr;
   std::uint32_t* d_lb_groups = nullptr;
   std::uint32_t lb_rows_cap = 64U;
-  // device lower bounds (batched engine): a second stream computes them
-  // ahead of admission into a pool of S rows; up to lbq_chunk queries per
-  // launch, one workgroup each, with [lbq_chunk][L] scratch arrays
-  hipStream_t lb_stream = nullptr;
+  // device lower bounds (batched engine): kLbWorkers threads, each with
+  // its own stream and scratch, compute them ahead of admission into a
+  // pool of S rows (one workgroup per query, up to `chunk` per launch)
+  struct lb_worker_res {
+    hipStream_t stream = nullptr;
+    std::uint32_t chunk = 0U;
+    std::uint32_t* out_row = nullptr;
+    std::uint32_t* dist = nullptr;
+    std::uint32_t* stamp_near = nullptr;
+    std::uint32_t* stamp_far = nullptr;
+    std::uint32_t* near0 = nullptr;
+    std::uint32_t* near1 = nullptr;
+    std::uint32_t* far0 = nullptr;
+    std::uint32_t* far1 = nullptr;
+    std::uint32_t* seed_off = nullptr;
+    std::uint32_t* seed_loc = nullptr;
+    std::uint16_t* seed_dur = nullptr;
+    std::size_t seed_cap = 0U;
+  };
+  std::vector<lb_worker_res> lbw;
   std::uint16_t* d_lb_pool = nullptr;
-  std::uint32_t* d_lbq_out_row = nullptr;
-  std::uint32_t lbq_chunk = 0U;
-  std::uint32_t* d_lbq_dist = nullptr;
-  std::uint32_t* d_lbq_stamp_near = nullptr;
-  std::uint32_t* d_lbq_stamp_far = nullptr;
-  std::uint32_t* d_lbq_near0 = nullptr;
-  std::uint32_t* d_lbq_near1 = nullptr;
-  std::uint32_t* d_lbq_far0 = nullptr;
-  std::uint32_t* d_lbq_far1 = nullptr;
-  std::uint32_t* d_seed_off = nullptr;
-  std::uint32_t* d_seed_loc = nullptr;
-  std::uint16_t* d_seed_dur = nullptr;
-  std::size_t seed_cap = 0U;
   // growable device buffers for starts / dest ops
   std::uint32_t* d_start_slot = nullptr;
   std::uint32_t* d_start_loc = nullptr;
@@ -172,7 +175,9 @@ struct nigiri_amd_engine {
     for (auto ev : events) (void)hipEventDestroy(ev);
     for (auto* p : grow_ptrs) (void)hipFree(p);
     if (stream != nullptr) (void)hipStreamDestroy(stream);
-    if (lb_stream != nullptr) (void)hipStreamDestroy(lb_stream);
+    for (auto& w : lbw) {
+      if (w.stream != nullptr) (void)hipStreamDestroy(w.stream);
+    }
   }
 
   template <typename T>
@@ -271,9 +276,13 @@ struct slot_info {
 };
 
 // Lower bounds of queued queries (batched engine), computed ahead of
-// admission by a worker thread on the engine's second stream: queries are
-// submitted once prepared, become ready when their row of the pool holds
-// their bounds, and give the row back after admission has transposed it.
+// admission by kLbWorkers worker threads, each on its own stream: queries
+// are submitted once prepared, become ready when their row of the pool
+// holds their bounds, and give the row back after admission has transposed
+// it.  A query waits at most for one launch of a free worker.
+// threads (and streams) computing lower bounds ahead of admission
+constexpr std::size_t kLbWorkers = 3U;
+
 static_assert(kMaxQueryDests == kMaxDestBits && kMaxQueryEgress == kMaxDist,
               "query limits of the host preparation and the device state agree");
 
@@ -289,11 +298,13 @@ struct lb_pipeline {
   std::uint32_t n_finished = 0U;
   bool stop = false;
   std::exception_ptr err;
-  std::thread worker;
+  std::vector<std::thread> workers;
 
   lb_pipeline(nigiri_amd_engine& eng, bool f, std::vector<prepared_query>& p) : e{eng}, fwd{f}, pq{p} {
     for (std::uint32_t r = 0; r != e.S; ++r) free_rows.push_back(e.S - 1U - r);
-    worker = std::thread{[this]() { run(); }};
+    for (std::size_t w = 0; w != e.lbw.size(); ++w) {
+      workers.emplace_back([this, w]() { run(e.lbw[w]); });
+    }
   }
   ~lb_pipeline() {
     {
@@ -301,7 +312,7 @@ struct lb_pipeline {
       stop = true;
     }
     cv.notify_all();
-    worker.join();
+    for (auto& t : workers) t.join();
   }
   void submit(std::int32_t q) {
     {
@@ -347,7 +358,8 @@ struct lb_pipeline {
     cv.notify_all();
   }
 
-  void run() {
+  void run(nigiri_amd_engine::lb_worker_res const& w) {
+    std::size_t taken = 0U;
     try {
       HIP_CHECK(hipSetDevice(e.img->device));
       while (true) {
@@ -355,9 +367,9 @@ struct lb_pipeline {
         std::vector<std::uint32_t> rows;
         {
           std::unique_lock<std::mutex> lock{mu};
-          cv.wait(lock, [&]() { return stop || (!waiting.empty() && !free_rows.empty()); });
-          if (stop) return;
-          while (!waiting.empty() && !free_rows.empty() && batch.size() < e.lbq_chunk) {
+          cv.wait(lock, [&]() { return stop || err || (!waiting.empty() && !free_rows.empty()); });
+          if (stop || err) return;
+          while (!waiting.empty() && !free_rows.empty() && batch.size() < w.chunk) {
             auto const q = waiting.front();
             waiting.pop_front();
             auto const row = free_rows.back();
@@ -366,29 +378,32 @@ struct lb_pipeline {
             batch.push_back(q);
             rows.push_back(row);
           }
-          in_flight = batch.size();
+          taken = batch.size();
+          in_flight += taken;
         }
-        compute(batch, rows);
+        compute(w, batch, rows);
         {
           std::lock_guard<std::mutex> lock{mu};
           for (auto const q : batch) ready.push_back(q);
           n_finished += static_cast<std::uint32_t>(batch.size());
-          in_flight = 0U;
+          in_flight -= taken;
+          taken = 0U;
         }
         cv.notify_all();
       }
     } catch (...) {
       {
         std::lock_guard<std::mutex> lock{mu};
-        err = std::current_exception();
-        in_flight = 0U;
+        if (!err) err = std::current_exception();
+        in_flight -= taken;
       }
       cv.notify_all();
     }
   }
 
-  void compute(std::vector<std::int32_t> const& batch, std::vector<std::uint32_t> const& rows) {
-    auto const stream = e.lb_stream;
+  void compute(nigiri_amd_engine::lb_worker_res const& w, std::vector<std::int32_t> const& batch,
+               std::vector<std::uint32_t> const& rows) {
+    auto const stream = w.stream;
     std::vector<std::uint32_t> seed_off{0U}, seed_loc;
     std::vector<std::uint16_t> seed_dur;
     for (auto const q : batch) {
@@ -398,30 +413,30 @@ struct lb_pipeline {
       }
       seed_off.push_back(static_cast<std::uint32_t>(seed_loc.size()));
     }
-    if (seed_loc.size() > e.seed_cap) {
+    if (seed_loc.size() > w.seed_cap) {
       throw status_error{NIGIRI_AMD_EUNSUPPORTED, "too many destination offsets"};
     }
-    HIP_CHECK(hipMemcpyAsync(e.d_seed_off, seed_off.data(), 4U * seed_off.size(), hipMemcpyHostToDevice, stream));
-    HIP_CHECK(hipMemcpyAsync(e.d_lbq_out_row, rows.data(), 4U * rows.size(), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(w.seed_off, seed_off.data(), 4U * seed_off.size(), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(w.out_row, rows.data(), 4U * rows.size(), hipMemcpyHostToDevice, stream));
     if (!seed_loc.empty()) {
-      HIP_CHECK(hipMemcpyAsync(e.d_seed_loc, seed_loc.data(), 4U * seed_loc.size(), hipMemcpyHostToDevice, stream));
-      HIP_CHECK(hipMemcpyAsync(e.d_seed_dur, seed_dur.data(), 2U * seed_dur.size(), hipMemcpyHostToDevice, stream));
+      HIP_CHECK(hipMemcpyAsync(w.seed_loc, seed_loc.data(), 4U * seed_loc.size(), hipMemcpyHostToDevice, stream));
+      HIP_CHECK(hipMemcpyAsync(w.seed_dur, seed_dur.data(), 2U * seed_dur.size(), hipMemcpyHostToDevice, stream));
     }
     lb_args q{};
     q.im = e.img->d;
     q.L = e.st.L;
     q.n_q = static_cast<std::uint32_t>(batch.size());
-    q.seed_off = e.d_seed_off;
-    q.seed_loc = e.d_seed_loc;
-    q.seed_dur = e.d_seed_dur;
-    q.dist = e.d_lbq_dist;
-    q.stamp_near = e.d_lbq_stamp_near;
-    q.stamp_far = e.d_lbq_stamp_far;
-    q.near0 = e.d_lbq_near0;
-    q.near1 = e.d_lbq_near1;
-    q.far0 = e.d_lbq_far0;
-    q.far1 = e.d_lbq_far1;
-    q.out_row = e.d_lbq_out_row;
+    q.seed_off = w.seed_off;
+    q.seed_loc = w.seed_loc;
+    q.seed_dur = w.seed_dur;
+    q.dist = w.dist;
+    q.stamp_near = w.stamp_near;
+    q.stamp_far = w.stamp_far;
+    q.near0 = w.near0;
+    q.near1 = w.near1;
+    q.far0 = w.far0;
+    q.far1 = w.far1;
+    q.out_row = w.out_row;
     q.rows = e.d_lb_pool;
     launch_lb_queries(fwd, q, stream);
     HIP_CHECK(hipGetLastError());
@@ -437,6 +452,7 @@ struct runner {
   std::function<std::int32_t()> next;            // next query id or -1
   std::function<void(std::int32_t)> on_done;      // query finished
   std::function<void(std::uint32_t)> release_row; // lower-bound row consumed
+  std::function<void(std::int32_t)> on_admit;     // query takes a slot
   std::vector<slot_info> slots;
   std::uint64_t rounds = 0U, epochs = 0U;
 
@@ -644,6 +660,7 @@ struct runner {
       si.epoch = 0U;
       si.is_new = true;
       starting.push_back(s);
+      if (on_admit) on_admit(admitted[j]);
     }
     if (!starting.empty()) begin_epochs(starting);
 
@@ -824,7 +841,7 @@ void prepare_all(timetable const& tt, nigiri_query const* qs, std::uint32_t n,
 std::uint32_t write_results(std::vector<prepared_query> const& pq, std::vector<int> const& status,
                             nigiri_result* res, nigiri_journey* journeys, std::uint32_t cap,
                             std::vector<std::uint32_t> const* epochs,
-                            std::vector<double> const* latency) {
+                            std::vector<double> const* latency, std::vector<double> const* queue = nullptr) {
   std::uint32_t pos = 0U;
   int overflow = 0;
   for (std::size_t i = 0; i != pq.size(); ++i) {
@@ -843,6 +860,7 @@ std::uint32_t write_results(std::vector<prepared_query> const& pq, std::vector<i
     r.slot = p.slot_;
     if (epochs != nullptr) r.n_rounds = (*epochs)[i];
     if (latency != nullptr) r.latency_ms = (*latency)[i];
+    if (queue != nullptr) r.queue_ms = (*queue)[i];
     for (auto const& j : p.results_) {
       if (pos < cap && journeys != nullptr) {
         journeys[pos] = nigiri_journey{j.start_time_, j.dest_time_, j.dest_, j.transfers_, {}};
@@ -1073,26 +1091,29 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     e->d_lb_row = m.alloc<std::int32_t>(S);
     e->d_lb_groups = m.alloc<std::uint32_t>(G);
     if (W == 64U) {
-      // device lower-bound scratch: 30 bytes per location and query, at
-      // most about 4 GiB, at least one query per launch
-      auto const per_q = 30ULL * L;
-      e->lbq_chunk = static_cast<std::uint32_t>(
-          std::max<std::uint64_t>(1U, std::min<std::uint64_t>(S, (4ULL << 30) / per_q)));
-      auto const n = static_cast<std::size_t>(e->lbq_chunk) * L;
-      e->d_lbq_dist = m.alloc<std::uint32_t>(n);
-      e->d_lbq_stamp_near = m.alloc<std::uint32_t>(n);
-      e->d_lbq_stamp_far = m.alloc<std::uint32_t>(n);
-      e->d_lbq_near0 = m.alloc<std::uint32_t>(n);
-      e->d_lbq_near1 = m.alloc<std::uint32_t>(n);
-      e->d_lbq_far0 = m.alloc<std::uint32_t>(n);
-      e->d_lbq_far1 = m.alloc<std::uint32_t>(n);
+      // device lower-bound scratch: 28 bytes per location and query and
+      // worker, about 4 GiB in all, at least one query per launch
+      auto const per_q = 28ULL * L;
       e->d_lb_pool = m.alloc<std::uint16_t>(static_cast<std::size_t>(S) * L);
-      e->d_lbq_out_row = m.alloc<std::uint32_t>(e->lbq_chunk);
-      e->d_seed_off = m.alloc<std::uint32_t>(static_cast<std::size_t>(e->lbq_chunk) + 1U);
-      e->seed_cap = static_cast<std::size_t>(e->lbq_chunk) * (kMaxDist + kMaxDestBits);
-      e->d_seed_loc = m.alloc<std::uint32_t>(e->seed_cap);
-      e->d_seed_dur = m.alloc<std::uint16_t>(e->seed_cap);
-      HIP_CHECK(hipStreamCreateWithFlags(&e->lb_stream, hipStreamNonBlocking));
+      e->lbw.resize(kLbWorkers);
+      for (auto& w : e->lbw) {
+        w.chunk = static_cast<std::uint32_t>(std::max<std::uint64_t>(
+            1U, std::min<std::uint64_t>(S, (4ULL << 30) / (kLbWorkers * per_q))));
+        auto const n = static_cast<std::size_t>(w.chunk) * L;
+        w.dist = m.alloc<std::uint32_t>(n);
+        w.stamp_near = m.alloc<std::uint32_t>(n);
+        w.stamp_far = m.alloc<std::uint32_t>(n);
+        w.near0 = m.alloc<std::uint32_t>(n);
+        w.near1 = m.alloc<std::uint32_t>(n);
+        w.far0 = m.alloc<std::uint32_t>(n);
+        w.far1 = m.alloc<std::uint32_t>(n);
+        w.out_row = m.alloc<std::uint32_t>(w.chunk);
+        w.seed_off = m.alloc<std::uint32_t>(static_cast<std::size_t>(w.chunk) + 1U);
+        w.seed_cap = static_cast<std::size_t>(w.chunk) * (kMaxDist + kMaxDestBits);
+        w.seed_loc = m.alloc<std::uint32_t>(w.seed_cap);
+        w.seed_dur = m.alloc<std::uint16_t>(w.seed_cap);
+        HIP_CHECK(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
+      }
     }
     e->h_lb.reserve(static_cast<std::size_t>(e->lb_rows_cap) * L);
     e->h_active.reserve(G);
@@ -1226,7 +1247,7 @@ int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs, const double*
     using clock = std::chrono::steady_clock;
     std::vector<prepared_query> pq(n);
     std::vector<int> status(n, NIGIRI_AMD_OK);
-    std::vector<double> latency(n, 0.0);
+    std::vector<double> latency(n, 0.0), queue(n, 0.0);
     std::mutex mu;
     std::deque<std::int32_t> ready;
     std::atomic<std::uint32_t> next_prep{0U};
@@ -1293,6 +1314,9 @@ int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs, const double*
       latency[static_cast<std::size_t>(q)] = ms_since(clock::now()) - arrival_ms[q];
       n_done.fetch_add(1U);
     };
+    r.on_admit = [&](std::int32_t q) {
+      queue[static_cast<std::size_t>(q)] = ms_since(clock::now()) - arrival_ms[q];
+    };
     std::atomic<std::uint32_t>& n_prepared = n_prepared_ctr;
     try {
       while (n_done.load() < n) {
@@ -1329,7 +1353,7 @@ int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs, const double*
     }
     for (auto& t : workers) t.join();
     if (wall_ms != nullptr) *wall_ms = ms_since(clock::now());
-    auto const w = write_results(pq, status, res, journeys, journey_capacity, nullptr, &latency);
+    auto const w = write_results(pq, status, res, journeys, journey_capacity, nullptr, &latency, &queue);
     if (w == ~0U) {
       set_error("serve: journey_capacity too small");
       return NIGIRI_AMD_ECAPACITY;

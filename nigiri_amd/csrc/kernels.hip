@@ -71,6 +71,23 @@ __device__ inline void list_push(std::uint64_t* list, std::uint32_t* count, std:
   }
 }
 
+// Work items of a wavefront from a shared cursor, Chunk at a time:
+// wavefronts that start late (their CU was busy) simply take fewer items.
+template <std::uint32_t Chunk>
+struct item_claim {
+  std::uint32_t* cursor;
+  std::uint32_t next = 0U, end = 0U;
+  __device__ std::uint32_t get() {
+    if (next == end) {
+      std::uint32_t v = 0U;
+      if ((threadIdx.x & 63U) == 0U) v = atomicAdd(cursor, Chunk);
+      next = static_cast<std::uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+      end = next + Chunk;
+    }
+    return next++;
+  }
+};
+
 // adds a per-lane count to a global counter with one atomic per wavefront
 __device__ inline void wave_count(unsigned long long* counter, unsigned long long v) {
   for (int off = 32; off > 0; off >>= 1) {
@@ -829,8 +846,8 @@ __global__ void __launch_bounds__(256, 8) k_route_scan(round_args a) {
     std::uint32_t const lane = threadIdx.x & 63U;
     auto const n_items = clamp_count(s.counts[5], s.cap_route);
     std::uint32_t c_stops = 0U, c_tmp = 0U, c_ev = 0U, c_rd = 0U, c_tt = 0U, c_ws = 0U;
-    for (std::uint32_t item = (blockIdx.x * blockDim.x + threadIdx.x) / 64U; item < n_items;
-         item += gridDim.x * kWavesPerBlock) {
+    item_claim<1U> claim{&s.counts[6]};
+    for (std::uint32_t item = claim.get(); item < n_items; item = claim.get()) {
       auto const info = s.packed_info[item];
       auto const r = uni(static_cast<std::uint32_t>(info >> 32));
       auto const n_lanes = uni(static_cast<std::uint32_t>(info));
@@ -1159,13 +1176,16 @@ __global__ void k_transfer_fp(round_args a) {
   auto* overflow = &s.counts[4];
   auto const n_items = clamp_count(s.counts[1], s.cap_r);
   std::uint32_t const lane_id = W == 64 ? (threadIdx.x & 63U) : 0U;
-  std::uint32_t const step = W == 64 ? gridDim.x * kWavesPerBlock : gridDim.x * blockDim.x;
-  std::uint32_t item = W == 64 ? (blockIdx.x * blockDim.x + threadIdx.x) / 64U
-                               : blockIdx.x * blockDim.x + threadIdx.x;
+  // batched engine: items claimed per wavefront; plain engine: grid stride
+  item_claim<16U> claim{&s.counts[7]};
+  auto const next_item = [&](std::uint32_t i) {
+    return W == 64 ? claim.get() : i + gridDim.x * blockDim.x;
+  };
+  std::uint32_t item = W == 64 ? claim.get() : blockIdx.x * blockDim.x + threadIdx.x;
   auto const* off = Fwd ? im.fp_out_offsets : im.fp_in_offsets;
   auto const* fps = Fwd ? im.fp_out : im.fp_in;
   unsigned long long c_st = 0ULL, c_fp = 0ULL;
-  for (; item < n_items; item += step) {
+  for (; item < n_items; item = next_item(item)) {
     auto const e = s.list_r[item];
     auto const g = static_cast<std::uint32_t>(e >> 32), i = static_cast<std::uint32_t>(e);
     bool on;
@@ -1681,7 +1701,8 @@ void round_impl(round_args const& a, std::uint32_t const* sweep_groups, std::uin
   hipLaunchKernelGGL((k_hazards<Fwd, W>), dim3(1024), dim3(kBlock), 0, st, a);
   hipLaunchKernelGGL((k_clear_marks<W>), dim3(1024), dim3(kBlock), 0, st, a.st, 1);
   hipLaunchKernelGGL((k_clear_marks<W>), dim3(256), dim3(kBlock), 0, st, a.st, 2);
-  hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 2U | 8U);  // list_r, hazards
+  // list_r, hazards, work cursors of the route scan and transfers
+  hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 2U | 8U | 64U | 128U);
   hipLaunchKernelGGL((k_round_finish<W>), dim3(slot_blocks), dim3(kBlock), 0, st, a, n_slots);
 }
 

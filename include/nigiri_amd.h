@@ -134,14 +134,35 @@ typedef struct nigiri_query {
   double fastest_direct_factor;  /* default 1.0                             */
 } nigiri_query;
 
-/* A journey of the result Pareto set (journey.h:135-141, legs omitted). */
+/* A journey of the result Pareto set (journey.h:104-111).  Its legs, in
+ * travel order, are legs[leg_offset .. leg_offset + n_legs) of the legs
+ * array passed with it (n_legs = 0 when no legs array was given). */
 typedef struct nigiri_journey {
   int32_t start_time;  /* unix minutes */
   int32_t dest_time;   /* unix minutes */
   uint32_t dest;       /* location index (END = 1 for intermodal)           */
   uint8_t transfers;
   uint8_t pad_[3];
+  uint32_t leg_offset;
+  uint32_t n_legs;
 } nigiri_journey;
+
+/* journey::leg (journey.h:44-75); `kind` names the alternative of uses_. */
+#define NIGIRI_AMD_LEG_RUN 0      /* journey::run_enter_exit               */
+#define NIGIRI_AMD_LEG_FOOTPATH 1 /* footpath                              */
+#define NIGIRI_AMD_LEG_OFFSET 2   /* routing::offset (intermodal)          */
+typedef struct nigiri_leg {
+  int32_t kind;
+  uint32_t from, to;           /* locations (START = 0, END = 1)          */
+  int32_t dep_time, arr_time;  /* unix minutes                            */
+  uint32_t transport;          /* RUN: transport index                    */
+  uint32_t route;              /* RUN: route of the transport             */
+  uint16_t day;                /* RUN: traffic day (day_idx_t)            */
+  uint16_t stop_from;          /* RUN: stop range [stop_from, stop_to) of */
+  uint16_t stop_to;            /*      the route                          */
+  int16_t duration;            /* FOOTPATH / OFFSET: minutes              */
+  uint32_t target;             /* FOOTPATH / OFFSET: target location      */
+} nigiri_leg;
 
 /* Per-query outcome of a batched search. */
 typedef struct nigiri_result {

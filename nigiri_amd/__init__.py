@@ -72,7 +72,27 @@ class Query(C.Structure):
 
 class Journey(C.Structure):
     _fields_ = [("start_time", C.c_int32), ("dest_time", C.c_int32), ("dest", C.c_uint32),
-                ("transfers", C.c_uint8), ("pad_", C.c_uint8 * 3)]
+                ("transfers", C.c_uint8), ("pad_", C.c_uint8 * 3),
+                ("leg_offset", C.c_uint32), ("n_legs", C.c_uint32)]
+
+
+LEG_RUN, LEG_FOOTPATH, LEG_OFFSET = 0, 1, 2
+
+
+class Leg(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("from_", C.c_uint32), ("to", C.c_uint32),
+                ("dep_time", C.c_int32), ("arr_time", C.c_int32), ("transport", C.c_uint32),
+                ("route", C.c_uint32), ("day", C.c_uint16), ("stop_from", C.c_uint16),
+                ("stop_to", C.c_uint16), ("duration", C.c_int16), ("target", C.c_uint32)]
+
+    def as_tuple(self):
+        """(kind, from, to, dep, arr, what): what is (transport, route, day, stop_from, stop_to)
+        for a run and (target, duration) for a footpath or an offset."""
+        if self.kind == LEG_RUN:
+            what = (self.transport, self.route, self.day, self.stop_from, self.stop_to)
+        else:
+            what = (self.target, self.duration)
+        return (self.kind, self.from_, self.to, self.dep_time, self.arr_time, what)
 
 
 class Result(C.Structure):

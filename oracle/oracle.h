@@ -38,16 +38,24 @@ typedef struct oracle_stats {
   uint64_t alg_bytes; /* algorithmic bytes (model in oracle.cc)          */
 } oracle_stats;
 
-/* Full search of one query (search<Dir, raptor<...>>::execute).
- * Journeys go to out[0 .. *n_out).  State after the last epoch goes to the
+/* Full search of one query (search<Dir, raptor<...>>::execute), including
+ * journey reconstruction (raptor/reconstruct.cc, optimize_footpaths.cc).
+ * Journeys go to out[0 .. *n_out), their legs to the optional legs array
+ * (*n_legs = legs needed).  State after the last epoch goes to the
  * optional arrays: round_times[16][n_locations], best/tmp[n_locations],
  * time_at_dest[16], lb[n_locations].  info = {base_day, fastest_direct,
  * interval_from, interval_to}. */
 int oracle_search(const nigiri_tt_view* tt, const nigiri_query* q,
                   nigiri_journey* out, uint32_t cap, uint32_t* n_out,
+                  nigiri_leg* legs, uint32_t leg_cap, uint32_t* n_legs,
                   int16_t* round_times, int16_t* best, int16_t* tmp,
                   int16_t* time_at_dest, uint16_t* lb, int32_t* info,
                   oracle_stats* stats);
+
+/* optimize_footpaths (raptor/optimize_footpaths.cc:444-453) on n legs in
+ * travel order, in place; q is the journey-direction query. */
+int oracle_optimize_footpaths(const nigiri_tt_view* tt, const nigiri_query* q,
+                              nigiri_leg* legs, uint32_t n);
 
 /* Lower bounds only (routing/dijkstra.cc:18-67). */
 int oracle_lb(const nigiri_tt_view* tt, const nigiri_query* q, uint16_t* lb);

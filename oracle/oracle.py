@@ -41,13 +41,16 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        from nigiri_amd import TtView, Query, Journey  # struct layouts only
+        from nigiri_amd import TtView, Query, Journey, Leg  # struct layouts only
         h = C.CDLL(LIB_PATH)
         P = C.POINTER
         h.oracle_search.restype = C.c_int
         h.oracle_search.argtypes = [P(TtView), P(Query), P(Journey), C.c_uint32, P(C.c_uint32),
+                                    P(Leg), C.c_uint32, P(C.c_uint32),
                                     P(C.c_int16), P(C.c_int16), P(C.c_int16), P(C.c_int16),
                                     P(C.c_uint16), P(C.c_int32), P(Stats)]
+        h.oracle_optimize_footpaths.restype = C.c_int
+        h.oracle_optimize_footpaths.argtypes = [P(TtView), P(Query), P(Leg), C.c_uint32]
         h.oracle_lb.restype = C.c_int
         h.oracle_lb.argtypes = [P(TtView), P(Query), P(C.c_uint16)]
         h.oracle_fastest_direct.restype = C.c_int
@@ -66,12 +69,15 @@ def _p(a, t):
     return a.ctypes.data_as(C.POINTER(t))
 
 
-def search(tt, q, state=False, cap=4096):
-    """Runs one query; returns journeys (start, dest_time, dest, transfers) and optionally state."""
-    from nigiri_amd import Journey
+def search(tt, q, state=False, cap=4096, leg_cap=1 << 15):
+    """Runs one query; returns journeys (start, dest_time, dest, transfers), their legs
+    (one list of Leg.as_tuple() per journey) and optionally the state."""
+    from nigiri_amd import Journey, Leg
     L = tt.n_locations
     jr = (Journey * cap)()
+    lg = (Leg * leg_cap)()
     n = C.c_uint32()
+    n_legs = C.c_uint32()
     info = np.zeros(4, np.int32)
     stats = Stats()
     rt = best = tmp = tad = lb = None
@@ -84,17 +90,31 @@ def search(tt, q, state=False, cap=4096):
         lb = np.zeros(L, np.uint16)
         args = [_p(rt, C.c_int16), _p(best, C.c_int16), _p(tmp, C.c_int16), _p(tad, C.c_int16),
                 _p(lb, C.c_uint16)]
-    rc = lib().oracle_search(C.byref(tt.view), C.byref(q), jr, cap, C.byref(n), *args,
-                             _p(info, C.c_int32), C.byref(stats))
+    rc = lib().oracle_search(C.byref(tt.view), C.byref(q), jr, cap, C.byref(n), lg, leg_cap,
+                             C.byref(n_legs), *args, _p(info, C.c_int32), C.byref(stats))
     if rc != 0:
         raise RuntimeError("oracle_search failed")
-    js = [(jr[i].start_time, jr[i].dest_time, jr[i].dest, jr[i].transfers)
-          for i in range(min(n.value, cap))]
-    out = {"journeys": js, "base_day": int(info[0]), "fastest_direct": int(info[1]),
-           "stats": stats.as_dict()}
+    if n.value > cap or n_legs.value > leg_cap:
+        raise RuntimeError("oracle_search: output capacity exceeded")
+    k = n.value
+    js = [(jr[i].start_time, jr[i].dest_time, jr[i].dest, jr[i].transfers) for i in range(k)]
+    legs = [[lg[jr[i].leg_offset + x].as_tuple() for x in range(jr[i].n_legs)]
+            for i in range(k)]
+    out = {"journeys": js, "legs": legs, "base_day": int(info[0]),
+           "fastest_direct": int(info[1]), "stats": stats.as_dict()}
     if state:
         out.update(round_times=rt, best=best, tmp=tmp, time_at_dest=tad, lb=lb)
     return out
+
+
+def optimize_footpaths(tt, q, legs):
+    """Runs optimize_footpaths on a list of Leg structures (travel order); returns the
+    optimised legs as Leg.as_tuple() tuples."""
+    from nigiri_amd import Leg
+    arr = (Leg * max(len(legs), 1))(*legs)
+    if lib().oracle_optimize_footpaths(C.byref(tt.view), C.byref(q), arr, len(legs)) != 0:
+        raise RuntimeError("oracle_optimize_footpaths failed")
+    return [arr[i].as_tuple() for i in range(len(legs))]
 
 
 def lower_bounds(tt, q):

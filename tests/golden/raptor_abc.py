@@ -39,3 +39,29 @@ BWD_EXPECTED = [("03:00", "05:15", 1), ("03:30", "05:45", 1)]
 INTERMODAL_START = ("A", 10)
 INTERMODAL_DEST = ("C", 15)
 INTERMODAL_FWD_EXPECTED = [("05:20", "08:00", 1), ("05:50", "08:30", 1)]
+
+# The legs of the journeys above as nigiri's tests print them (the
+# reconstruction, src/routing/raptor/reconstruct.cc, then optimize_footpaths):
+# ("run", from, to, dep, arr, train) with the stop range [0, 2) of the
+# two-stop line, ("footpath", from, to, dep, arr, duration),
+# ("offset", from, to, dep, arr, target, duration); times "HH:MM" UTC.
+FWD_LEGS = [
+    [("run", "A", "B", "05:00", "06:00", "1337"), ("footpath", "B", "B", "06:00", "06:02", 2),
+     ("run", "B", "C", "06:15", "07:15", "7331")],
+    [("run", "A", "B", "05:30", "06:30", "1337"), ("footpath", "B", "B", "06:30", "06:32", 2),
+     ("run", "B", "C", "06:45", "07:45", "7331")],
+]
+BWD_LEGS = [
+    [("run", "A", "B", "03:00", "04:00", "1337"), ("footpath", "B", "B", "04:00", "04:02", 2),
+     ("run", "B", "C", "04:15", "05:15", "7331")],
+    [("run", "A", "B", "03:30", "04:30", "1337"), ("footpath", "B", "B", "04:30", "04:32", 2),
+     ("run", "B", "C", "04:45", "05:45", "7331")],
+]
+INTERMODAL_FWD_LEGS = [
+    [("offset", "START", "A", "05:20", "05:30", "A", 10),
+     ("run", "A", "B", "05:30", "06:30", "1337"), ("footpath", "B", "B", "06:30", "06:32", 2),
+     ("run", "B", "C", "06:45", "07:45", "7331"), ("offset", "C", "END", "07:45", "08:00", "C", 15)],
+    [("offset", "START", "A", "05:50", "06:00", "A", 10),
+     ("run", "A", "B", "06:00", "07:00", "1337"), ("footpath", "B", "B", "07:00", "07:02", 2),
+     ("run", "B", "C", "07:15", "08:15", "7331"), ("offset", "C", "END", "08:15", "08:30", "C", 15)],
+]

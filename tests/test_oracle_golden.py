@@ -116,3 +116,48 @@ def test_raptor_intermodal_forward_golden():
     js = run(tt, qs[0])
     want = [(day0 + hm(a), day0 + hm(b), 1, k) for a, b, k in G.INTERMODAL_FWD_EXPECTED]
     assert sorted(js) == sorted(want)
+
+
+def expected_legs(spec_legs, loc, day0, traffic_day):
+    name = {"START": 0, "END": 1, **loc}
+    out = []
+    for leg in spec_legs:
+        kind, fr, to, dep, arr = leg[:5]
+        row = [name[fr], name[to], day0 + hm(dep), day0 + hm(arr)]
+        if kind == "run":
+            # the line is fixed by the stops (A -> B only on 1337, B -> C only on 7331)
+            out.append((na.LEG_RUN, *row, (None, None, traffic_day, 0, 2)))
+        elif kind == "footpath":
+            out.append((na.LEG_FOOTPATH, *row, (name[fr], leg[5])))
+        else:
+            out.append((na.LEG_OFFSET, *row, (name[leg[5]], leg[6])))
+    return out
+
+
+def comparable(legs):
+    # runs: compare traffic day and stop range; with the stops and times they fix the trip
+    return [l if l[0] != na.LEG_RUN else (*l[:5], l[5][2:]) for l in legs]
+
+
+@pytest.mark.parametrize("direction", ["fwd", "bwd", "intermodal"])
+def test_reconstructed_legs_golden(direction):
+    # legs printed by raptor_test.cc:18-112 and raptor_intermodal_test.cc:16-50
+    tt, loc, day0 = abc_timetable()
+    traffic_day = (day0 - tt.view.internal_from) // 1440
+    iv = (day0 + hm(G.INTERVAL[0]), day0 + hm(G.INTERVAL[1]))
+    if direction == "fwd":
+        spec = dict(start=[(loc["A"], 0)], dest=[(loc["C"], 0)], interval=iv, dir=na.DIR_FWD)
+        want = G.FWD_LEGS
+    elif direction == "bwd":
+        spec = dict(start=[(loc["C"], 0)], dest=[(loc["A"], 0)], interval=iv, dir=na.DIR_BWD)
+        want = G.BWD_LEGS
+    else:
+        s, sd = G.INTERMODAL_START
+        d, dd = G.INTERMODAL_DEST
+        spec = dict(start=[(loc[s], sd)], dest=[(loc[d], dd)], interval=iv,
+                    start_match_mode=na.MATCH_INTERMODAL, dest_match_mode=na.MATCH_INTERMODAL)
+        want = G.INTERMODAL_FWD_LEGS
+    out = oracle.search(tt, na.Queries.build([spec])[0])
+    order = sorted(range(len(out["journeys"])), key=lambda i: min(out["journeys"][i][:2]))
+    got = [comparable(out["legs"][i]) for i in order]
+    assert got == [comparable(expected_legs(w, loc, day0, traffic_day)) for w in want]

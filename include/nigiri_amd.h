@@ -28,6 +28,9 @@ extern "C" {
 #define NIGIRI_AMD_EDEVICE (-3)  /* HIP runtime error                        */
 #define NIGIRI_AMD_ECAPACITY (-4) /* request exceeds slots / state memory   */
 #define NIGIRI_AMD_EUNSUPPORTED (-5) /* query feature not supported here    */
+#define NIGIRI_AMD_ERECONSTRUCT (-6) /* no legs for the journey: nigiri's
+                                        reconstruct throws (reconstruct.cc:
+                                        159, 738-741, 794-796)              */
 
 #define NIGIRI_AMD_DIR_FWD 0
 #define NIGIRI_AMD_DIR_BWD 1
@@ -242,12 +245,17 @@ void nigiri_amd_engine_destroy(nigiri_amd_engine* e);
 
 /* Full search of n queries (replaces routing::raptor_search,
  * include/nigiri/routing/raptor_search.h:9-17, and search<>::execute,
- * search.h:204-345).  Journeys of query i are written to
- * journeys[res[i].journey_offset ...]; journey_capacity bounds the total.
- * Queries run batched: up to lanes * n_groups at once, epoch-synchronous. */
+ * search.h:204-345), including the reconstruction of every journey's legs
+ * (search.h:457-475; journeys without legs are dropped as nigiri drops
+ * them, search.h:335-336).  Journeys of query i are written to
+ * journeys[res[i].journey_offset ...], their legs to legs[journey.leg_offset
+ * ...]; journey_capacity and leg_capacity bound the totals (legs may be
+ * NULL with leg_capacity 0: legs are then not written).  Queries run
+ * batched: up to lanes * n_groups at once. */
 int nigiri_amd_search(nigiri_amd_engine* e, const nigiri_query* qs,
                       uint32_t n, nigiri_result* res, nigiri_journey* journeys,
-                      uint32_t journey_capacity);
+                      uint32_t journey_capacity, nigiri_leg* legs,
+                      uint32_t leg_capacity);
 
 /* ---- Algo-level entry points for one slot: the method set of
  * raptor<> (raptor.h:174-414) / gpu_raptor<> (gpu/raptor.h:83-99) that
@@ -281,6 +289,16 @@ int nigiri_amd_algo_execute(nigiri_amd_engine* e, uint32_t slot,
                             int32_t start_time, uint8_t max_transfers,
                             int32_t worst_time_at_dest, int16_t* dest_times,
                             uint32_t* n_rounds);
+/* Reconstructs the legs of journey *j (its start_time, dest_time, dest and
+ * transfers) from the slot's state as the last execute left it (replaces
+ * raptor<>::reconstruct, raptor.h:416-423, i.e. reconstruct_journey,
+ * reconstruct.cc:879-902, for queries without vias).  q is the search's
+ * query (its offsets, match modes and is_interval are read).  Legs go to
+ * legs[0 .. j->n_legs); returns NIGIRI_AMD_ERECONSTRUCT where nigiri's
+ * reconstruct throws. */
+int nigiri_amd_algo_reconstruct(nigiri_amd_engine* e, uint32_t slot,
+                                const nigiri_query* q, nigiri_journey* j,
+                                nigiri_leg* legs, uint32_t leg_capacity);
 /* Copies the slot's state to host arrays (any pointer may be NULL):
  * round_times[max_k][n_locations], best[n_locations], tmp[n_locations],
  * time_at_dest[max_k], lb[n_locations] (the lower bounds the search used;
@@ -303,6 +321,10 @@ typedef struct nigiri_engine_stats {
   uint64_t transfer_launches;
   double transfer_ms;
   uint64_t work[16];
+  uint64_t reconstructed;   /* journeys handed to the reconstruction        */
+  double reconstruct_ms;    /* host wall time of the reconstruction steps   */
+  double legs_ms;           /*   of which writing the legs on the host      */
+  double results_ms;        /* host wall time of the journey extraction     */
 } nigiri_engine_stats;
 
 int nigiri_amd_engine_profile(nigiri_amd_engine* e, int enable);
@@ -331,7 +353,8 @@ int nigiri_amd_engine_configure(nigiri_amd_engine* e,
 int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs,
                      const double* arrival_ms, uint32_t n,
                      nigiri_result* res, nigiri_journey* journeys,
-                     uint32_t journey_capacity, double* wall_ms);
+                     uint32_t journey_capacity, nigiri_leg* legs,
+                     uint32_t leg_capacity, double* wall_ms);
 
 #ifdef __cplusplus
 }

@@ -13,7 +13,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libnigiri_amd.so")
 
-OK, EINVAL, ENOMEM, EDEVICE, ECAPACITY, EUNSUPPORTED = 0, -1, -2, -3, -4, -5
+OK, EINVAL, ENOMEM, EDEVICE, ECAPACITY, EUNSUPPORTED, ERECONSTRUCT = 0, -1, -2, -3, -4, -5, -6
 DIR_FWD, DIR_BWD = 0, 1
 MATCH_EXACT, MATCH_INTERMODAL = 0, 1
 N_SPECIAL = 9
@@ -116,7 +116,9 @@ class EngineStats(C.Structure):
     _fields_ = [("epochs", C.c_uint64), ("rounds", C.c_uint64),
                 ("route_scan_launches", C.c_uint64), ("route_scan_ms", C.c_double),
                 ("transfer_launches", C.c_uint64), ("transfer_ms", C.c_double),
-                ("work", C.c_uint64 * 16)]
+                ("work", C.c_uint64 * 16), ("reconstructed", C.c_uint64),
+                ("reconstruct_ms", C.c_double), ("legs_ms", C.c_double),
+                ("results_ms", C.c_double)]
 
 
 class EngineConfig(C.Structure):
@@ -148,13 +150,15 @@ SIGNATURES = {
     "nigiri_amd_engine_state_bytes": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "nigiri_amd_engine_destroy": (None, [_P]),
     "nigiri_amd_search": (C.c_int, [_P, C.POINTER(Query), C.c_uint32, C.POINTER(Result),
-                                    C.POINTER(Journey), C.c_uint32]),
+                                    C.POINTER(Journey), C.c_uint32, C.POINTER(Leg), C.c_uint32]),
     "nigiri_amd_algo_init": (C.c_int, [_P, C.c_uint32, C.POINTER(AlgoInit)]),
     "nigiri_amd_algo_reset_arrivals": (C.c_int, [_P, C.c_uint32]),
     "nigiri_amd_algo_next_start_time": (C.c_int, [_P, C.c_uint32]),
     "nigiri_amd_algo_add_start": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_int32]),
     "nigiri_amd_algo_execute": (C.c_int, [_P, C.c_uint32, C.c_int32, C.c_uint8, C.c_int32,
                                           C.POINTER(C.c_int16), C.POINTER(C.c_uint32)]),
+    "nigiri_amd_algo_reconstruct": (C.c_int, [_P, C.c_uint32, C.POINTER(Query),
+                                              C.POINTER(Journey), C.POINTER(Leg), C.c_uint32]),
     "nigiri_amd_algo_download": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_int16),
                                            C.POINTER(C.c_int16), C.POINTER(C.c_int16),
                                            C.POINTER(C.c_int16), C.POINTER(C.c_uint16)]),
@@ -163,7 +167,7 @@ SIGNATURES = {
     "nigiri_amd_engine_stats": (C.c_int, [_P, C.POINTER(EngineStats), C.c_int]),
     "nigiri_amd_serve": (C.c_int, [_P, C.POINTER(Query), C.POINTER(C.c_double), C.c_uint32,
                                    C.POINTER(Result), C.POINTER(Journey), C.c_uint32,
-                                   C.POINTER(C.c_double)]),
+                                   C.POINTER(Leg), C.c_uint32, C.POINTER(C.c_double)]),
 }
 
 _lib_handle = None
@@ -391,14 +395,17 @@ class Image:
             pass
 
 
-def _collect(res, jr, n):
+def _collect(res, jr, n, lg=None):
     out = []
     for i in range(n):
         r = res[i]
-        js = [(jr[r.journey_offset + j].start_time, jr[r.journey_offset + j].dest_time,
-               jr[r.journey_offset + j].dest, jr[r.journey_offset + j].transfers)
-              for j in range(r.n_journeys)]
-        out.append({"status": r.status, "journeys": js, "n_epochs": r.n_epochs,
+        js, legs = [], []
+        for x in range(r.n_journeys):
+            j = jr[r.journey_offset + x]
+            js.append((j.start_time, j.dest_time, j.dest, j.transfers))
+            if lg is not None:
+                legs.append([lg[j.leg_offset + y].as_tuple() for y in range(j.n_legs)])
+        out.append({"status": r.status, "journeys": js, "legs": legs, "n_epochs": r.n_epochs,
                     "base_day": r.base_day, "fastest_direct": r.fastest_direct, "slot": r.slot,
                     "latency_ms": r.latency_ms, "queue_ms": r.queue_ms})
     return out
@@ -439,24 +446,40 @@ class Engine:
         d["work"] = {n: st.work[i] for i, n in enumerate(WORK_NAMES)}
         return d
 
-    def search(self, queries, journey_capacity=None):
+    def search(self, queries, journey_capacity=None, legs=True, leg_capacity=None):
+        """Runs the queries; each result holds its journeys and (legs=True) their legs as
+        Leg.as_tuple() lists.  legs="raw" has the library write the legs but leaves them in
+        the ctypes array (returned as the "legs" entry of the first result)."""
         n = len(queries)
-        cap = journey_capacity or max(64, 64 * n)
+        cap = journey_capacity or max(64, 32 * n)
         res = (Result * max(n, 1))()
         jr = (Journey * cap)()
-        _check(lib().nigiri_amd_search(self.handle, queries.q, n, res, jr, cap))
-        return _collect(res, jr, n)
+        leg_cap = (leg_capacity or 24 * cap) if legs else 0
+        lg = (Leg * max(leg_cap, 1))() if legs else None
+        _check(lib().nigiri_amd_search(self.handle, queries.q, n, res, jr, cap, lg, leg_cap))
+        return _collect(res, jr, n, lg if legs is True else None)
 
-    def serve(self, queries, arrival_ms, journey_capacity=None):
+    def serve(self, queries, arrival_ms, journey_capacity=None, legs=True, leg_capacity=None):
         n = len(queries)
-        cap = journey_capacity or max(64, 64 * n)
+        cap = journey_capacity or max(64, 32 * n)
         res = (Result * max(n, 1))()
         jr = (Journey * cap)()
+        leg_cap = (leg_capacity or 24 * cap) if legs else 0
+        lg = (Leg * max(leg_cap, 1))() if legs else None
         arr = np.ascontiguousarray(arrival_ms, np.float64)
         wall = C.c_double()
         _check(lib().nigiri_amd_serve(self.handle, queries.q, ptr(arr, C.c_double), n, res, jr,
-                                      cap, C.byref(wall)))
-        return _collect(res, jr, n), wall.value
+                                      cap, lg, leg_cap, C.byref(wall)))
+        return _collect(res, jr, n, lg if legs is True else None), wall.value
+
+    def reconstruct(self, slot, query, journey, leg_capacity=64):
+        """Legs of journey (start, dest_time, dest, transfers) from the slot's state as its
+        last epoch left it (nigiri's raptor::reconstruct); query is a Query struct."""
+        j = Journey(journey[0], journey[1], journey[2], journey[3])
+        lg = (Leg * leg_capacity)()
+        _check(lib().nigiri_amd_algo_reconstruct(self.handle, slot, C.byref(query), C.byref(j),
+                                                 lg, leg_capacity))
+        return [lg[i].as_tuple() for i in range(j.n_legs)]
 
     def download(self, slot):
         L = self.image.tt.n_locations

@@ -52,6 +52,12 @@ struct dev_image {
   std::uint32_t const* fp_in;
   std::uint32_t const* fp_in_to_out;  // in-edge index -> out-edge index
   std::uint32_t const* fp_out_to_in;  // out-edge index -> in-edge index
+  // the stops of every location's routes: for location l, in the order of
+  // location_routes[l] and then of the stops, each (route, stop index) where
+  // the route visits l (journey reconstruction, reconstruct.cc:456-519)
+  std::uint32_t const* lrs_offsets;
+  std::uint32_t const* lrs_route;
+  std::uint16_t const* lrs_stop;
   // lower-bound graphs (loader/build_lb_graph.h), packed like footpaths
   std::uint32_t const* lb_fwd_offsets;
   std::uint32_t const* lb_fwd;

@@ -29,6 +29,7 @@
 #include "capi_util.h"
 #include "handles.h"
 #include "device.h"
+#include "journey_legs.h"
 #include "kernels.h"
 #include "search_host.h"
 #include "timetable.h"
@@ -99,6 +100,96 @@ struct nigiri_amd_image {
   dev_alloc mem;
 };
 
+// host threads that write the legs of a step's journeys (with the calling
+// thread); the lower-bound workers and query preparation use others
+constexpr unsigned kLegWorkers = 9U;
+
+// Fork-join pool of host threads for the per-journey work of a step (the
+// legs of reconstructed journeys).  parallel_for runs f(0 .. n-1) on the
+// workers and the calling thread and returns when all calls are done; the
+// first exception is rethrown in the caller.
+class host_pool {
+ public:
+  explicit host_pool(unsigned n) {
+    for (unsigned i = 0; i != n; ++i) workers_.emplace_back([this]() { work(); });
+  }
+  ~host_pool() {
+    {
+      std::lock_guard<std::mutex> lk{mu_};
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : workers_) t.join();
+  }
+  void parallel_for(std::size_t n, std::function<void(std::size_t)> f) {
+    start(n, std::move(f));
+    wait();
+  }
+  // starts f(0 .. n-1) on the workers; wait() joins (the caller helps)
+  void start(std::size_t n, std::function<void(std::size_t)> f) {
+    wait();
+    std::lock_guard<std::mutex> lk{mu_};
+    job_ = std::move(f);
+    n_ = n;
+    next_.store(0U);
+    busy_ = workers_.size();
+    error_ = nullptr;
+    active_ = true;
+    ++gen_;
+    cv_.notify_all();
+  }
+  void wait() {
+    if (!active_) return;
+    drain();
+    std::unique_lock<std::mutex> lk{mu_};
+    done_cv_.wait(lk, [&]() { return busy_ == 0U; });
+    active_ = false;
+    job_ = nullptr;
+    if (error_) {
+      auto e = error_;
+      error_ = nullptr;
+      std::rethrow_exception(e);
+    }
+  }
+
+ private:
+  void drain() {
+    for (auto i = next_.fetch_add(1U); i < n_; i = next_.fetch_add(1U)) {
+      try {
+        job_(i);
+      } catch (...) {
+        std::lock_guard<std::mutex> lk{mu_};
+        if (!error_) error_ = std::current_exception();
+      }
+    }
+  }
+  void work() {
+    std::uint64_t seen = 0U;
+    while (true) {
+      {
+        std::unique_lock<std::mutex> lk{mu_};
+        cv_.wait(lk, [&]() { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+      }
+      drain();
+      std::lock_guard<std::mutex> lk{mu_};
+      if (--busy_ == 0U) done_cv_.notify_all();
+    }
+  }
+  std::vector<std::thread> workers_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  std::function<void(std::size_t)> job_;
+  bool active_ = false;  // a job was started and not yet joined (caller thread only)
+  std::size_t n_ = 0U;
+  std::atomic<std::size_t> next_{0U};
+  std::size_t busy_ = 0U;
+  std::uint64_t gen_ = 0U;
+  bool stop_ = false;
+  std::exception_ptr error_;
+};
+
 struct nigiri_amd_engine {
   nigiri_amd_image* img = nullptr;
   std::uint32_t W = 64U, G = 1U, K = 16U, S = 64U;
@@ -152,6 +243,15 @@ struct nigiri_amd_engine {
   std::uint64_t* d_dest_ops = nullptr;
   std::uint8_t* d_dest_set = nullptr;
   std::size_t dest_ops_cap = 0U;
+  // journey reconstruction: jobs, egress offsets of intermodal jobs, records
+  rc_job* d_rc_jobs = nullptr;
+  rc_out* d_rc_out = nullptr;
+  std::size_t rc_cap = 0U;
+  std::uint32_t* d_rc_off_loc = nullptr;
+  std::int16_t* d_rc_off_dur = nullptr;
+  std::size_t rc_off_cap = 0U;
+  std::vector<rc_out> h_rc_out;
+  std::unique_ptr<host_pool> pool;  // writes the legs of a step's journeys
   std::vector<void*> grow_ptrs;
   // pinned host staging
   pinned<std::uint16_t> h_lb;
@@ -255,10 +355,34 @@ void build_image(nigiri_amd_image& img, timetable const& tt, hipStream_t s) {
   }
   d.fp_in_to_out = m.upload(in_to_out, s);
   d.fp_out_to_in = m.upload(out_to_in, s);
+  // (route, stop) visits of every location, in location_routes order
+  {
+    std::vector<std::uint32_t> off{0U}, route;
+    std::vector<std::uint16_t> stop;
+    for (auto l = 0U; l != tt.n_locations; ++l) {
+      for (auto j = tt.location_route_offsets[l]; j != tt.location_route_offsets[l + 1U]; ++j) {
+        auto const r = tt.location_routes[j];
+        auto const s0 = tt.route_stop_offsets[r], n = tt.route_stop_offsets[r + 1U] - s0;
+        if (n > 0xFFFFU) throw status_error{NIGIRI_AMD_EUNSUPPORTED, "route with more than 65535 stops"};
+        for (auto i = 0U; i != n; ++i) {
+          if ((tt.route_stops[s0 + i] & kStopLocMask) != l) continue;
+          route.push_back(r);
+          stop.push_back(static_cast<std::uint16_t>(i));
+        }
+      }
+      off.push_back(static_cast<std::uint32_t>(route.size()));
+    }
+    d.lrs_offsets = m.upload(off, s);
+    d.lrs_route = m.upload(route, s);
+    d.lrs_stop = m.upload(stop, s);
+    // the uploads are asynchronous: keep the host vectors until they are done
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
   d.lb_fwd_offsets = m.upload(tt.lb_fwd_offsets, s);
   d.lb_fwd = m.upload(tt.lb_fwd, s);
   d.lb_bwd_offsets = m.upload(tt.lb_bwd_offsets, s);
   d.lb_bwd = m.upload(tt.lb_bwd, s);
+  HIP_CHECK(hipStreamSynchronize(s));  // host vectors above go out of scope
 }
 
 std::int16_t clamp16(long long v) {
@@ -444,6 +568,48 @@ struct lb_pipeline {
   }
 };
 
+// Runs the device journey reconstruction of `jobs` on the engine's stream;
+// their records are in e.h_rc_out afterwards.
+void run_reconstruct(nigiri_amd_engine& e, bool fwd, std::vector<rc_job> const& jobs,
+                     std::vector<std::uint32_t> const& off_loc,
+                     std::vector<std::int16_t> const& off_dur) {
+  auto const stream = e.stream;
+  for (auto const& j : jobs) {
+    if (j.slot >= e.S || j.dest >= e.st.L || j.transfers + 2U > e.K ||
+        static_cast<std::size_t>(j.off_begin) + j.n_off > off_loc.size()) {
+      throw status_error{NIGIRI_AMD_EINVAL, "reconstruct: job out of range"};
+    }
+  }
+  if (jobs.size() > e.rc_cap) {
+    e.rc_cap = jobs.size() * 2U;
+    e.d_rc_jobs = e.grow(e.d_rc_jobs, e.rc_cap);
+    e.d_rc_out = e.grow(e.d_rc_out, e.rc_cap);
+  }
+  if (off_loc.size() > e.rc_off_cap) {
+    e.rc_off_cap = off_loc.size() * 2U;
+    e.d_rc_off_loc = e.grow(e.d_rc_off_loc, e.rc_off_cap);
+    e.d_rc_off_dur = e.grow(e.d_rc_off_dur, e.rc_off_cap);
+  }
+  HIP_CHECK(hipMemcpyAsync(e.d_rc_jobs, jobs.data(), sizeof(rc_job) * jobs.size(), hipMemcpyHostToDevice, stream));
+  if (!off_loc.empty()) {
+    HIP_CHECK(hipMemcpyAsync(e.d_rc_off_loc, off_loc.data(), 4U * off_loc.size(), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(e.d_rc_off_dur, off_dur.data(), 2U * off_dur.size(), hipMemcpyHostToDevice, stream));
+  }
+  rc_args a{};
+  a.im = e.img->d;
+  a.st = e.st;
+  a.jobs = e.d_rc_jobs;
+  a.n_jobs = static_cast<std::uint32_t>(jobs.size());
+  a.off_loc = e.d_rc_off_loc;
+  a.off_dur = e.d_rc_off_dur;
+  a.out = e.d_rc_out;
+  launch_reconstruct(fwd, e.W, a, stream);
+  HIP_CHECK(hipGetLastError());
+  e.h_rc_out.resize(jobs.size());
+  HIP_CHECK(hipMemcpyAsync(e.h_rc_out.data(), e.d_rc_out, sizeof(rc_out) * jobs.size(), hipMemcpyDeviceToHost, stream));
+  HIP_CHECK(hipStreamSynchronize(stream));
+}
+
 struct runner {
   nigiri_amd_engine& e;
   timetable const& tt;
@@ -455,9 +621,23 @@ struct runner {
   std::function<void(std::int32_t)> on_admit;     // query takes a slot
   std::vector<slot_info> slots;
   std::uint64_t rounds = 0U, epochs = 0U;
+  // legs being written on the host pool, and the queries waiting for them
+  bool legs_pending = false;
+  std::vector<std::pair<std::int32_t, std::size_t>> rc_owner;
+  std::vector<std::int32_t> completing;
 
   runner(nigiri_amd_engine& eng, timetable const& t, bool f, std::vector<prepared_query>& p)
       : e{eng}, tt{t}, fwd{f}, pq{p}, slots(eng.S) {}
+  runner(runner const&) = delete;
+  runner& operator=(runner const&) = delete;
+  ~runner() {
+    // the pool's job refers to this runner: join it (errors were reported
+    // through the exception that ends the runner early)
+    try {
+      e.pool->wait();
+    } catch (...) {
+    }
+  }
 
   round_args rargs() const {
     round_args a{};
@@ -672,7 +852,7 @@ struct runner {
       // after its first epoch a slot sees earlier rows in round_times
       if (slots[s].epoch != 0U) sweep[s / W] = 1U;
     }
-    if (!any) return false;
+    if (!any) return settle();
     auto const stream = e.stream;
     auto const& tr = e.st;
     std::vector<std::uint32_t> sweep_groups;
@@ -685,6 +865,7 @@ struct runner {
     launch_round(fwd, W, rargs(), e.d_sweep_groups, static_cast<std::uint32_t>(sweep_groups.size()), S,
                  static_cast<int>(e.grid_waves), stream);
     HIP_CHECK(hipGetLastError());
+    settle();  // while the round runs on the device
     ++rounds;
     ++e.stats.rounds;
     HIP_CHECK(hipMemcpyAsync(e.h_active.p, tr.grp_active, 8U * G, hipMemcpyDeviceToHost, stream));
@@ -711,17 +892,90 @@ struct runner {
     auto const per_slot = static_cast<std::size_t>(kMaxDestBits) * K;
     HIP_CHECK(hipMemcpyAsync(e.h_rows.p, tr.dest_rows, 2U * per_slot * finished.size(), hipMemcpyDeviceToHost, stream));
     HIP_CHECK(hipStreamSynchronize(stream));
+    auto const t0 = std::chrono::steady_clock::now();
     for (std::size_t j = 0; j != finished.size(); ++j) {
       auto& si = slots[finished[j]];
       si.running = false;
-      auto& p = pq[static_cast<std::size_t>(si.q)];
-      add_epoch_results(tt, p, si.epoch, e.h_rows.p + j * per_slot, K);
-      if (si.epoch + 1U == p.epochs_.size()) {
-        finish(p);
-        on_done(si.q);
+      add_epoch_results(tt, pq[static_cast<std::size_t>(si.q)], si.epoch, e.h_rows.p + j * per_slot, K);
+    }
+    auto const t1 = std::chrono::steady_clock::now();
+    reconstruct_new_journeys(finished);
+    e.stats.results_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    e.stats.reconstruct_ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
+    // queries whose last epoch ended complete once their legs are written
+    for (auto const s : finished) {
+      auto const& si = slots[s];
+      if (si.epoch + 1U == pq[static_cast<std::size_t>(si.q)].epochs_.size()) {
+        completing.push_back(si.q);
       }
     }
+    if (!legs_pending) settle();
     return true;
+  }
+
+  // Joins the leg writing started by the last step and completes the queries
+  // that waited for it.  Returns whether any query completed.
+  bool settle() {
+    if (legs_pending) {
+      auto const t0 = std::chrono::steady_clock::now();
+      e.pool->wait();
+      legs_pending = false;
+      e.stats.legs_ms +=
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    auto const any = !completing.empty();
+    for (auto const q : completing) {
+      finish(pq[static_cast<std::size_t>(q)]);
+      on_done(q);
+    }
+    completing.clear();
+    return any;
+  }
+
+  // Reconstructs the journeys that the epochs just ended added to their
+  // queries' result sets (search.h:457-475), against the state those epochs
+  // left: before any of these slots starts its next epoch.
+  void reconstruct_new_journeys(std::vector<std::uint32_t> const& finished) {
+    std::vector<rc_job> jobs;
+    std::vector<std::pair<std::int32_t, std::size_t>> owner;  // (query, journey)
+    std::vector<std::uint32_t> off_loc;
+    std::vector<std::int16_t> off_dur;
+    std::vector<std::size_t> cand;
+    for (auto const s : finished) {
+      auto const q = slots[s].q;
+      auto const& p = pq[static_cast<std::size_t>(q)];
+      reconstruction_candidates(p, cand);
+      if (cand.empty()) continue;
+      auto const off_begin = static_cast<std::uint32_t>(off_loc.size());
+      std::uint16_t n_off = 0U;
+      if (p.intermodal_dest_) {
+        for (auto const& [loc, dur] : p.dest_offsets_) {
+          off_loc.push_back(loc);
+          off_dur.push_back(dur);
+        }
+        n_off = static_cast<std::uint16_t>(p.dest_offsets_.size());
+      }
+      for (auto const i : cand) {
+        auto const& j = p.results_[i];
+        jobs.push_back(rc_job{s, j.dest_, off_begin, n_off, j.transfers_,
+                              static_cast<std::uint8_t>(p.intermodal_dest_ ? 1U : 0U)});
+        owner.emplace_back(q, i);
+      }
+    }
+    if (jobs.empty()) return;
+    run_reconstruct(e, fwd, jobs, off_loc, off_dur);
+    // the legs are written by the pool while the next round runs on the
+    // device (settle() joins): journeys are independent, each writes only
+    // its own result element, the timetable is read-only, and neither these
+    // queries' results nor e.h_rc_out change before the join
+    rc_owner = std::move(owner);
+    e.pool->start(jobs.size(), [this](std::size_t x) {
+      auto& p = pq[static_cast<std::size_t>(rc_owner[x].first)];
+      assemble_legs(tt, p, e.h_rc_out[x], p.results_[rc_owner[x].second]);
+    });
+    legs_pending = true;
+    e.stats.reconstructed += jobs.size();
   }
 
   bool any_running() const {
@@ -840,9 +1094,10 @@ void prepare_all(timetable const& tt, nigiri_query const* qs, std::uint32_t n,
 
 std::uint32_t write_results(std::vector<prepared_query> const& pq, std::vector<int> const& status,
                             nigiri_result* res, nigiri_journey* journeys, std::uint32_t cap,
+                            nigiri_leg* legs, std::uint32_t leg_cap,
                             std::vector<std::uint32_t> const* epochs,
                             std::vector<double> const* latency, std::vector<double> const* queue = nullptr) {
-  std::uint32_t pos = 0U;
+  std::uint32_t pos = 0U, leg_pos = 0U;
   int overflow = 0;
   for (std::size_t i = 0; i != pq.size(); ++i) {
     auto& r = res[i];
@@ -863,7 +1118,17 @@ std::uint32_t write_results(std::vector<prepared_query> const& pq, std::vector<i
     if (queue != nullptr) r.queue_ms = (*queue)[i];
     for (auto const& j : p.results_) {
       if (pos < cap && journeys != nullptr) {
-        journeys[pos] = nigiri_journey{j.start_time_, j.dest_time_, j.dest_, j.transfers_, {}};
+        auto& out = journeys[pos];
+        out = nigiri_journey{j.start_time_, j.dest_time_, j.dest_, j.transfers_, {}, leg_pos, 0U};
+        if (legs != nullptr) {
+          if (leg_pos + j.legs_.size() <= leg_cap) {
+            std::copy(j.legs_.begin(), j.legs_.end(), legs + leg_pos);
+            out.n_legs = static_cast<std::uint32_t>(j.legs_.size());
+            leg_pos += out.n_legs;
+          } else {
+            overflow = 1;
+          }
+        }
         ++pos;
         ++r.n_journeys;
       } else {
@@ -912,18 +1177,6 @@ void run_batched(nigiri_amd_engine& e, bool fwd, std::vector<prepared_query>& pq
       pipe.wait_ready(1U);
     }
   }
-}
-
-void engine_prepare_all(timetable const& tt, nigiri_query const* qs, std::uint32_t n,
-                        std::vector<prepared_query>& pq, std::vector<int>& status,
-                        std::vector<std::string>& errors) {
-  prepare_all(tt, qs, n, pq, status, errors, true);
-}
-
-std::uint32_t engine_write_results(std::vector<prepared_query> const& pq, std::vector<int> const& status,
-                                   nigiri_result* res, nigiri_journey* journeys, std::uint32_t cap,
-                                   std::vector<double> const* latency) {
-  return write_results(pq, status, res, journeys, cap, nullptr, latency);
 }
 
 }  // namespace nigiri_amd
@@ -989,6 +1242,7 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
   return guarded([&]() {
     HIP_CHECK(hipSetDevice(img->device));
     auto e = std::make_unique<nigiri_amd_engine>();
+    e->pool = std::make_unique<host_pool>(kLegWorkers);
     e->img = img;
     e->W = lanes;
     e->G = n_groups;
@@ -1175,7 +1429,8 @@ int nigiri_amd_engine_stats(nigiri_amd_engine* e, nigiri_engine_stats* out, int 
 }
 
 int nigiri_amd_search(nigiri_amd_engine* e, const nigiri_query* qs, uint32_t n, nigiri_result* res,
-                      nigiri_journey* journeys, uint32_t journey_capacity) {
+                      nigiri_journey* journeys, uint32_t journey_capacity, nigiri_leg* legs,
+                      uint32_t leg_capacity) {
   if (e == nullptr || (n != 0U && (qs == nullptr || res == nullptr))) {
     set_error("search: null argument");
     return NIGIRI_AMD_EINVAL;
@@ -1205,9 +1460,10 @@ int nigiri_amd_search(nigiri_amd_engine* e, const nigiri_query* qs, uint32_t n, 
           *e, dir == 0, pq,
           [&]() { return head < queue.size() ? queue[head++] : -1; }, [](std::int32_t) {}, {});
     }
-    auto const w = write_results(pq, status, res, journeys, journey_capacity, nullptr, nullptr);
+    auto const w = write_results(pq, status, res, journeys, journey_capacity, legs, leg_capacity,
+                                 nullptr, nullptr);
     if (w == ~0U) {
-      set_error("search: journey_capacity too small");
+      set_error("search: journey_capacity or leg_capacity too small");
       return NIGIRI_AMD_ECAPACITY;
     }
     for (std::uint32_t i = 0; i != n; ++i) {
@@ -1226,7 +1482,7 @@ int nigiri_amd_search(nigiri_amd_engine* e, const nigiri_query* qs, uint32_t n, 
 // boundary while other slots keep running (continuous batching).
 int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs, const double* arrival_ms, uint32_t n,
                      nigiri_result* res, nigiri_journey* journeys, uint32_t journey_capacity,
-                     double* wall_ms) {
+                     nigiri_leg* legs, uint32_t leg_capacity, double* wall_ms) {
   if (e == nullptr || (n != 0U && (qs == nullptr || arrival_ms == nullptr || res == nullptr))) {
     set_error("serve: null argument");
     return NIGIRI_AMD_EINVAL;
@@ -1353,7 +1609,8 @@ int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs, const double*
     }
     for (auto& t : workers) t.join();
     if (wall_ms != nullptr) *wall_ms = ms_since(clock::now());
-    auto const w = write_results(pq, status, res, journeys, journey_capacity, nullptr, &latency, &queue);
+    auto const w = write_results(pq, status, res, journeys, journey_capacity, legs, leg_capacity,
+                                 nullptr, &latency, &queue);
     if (w == ~0U) {
       set_error("serve: journey_capacity too small");
       return NIGIRI_AMD_ECAPACITY;
@@ -1569,6 +1826,58 @@ int nigiri_amd_algo_execute(nigiri_amd_engine* e, uint32_t slot, int32_t start_t
     e->algo_new[slot] = 0U;
     e->algo_fresh[slot] = 0U;
     e->pending_starts[slot].clear();
+    return NIGIRI_AMD_OK;
+  });
+}
+
+int nigiri_amd_algo_reconstruct(nigiri_amd_engine* e, uint32_t slot, const nigiri_query* q,
+                                nigiri_journey* j, nigiri_leg* legs, uint32_t leg_capacity) {
+  if (e == nullptr || q == nullptr || j == nullptr || slot >= e->S || j->dest >= e->st.L ||
+      j->transfers + 2U > e->K || (legs == nullptr && leg_capacity != 0U)) {
+    set_error("algo_reconstruct: bad argument");
+    return NIGIRI_AMD_EINVAL;
+  }
+  return guarded([&]() {
+    HIP_CHECK(hipSetDevice(e->img->device));
+    auto const& tt = *e->img->tt;
+    auto const& sp = e->h_param[slot];
+    prepared_query p;
+    p.fwd_ = q->dir == NIGIRI_AMD_DIR_FWD;
+    p.ontrip_ = q->is_interval == 0;
+    p.intermodal_dest_ = sp.intermodal != 0U;
+    p.start_intermodal_ = q->start_match_mode == NIGIRI_AMD_MATCH_INTERMODAL;
+    p.base_ = sp.base;
+    p.tts_ = host_tts{sp.tts_default != 0U, sp.tts_min, sp.tts_add, sp.tts_factor};
+    for (auto i = 0U; i != q->n_start; ++i) p.start_offsets_.emplace_back(q->start_loc[i], q->start_dur[i]);
+    for (auto i = 0U; i != q->n_dest; ++i) p.dest_offsets_.emplace_back(q->dest_loc[i], q->dest_dur[i]);
+    auto const by_dur = [](auto const& a, auto const& b) { return a.second < b.second; };
+    std::stable_sort(p.start_offsets_.begin(), p.start_offsets_.end(), by_dur);
+    std::stable_sort(p.dest_offsets_.begin(), p.dest_offsets_.end(), by_dur);
+    std::vector<std::uint32_t> off_loc;
+    std::vector<std::int16_t> off_dur;
+    if (p.intermodal_dest_) {
+      for (auto const& [loc, dur] : p.dest_offsets_) {
+        off_loc.push_back(loc);
+        off_dur.push_back(dur);
+      }
+    }
+    std::vector<rc_job> jobs{rc_job{slot, j->dest, 0U, static_cast<std::uint16_t>(off_loc.size()),
+                                    j->transfers, sp.intermodal}};
+    run_reconstruct(*e, p.fwd_, jobs, off_loc, off_dur);
+    host_journey hj{j->start_time, j->dest_time, j->dest, j->transfers};
+    assemble_legs(tt, p, e->h_rc_out[0], hj);
+    j->leg_offset = 0U;
+    j->n_legs = 0U;
+    if (hj.error_) {
+      set_error("algo_reconstruct: reconstruction failed");
+      return NIGIRI_AMD_ERECONSTRUCT;
+    }
+    if (hj.legs_.size() > leg_capacity) {
+      set_error("algo_reconstruct: leg_capacity too small");
+      return NIGIRI_AMD_ECAPACITY;
+    }
+    std::copy(hj.legs_.begin(), hj.legs_.end(), legs);
+    j->n_legs = static_cast<std::uint32_t>(hj.legs_.size());
     return NIGIRI_AMD_OK;
   });
 }

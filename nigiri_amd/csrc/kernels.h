@@ -31,6 +31,7 @@
 #include <cstdint>
 
 #include "device.h"
+#include "rc_record.h"
 
 namespace nigiri_amd {
 
@@ -122,5 +123,26 @@ void launch_gather_slot(std::uint32_t W, dev_state const& st, std::uint32_t slot
 // extraction input, raptor.h:393-413)
 void launch_extract(std::uint32_t W, dev_state const& st, std::uint32_t const* slots,
                     std::uint32_t n_slots, hipStream_t stream);
+
+// Journey reconstruction (reconstruct.cc:162-877, no vias, no real-time
+// data, no time-dependent offsets) against a slot's round_times, right after
+// the epoch that found the journey.  One wavefront per job walks the rounds
+// k = transfers + 1 .. 1 back from the destination; per round it records the
+// walk (transfer, footpath or egress offset) and the trip that reach the
+// round's location, searching routes, trips and boarding stops 64 at a time
+// in the reference's order.  The host turns the records into legs.
+static_assert(kRcMaxSteps == kRowsMax - 1, "one step per round k = 1 .. 15");
+
+struct rc_args {
+  dev_image im;
+  dev_state st;
+  rc_job const* jobs;
+  std::uint32_t n_jobs;
+  std::uint32_t const* off_loc;
+  std::int16_t const* off_dur;
+  rc_out* out;
+};
+
+void launch_reconstruct(bool fwd, std::uint32_t W, rc_args const& a, hipStream_t stream);
 
 }  // namespace nigiri_amd

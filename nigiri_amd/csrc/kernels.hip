@@ -1720,6 +1720,199 @@ void epoch_begin_impl(epoch_args const& a, hipStream_t st) {
   }
 }
 
+
+// ======================================================================
+// Journey reconstruction (reconstruct.cc:162-877).  One wavefront per job;
+// every search below runs 64 candidates at a time and takes the first hit
+// in the reference's iteration order, so the result equals the sequential
+// walk of get_transport -> get_route_transport -> find_start_in_prev_round.
+template <bool Fwd, int W>
+struct rc_walker {
+  using D = dirop<Fwd>;
+  dev_image const& im;
+  dev_state const& st;
+  slot_param const& p;
+  std::uint32_t g, slane, tl;
+
+  __device__ std::int16_t rt(std::uint32_t k, std::uint32_t l) const {
+    return st.round_times[lay<W>::rt(st, g, k, l, slane)];
+  }
+  // unix_to_delta of a trip event: (traffic day - base day) * 1440 + event
+  __device__ std::int16_t ev_delta(std::uint32_t r, std::uint32_t to, std::uint32_t i, bool arr,
+                                   std::uint32_t day) const {
+    auto const n_t = im.route_transport_from[r + 1U] - im.route_transport_from[r];
+    auto const d = im.event_times[im.route_stop_time_from[r] + n_t * (2U * i - (arr ? 1U : 0U)) + to];
+    auto const cnt = static_cast<int>(d & 31U) * 1440 + static_cast<int>(d >> 5);
+    return clamp16((static_cast<int>(day) - static_cast<int>(p.base)) * 1440 + cnt);
+  }
+
+  // find_start_in_prev_round (reconstruct.cc:187-293): the stop nearest to
+  // `finish` (before it forward, after it backward) where round k - 1 allows
+  // boarding; -1 if none
+  __device__ int find_start(std::uint32_t k, std::uint32_t r, std::uint32_t to, std::uint32_t day,
+                            std::uint32_t finish) const {
+    auto const s0 = im.route_stop_offsets[r];
+    auto const n = im.route_stop_offsets[r + 1U] - s0;
+    auto const n_cand = Fwd ? finish : n - 1U - finish;
+    for (std::uint32_t c0 = 0U; c0 < n_cand; c0 += 64U) {
+      auto const x = c0 + tl;
+      bool ok = false;
+      if (x < n_cand) {
+        auto const j = Fwd ? finish - 1U - x : finish + 1U + x;
+        auto const stp = im.route_stops[s0 + j];
+        bool const allowed = ((stp >> (Fwd ? 28U : 29U)) & 1U) != 0U;
+        if (allowed) {
+          ok = D::better_eq(rt(k - 1U, stp & kLocMask), ev_delta(r, to, j, !Fwd, day));
+        }
+      }
+      auto const m = __ballot(ok);
+      if (m != 0ULL) {
+        auto const x0 = c0 + static_cast<std::uint32_t>(__ffsll(static_cast<unsigned long long>(m)) - 1);
+        return static_cast<int>(Fwd ? finish - 1U - x0 : finish + 1U + x0);
+      }
+    }
+    return -1;
+  }
+
+  // get_transport + get_route_transport (reconstruct.cc:304-374, 376-521):
+  // over the (route, stop) visits of l in the reference's order, the trips
+  // whose event at the stop falls on (day, mam).  A stop's events are sorted
+  // by minute of the day (FIFO modulo 1440, checked when the timetable is
+  // built), so each visit's matching trips are one range found by binary
+  // search; the lanes search 64 visits at a time, then the visits with
+  // matches are tried in order, their trips in route order.
+  __device__ bool transport(std::uint32_t k, std::uint32_t l, std::int16_t time, rc_step& out) const {
+    auto const minutes = static_cast<int>(p.base) * 1440 + time;  // split_day_mam
+    auto const day = minutes / 1440, mam = minutes % 1440;
+    auto const v0 = im.lrs_offsets[l], v1 = im.lrs_offsets[l + 1U];
+    for (auto c0 = v0; c0 < v1; c0 += 64U) {
+      auto const v = c0 + tl;
+      std::uint32_t r = 0U, i = 0U, lo = 0U, hi = 0U;
+      if (v < v1 && mam >= 0) {
+        r = im.lrs_route[v];
+        i = im.lrs_stop[v];
+        auto const s0 = im.route_stop_offsets[r];
+        auto const n = im.route_stop_offsets[r + 1U] - s0;
+        auto const stp = im.route_stops[s0 + i];
+        bool const allowed = Fwd ? (i != 0U && ((stp >> 29) & 1U) != 0U)
+                                 : (i != n - 1U && ((stp >> 28) & 1U) != 0U);
+        if (allowed) {
+          auto const n_t = im.route_transport_from[r + 1U] - im.route_transport_from[r];
+          auto const* ev = im.event_times + im.route_stop_time_from[r] + n_t * (2U * i - (Fwd ? 1U : 0U));
+          // lo = first trip with minute >= mam, hi = first with minute > mam
+          std::uint32_t a = 0U, b = n_t;
+          while (a < b) {
+            auto const mid = (a + b) / 2U;
+            if (static_cast<int>(ev[mid] >> 5) < mam) a = mid + 1U; else b = mid;
+          }
+          lo = a;
+          b = n_t;
+          while (a < b) {
+            auto const mid = (a + b) / 2U;
+            if (static_cast<int>(ev[mid] >> 5) <= mam) a = mid + 1U; else b = mid;
+          }
+          hi = a;
+        }
+      }
+      auto m = __ballot(lo < hi);
+      while (m != 0ULL) {
+        auto const b = __ffsll(static_cast<unsigned long long>(m)) - 1;
+        m &= m - 1ULL;
+        auto const sr = static_cast<std::uint32_t>(__shfl(static_cast<int>(r), b, 64));
+        auto const si = static_cast<std::uint32_t>(__shfl(static_cast<int>(i), b, 64));
+        auto const slo = static_cast<std::uint32_t>(__shfl(static_cast<int>(lo), b, 64));
+        auto const shi = static_cast<std::uint32_t>(__shfl(static_cast<int>(hi), b, 64));
+        auto const t0 = im.route_transport_from[sr];
+        auto const n_t = im.route_transport_from[sr + 1U] - t0;
+        auto const blk = im.route_stop_time_from[sr] + n_t * (2U * si - (Fwd ? 1U : 0U));
+        for (auto to = slo; to < shi; ++to) {
+          auto const d = im.event_times[blk + to];
+          auto const td = static_cast<std::uint16_t>(day - static_cast<int>(d & 31U));
+          if (!bit_test(im.bitfields, im.transport_traffic_days[t0 + to], td)) continue;
+          auto const j = find_start(k, sr, to, td, si);
+          if (j >= 0) {
+            out.transport = t0 + to;
+            out.route = sr;
+            out.day = td;
+            out.stop_idx = static_cast<std::uint16_t>(j);
+            out.finish_idx = static_cast<std::uint16_t>(si);
+            return true;
+          }
+        }
+      }
+    }
+    return false;
+  }
+
+  // check_fp (reconstruct.cc:523-649): a walk of `dur` minutes from `target`
+  // to the round's location ends at curr
+  __device__ bool walk(std::uint32_t k, std::int16_t curr, std::uint32_t target, std::int16_t dur,
+                       rc_step& out) const {
+    return transport(k, target, static_cast<std::int16_t>(curr - D::dir(dur)), out);
+  }
+};
+
+template <bool Fwd, int W>
+__global__ __launch_bounds__(64) void k_reconstruct(rc_args a) {
+  auto const& im = a.im;
+  auto const& st = a.st;
+  auto const tl = threadIdx.x & 63U;
+  for (auto job_i = blockIdx.x; job_i < a.n_jobs; job_i += gridDim.x) {
+    auto const job = a.jobs[job_i];
+    auto const& p = st.param[job.slot];
+    rc_walker<Fwd, W> w{im, st, p, job.slot / W, job.slot % W, tl};
+    auto& out = a.out[job_i];
+    auto const top = static_cast<std::uint32_t>(job.transfers) + 1U;
+    auto l = job.dest;
+    std::int32_t status = 0;
+    std::uint32_t n_steps = 0U;
+    for (std::uint32_t i = 0U; i < top; ++i) {
+      auto const k = top - i;
+      auto const curr = w.rt(k, l);
+      rc_step s{};
+      std::int32_t walk_id = -1;
+      bool found = false;
+      if (job.intermodal != 0U && k == top) {
+        // find_dest_leg (reconstruct.cc:651-679) per egress offset
+        for (std::uint32_t o = 0U; o < job.n_off && !found; ++o) {
+          auto const dur = a.off_dur[job.off_begin + o];
+          if (dur >= 511) continue;  // footpath::kMaxDuration
+          found = w.walk(k, curr, a.off_loc[job.off_begin + o], dur, s);
+          walk_id = static_cast<std::int32_t>(o);
+        }
+      } else {
+        // transfer at l (duration 0 into the destination), then footpaths
+        auto const tr = static_cast<std::int16_t>(k == top ? 0 : adjusted<std::uint8_t>(p, im.transfer_time[l]));
+        found = w.walk(k, curr, l, tr, s);
+        auto const* off = Fwd ? im.fp_in_offsets : im.fp_out_offsets;
+        auto const* fps = Fwd ? im.fp_in : im.fp_out;
+        for (auto e = off[l]; e < off[l + 1U] && !found; ++e) {
+          auto const fp = fps[e];
+          auto const dur = adjusted<std::int16_t>(p, static_cast<std::int16_t>(fp >> 23));
+          found = w.walk(k, curr, fp & ((1U << 23) - 1U), dur, s);
+          walk_id = static_cast<std::int32_t>(e - off[l]);
+        }
+      }
+      if (!found) {
+        status = 1;
+        break;
+      }
+      s.curr_time = curr;
+      s.walk = walk_id;
+      if (tl == 0U) out.steps[i] = s;
+      ++n_steps;
+      l = im.route_stops[im.route_stop_offsets[s.route] + s.stop_idx] & kLocMask;
+    }
+    if (tl == 0U) {
+      out.status = status;
+      out.n_steps = n_steps;
+      out.start_loc = l;
+      out.rt0 = w.rt(0U, l);
+      out.pad = 0;
+    }
+  }
+}
+
 }  // namespace
 
 void launch_round(bool fwd, std::uint32_t W, round_args const& a, std::uint32_t const* sweep_groups,
@@ -1777,6 +1970,24 @@ void launch_extract(std::uint32_t W, dev_state const& st, std::uint32_t const* s
     hipLaunchKernelGGL((k_extract<64>), dim3(grid_for(work, kBlock, 4096)), dim3(kBlock), 0, stream, st, slots, n_slots);
   } else {
     hipLaunchKernelGGL((k_extract<1>), dim3(grid_for(work, kBlock, 4096)), dim3(kBlock), 0, stream, st, slots, n_slots);
+  }
+}
+
+void launch_reconstruct(bool fwd, std::uint32_t W, rc_args const& a, hipStream_t stream) {
+  if (a.n_jobs == 0U) return;
+  auto const grid = dim3(std::min<std::uint32_t>(a.n_jobs, 8192U));
+  if (W == 64U) {
+    if (fwd) {
+      hipLaunchKernelGGL((k_reconstruct<true, 64>), grid, dim3(64), 0, stream, a);
+    } else {
+      hipLaunchKernelGGL((k_reconstruct<false, 64>), grid, dim3(64), 0, stream, a);
+    }
+  } else {
+    if (fwd) {
+      hipLaunchKernelGGL((k_reconstruct<true, 1>), grid, dim3(64), 0, stream, a);
+    } else {
+      hipLaunchKernelGGL((k_reconstruct<false, 1>), grid, dim3(64), 0, stream, a);
+    }
   }
 }
 

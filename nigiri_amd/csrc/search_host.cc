@@ -220,6 +220,13 @@ void prepare(timetable const& tt, nigiri_query const& q, prepared_query& p,
   p.iv_from_ = p.ontrip_ ? q.start_time : q.interval_from;
   p.iv_to_ = p.ontrip_ ? q.start_time : q.interval_to;
 
+  p.start_intermodal_ = start_im;
+  for (auto i = 0U; i != q.n_start; ++i) p.start_offsets_.emplace_back(q.start_loc[i], q.start_dur[i]);
+  for (auto i = 0U; i != q.n_dest; ++i) p.dest_offsets_.emplace_back(q.dest_loc[i], q.dest_dur[i]);
+  auto const by_dur = [](auto const& a, auto const& b) { return a.second < b.second; };
+  std::stable_sort(p.start_offsets_.begin(), p.start_offsets_.end(), by_dur);
+  std::stable_sort(p.dest_offsets_.begin(), p.dest_offsets_.end(), by_dur);
+
   host_tts raw{q.tts_default != 0, q.tts_min_transfer_time, q.tts_additional_time, q.tts_factor};
   p.fastest_direct_ = fastest_direct(tt, q, fwd, raw);
 
@@ -393,13 +400,16 @@ void add_epoch_results(timetable const& tt, prepared_query& p, std::size_t epoch
       res.back() = el;
     }
   }
-  // reconstruction gate (search.h:457-475); reconstruction is assumed to
-  // succeed for journeys that pass it
-  for (auto& j : res) {
-    if (!j.reconstructed_ &&
+}
+
+void reconstruction_candidates(prepared_query const& p, std::vector<std::size_t>& out) {
+  out.clear();
+  for (std::size_t i = 0; i != p.results_.size(); ++i) {
+    auto const& j = p.results_[i];
+    if (!j.reconstructed_ && !j.error_ &&
         (p.ontrip_ || (j.start_time_ >= p.iv_from_ && j.start_time_ < p.iv_to_)) &&
         j.travel_time() < p.fastest_direct_) {
-      j.reconstructed_ = true;
+      out.push_back(i);
     }
   }
 }

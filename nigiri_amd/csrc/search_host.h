@@ -29,6 +29,8 @@ struct host_journey {
   std::uint32_t dest_;
   std::uint8_t transfers_;
   bool reconstructed_{false};
+  bool error_{false};  // reconstruction found no legs (search.h:463-473)
+  std::vector<nigiri_leg> legs_{};  // travel order
   bool dominates(host_journey const& o) const {  // journey.h:80-88
     if (start_time_ <= dest_time_) {
       return transfers_ <= o.transfers_ && start_time_ >= o.start_time_ &&
@@ -63,6 +65,9 @@ struct prepared_query {
   std::uint16_t base_{0};
   std::int32_t iv_from_{0}, iv_to_{0};
   host_tts tts_;
+  bool start_intermodal_{false};
+  // query offsets in sanitized order (query.cc:59-67: sorted by duration)
+  std::vector<std::pair<std::uint32_t, std::int16_t>> start_offsets_, dest_offsets_;
   std::vector<std::uint32_t> dests_;  // ascending (non-intermodal)
   std::vector<std::uint32_t> dist_loc_;  // ascending (intermodal)
   std::vector<std::uint16_t> dist_min_;
@@ -98,12 +103,15 @@ struct host_scratch {
 void prepare(timetable const& tt, nigiri_query const& q, prepared_query& p,
              host_scratch& s, bool host_lb = true);
 
-// Journey extraction of one epoch (raptor.h:393-413) followed by the
-// reconstruction gate of search_interval() (search.h:457-475).
+// Journey extraction of one epoch (raptor.h:393-413).
 // dest_rows[j * max_k + k] is round_times[k][dest bit j] (delta).
 void add_epoch_results(timetable const& tt, prepared_query& p,
                        std::size_t epoch_idx, std::int16_t const* dest_rows,
                        std::uint32_t max_k);
+
+// The reconstruction gate of search_interval() (search.h:457-475): indices
+// of the journeys of p.results_ to reconstruct after the current epoch.
+void reconstruction_candidates(prepared_query const& p, std::vector<std::size_t>& out);
 
 // Final filtering of search::execute (search.h:319-336).
 void finish(prepared_query& p);

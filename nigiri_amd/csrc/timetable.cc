@@ -229,8 +229,36 @@ void build_lb_graph(timetable& tt, bool fwd) {
 
 }  // namespace
 
+namespace {
+
+void build_importance(timetable& tt) {
+  std::vector<unsigned> route_days(tt.n_routes(), 0U);
+  for (auto r = 0U; r != tt.n_routes(); ++r) {
+    for (auto t = tt.route_transport_from[r]; t != tt.route_transport_from[r + 1U]; ++t) {
+      auto const* bf = &tt.bitfields[static_cast<std::size_t>(tt.transport_traffic_days[t]) * kBitfieldWords];
+      for (auto w = 0U; w != kBitfieldWords; ++w) {
+        route_days[r] += static_cast<unsigned>(__builtin_popcountll(bf[w]));
+      }
+    }
+  }
+  tt.location_importance.assign(tt.n_locations, 0U);
+  tt.max_importance = 0U;
+  for (auto l = 0U; l != tt.n_locations; ++l) {
+    unsigned count = 0U;
+    for (auto j = tt.location_route_offsets[l]; j != tt.location_route_offsets[l + 1U]; ++j) {
+      count += route_days[tt.location_routes[j]];
+    }
+    auto& x = tt.location_importance[l];
+    x += 1.0F * static_cast<float>(count);
+    tt.max_importance = std::max(tt.max_importance, x);
+  }
+}
+
+}  // namespace
+
 void finalize(timetable& tt) {
   build_location_routes(tt);
+  build_importance(tt);
   build_route_traffic_days(tt);
   build_footpaths_in(tt);
   build_lb_graph(tt, true);

@@ -66,6 +66,11 @@ struct timetable {
   std::vector<std::uint32_t> fp_out_offsets, fp_out, fp_in_offsets, fp_in;
   std::vector<std::uint32_t> lb_fwd_offsets, lb_fwd, lb_bwd_offsets, lb_bwd;
   std::vector<float> coord_x, coord_y;
+  // location_importance_ / max_importance_ as assign_importance computes
+  // them (src/loader/init_finish.cc:64-101).  Routes carry no transport
+  // class here: every trip counts with the priority of class kOther (1).
+  std::vector<std::uint32_t> location_importance;
+  std::uint32_t max_importance = 0U;
 
   std::uint32_t n_routes() const {
     return static_cast<std::uint32_t>(route_stop_offsets.size() - 1U);

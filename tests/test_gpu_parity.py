@@ -41,6 +41,7 @@ def compare(tt, eng, qs, idx=None, check_state=True):
         got = res[i]
         assert got["status"] == 0
         assert got["journeys"] == ref["journeys"], (i, got["journeys"], ref["journeys"])
+        assert got["legs"] == ref["legs"], (i, got["legs"], ref["legs"])
         assert got["base_day"] == ref["base_day"]
         if check_state and len(qs) <= eng.slots:
             st = eng.download(got["slot"])
@@ -202,3 +203,19 @@ def test_gpu_batched_engine_rejects_routes_beyond_its_lds_staging():
     plain = na.Engine(img, lanes=1, groups=1, max_k=8)
     qs = na.Queries.build([dict(start=[(9, 0)], dest=[(11, 0)], time=day * 1440 + 100)])
     compare(tt, plain, qs)
+
+
+def test_gpu_algo_reconstruct_equals_search_legs():
+    # raptor::reconstruct through the Algo entry point, against the state the
+    # (single) epoch of each point query left in its slot
+    tt, img = image_for("w1", lambda: na.Timetable.generate(1))
+    eng = na.Engine(img, lanes=64, groups=1, max_k=16)
+    qs = na.Queries.generate(tt, 1, 64, seed=0x7172 + 5)
+    say("algo reconstruct workload1 n=64")
+    res = compare(tt, eng, qs, check_state=False)
+    n = 0
+    for i in range(len(qs)):
+        for j, legs in zip(res[i]["journeys"], res[i]["legs"]):
+            assert eng.reconstruct(res[i]["slot"], qs[i], j) == legs
+            n += 1
+    assert n > 0

@@ -344,6 +344,9 @@ class Queries:
             q.tts_additional_time = s.get("tts_additional_time", 0)
             q.tts_factor = s.get("tts_factor", 1.0)
             q.fastest_direct_factor = s.get("fastest_direct_factor", 1.0)
+            q.min_connection_count = s.get("min_connection_count", 0)
+            q.extend_interval_earlier = int(s.get("extend_interval_earlier", 0))
+            q.extend_interval_later = int(s.get("extend_interval_later", 0))
             if "fastest_direct" in s:
                 q.has_fastest_direct = 1
                 q.fastest_direct = s["fastest_direct"]

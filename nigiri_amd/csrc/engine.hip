@@ -679,6 +679,9 @@ struct runner {
       auto& p = pq[static_cast<std::size_t>(si.q)];
       auto const g = s / W, lane = s % W;
       bt_mask[g] |= 1ULL << lane;
+      // reset_arrivals of an interval extension (search.h:299-311)
+      auto const reset = !si.is_new && p.epochs_[si.epoch].reset_;
+      if (reset) rt_mask[g] |= 1ULL << lane;
       if (si.is_new) {
         rt_mask[g] |= 1ULL << lane;
         new_slots.push_back(s);
@@ -716,7 +719,7 @@ struct runner {
       }
       auto const& ep = p.epochs_[si.epoch];
       auto const base = p.base_minutes(tt);
-      flags[s] = static_cast<std::uint8_t>(1U | (si.is_new ? 2U : 0U));
+      flags[s] = static_cast<std::uint8_t>(1U | (si.is_new || reset ? 2U : 0U));
       worst[s] = clamp16(static_cast<long long>(ep.worst_time_at_dest_) - base);
       for (std::size_t j = 0; j != ep.loc_.size(); ++j) {
         st_slot.push_back(s);
@@ -903,27 +906,38 @@ struct runner {
     e.stats.results_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
     e.stats.reconstruct_ms +=
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
-    // queries whose last epoch ended complete once their legs are written
+    // after each epoch the search may stop early or, after its last one,
+    // widen the interval and go on (search.h:237-317, 477-491); queries whose
+    // last epoch ended complete once their legs are written
     for (auto const s : finished) {
       auto const& si = slots[s];
-      if (si.epoch + 1U == pq[static_cast<std::size_t>(si.q)].epochs_.size()) {
-        completing.push_back(si.q);
+      auto& p = pq[static_cast<std::size_t>(si.q)];
+      early_termination(p, si.epoch);
+      if (si.epoch + 1U != p.epochs_.size()) continue;
+      if (wants_extension(p)) {
+        // extending may remove results the leg writers still refer to
+        join_legs();
+        if (extend_interval(tt, p)) continue;  // the slot runs the new epochs
       }
+      completing.push_back(si.q);
     }
     if (!legs_pending) settle();
     return true;
   }
 
+  void join_legs() {
+    if (!legs_pending) return;
+    auto const t0 = std::chrono::steady_clock::now();
+    e.pool->wait();
+    legs_pending = false;
+    e.stats.legs_ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+
   // Joins the leg writing started by the last step and completes the queries
   // that waited for it.  Returns whether any query completed.
   bool settle() {
-    if (legs_pending) {
-      auto const t0 = std::chrono::steady_clock::now();
-      e.pool->wait();
-      legs_pending = false;
-      e.stats.legs_ms +=
-          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    }
+    join_legs();
     auto const any = !completing.empty();
     for (auto const q : completing) {
       finish(pq[static_cast<std::size_t>(q)]);

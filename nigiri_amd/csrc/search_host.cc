@@ -179,13 +179,151 @@ void starts_at_stop(timetable const& tt, bool fwd, std::uint32_t r, std::uint32_
   }
 }
 
+// start labels of the interval [from, to) (get_starts, start_times.cc:241-319;
+// the start map is taken in ascending location order)
+void interval_labels(timetable const& tt, prepared_query const& p, std::int32_t from,
+                     std::int32_t to, bool add_ontrip, std::vector<start_label>& starts) {
+  auto const fwd = p.fwd_;
+  for (auto const& [l, o] : p.start_map_) {
+    auto const stop_from = fwd ? from : from - p.max_start_offset_;
+    auto const stop_to = fwd ? to + p.max_start_offset_ : to;
+    for (auto j = tt.location_route_offsets[l]; j != tt.location_route_offsets[l + 1U]; ++j) {
+      auto const r = tt.location_routes[j];
+      auto const s0 = tt.route_stop_offsets[r];
+      auto const n = tt.route_stop_offsets[r + 1U] - s0;
+      for (auto i = 0U; i != n; ++i) {
+        auto const stp = tt.route_stops[s0 + i];
+        if ((stp & kStopLocMask) != l) continue;
+        auto const in = ((stp >> 28) & 1U) != 0U, out = ((stp >> 29) & 1U) != 0U;
+        if ((!fwd && (i == 0U || !out)) || (fwd && (i == n - 1U || !in))) continue;
+        starts_at_stop(tt, fwd, r, i, l, from, to, stop_from, stop_to, o, starts);
+      }
+    }
+    // ontrip label at the interval end (start_times.cc:225-238)
+    if (add_ontrip && o != static_cast<std::int16_t>(kFpMaxDuration)) {
+      auto const at_start = fwd ? to : from - 1;
+      starts.push_back({at_start, fwd ? to + o : from - 1 - o, l});
+    }
+  }
+}
+
+// sorted labels (search.h:408-410) grouped by start time into epochs
+void append_epochs(prepared_query& p, std::vector<start_label>& starts, bool reset) {
+  auto const fwd = p.fwd_;
+  std::sort(starts.begin(), starts.end(), [&](start_label const& a, start_label const& b) {
+    return fwd ? b < a : a < b;
+  });
+  for (std::size_t from = 0U; from != starts.size();) {
+    auto to = from;
+    while (to != starts.size() && starts[to].time_at_start == starts[from].time_at_start) ++to;
+    epoch ep;
+    ep.start_time_ = starts[from].time_at_start;
+    ep.worst_time_at_dest_ =
+        ep.start_time_ + (fwd ? 1 : -1) * (std::min(p.fastest_direct_, p.max_travel_time_) + 1);
+    ep.reset_ = reset && from == 0U;
+    for (auto i = from; i != to; ++i) {
+      ep.loc_.push_back(starts[i].stop);
+      ep.time_.push_back(starts[i].time_at_stop);
+    }
+    p.epochs_.push_back(std::move(ep));
+    from = to;
+  }
+}
+
+// interval_estimator (interval_estimate.h:12-156)
+bool can_extend_earlier(prepared_query const& p, std::int32_t from) {
+  return p.extend_earlier_ && from != p.ext_from_;
+}
+bool can_extend_later(prepared_query const& p, std::int32_t to) {
+  return p.extend_later_ && to != p.ext_to_;
+}
+void clamp_interval(prepared_query const& p, std::int32_t& from, std::int32_t& to) {
+  from = std::clamp(std::clamp(from, p.ext_from_, p.ext_to_), p.max_from_, p.max_to_);
+  to = std::clamp(std::clamp(to, p.ext_from_, p.ext_to_), p.max_from_, p.max_to_);
+}
+
+unsigned results_in_interval(prepared_query const& p) {  // search.h:382-391
+  if (p.ontrip_) return static_cast<unsigned>(p.results_.size());
+  return static_cast<unsigned>(std::count_if(
+      p.results_.begin(), p.results_.end(),
+      [&](host_journey const& j) { return j.start_time_ >= p.iv_from_ && j.start_time_ < p.iv_to_; }));
+}
+
 }  // namespace
+
+void early_termination(prepared_query& p, std::size_t epoch_idx) {
+  if (p.min_connection_count_ == 0U || results_in_interval(p) < p.min_connection_count_) return;
+  if (!((p.fwd_ && p.extend_earlier_ && !p.extend_later_) ||
+        (!p.fwd_ && !p.extend_earlier_ && p.extend_later_))) {
+    return;
+  }
+  auto const start_time = p.epochs_[epoch_idx].start_time_;
+  if (p.fwd_) {
+    p.iv_from_ = start_time;
+  } else {
+    p.iv_to_ = start_time + 1;
+  }
+  p.epochs_.resize(epoch_idx + 1U);
+}
+
+bool wants_extension(prepared_query const& p) {
+  return !p.results_.empty() && !p.ontrip_ && results_in_interval(p) < p.min_connection_count_;
+}
+
+bool extend_interval(timetable const& tt, prepared_query& p) {
+  bool reset = false;
+  while (wants_extension(p)) {
+    // interval_estimator::extension (interval_estimate.h:83-107)
+    auto const need = p.min_connection_count_ - results_in_interval(p);
+    if (!p.extend_earlier_ && !p.extend_later_) return false;
+    auto from = p.iv_from_, to = p.iv_to_;
+    auto const ext = static_cast<std::int32_t>(static_cast<unsigned>(to - from) * need);
+    if (can_extend_earlier(p, from) && can_extend_later(p, to)) {
+      from -= ext / 2;
+      to += ext / 2;
+    } else {
+      if (p.extend_earlier_) from -= ext;
+      if (p.extend_later_) to += ext;
+    }
+    clamp_interval(p, from, to);
+    if (from == p.iv_from_ && to == p.iv_to_) return false;
+    std::vector<start_label> starts;
+    auto const remove_outside = [&]() {  // remove_ontrip_results (search.h:413-417)
+      auto& e = p.results_;
+      e.erase(std::remove_if(e.begin(), e.end(),
+                             [&](host_journey const& j) {
+                               return !(j.start_time_ >= p.iv_from_ && j.start_time_ < p.iv_to_);
+                             }),
+              e.end());
+    };
+    if (from != p.iv_from_) {
+      interval_labels(tt, p, from, p.iv_from_, !p.fwd_, starts);
+      if (!p.fwd_) {
+        reset = true;
+        remove_outside();
+      }
+    }
+    if (to != p.iv_to_) {
+      interval_labels(tt, p, p.iv_to_, to, p.fwd_, starts);
+      if (p.fwd_) {
+        reset = true;
+        remove_outside();
+      }
+    }
+    p.iv_from_ = from;
+    p.iv_to_ = to;
+    if (!starts.empty()) {
+      append_epochs(p, starts, reset);
+      return true;
+    }
+  }
+  return false;
+}
 
 void prepare(timetable const& tt, nigiri_query const& q, prepared_query& p,
              host_scratch& s, bool host_lb) {
-  if (q.min_connection_count != 0U || q.prf_idx != 0) {
-    throw status_error{NIGIRI_AMD_EUNSUPPORTED,
-                       "min_connection_count > 0 and profiles != 0 are not supported"};
+  if (q.prf_idx != 0) {
+    throw status_error{NIGIRI_AMD_EUNSUPPORTED, "profiles other than 0 are not supported"};
   }
   if (q.dir != NIGIRI_AMD_DIR_FWD && q.dir != NIGIRI_AMD_DIR_BWD) {
     throw status_error{NIGIRI_AMD_EINVAL, "bad direction"};
@@ -317,51 +455,63 @@ void prepare(timetable const& tt, nigiri_query const& q, prepared_query& p,
       }
     }
   }
-  std::vector<start_label> starts;
-  auto const iv_size = p.iv_to_ - p.iv_from_;
-  auto const with_interval = iv_size != 0 || !p.ontrip_;
-  for (auto const& [l, o] : shortest) {
-    if (!with_interval) {
-      starts.push_back({q.start_time, fwd ? q.start_time + o : q.start_time - o, l});
-      continue;
-    }
-    auto const stop_from = fwd ? p.iv_from_ : p.iv_from_ - q.max_start_offset;
-    auto const stop_to = fwd ? p.iv_to_ + q.max_start_offset : p.iv_to_;
-    for (auto j = tt.location_route_offsets[l]; j != tt.location_route_offsets[l + 1U]; ++j) {
-      auto const r = tt.location_routes[j];
-      auto const s0 = tt.route_stop_offsets[r];
-      auto const n = tt.route_stop_offsets[r + 1U] - s0;
-      for (auto i = 0U; i != n; ++i) {
-        auto const stp = tt.route_stops[s0 + i];
-        if ((stp & kStopLocMask) != l) continue;
-        auto const in = ((stp >> 28) & 1U) != 0U, out = ((stp >> 29) & 1U) != 0U;
-        if ((!fwd && (i == 0U || !out)) || (fwd && (i == n - 1U || !in))) continue;
-        starts_at_stop(tt, fwd, r, i, l, p.iv_from_, p.iv_to_, stop_from, stop_to, o, starts);
+  p.start_map_.assign(shortest.begin(), shortest.end());
+  p.max_start_offset_ = q.max_start_offset;
+
+  // interval_estimator (interval_estimate.h:12-156) and its initial
+  // widening of a pretrip interval (search.h:216-218)
+  p.min_connection_count_ = q.min_connection_count;
+  p.extend_earlier_ = q.extend_interval_earlier != 0;
+  p.extend_later_ = q.extend_interval_later != 0;
+  p.ext_from_ = tt.date_range_from;
+  p.ext_to_ = tt.date_range_to;
+  {
+    constexpr std::int64_t kMaxSearchIntervalSize = (32767 / 1440 - 7) * 1440;  // limits.h:11-13
+    auto const from = static_cast<std::int64_t>(p.iv_from_), to = static_cast<std::int64_t>(p.iv_to_);
+    auto const days = q.has_fastest_direct != 0 ? 1 : 2;
+    p.max_from_ = static_cast<std::int32_t>(
+        std::max(round_to_days(from - days * 1440), round_to_days(from + (to - from - kMaxSearchIntervalSize) / 2)) * 1440);
+    p.max_to_ = static_cast<std::int32_t>(
+        std::min(round_to_days(to + days * 1440), round_to_days(from + (to - from + kMaxSearchIntervalSize) / 2)) * 1440);
+  }
+  if (!p.ontrip_ && p.min_connection_count_ != 0U && (p.extend_earlier_ || p.extend_later_)) {
+    auto from = p.iv_from_, to = p.iv_to_;
+    auto const ext = 60 * static_cast<std::int32_t>(p.min_connection_count_);
+    if (fwd ? can_extend_later(p, to) : can_extend_earlier(p, from)) {
+      if (fwd) {
+        if (can_extend_earlier(p, from)) {
+          from -= 60;
+        } else {
+          to += 60;
+        }
+        to += ext;
+      } else {
+        if (can_extend_later(p, to)) {
+          to += 60;
+        } else {
+          from -= 60;
+        }
+        from -= ext;
       }
+    } else if (fwd) {
+      if (p.extend_earlier_) from -= 60 + ext;
+    } else {
+      if (p.extend_later_) to += 60 + ext;
     }
-    // ontrip label at the interval end (start_times.cc:225-238)
-    auto const at_start = fwd ? p.iv_to_ : p.iv_from_ - 1;
-    if (o != static_cast<std::int16_t>(kFpMaxDuration)) {
-      starts.push_back({at_start, fwd ? p.iv_to_ + o : p.iv_from_ - 1 - o, l});
-    }
+    clamp_interval(p, from, to);
+    p.iv_from_ = from;
+    p.iv_to_ = to;
   }
-  std::sort(starts.begin(), starts.end(), [&](start_label const& a, start_label const& b) {
-    return fwd ? b < a : a < b;
-  });
-  for (std::size_t from = 0U; from != starts.size();) {
-    auto to = from;
-    while (to != starts.size() && starts[to].time_at_start == starts[from].time_at_start) ++to;
-    epoch ep;
-    ep.start_time_ = starts[from].time_at_start;
-    ep.worst_time_at_dest_ =
-        ep.start_time_ + (fwd ? 1 : -1) * (std::min(p.fastest_direct_, p.max_travel_time_) + 1);
-    for (auto i = from; i != to; ++i) {
-      ep.loc_.push_back(starts[i].stop);
-      ep.time_.push_back(starts[i].time_at_stop);
+
+  std::vector<start_label> starts;
+  if (p.ontrip_) {
+    for (auto const& [l, o] : p.start_map_) {
+      starts.push_back({q.start_time, fwd ? q.start_time + o : q.start_time - o, l});
     }
-    p.epochs_.push_back(std::move(ep));
-    from = to;
+  } else {
+    interval_labels(tt, p, p.iv_from_, p.iv_to_, true, starts);
   }
+  append_epochs(p, starts, false);
 }
 
 void add_epoch_results(timetable const& tt, prepared_query& p, std::size_t epoch_idx,

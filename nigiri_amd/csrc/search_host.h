@@ -49,6 +49,7 @@ struct host_journey {
 struct epoch {
   std::int32_t start_time_;
   std::int32_t worst_time_at_dest_;
+  bool reset_ = false;  // reset_arrivals before this epoch (search.h:299-311)
   std::vector<std::uint32_t> loc_;
   std::vector<std::int32_t> time_;  // time at stop (unix minutes)
 };
@@ -63,7 +64,15 @@ struct prepared_query {
   std::int16_t fastest_direct_{32767};
   std::int16_t max_travel_time_{7200};
   std::uint16_t base_{0};
-  std::int32_t iv_from_{0}, iv_to_{0};
+  std::int32_t iv_from_{0}, iv_to_{0};  // the search interval (grows, search.h:237-317)
+  // interval extension (search.h:237-317, interval_estimate.h)
+  std::uint32_t min_connection_count_{0U};
+  bool extend_earlier_{false}, extend_later_{false};
+  std::int32_t ext_from_{0}, ext_to_{0};  // timetable's external interval
+  std::int32_t max_from_{0}, max_to_{0};  // interval the int16 deltas can hold
+  std::int16_t max_start_offset_{7200};
+  // shortest start offset per location (get_starts, start_times.cc:251-271)
+  std::vector<std::pair<std::uint32_t, std::int16_t>> start_map_;
   host_tts tts_;
   bool start_intermodal_{false};
   // query offsets in sanitized order (query.cc:59-67: sorted by duration)
@@ -112,6 +121,22 @@ void add_epoch_results(timetable const& tt, prepared_query& p,
 // The reconstruction gate of search_interval() (search.h:457-475): indices
 // of the journeys of p.results_ to reconstruct after the current epoch.
 void reconstruction_candidates(prepared_query const& p, std::vector<std::size_t>& out);
+
+// Early termination of search_interval() (search.h:477-491) after epoch
+// epoch_idx: with enough journeys and a one-sided extension the remaining
+// epochs are dropped and the search interval shrinks to the epoch's start.
+void early_termination(prepared_query& p, std::size_t epoch_idx);
+
+// Whether execute() (search.h:240-270) goes on after the last epoch:
+// a pretrip search with fewer than min_connection_count journeys in it.
+bool wants_extension(prepared_query const& p);
+
+// The interval extension step of execute() (search.h:282-317): widens the
+// search interval and appends the epochs of its new parts (the first one
+// with reset_ where the reference resets the arrivals), removing journeys
+// outside the interval when it resets.  Repeats while a step adds no
+// epoch.  Returns whether epochs were appended; false ends the search.
+bool extend_interval(timetable const& tt, prepared_query& p);
 
 // Final filtering of search::execute (search.h:319-336).
 void finish(prepared_query& p);

@@ -1251,6 +1251,7 @@ struct interval_t {
   std::int32_t from_, to_;
   bool contains(std::int32_t t) const { return t >= from_ && t < to_; }
   std::int32_t size() const { return to_ - from_; }
+  bool operator==(interval_t const& o) const { return from_ == o.from_ && to_ == o.to_; }
 };
 
 // add_start_times_at_stop (start_times.cc:40-121)
@@ -1391,6 +1392,90 @@ std::int64_t round_days(std::int64_t minutes) {
   return (floor_d % 2 == 0) ? floor_d : floor_d + 1;
 }
 
+// interval_estimator (include/nigiri/routing/interval_estimate.h:12-156):
+// the search interval grows until min_connection_count journeys start in it,
+// within the timetable's external interval and the int16 delta range.
+template <bool Fwd>
+struct interval_estimator {
+  static constexpr std::int32_t kMaxSearchIntervalSize = (32767 / 1440 - 7) * 1440;  // limits.h:11-13
+
+  interval_estimator(tt_t const& tt, query_t const& q)
+      : q_{q}, ext_{tt.v_.date_range_from, tt.v_.date_range_to} {
+    auto const start = q.is_interval_ ? interval_t{q.iv_from_, q.iv_to_}
+                                      : interval_t{q.start_time_, q.start_time_};
+    auto const days = q.has_fastest_direct_ ? 1 : 2;
+    auto const size = static_cast<std::int64_t>(start.to_) - start.from_;
+    auto const rd = [](std::int64_t minutes) {
+      return static_cast<std::int32_t>(round_days(minutes) * 1440);
+    };
+    max_.from_ = std::max(rd(static_cast<std::int64_t>(start.from_) - days * 1440),
+                          rd(start.from_ + (size - kMaxSearchIntervalSize) / 2));
+    max_.to_ = std::min(rd(static_cast<std::int64_t>(start.to_) + days * 1440),
+                        rd(start.from_ + (size + kMaxSearchIntervalSize) / 2));
+  }
+
+  interval_t initial(interval_t const& itv) const {
+    if (q_.min_connection_count_ == 0U || (!q_.extend_earlier_ && !q_.extend_later_)) return itv;
+    auto n = itv;
+    auto const ext = 60 * static_cast<std::int32_t>(q_.min_connection_count_);
+    if (Fwd ? can_extend_later(itv) : can_extend_earlier(itv)) {
+      if (Fwd) {
+        if (can_extend_earlier(itv)) {
+          n.from_ -= 60;
+        } else {
+          n.to_ += 60;
+        }
+        n.to_ += ext;
+      } else {
+        if (can_extend_later(itv)) {
+          n.to_ += 60;
+        } else {
+          n.from_ -= 60;
+        }
+        n.from_ -= ext;
+      }
+    } else if (Fwd) {
+      if (q_.extend_earlier_) n.from_ -= 60 + ext;
+    } else {
+      if (q_.extend_later_) n.to_ += 60 + ext;
+    }
+    clamp(n);
+    return n;
+  }
+
+  interval_t extension(interval_t const& itv, unsigned num_con_req) const {
+    if (num_con_req == 0U || (!q_.extend_earlier_ && !q_.extend_later_)) return itv;
+    auto n = itv;
+    auto const ext = static_cast<std::int32_t>(static_cast<unsigned>(itv.size()) * num_con_req);
+    if (can_extend_earlier(itv) && can_extend_later(itv)) {
+      n.from_ -= ext / 2;
+      n.to_ += ext / 2;
+    } else {
+      if (q_.extend_earlier_) n.from_ -= ext;
+      if (q_.extend_later_) n.to_ += ext;
+    }
+    clamp(n);
+    return n;
+  }
+
+ private:
+  bool can_extend_earlier(interval_t const& itv) const {
+    return q_.extend_earlier_ && itv.from_ != ext_.from_;
+  }
+  bool can_extend_later(interval_t const& itv) const {
+    return q_.extend_later_ && itv.to_ != ext_.to_;
+  }
+  void clamp(interval_t& itv) const {
+    itv.from_ = std::clamp(itv.from_, ext_.from_, ext_.to_);
+    itv.to_ = std::clamp(itv.to_, ext_.from_, ext_.to_);
+    itv.from_ = std::clamp(itv.from_, max_.from_, max_.to_);
+    itv.to_ = std::clamp(itv.to_, max_.from_, max_.to_);
+  }
+
+  query_t const& q_;
+  interval_t ext_, max_{};
+};
+
 // search<Dir, raptor<...>>::search(...) + execute() (search.h:84-345) as
 // driven by raptor_search_with_dir (raptor_search.cc:51-79).
 template <bool Fwd>
@@ -1437,48 +1522,104 @@ void run_search(tt_t const& tt, query_t q, raptor_state& rs, search_result& out,
       }
     }
   }
-  if (q.min_connection_count_ != 0U) {
-    throw std::invalid_argument("min_connection_count > 0 is not supported");
-  }
-  // interval_estimator::initial is the identity for min_connection_count 0
+  interval_estimator<Fwd> const est{tt, q};
+  if (!is_ontrip) search_interval = est.initial(search_interval);
 
+  // add_start_labels (search.h:400-411)
   std::vector<start_t> starts;
+  auto const add_start_labels = [&](interval_t const& iv, bool add_ontrip) {
+    get_starts(Fwd, tt, true, 0, iv, q, starts, add_ontrip);
+    std::sort(starts.begin(), starts.end(),
+              [](start_t const& a, start_t const& b) { return Fwd ? b < a : a < b; });
+  };
   if (search_interval.size() != 0) {
-    get_starts(Fwd, tt, true, 0, search_interval, q, starts, true);
+    add_start_labels(search_interval, true);
   } else if (q.is_interval_) {
-    get_starts(Fwd, tt, true, 0, interval_t{q.iv_from_, q.iv_to_}, q, starts, true);
+    add_start_labels(interval_t{q.iv_from_, q.iv_to_}, true);
   } else {
     get_starts(Fwd, tt, false, q.start_time_, search_interval, q, starts, true);
+    std::sort(starts.begin(), starts.end(),
+              [](start_t const& a, start_t const& b) { return Fwd ? b < a : a < b; });
   }
-  std::sort(starts.begin(), starts.end(),
-            [](start_t const& a, start_t const& b) { return Fwd ? b < a : a < b; });
 
-  // search_interval() (search.h:419-494)
   auto& results = out.results_;
-  for (auto from = 0U; from != starts.size();) {
-    auto to = from;
-    while (to != starts.size() && starts[to].time_at_start_ == starts[from].time_at_start_) ++to;
-    ++c.s.n_epochs;
-    algo.next_start_time();
-    auto const start_time = starts[from].time_at_start_;
-    for (auto i = from; i != to; ++i) algo.add_start(starts[i].stop_, starts[i].time_at_stop_);
-    auto const worst_time_at_dest =
-        start_time + (Fwd ? 1 : -1) * (std::min(fastest_direct, q.max_travel_time_) + 1);
-    algo.execute(start_time, q.max_transfers_, worst_time_at_dest, results);
-    for (auto& j : results.els_) {
-      if (!j.is_reconstructed_ && !j.error_ &&
-          (is_ontrip || search_interval.contains(j.start_time_)) &&
-          j.travel_time() < fastest_direct) {
-        try {
-          reconstructor<Fwd>{tt, q, rs, base_day, j}.run();
-        } catch (std::exception const&) {
-          j.error_ = true;
+  auto const n_results_in_interval = [&]() {  // search.h:382-391
+    if (!q.is_interval_) return static_cast<unsigned>(results.els_.size());
+    return static_cast<unsigned>(std::count_if(
+        results.els_.begin(), results.els_.end(),
+        [&](journey_t const& j) { return search_interval.contains(j.start_time_); }));
+  };
+  auto const min_count = static_cast<unsigned>(q.min_connection_count_);
+
+  // execute() (search.h:204-320)
+  while (true) {
+    // search_interval() (search.h:419-494)
+    auto early_termination = false;
+    for (auto from = 0U; from != starts.size();) {
+      auto to = from;
+      while (to != starts.size() && starts[to].time_at_start_ == starts[from].time_at_start_) ++to;
+      if (early_termination) {
+        from = to;
+        continue;
+      }
+      ++c.s.n_epochs;
+      algo.next_start_time();
+      auto const start_time = starts[from].time_at_start_;
+      for (auto i = from; i != to; ++i) algo.add_start(starts[i].stop_, starts[i].time_at_stop_);
+      auto const worst_time_at_dest =
+          start_time + (Fwd ? 1 : -1) * (std::min(fastest_direct, q.max_travel_time_) + 1);
+      algo.execute(start_time, q.max_transfers_, worst_time_at_dest, results);
+      for (auto& j : results.els_) {
+        if (!j.is_reconstructed_ && !j.error_ &&
+            (is_ontrip || search_interval.contains(j.start_time_)) &&
+            j.travel_time() < fastest_direct) {
+          try {
+            reconstructor<Fwd>{tt, q, rs, base_day, j}.run();
+          } catch (std::exception const&) {
+            j.error_ = true;
+          }
         }
       }
+      if (min_count > 0U && n_results_in_interval() >= min_count &&
+          ((Fwd && q.extend_earlier_ && !q.extend_later_) ||
+           (!Fwd && !q.extend_earlier_ && q.extend_later_))) {
+        early_termination = true;
+        if (Fwd) {
+          search_interval.from_ = start_time;
+        } else {
+          search_interval.to_ = start_time + 1;
+        }
+      }
+      from = to;
     }
-    from = to;
+
+    if (results.els_.empty() || is_ontrip || n_results_in_interval() >= min_count) break;
+
+    auto const new_interval = est.extension(search_interval, min_count - n_results_in_interval());
+    if (new_interval == search_interval) break;
+    starts.clear();
+    auto const remove_ontrip_results = [&]() {
+      auto& e = results.els_;
+      e.erase(std::remove_if(e.begin(), e.end(),
+                             [&](journey_t const& j) { return !search_interval.contains(j.start_time_); }),
+              e.end());
+    };
+    if (new_interval.from_ != search_interval.from_) {
+      add_start_labels(interval_t{new_interval.from_, search_interval.from_}, !Fwd);
+      if (!Fwd) {
+        algo.reset_arrivals();
+        remove_ontrip_results();
+      }
+    }
+    if (new_interval.to_ != search_interval.to_) {
+      add_start_labels(interval_t{search_interval.to_, new_interval.to_}, Fwd);
+      if (Fwd) {
+        algo.reset_arrivals();
+        remove_ontrip_results();
+      }
+    }
+    search_interval = new_interval;
   }
-  // the extension loop ends after one pass when min_connection_count == 0
 
   if (!is_ontrip) {
     auto& e = results.els_;

@@ -65,3 +65,33 @@ INTERMODAL_FWD_LEGS = [
      ("run", "A", "B", "06:00", "07:00", "1337"), ("footpath", "B", "B", "07:00", "07:02", 2),
      ("run", "B", "C", "07:15", "08:15", "7331"), ("offset", "C", "END", "08:15", "08:30", "C", 15)],
 ]
+
+# test/routing/raptor_intermodal_test.cc:85-251 (raptor_intermodal_backward)
+# on the same timetable: start offset {C, 15 min}, destination offset
+# {A, 10 min}, interval [05:00, 06:00) UTC, backward, min_connection_count 3,
+# extending the interval earlier and later.  The printed journeys run from
+# END (departure) to START (arrival), every 30 minutes from 00:20 to 03:50:
+# (departure at END, arrival at START, transfers).
+INTERMODAL_BWD_START = ("C", 15)
+INTERMODAL_BWD_DEST = ("A", 10)
+INTERMODAL_BWD_MIN_CONNECTIONS = 3
+INTERMODAL_BWD_EXPECTED = [
+    ("00:20", "03:00", 1), ("00:50", "03:30", 1), ("01:20", "04:00", 1), ("01:50", "04:30", 1),
+    ("02:20", "05:00", 1), ("02:50", "05:30", 1), ("03:20", "06:00", 1), ("03:50", "06:30", 1),
+]
+
+
+def _hm(m):
+    return f"{m // 60:02d}:{m % 60:02d}"
+
+
+# their legs: END -> A offset (10), A -> B (1337), transfer at B (2),
+# B -> C (7331), C -> START offset (15)
+INTERMODAL_BWD_LEGS = [
+    [("offset", "END", "A", _hm(d), _hm(d + 10), "A", 10),
+     ("run", "A", "B", _hm(d + 10), _hm(d + 70), "1337"),
+     ("footpath", "B", "B", _hm(d + 70), _hm(d + 72), 2),
+     ("run", "B", "C", _hm(d + 85), _hm(d + 145), "7331"),
+     ("offset", "C", "START", _hm(d + 145), _hm(d + 160), "C", 15)]
+    for d in range(20, 20 + 8 * 30, 30)
+]

@@ -219,3 +219,37 @@ def test_gpu_algo_reconstruct_equals_search_legs():
             assert eng.reconstruct(res[i]["slot"], qs[i], j) == legs
             n += 1
     assert n > 0
+
+
+def test_gpu_golden_intermodal_bwd_interval_extension():
+    # raptor_intermodal_test.cc:85-251: min_connection_count 3 widens the
+    # interval; the epochs of the new parts run on the same slot
+    from test_oracle_golden import intermodal_bwd_query
+    tt, loc, day0 = abc_timetable()
+    img = na.Image(tt)
+    eng = na.Engine(img, lanes=64, groups=1, max_k=16)
+    say("golden abc intermodal bwd with interval extension")
+    res = compare(tt, eng, intermodal_bwd_query(loc, day0))
+    assert len(res[0]["journeys"]) == len(G.INTERMODAL_BWD_EXPECTED)
+
+
+@pytest.mark.parametrize("lanes", [64, 1])
+def test_gpu_interval_extension_workload3(lanes):
+    # min_connection_count with each extension direction, forward and backward
+    # (early termination for one-sided extensions, resets when extending the
+    # wrong way)
+    tt, img = image_for("w3s", lambda: na.Timetable.generate(3, scale=0.03))
+    eng = na.Engine(img, lanes=lanes, groups=1 if lanes == 64 else 8, max_k=10)
+    n = 48 if lanes == 64 else 8
+    qs = na.Queries.generate(tt, 3, n, seed=0x7172 + 31)
+    for i in range(n):
+        qs[i].min_connection_count = (2, 3, 5)[i % 3]
+        qs[i].extend_interval_earlier, qs[i].extend_interval_later = ((1, 0), (0, 1), (1, 1))[(i // 3) % 3]
+    # one direction per search call: a search runs its backward queries after
+    # the forward ones, on the same slots, so only one direction's states remain
+    say(f"workload3 interval extension lanes={lanes} n={n} fwd")
+    compare(tt, eng, qs)
+    for i in range(n):
+        qs[i].dir = na.DIR_BWD
+    say(f"workload3 interval extension lanes={lanes} n={n} bwd")
+    compare(tt, eng, qs)

@@ -161,3 +161,51 @@ def test_reconstructed_legs_golden(direction):
     order = sorted(range(len(out["journeys"])), key=lambda i: min(out["journeys"][i][:2]))
     got = [comparable(out["legs"][i]) for i in order]
     assert got == [comparable(expected_legs(w, loc, day0, traffic_day)) for w in want]
+
+
+def intermodal_bwd_query(loc, day0):
+    iv = (day0 + hm(G.INTERVAL[0]), day0 + hm(G.INTERVAL[1]))
+    (s, sd), (d, dd) = G.INTERMODAL_BWD_START, G.INTERMODAL_BWD_DEST
+    return na.Queries.build([dict(start=[(loc[s], sd)], dest=[(loc[d], dd)], interval=iv,
+                                  dir=na.DIR_BWD, start_match_mode=na.MATCH_INTERMODAL,
+                                  dest_match_mode=na.MATCH_INTERMODAL,
+                                  min_connection_count=G.INTERMODAL_BWD_MIN_CONNECTIONS,
+                                  extend_interval_earlier=1, extend_interval_later=1)])
+
+
+def test_raptor_intermodal_backward_golden():
+    # raptor_intermodal_test.cc:85-251: the interval extension (search.h:237-317,
+    # interval_estimate.h) widens [05:00, 06:00) until 3 journeys start in it
+    tt, loc, day0 = abc_timetable()
+    traffic_day = (day0 - tt.view.internal_from) // 1440
+    out = oracle.search(tt, intermodal_bwd_query(loc, day0)[0])
+    got = sorted(zip(out["journeys"], out["legs"]), key=lambda x: x[0][1])
+    want = [(day0 + hm(a), day0 + hm(d), 1, k) for d, a, k in G.INTERMODAL_BWD_EXPECTED]
+    assert [j for j, _ in got] == want
+    assert [comparable(l) for _, l in got] == [
+        comparable(expected_legs(w, loc, day0, traffic_day)) for w in G.INTERMODAL_BWD_LEGS]
+
+
+def egress_timetable():
+    from golden import intermodal_egress as E
+    b = Builder(day_number(*E.RANGE_FROM), E.RANGE_DAYS, 4)
+    loc = {"A": 9, "A2": 10, "B": 11, "C": 12}
+    day = day_number(*E.DATE)
+    for stops, times in E.TRIPS.values():
+        b.add_line([loc[x] for x in stops], [(times, [day])])
+    return b.build(), loc, day * 1440, E
+
+
+def test_intermodal_egress_at_bound_golden():
+    # gpu_intermodal_egress_test.cc:89-119: the one-transfer journey via A and B
+    # leaves the start at 09:55 and reaches END at 11:00
+    tt, loc, day0, E = egress_timetable()
+    q = na.Queries.build([dict(start=[(loc[s], d) for s, d in E.START],
+                               dest=[(loc[s], d) for s, d in E.DEST],
+                               interval=(day0 + hm(E.INTERVAL[0]), day0 + hm(E.INTERVAL[1])),
+                               start_match_mode=na.MATCH_INTERMODAL,
+                               dest_match_mode=na.MATCH_INTERMODAL,
+                               min_connection_count=E.MIN_CONNECTIONS)])
+    js = oracle.search(tt, q[0])["journeys"]
+    dep, arr, k = E.MUST_FIND
+    assert (day0 + hm(dep), day0 + hm(arr), 1, k) in js

@@ -14,16 +14,25 @@
 //   include/nigiri/common/dial.h                    bucket queue
 //   src/routing/get_fastest_direct.cc               fastest direct connection
 //   include/nigiri/routing/interval_estimate.h      initial() / extension()
+//   src/routing/raptor/reconstruct.cc               reconstruct_journey
+//   src/routing/raptor/optimize_footpaths.cc        optimize_footpaths
 //   include/nigiri/routing/pareto_set.h, journey.h  result set
 //   include/nigiri/common/delta_t.h                 clamp, split_day_mam, ...
 //   include/nigiri/routing/transfer_time_settings.h adjusted_transfer_time
 //
 // How each part is pinned (tests/test_oracle_*.py):
 //   raptor execute / search driver: golden journeys of the reference tests
-//     (test/routing/raptor_test.cc:18-112 and the forward case of
-//     test/routing/raptor_intermodal_test.cc:16-83, transcribed into our
-//     builder), closed forms on periodic line networks, and brute force
-//     (time-expanded exhaustive search) on tiny random networks.
+//     (test/routing/raptor_test.cc:18-112, test/routing/
+//     raptor_intermodal_test.cc:16-251 including the backward search with
+//     interval extension, test/routing/gpu_intermodal_egress_test.cc,
+//     transcribed into our builder), closed forms on periodic line
+//     networks, and brute force (time-expanded exhaustive search) on tiny
+//     random networks.
+//   reconstruct_journey / optimize_footpaths: the printed legs of the same
+//     golden journeys; every reconstructed journey is a feasible itinerary
+//     of the timetable; optimize_footpaths on the scenarios of
+//     test/routing/optmize_footpaths_test.cc without their via stops, with
+//     expected legs derived by hand.
 //   dijkstra lb: brute-force Bellman-Ford on random graphs, and lb <= the
 //     travel time of every journey.  (test/routing/dijkstra_test.cc needs
 //     parent-station equivalence, which this timetable model does not have.)

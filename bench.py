@@ -72,7 +72,7 @@ def serve_main(args):
     img = na.Image(tt)
     max_k = 16 if cfg in (1, 2) else 10
     eng = na.Engine(img, lanes=64, groups=args.groups, max_k=max_k)
-    eng.configure(admission_wait_ms=1.0)
+    eng.configure(admission_wait_ms=1.0, graphs=not args.eager)
     kinds = np.array(["intermodal" if qs[i].dest_match_mode == na.MATCH_INTERMODAL
                       else ("range" if qs[i].is_interval else "point") for i in range(n)])
     eng.serve(qs.subset(list(range(min(64, n)))), np.zeros(min(64, n)))  # warm-up
@@ -134,6 +134,8 @@ def main():
                     help="serving measurement instead: Poisson arrivals at RATE queries/s "
                          "of the workload's query mix through the continuous-batching loop")
     ap.add_argument("--groups", type=int, default=16, help="64-query groups (serving mode)")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch each round's kernels one by one instead of replaying captured hipGraphs")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -169,7 +171,7 @@ def main():
     eng.profile(True)
     # a step submits a whole batch at once: let preparation fill every slot
     # before the first epoch starts
-    eng.configure(admission_wait_ms=args.admission_wait_ms)
+    eng.configure(admission_wait_ms=args.admission_wait_ms, graphs=not args.eager)
     B = groups * 64
     n_q = len(qs)
 
@@ -204,6 +206,7 @@ def main():
 
     # plain one-query-per-launch GPU path on a sample of the same queries
     plain = na.Engine(img, lanes=1, groups=1, max_k=max_k)
+    plain.configure(graphs=not args.eager)
     ps = qs.subset(list(range(min(args.plain_sample, n_q))))
     plain.search(ps.subset([0]))  # warm-up
     sync()
@@ -260,6 +263,8 @@ def main():
         "journeys": n_journeys,
         "lane_utilization": round(st["work"]["route_stops"] / max(1, 64 * st["work"]["wave_stops"]), 4),
         "rounds": st["rounds"], "epochs": st["epochs"],
+        "round_launch": "eager" if args.eager else "hipGraph",
+        "graph_rounds": int(st["graph_rounds"]), "graph_captures": int(st["graph_captures"]),
         "work": {k: int(v) for k, v in st["work"].items()},
         "host_ms_per_step": {"serve_call": round(serve_wall / args.steps, 1),
                              "journey_extraction": round(st["results_ms"] / args.steps, 1),

@@ -325,6 +325,8 @@ typedef struct nigiri_engine_stats {
   double reconstruct_ms;    /* host wall time of the reconstruction steps   */
   double legs_ms;           /*   of which writing the legs on the host      */
   double results_ms;        /* host wall time of the journey extraction     */
+  uint64_t graph_rounds;    /* rounds replayed from captured hipGraphs      */
+  uint64_t graph_captures;  /* round graphs captured (engine lifetime)      */
 } nigiri_engine_stats;
 
 int nigiri_amd_engine_profile(nigiri_amd_engine* e, int enable);
@@ -335,11 +337,14 @@ int nigiri_amd_engine_stats(nigiri_amd_engine* e, nigiri_engine_stats* out,
  * admission_wait_ms: serving loop; at an epoch boundary, wait up to this
  *   long for queries that have arrived but are still being prepared, so
  *   that free slots start the epoch filled (default 2 ms).
- * grid_waves: wavefronts per persistent-kernel launch (0 = 32 per CU).   */
+ * grid_waves: wavefronts per persistent-kernel launch (0 = 32 per CU).
+ * eager_rounds: 0 = each round's kernel sequence is captured once as a
+ *   hipGraph and replayed (default); 1 = kernels are launched one by one. */
 typedef struct nigiri_engine_config {
   double admission_wait_ms;
   uint32_t grid_waves;
-  uint32_t reserved[7];
+  uint32_t eager_rounds;
+  uint32_t reserved[6];
 } nigiri_engine_config;
 
 int nigiri_amd_engine_configure(nigiri_amd_engine* e,

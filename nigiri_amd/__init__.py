@@ -118,12 +118,13 @@ class EngineStats(C.Structure):
                 ("transfer_launches", C.c_uint64), ("transfer_ms", C.c_double),
                 ("work", C.c_uint64 * 16), ("reconstructed", C.c_uint64),
                 ("reconstruct_ms", C.c_double), ("legs_ms", C.c_double),
-                ("results_ms", C.c_double)]
+                ("results_ms", C.c_double), ("graph_rounds", C.c_uint64),
+                ("graph_captures", C.c_uint64)]
 
 
 class EngineConfig(C.Structure):
     _fields_ = [("admission_wait_ms", C.c_double), ("grid_waves", C.c_uint32),
-                ("reserved", C.c_uint32 * 7)]
+                ("eager_rounds", C.c_uint32), ("reserved", C.c_uint32 * 6)]
 
 
 WORK_NAMES = ["route_stops", "events", "route_day_tests", "transport_day_tests", "tmp_writes",
@@ -433,10 +434,11 @@ class Engine:
         _check(lib().nigiri_amd_engine_state_bytes(self.handle, C.byref(b)))
         return b.value
 
-    def configure(self, admission_wait_ms=2.0, grid_waves=0):
+    def configure(self, admission_wait_ms=2.0, grid_waves=0, graphs=True):
         c = EngineConfig()
         c.admission_wait_ms = admission_wait_ms
         c.grid_waves = grid_waves
+        c.eager_rounds = 0 if graphs else 1
         _check(lib().nigiri_amd_engine_configure(self.handle, C.byref(c)))
 
     def profile(self, enable=True):

@@ -266,12 +266,16 @@ struct nigiri_amd_engine {
   // configuration
   double admission_wait_ms = 2.0;
   std::uint32_t grid_waves = 8192U;
+  // rounds replayed from captured hipGraphs (off: kernels launched one by one)
+  bool use_graphs = true;
+  round_graphs* graphs = round_graphs_create();
   // instrumentation
   bool profile = false;
   std::vector<hipEvent_t> events;  // 4 per round of one epoch
   nigiri_engine_stats stats{};
 
   ~nigiri_amd_engine() {
+    round_graphs_destroy(graphs);
     for (auto ev : events) (void)hipEventDestroy(ev);
     for (auto* p : grow_ptrs) (void)hipFree(p);
     if (stream != nullptr) (void)hipStreamDestroy(stream);
@@ -858,15 +862,23 @@ struct runner {
     if (!any) return settle();
     auto const stream = e.stream;
     auto const& tr = e.st;
-    std::vector<std::uint32_t> sweep_groups;
+    // the sweep list in device memory: [count, groups...]
+    std::vector<std::uint32_t> sweep_groups{0U};
     for (std::uint32_t g = 0; g != G; ++g) {
       if (sweep[g] != 0U) sweep_groups.push_back(g);
     }
-    if (!sweep_groups.empty()) {
+    sweep_groups[0] = static_cast<std::uint32_t>(sweep_groups.size() - 1U);
+    bool const has_sweep = sweep_groups[0] != 0U;
+    if (has_sweep) {
       HIP_CHECK(hipMemcpyAsync(e.d_sweep_groups, sweep_groups.data(), 4U * sweep_groups.size(), hipMemcpyHostToDevice, stream));
     }
-    launch_round(fwd, W, rargs(), e.d_sweep_groups, static_cast<std::uint32_t>(sweep_groups.size()), S,
-                 static_cast<int>(e.grid_waves), stream);
+    if (e.use_graphs) {
+      HIP_CHECK(launch_round_graph(*e.graphs, fwd, W, rargs(), e.d_sweep_groups, has_sweep, S,
+                                   static_cast<int>(e.grid_waves), stream));
+      ++e.stats.graph_rounds;
+    } else {
+      launch_round(fwd, W, rargs(), e.d_sweep_groups, has_sweep, S, static_cast<int>(e.grid_waves), stream);
+    }
     HIP_CHECK(hipGetLastError());
     settle();  // while the round runs on the device
     ++rounds;
@@ -1351,7 +1363,7 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     e->d_bt_mask = m.alloc<std::uint64_t>(G);
     e->d_flags = m.alloc<std::uint8_t>(S);
     e->d_worst = m.alloc<std::int16_t>(S);
-    e->d_sweep_groups = m.alloc<std::uint32_t>(G);
+    e->d_sweep_groups = m.alloc<std::uint32_t>(G + 1U);
     e->d_slot_list = m.alloc<std::uint32_t>(S);
     e->lb_rows_cap = std::min<std::uint32_t>(S, 64U);
     e->d_lb_staging = m.alloc<std::uint16_t>(static_cast<std::size_t>(e->lb_rows_cap) * L);
@@ -1409,6 +1421,7 @@ int nigiri_amd_engine_configure(nigiri_amd_engine* e, const nigiri_engine_config
   if (e == nullptr || c == nullptr || c->admission_wait_ms < 0.0) return NIGIRI_AMD_EINVAL;
   e->admission_wait_ms = c->admission_wait_ms;
   if (c->grid_waves != 0U) e->grid_waves = c->grid_waves;
+  e->use_graphs = c->eager_rounds == 0U;
   return NIGIRI_AMD_OK;
 }
 
@@ -1433,6 +1446,7 @@ int nigiri_amd_engine_stats(nigiri_amd_engine* e, nigiri_engine_stats* out, int 
     unsigned long long w[kWorkCount];
     HIP_CHECK(hipMemcpy(w, e->st.work, sizeof(w), hipMemcpyDeviceToHost));
     *out = e->stats;
+    out->graph_captures = round_graphs_captures(e->graphs);
     for (int i = 0; i != kWorkCount; ++i) out->work[i] = w[i];
     if (reset != 0) {
       e->stats = nigiri_engine_stats{};
@@ -1813,9 +1827,13 @@ int nigiri_amd_algo_execute(nigiri_amd_engine* e, uint32_t slot, int32_t start_t
     ra.st = tr;
     ra.invalid = a.invalid;
     // a slot after its first epoch sees earlier rows in round_times
-    std::uint32_t const n_sweep = e->algo_new[slot] != 0U ? 0U : 1U;
+    bool const has_sweep = e->algo_new[slot] == 0U;
+    std::uint32_t const sw[2] = {1U, g};
+    if (has_sweep) {
+      HIP_CHECK(hipMemcpyAsync(e->d_sweep_groups, sw, sizeof(sw), hipMemcpyHostToDevice, stream));
+    }
     while (true) {
-      launch_round(fwd, e->W, ra, e->d_reset_groups, n_sweep, S, 2048, stream);
+      launch_round(fwd, e->W, ra, e->d_sweep_groups, has_sweep, S, 2048, stream);
       HIP_CHECK(hipGetLastError());
       ++rounds;
       HIP_CHECK(hipMemcpyAsync(e->h_active.p, tr.grp_active, 8U * G, hipMemcpyDeviceToHost, stream));

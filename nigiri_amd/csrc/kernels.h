@@ -75,9 +75,24 @@ void launch_lb_queries(bool fwd, lb_args const& a, hipStream_t stream);
 constexpr std::uint32_t kBatchedMaxRouteTransports = 256U;
 
 // Launch helpers (defined in kernels.hip), all asynchronous on `stream`.
+// One round for all running slots.  `sweep` is device memory: sweep[0] =
+// number of groups whose best row is swept, sweep[1..] = those groups; it is
+// read only when has_sweep is set.
 void launch_round(bool fwd, std::uint32_t W, round_args const& a,
-                  std::uint32_t const* sweep_groups, std::uint32_t n_sweep_groups,
+                  std::uint32_t const* sweep, bool has_sweep,
                   std::uint32_t n_slots, int grid_waves, hipStream_t stream);
+
+// The same round replayed from hipGraphs captured on first use (per
+// direction, lanes, sweep on/off, grid and round_args; recaptured when the
+// arguments change).  When a.ev is set the round is held as one graph per
+// segment and the events are recorded between the graphs.
+struct round_graphs;
+round_graphs* round_graphs_create();
+void round_graphs_destroy(round_graphs* g);
+std::uint64_t round_graphs_captures(round_graphs const* g);
+hipError_t launch_round_graph(round_graphs& g, bool fwd, std::uint32_t W, round_args const& a,
+                              std::uint32_t const* sweep, bool has_sweep, std::uint32_t n_slots,
+                              int grid_waves, hipStream_t stream);
 
 struct epoch_args {
   dev_state st;

@@ -2,6 +2,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
+#include <vector>
 
 #include "kernels.h"
 
@@ -253,10 +255,14 @@ __global__ void k_round_begin(round_args a, std::uint32_t n_slots) {
   }
 }
 
-// best = min(round_times[k], best) for groups holding later-epoch slots
+// best = min(round_times[k], best) for groups holding later-epoch slots;
+// sweep[0] is the number of groups, sweep[1..] their indices (in device
+// memory, so that a captured round replays with the current list)
 template <bool Fwd, int W>
-__global__ void k_best_sweep(round_args a, std::uint32_t const* groups, std::uint32_t n_groups) {
+__global__ void k_best_sweep(round_args a, std::uint32_t const* sweep) {
   auto const& s = a.st;
+  auto const n_groups = sweep[0];
+  auto const* groups = sweep + 1;
   auto const per_group = static_cast<std::size_t>(s.L) * W;
   auto const total = per_group * n_groups;
   for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < total;
@@ -1664,46 +1670,81 @@ void compact(std::uint8_t* flags, std::uint32_t n_minor, std::uint32_t n_major, 
                      n_per_group, n_words, list, count, cap, &s.counts[4]);
 }
 
+// One round (raptor.h:218-387) as five segments of kernels.  The route scan
+// (segment 1) and the transfer kernel (segment 3) are segments of their
+// own, so that profiling can record events between segments.
+constexpr int kRoundSegments = 5;
+
 template <bool Fwd, int W>
-void round_impl(round_args const& a, std::uint32_t const* sweep_groups, std::uint32_t n_sweep,
-                std::uint32_t n_slots, int grid_waves, hipStream_t st) {
+void round_segment(int seg, round_args const& a, std::uint32_t const* sweep, bool has_sweep,
+                   std::uint32_t n_slots, int grid_waves, hipStream_t st) {
   auto const slot_blocks = grid_for(n_slots, kBlock, 1 << 20);
   auto const item_blocks = W == 64 ? (grid_waves + kWavesPerBlock - 1) / kWavesPerBlock
                                    : (grid_waves * 64 + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL((k_round_end_check<W>), dim3(slot_blocks), dim3(kBlock), 0, st, a, n_slots);
-  if (n_sweep != 0U) {
-    hipLaunchKernelGGL((k_best_sweep<Fwd, W>), dim3(2048), dim3(kBlock), 0, st, a, sweep_groups, n_sweep);
+  switch (seg) {
+    case 0:
+      hipLaunchKernelGGL((k_round_end_check<W>), dim3(slot_blocks), dim3(kBlock), 0, st, a, n_slots);
+      if (has_sweep) {
+        hipLaunchKernelGGL((k_best_sweep<Fwd, W>), dim3(2048), dim3(kBlock), 0, st, a, sweep);
+      }
+      hipLaunchKernelGGL((k_round_begin<Fwd, W>), dim3(slot_blocks), dim3(kBlock), 0, st, a, n_slots);
+      hipLaunchKernelGGL((k_mark_routes<W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
+      if constexpr (W == 64) {
+        compact(a.st.flag_route, a.st.G, a.st.R, a.st.list_route, &a.st.counts[2], a.st.cap_route, a.st, st);
+        hipLaunchKernelGGL(k_route_pack, dim3(item_blocks), dim3(kBlock), 0, st, a);
+      }
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_route_scan<Fwd, W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_clear_marks<W>), dim3(1024), dim3(kBlock), 0, st, a.st, 0);
+      hipLaunchKernelGGL((k_group_update), dim3(grid_for(n_slots, kBlock, 1 << 20)), dim3(kBlock), 0, st, a.st);
+      // list_t, routes, packed
+      hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 1U | 4U | 32U);
+      if constexpr (W == 64) {
+        compact(a.st.flag_r, a.st.L, a.st.G, a.st.list_r, &a.st.counts[1], a.st.cap_r, a.st, st);
+      }
+      hipLaunchKernelGGL((k_dest_pass<Fwd, W>), dim3(grid_for(n_slots, 64, 1 << 20)), dim3(64), 0, st, a, n_slots);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_transfer_fp<Fwd, W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
+      break;
+    default:
+      if constexpr (W == 64) {
+        compact(a.st.flag_t, a.st.L, a.st.G, a.st.list_t, &a.st.counts[0], a.st.cap_t, a.st, st);
+        compact(a.st.flag_hz, a.st.L, a.st.G, a.st.list_hz, &a.st.counts[3], a.st.cap_hz, a.st, st);
+      }
+      hipLaunchKernelGGL((k_rt_fixup<Fwd, W>), dim3(1024), dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((k_hazards<Fwd, W>), dim3(1024), dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((k_clear_marks<W>), dim3(1024), dim3(kBlock), 0, st, a.st, 1);
+      hipLaunchKernelGGL((k_clear_marks<W>), dim3(256), dim3(kBlock), 0, st, a.st, 2);
+      // list_r, hazards, work cursors of the route scan and transfers
+      hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 2U | 8U | 64U | 128U);
+      hipLaunchKernelGGL((k_round_finish<W>), dim3(slot_blocks), dim3(kBlock), 0, st, a, n_slots);
+      break;
   }
-  hipLaunchKernelGGL((k_round_begin<Fwd, W>), dim3(slot_blocks), dim3(kBlock), 0, st, a, n_slots);
-  hipLaunchKernelGGL((k_mark_routes<W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
-  if constexpr (W == 64) {
-    compact(a.st.flag_route, a.st.G, a.st.R, a.st.list_route, &a.st.counts[2], a.st.cap_route, a.st, st);
-    hipLaunchKernelGGL(k_route_pack, dim3(item_blocks), dim3(kBlock), 0, st, a);
+}
+
+void round_segment_any(bool fwd, std::uint32_t W, int seg, round_args const& a, std::uint32_t const* sweep,
+                       bool has_sweep, std::uint32_t n_slots, int grid_waves, hipStream_t st) {
+  if (W == 64U) {
+    fwd ? round_segment<true, 64>(seg, a, sweep, has_sweep, n_slots, grid_waves, st)
+        : round_segment<false, 64>(seg, a, sweep, has_sweep, n_slots, grid_waves, st);
+  } else {
+    fwd ? round_segment<true, 1>(seg, a, sweep, has_sweep, n_slots, grid_waves, st)
+        : round_segment<false, 1>(seg, a, sweep, has_sweep, n_slots, grid_waves, st);
   }
-  if (a.ev[0] != nullptr) (void)hipEventRecord(a.ev[0], st);
-  hipLaunchKernelGGL((k_route_scan<Fwd, W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
-  if (a.ev[1] != nullptr) (void)hipEventRecord(a.ev[1], st);
-  hipLaunchKernelGGL((k_clear_marks<W>), dim3(1024), dim3(kBlock), 0, st, a.st, 0);
-  hipLaunchKernelGGL((k_group_update), dim3(grid_for(n_slots, kBlock, 1 << 20)), dim3(kBlock), 0, st, a.st);
-  hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 1U | 4U | 32U);  // list_t, routes, packed
-  if constexpr (W == 64) {
-    compact(a.st.flag_r, a.st.L, a.st.G, a.st.list_r, &a.st.counts[1], a.st.cap_r, a.st, st);
-  }
-  hipLaunchKernelGGL((k_dest_pass<Fwd, W>), dim3(grid_for(n_slots, 64, 1 << 20)), dim3(64), 0, st, a, n_slots);
-  if (a.ev[2] != nullptr) (void)hipEventRecord(a.ev[2], st);
-  hipLaunchKernelGGL((k_transfer_fp<Fwd, W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
-  if (a.ev[3] != nullptr) (void)hipEventRecord(a.ev[3], st);
-  if constexpr (W == 64) {
-    compact(a.st.flag_t, a.st.L, a.st.G, a.st.list_t, &a.st.counts[0], a.st.cap_t, a.st, st);
-    compact(a.st.flag_hz, a.st.L, a.st.G, a.st.list_hz, &a.st.counts[3], a.st.cap_hz, a.st, st);
-  }
-  hipLaunchKernelGGL((k_rt_fixup<Fwd, W>), dim3(1024), dim3(kBlock), 0, st, a);
-  hipLaunchKernelGGL((k_hazards<Fwd, W>), dim3(1024), dim3(kBlock), 0, st, a);
-  hipLaunchKernelGGL((k_clear_marks<W>), dim3(1024), dim3(kBlock), 0, st, a.st, 1);
-  hipLaunchKernelGGL((k_clear_marks<W>), dim3(256), dim3(kBlock), 0, st, a.st, 2);
-  // list_r, hazards, work cursors of the route scan and transfers
-  hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 2U | 8U | 64U | 128U);
-  hipLaunchKernelGGL((k_round_finish<W>), dim3(slot_blocks), dim3(kBlock), 0, st, a, n_slots);
+}
+
+// events around the route scan (0, 1) and the transfer kernel (2, 3)
+void record_before(round_args const& a, int seg, hipStream_t st) {
+  if (seg == 1 && a.ev[0] != nullptr) (void)hipEventRecord(a.ev[0], st);
+  if (seg == 3 && a.ev[2] != nullptr) (void)hipEventRecord(a.ev[2], st);
+}
+void record_after(round_args const& a, int seg, hipStream_t st) {
+  if (seg == 1 && a.ev[1] != nullptr) (void)hipEventRecord(a.ev[1], st);
+  if (seg == 3 && a.ev[3] != nullptr) (void)hipEventRecord(a.ev[3], st);
 }
 
 template <bool Fwd, int W>
@@ -1915,16 +1956,113 @@ __global__ __launch_bounds__(64) void k_reconstruct(rc_args a) {
 
 }  // namespace
 
-void launch_round(bool fwd, std::uint32_t W, round_args const& a, std::uint32_t const* sweep_groups,
-                  std::uint32_t n_sweep_groups, std::uint32_t n_slots, int grid_waves,
-                  hipStream_t stream) {
-  if (W == 64U) {
-    fwd ? round_impl<true, 64>(a, sweep_groups, n_sweep_groups, n_slots, grid_waves, stream)
-        : round_impl<false, 64>(a, sweep_groups, n_sweep_groups, n_slots, grid_waves, stream);
-  } else {
-    fwd ? round_impl<true, 1>(a, sweep_groups, n_sweep_groups, n_slots, grid_waves, stream)
-        : round_impl<false, 1>(a, sweep_groups, n_sweep_groups, n_slots, grid_waves, stream);
+void launch_round(bool fwd, std::uint32_t W, round_args const& a, std::uint32_t const* sweep,
+                  bool has_sweep, std::uint32_t n_slots, int grid_waves, hipStream_t stream) {
+  for (int seg = 0; seg != kRoundSegments; ++seg) {
+    record_before(a, seg, stream);
+    round_segment_any(fwd, W, seg, a, sweep, has_sweep, n_slots, grid_waves, stream);
+    record_after(a, seg, stream);
   }
+}
+
+// Captured rounds.  A graph holds the kernels of one round for fixed
+// arguments (direction, lanes, sweep on/off, grid, and the round_args
+// bytes apart from the events); it is captured on first use and replayed
+// while the arguments stay the same.  With events set, the round is held
+// as one graph per segment and the events are recorded between them.
+struct round_graphs {
+  struct entry {
+    bool fwd, has_sweep, segmented;
+    std::uint32_t W, n_slots;
+    int grid_waves;
+    std::uint32_t const* sweep;
+    round_args key;
+    hipGraphExec_t exec[kRoundSegments];
+    int n_exec;
+  };
+  std::vector<entry> entries;
+  std::uint64_t captures = 0U;
+  ~round_graphs() {
+    for (auto& e : entries) {
+      for (int i = 0; i != e.n_exec; ++i) (void)hipGraphExecDestroy(e.exec[i]);
+    }
+  }
+};
+
+round_graphs* round_graphs_create() { return new round_graphs{}; }
+void round_graphs_destroy(round_graphs* g) { delete g; }
+std::uint64_t round_graphs_captures(round_graphs const* g) { return g == nullptr ? 0U : g->captures; }
+
+namespace {
+
+hipError_t capture_segments(bool fwd, std::uint32_t W, int seg0, int seg1, round_args const& a,
+                            std::uint32_t const* sweep, bool has_sweep, std::uint32_t n_slots,
+                            int grid_waves, hipStream_t stream, hipGraphExec_t* out) {
+  auto err = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+  if (err != hipSuccess) return err;
+  for (int seg = seg0; seg != seg1; ++seg) {
+    round_segment_any(fwd, W, seg, a, sweep, has_sweep, n_slots, grid_waves, stream);
+  }
+  auto const launch_err = hipGetLastError();
+  hipGraph_t graph = nullptr;
+  err = hipStreamEndCapture(stream, &graph);
+  if (launch_err != hipSuccess) err = launch_err;
+  if (err == hipSuccess) err = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+  if (graph != nullptr) (void)hipGraphDestroy(graph);
+  return err;
+}
+
+}  // namespace
+
+hipError_t launch_round_graph(round_graphs& g, bool fwd, std::uint32_t W, round_args const& a,
+                              std::uint32_t const* sweep, bool has_sweep, std::uint32_t n_slots,
+                              int grid_waves, hipStream_t stream) {
+  bool const segmented = a.ev[0] != nullptr;
+  round_args key = a;
+  for (auto& ev : key.ev) ev = nullptr;
+  round_graphs::entry* hit = nullptr;
+  for (auto& e : g.entries) {
+    if (e.fwd == fwd && e.has_sweep == has_sweep && e.segmented == segmented && e.W == W &&
+        e.n_slots == n_slots && e.grid_waves == grid_waves && e.sweep == sweep) {
+      if (std::memcmp(&e.key, &key, sizeof(round_args)) == 0) {
+        hit = &e;
+      } else {  // arguments changed: drop the stale graphs
+        for (int i = 0; i != e.n_exec; ++i) (void)hipGraphExecDestroy(e.exec[i]);
+        e.n_exec = 0;
+        e.key = key;
+        hit = &e;
+      }
+      break;
+    }
+  }
+  if (hit == nullptr) {
+    g.entries.push_back(round_graphs::entry{fwd, has_sweep, segmented, W, n_slots, grid_waves, sweep,
+                                            key, {}, 0});
+    hit = &g.entries.back();
+  }
+  if (hit->n_exec == 0) {
+    ++g.captures;
+    if (segmented) {
+      for (int seg = 0; seg != kRoundSegments; ++seg) {
+        auto const err = capture_segments(fwd, W, seg, seg + 1, a, sweep, has_sweep, n_slots, grid_waves,
+                                          stream, &hit->exec[seg]);
+        if (err != hipSuccess) return err;
+        hit->n_exec = seg + 1;
+      }
+    } else {
+      auto const err = capture_segments(fwd, W, 0, kRoundSegments, a, sweep, has_sweep, n_slots, grid_waves,
+                                        stream, &hit->exec[0]);
+      if (err != hipSuccess) return err;
+      hit->n_exec = 1;
+    }
+  }
+  for (int i = 0; i != hit->n_exec; ++i) {
+    if (segmented) record_before(a, i, stream);
+    auto const err = hipGraphLaunch(hit->exec[i], stream);
+    if (err != hipSuccess) return err;
+    if (segmented) record_after(a, i, stream);
+  }
+  return hipSuccess;
 }
 
 void launch_lb_queries(bool fwd, lb_args const& a, hipStream_t stream) {

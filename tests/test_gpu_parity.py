@@ -253,3 +253,35 @@ def test_gpu_interval_extension_workload3(lanes):
         qs[i].dir = na.DIR_BWD
     say(f"workload3 interval extension lanes={lanes} n={n} bwd")
     compare(tt, eng, qs)
+
+
+def test_gpu_graph_rounds_equal_eager_rounds():
+    # the round sequence replayed from captured hipGraphs gives the eager
+    # launches' journeys, legs and state; each graph is replayed many times
+    # with the runner's own copies, epoch starts and syncs between replays
+    tt, img = image_for("w3s", lambda: na.Timetable.generate(3, scale=0.03))
+    qs = na.Queries.generate(tt, 3, 128, seed=0x7172 + 41)
+    eager = na.Engine(img, lanes=64, groups=2, max_k=10)
+    eager.configure(graphs=False)
+    graph = na.Engine(img, lanes=64, groups=2, max_k=10)
+    graph.stats(reset=True)
+    say("graph rounds vs eager rounds, workload3 mix n=128")
+    a = eager.search(qs)
+    b = compare(tt, graph, qs)
+    st = graph.stats()
+    assert st["graph_rounds"] == st["rounds"] > 0
+    # one graph per (direction, sweep on/off): captured once, replayed
+    assert 1 <= st["graph_captures"] <= 2, st["graph_captures"]
+    assert eager.stats()["graph_rounds"] == 0
+    for i in range(len(qs)):
+        assert a[i]["journeys"] == b[i]["journeys"], i
+        assert a[i]["legs"] == b[i]["legs"], i
+        sa, sb = eager.download(a[i]["slot"]), graph.download(b[i]["slot"])
+        for key in STATE_KEYS:
+            assert np.array_equal(sa[key], sb[key]), (i, key)
+    # profiling splits the round into per-segment graphs with events between
+    graph.profile(True)
+    c = graph.search(qs)
+    assert [r["journeys"] for r in c] == [r["journeys"] for r in a]
+    st = graph.stats()
+    assert st["route_scan_launches"] > 0 and st["route_scan_ms"] > 0.0

@@ -134,6 +134,8 @@ def main():
                     help="serving measurement instead: Poisson arrivals at RATE queries/s "
                          "of the workload's query mix through the continuous-batching loop")
     ap.add_argument("--groups", type=int, default=16, help="64-query groups (serving mode)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes instead of autotuning them on the first batch")
     ap.add_argument("--eager", action="store_true",
                     help="launch each round's kernels one by one instead of replaying captured hipGraphs")
     args = ap.parse_args()
@@ -180,6 +182,13 @@ def main():
         return qs.subset(idx)
 
     zeros = np.zeros(B)
+    tuned = None
+    if not args.no_autotune:
+        t_tune = time.time()
+        chosen, trials = eng.autotune(batch(0))
+        tuned = {"chosen": chosen, "seconds": round(time.time() - t_tune, 1),
+                 "trials": [[k, v, ms] for k, v, ms in trials]}
+        print(f"[bench] autotune: {chosen} ({tuned['seconds']} s)", file=sys.stderr, flush=True)
     for w in range(args.warmup):
         eng.serve(batch(w), zeros, journey_capacity=8 * B, leg_capacity=320 * B, legs="raw")
         print(f"[bench] warmup step {w} done", file=sys.stderr, flush=True)
@@ -264,6 +273,7 @@ def main():
         "lane_utilization": round(st["work"]["route_stops"] / max(1, 64 * st["work"]["wave_stops"]), 4),
         "rounds": st["rounds"], "epochs": st["epochs"],
         "round_launch": "eager" if args.eager else "hipGraph",
+        "launch_config": eng.get_config(), "autotune": tuned,
         "graph_rounds": int(st["graph_rounds"]), "graph_captures": int(st["graph_captures"]),
         "work": {k: int(v) for k, v in st["work"].items()},
         "host_ms_per_step": {"serve_call": round(serve_wall / args.steps, 1),

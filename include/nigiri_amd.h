@@ -339,16 +339,56 @@ int nigiri_amd_engine_stats(nigiri_amd_engine* e, nigiri_engine_stats* out,
  *   that free slots start the epoch filled (default 2 ms).
  * grid_waves: wavefronts per persistent-kernel launch (0 = 32 per CU).
  * eager_rounds: 0 = each round's kernel sequence is captured once as a
- *   hipGraph and replayed (default); 1 = kernels are launched one by one. */
+ *   hipGraph and replayed (default); 1 = kernels are launched one by one.
+ * Launch-shape knobs (0 = keep the current value; results never depend on
+ * them, only speed does; nigiri_amd_engine_autotune sets them):
+ * route_claim: packed route wavefronts a wavefront claims per atomic (1).
+ * transfer_claim: stations a wavefront claims per atomic in the transfer
+ *   and footpath kernel (16).
+ * pack_max_entries: (route, group) entries, i.e. 64-query groups, that one
+ *   packed route wavefront may mix (1-64; 64).
+ * lb_delta: threshold step in minutes of the near-far lower-bound kernel (8).
+ */
 typedef struct nigiri_engine_config {
   double admission_wait_ms;
   uint32_t grid_waves;
   uint32_t eager_rounds;
-  uint32_t reserved[6];
+  uint32_t route_claim;
+  uint32_t transfer_claim;
+  uint32_t pack_max_entries;
+  uint32_t lb_delta;
+  uint32_t reserved[2];
 } nigiri_engine_config;
 
 int nigiri_amd_engine_configure(nigiri_amd_engine* e,
                                 const nigiri_engine_config* c);
+/* The engine's current configuration. */
+int nigiri_amd_engine_get_config(const nigiri_amd_engine* e,
+                                 nigiri_engine_config* out);
+
+/* ---- autotuner (no counterpart in the reference) ------------------------
+ * Chooses the launch-shape knobs of nigiri_engine_config (grid_waves,
+ * route_claim, transfer_claim, pack_max_entries, lb_delta) for this engine
+ * and timetable on a sample of queries (ideally one batch of the real
+ * workload): one pass of coordinate descent in which every candidate value
+ * of a knob runs the sample through the engine `repeats` times and the value
+ * with the smallest median engine time is kept.  The chosen values stay set
+ * on the engine and are written to *out (may be NULL).  Each trial is
+ * reported as (knob, value, ms) in trials[0 .. min(n, trial_capacity)),
+ * with the total count in *n_trials (may be NULL).  Knob ids: 0 grid_waves,
+ * 1 route_claim, 2 transfer_claim, 3 pack_max_entries, 4 lb_delta.
+ * Results never depend on the knobs. */
+typedef struct nigiri_tune_trial {
+  uint32_t knob;
+  uint32_t value;
+  double ms;
+} nigiri_tune_trial;
+
+int nigiri_amd_engine_autotune(nigiri_amd_engine* e, const nigiri_query* qs,
+                               uint32_t n, uint32_t repeats,
+                               nigiri_engine_config* out,
+                               nigiri_tune_trial* trials,
+                               uint32_t trial_capacity, uint32_t* n_trials);
 
 /* ---- serving loop ------------------------------------------------------ */
 /* Runs a continuous-batching server over the engine: query i becomes

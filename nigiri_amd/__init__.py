@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libnigiri_amd.so")
+# NIGIRI_AMD_LIB selects another build of the library (e.g. to compare builds)
+LIB_PATH = os.environ.get("NIGIRI_AMD_LIB") or os.path.join(_HERE, "libnigiri_amd.so")
 
 OK, EINVAL, ENOMEM, EDEVICE, ECAPACITY, EUNSUPPORTED, ERECONSTRUCT = 0, -1, -2, -3, -4, -5, -6
 DIR_FWD, DIR_BWD = 0, 1
@@ -124,7 +125,16 @@ class EngineStats(C.Structure):
 
 class EngineConfig(C.Structure):
     _fields_ = [("admission_wait_ms", C.c_double), ("grid_waves", C.c_uint32),
-                ("eager_rounds", C.c_uint32), ("reserved", C.c_uint32 * 6)]
+                ("eager_rounds", C.c_uint32), ("route_claim", C.c_uint32),
+                ("transfer_claim", C.c_uint32), ("pack_max_entries", C.c_uint32),
+                ("lb_delta", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+
+
+class TuneTrial(C.Structure):
+    _fields_ = [("knob", C.c_uint32), ("value", C.c_uint32), ("ms", C.c_double)]
+
+
+TUNE_KNOBS = ["grid_waves", "route_claim", "transfer_claim", "pack_max_entries", "lb_delta"]
 
 
 WORK_NAMES = ["route_stops", "events", "route_day_tests", "transport_day_tests", "tmp_writes",
@@ -164,6 +174,10 @@ SIGNATURES = {
                                            C.POINTER(C.c_int16), C.POINTER(C.c_int16),
                                            C.POINTER(C.c_int16), C.POINTER(C.c_uint16)]),
     "nigiri_amd_engine_configure": (C.c_int, [_P, C.POINTER(EngineConfig)]),
+    "nigiri_amd_engine_get_config": (C.c_int, [_P, C.POINTER(EngineConfig)]),
+    "nigiri_amd_engine_autotune": (C.c_int, [_P, C.POINTER(Query), C.c_uint32, C.c_uint32,
+                                             C.POINTER(EngineConfig), C.POINTER(TuneTrial),
+                                             C.c_uint32, C.POINTER(C.c_uint32)]),
     "nigiri_amd_engine_profile": (C.c_int, [_P, C.c_int]),
     "nigiri_amd_engine_stats": (C.c_int, [_P, C.POINTER(EngineStats), C.c_int]),
     "nigiri_amd_serve": (C.c_int, [_P, C.POINTER(Query), C.POINTER(C.c_double), C.c_uint32,
@@ -434,12 +448,35 @@ class Engine:
         _check(lib().nigiri_amd_engine_state_bytes(self.handle, C.byref(b)))
         return b.value
 
-    def configure(self, admission_wait_ms=2.0, grid_waves=0, graphs=True):
+    def configure(self, admission_wait_ms=2.0, grid_waves=0, graphs=True, **knobs):
+        """Sets the engine configuration; launch-shape knobs (route_claim,
+        transfer_claim, pack_max_entries, lb_delta) left out keep their values."""
         c = EngineConfig()
         c.admission_wait_ms = admission_wait_ms
         c.grid_waves = grid_waves
         c.eager_rounds = 0 if graphs else 1
+        for k, v in knobs.items():
+            if k not in TUNE_KNOBS[1:]:
+                raise TypeError(f"unknown engine knob {k}")
+            setattr(c, k, v)
         _check(lib().nigiri_amd_engine_configure(self.handle, C.byref(c)))
+
+    def get_config(self):
+        c = EngineConfig()
+        _check(lib().nigiri_amd_engine_get_config(self.handle, C.byref(c)))
+        return {n: getattr(c, n) for n, _ in EngineConfig._fields_ if n != "reserved"}
+
+    def autotune(self, queries, repeats=1):
+        """Chooses the launch-shape knobs on a sample of queries; returns the
+        chosen configuration and the trials [(knob, value, ms)]."""
+        cfg = EngineConfig()
+        trials = (TuneTrial * 64)()
+        n_trials = C.c_uint32()
+        _check(lib().nigiri_amd_engine_autotune(self.handle, queries.q, len(queries), repeats,
+                                                C.byref(cfg), trials, 64, C.byref(n_trials)))
+        chosen = {n: getattr(cfg, n) for n in TUNE_KNOBS}
+        return chosen, [(TUNE_KNOBS[t.knob], t.value, round(t.ms, 3))
+                        for t in trials[:min(64, n_trials.value)]]
 
     def profile(self, enable=True):
         _check(lib().nigiri_amd_engine_profile(self.handle, int(enable)))

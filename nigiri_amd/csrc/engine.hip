@@ -268,6 +268,9 @@ struct nigiri_amd_engine {
   std::uint32_t grid_waves = 8192U;
   // rounds replayed from captured hipGraphs (off: kernels launched one by one)
   bool use_graphs = true;
+  // launch-shape knobs (nigiri_amd_engine_autotune chooses them)
+  round_tune tune{1U, 16U, 64U};
+  std::uint32_t lb_delta = 8U;
   round_graphs* graphs = round_graphs_create();
   // instrumentation
   bool profile = false;
@@ -553,6 +556,7 @@ struct lb_pipeline {
     lb_args q{};
     q.im = e.img->d;
     q.L = e.st.L;
+    q.delta = e.lb_delta;
     q.n_q = static_cast<std::uint32_t>(batch.size());
     q.seed_off = w.seed_off;
     q.seed_loc = w.seed_loc;
@@ -647,6 +651,7 @@ struct runner {
     round_args a{};
     a.im = e.img->d;
     a.st = e.st;
+    a.tune = e.tune;
     a.invalid = fwd ? SHRT_MAX : SHRT_MIN;
     if (e.profile && e.events.size() >= 4U) {
       for (int i = 0; i != 4; ++i) a.ev[i] = e.events[static_cast<std::size_t>(i)];
@@ -1422,7 +1427,118 @@ int nigiri_amd_engine_configure(nigiri_amd_engine* e, const nigiri_engine_config
   e->admission_wait_ms = c->admission_wait_ms;
   if (c->grid_waves != 0U) e->grid_waves = c->grid_waves;
   e->use_graphs = c->eager_rounds == 0U;
+  if (c->route_claim != 0U) e->tune.route_claim = c->route_claim;
+  if (c->transfer_claim != 0U) e->tune.transfer_claim = c->transfer_claim;
+  if (c->pack_max_entries != 0U) e->tune.pack_max_entries = std::min<std::uint32_t>(c->pack_max_entries, 64U);
+  if (c->lb_delta != 0U) e->lb_delta = c->lb_delta;
   return NIGIRI_AMD_OK;
+}
+
+namespace nigiri_amd {
+namespace {
+
+// search::execute for every query (search.h:250-340): host preparation,
+// then the engine runs the forward queries and then the backward ones.
+// Returns the wall time of the engine part in milliseconds.
+double run_search(nigiri_amd_engine& e, const nigiri_query* qs, std::uint32_t n, std::vector<prepared_query>& pq,
+                  std::vector<int>& status, std::vector<std::string>& errors) {
+  auto const& tt = *e.img->tt;
+  // the batched engine computes lower bounds on the device
+  prepare_all(tt, qs, n, pq, status, errors, e.W != 64U);
+  auto const t0 = std::chrono::steady_clock::now();
+  for (int dir = 0; dir != 2; ++dir) {
+    std::vector<std::int32_t> queue;
+    for (std::uint32_t i = 0; i != n; ++i) {
+      if (status[i] == NIGIRI_AMD_OK && (qs[i].dir == NIGIRI_AMD_DIR_FWD) == (dir == 0)) {
+        queue.push_back(static_cast<std::int32_t>(i));
+      }
+    }
+    if (queue.empty()) continue;
+    if (e.W == 64U) {
+      run_batched(e, dir == 0, pq, queue);
+      continue;
+    }
+    std::size_t head = 0U;
+    engine_run_dir(
+        e, dir == 0, pq, [&]() { return head < queue.size() ? queue[head++] : -1; }, [](std::int32_t) {}, {});
+  }
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+}  // namespace nigiri_amd
+
+int nigiri_amd_engine_get_config(const nigiri_amd_engine* e, nigiri_engine_config* out) {
+  if (e == nullptr || out == nullptr) return NIGIRI_AMD_EINVAL;
+  *out = nigiri_engine_config{};
+  out->admission_wait_ms = e->admission_wait_ms;
+  out->grid_waves = e->grid_waves;
+  out->eager_rounds = e->use_graphs ? 0U : 1U;
+  out->route_claim = e->tune.route_claim;
+  out->transfer_claim = e->tune.transfer_claim;
+  out->pack_max_entries = e->tune.pack_max_entries;
+  out->lb_delta = e->lb_delta;
+  return NIGIRI_AMD_OK;
+}
+
+int nigiri_amd_engine_autotune(nigiri_amd_engine* e, const nigiri_query* qs, uint32_t n, uint32_t repeats,
+                               nigiri_engine_config* out, nigiri_tune_trial* trials, uint32_t trial_capacity,
+                               uint32_t* n_trials) {
+  if (e == nullptr || qs == nullptr || n == 0U || repeats == 0U || (trial_capacity != 0U && trials == nullptr)) {
+    set_error("autotune: invalid argument");
+    return NIGIRI_AMD_EINVAL;
+  }
+  return guarded([&]() {
+    HIP_CHECK(hipSetDevice(e->img->device));
+    int cus = 0;
+    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->img->device));
+    auto const c = static_cast<std::uint32_t>(std::max(1, cus));
+    struct knob {
+      std::uint32_t id;
+      std::uint32_t* field;
+      std::vector<std::uint32_t> values;
+    };
+    // knob ids as in nigiri_tune_trial: 0 grid_waves, 1 route_claim,
+    // 2 transfer_claim, 3 pack_max_entries, 4 lb_delta
+    std::vector<knob> knobs{{0U, &e->grid_waves, {8U * c, 16U * c, 32U * c, 64U * c}},
+                            {1U, &e->tune.route_claim, {1U, 2U, 4U}},
+                            {2U, &e->tune.transfer_claim, {4U, 16U, 64U}}};
+    if (e->W == 64U) {  // packing and device lower bounds: batched engine only
+      knobs.push_back({3U, &e->tune.pack_max_entries, {1U, 2U, 4U, 8U, 64U}});
+      knobs.push_back({4U, &e->lb_delta, {4U, 8U, 16U, 32U, 64U}});
+    }
+    std::uint32_t nt = 0U;
+    std::vector<prepared_query> pq;
+    std::vector<int> status;
+    std::vector<std::string> errors;
+    // one untimed run: graphs captured, caches and allocations warm
+    (void)run_search(*e, qs, n, pq, status, errors);
+    // coordinate descent, one pass: each knob takes its fastest value with
+    // the knobs before it already set (median of `repeats` runs)
+    for (auto& k : knobs) {
+      auto best_v = *k.field;
+      double best_ms = 0.0;
+      bool first = true;
+      for (auto const v : k.values) {
+        *k.field = v;
+        std::vector<double> ms;
+        for (std::uint32_t r = 0; r != repeats; ++r) ms.push_back(run_search(*e, qs, n, pq, status, errors));
+        std::sort(ms.begin(), ms.end());
+        auto const med = ms[ms.size() / 2U];
+        if (nt < trial_capacity) trials[nt] = nigiri_tune_trial{k.id, v, med};
+        ++nt;
+        if (first || med < best_ms) {
+          best_ms = med;
+          best_v = v;
+          first = false;
+        }
+      }
+      *k.field = best_v;
+    }
+    if (n_trials != nullptr) *n_trials = nt;
+    if (out != nullptr) (void)nigiri_amd_engine_get_config(e, out);
+    return NIGIRI_AMD_OK;
+  });
 }
 
 int nigiri_amd_engine_profile(nigiri_amd_engine* e, int enable) {
@@ -1456,6 +1572,7 @@ int nigiri_amd_engine_stats(nigiri_amd_engine* e, nigiri_engine_stats* out, int 
   });
 }
 
+
 int nigiri_amd_search(nigiri_amd_engine* e, const nigiri_query* qs, uint32_t n, nigiri_result* res,
                       nigiri_journey* journeys, uint32_t journey_capacity, nigiri_leg* legs,
                       uint32_t leg_capacity) {
@@ -1465,29 +1582,10 @@ int nigiri_amd_search(nigiri_amd_engine* e, const nigiri_query* qs, uint32_t n, 
   }
   return guarded([&]() {
     HIP_CHECK(hipSetDevice(e->img->device));
-    auto const& tt = *e->img->tt;
     std::vector<prepared_query> pq;
     std::vector<int> status;
     std::vector<std::string> errors;
-    // the batched engine computes lower bounds on the device
-    prepare_all(tt, qs, n, pq, status, errors, e->W != 64U);
-    for (int dir = 0; dir != 2; ++dir) {
-      std::vector<std::int32_t> queue;
-      for (std::uint32_t i = 0; i != n; ++i) {
-        if (status[i] == NIGIRI_AMD_OK && (qs[i].dir == NIGIRI_AMD_DIR_FWD) == (dir == 0)) {
-          queue.push_back(static_cast<std::int32_t>(i));
-        }
-      }
-      if (queue.empty()) continue;
-      if (e->W == 64U) {
-        run_batched(*e, dir == 0, pq, queue);
-        continue;
-      }
-      std::size_t head = 0U;
-      engine_run_dir(
-          *e, dir == 0, pq,
-          [&]() { return head < queue.size() ? queue[head++] : -1; }, [](std::int32_t) {}, {});
-    }
+    run_search(*e, qs, n, pq, status, errors);
     auto const w = write_results(pq, status, res, journeys, journey_capacity, legs, leg_capacity,
                                  nullptr, nullptr);
     if (w == ~0U) {
@@ -1825,6 +1923,7 @@ int nigiri_amd_algo_execute(nigiri_amd_engine* e, uint32_t slot, int32_t start_t
     round_args ra{};
     ra.im = e->img->d;
     ra.st = tr;
+    ra.tune = e->tune;
     ra.invalid = a.invalid;
     // a slot after its first epoch sees earlier rows in round_times
     bool const has_sweep = e->algo_new[slot] == 0U;

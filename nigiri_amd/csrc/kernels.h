@@ -35,9 +35,18 @@
 
 namespace nigiri_amd {
 
+// Launch-shape knobs chosen by the autotuner (runtime values; the results
+// do not depend on them).
+struct round_tune {
+  std::uint32_t route_claim;       // packed route wavefronts claimed per atomic
+  std::uint32_t transfer_claim;    // stations claimed per atomic (transfers)
+  std::uint32_t pack_max_entries;  // (route, group) entries per packed wavefront
+};
+
 struct round_args {
   dev_image im;
   dev_state st;
+  round_tune tune;
   std::int16_t invalid;
   // optional timing events around the route scan / transfer kernels
   hipEvent_t ev[4];
@@ -53,6 +62,7 @@ struct lb_args {
   dev_image im;
   std::uint32_t L;
   std::uint32_t n_q;
+  std::uint32_t delta;  // near-far threshold step in minutes (any value >= 1 is exact)
   std::uint32_t const* seed_off;
   std::uint32_t const* seed_loc;
   std::uint16_t const* seed_dur;

@@ -73,18 +73,18 @@ __device__ inline void list_push(std::uint64_t* list, std::uint32_t* count, std:
   }
 }
 
-// Work items of a wavefront from a shared cursor, Chunk at a time:
+// Work items of a wavefront from a shared cursor, `chunk` at a time:
 // wavefronts that start late (their CU was busy) simply take fewer items.
-template <std::uint32_t Chunk>
 struct item_claim {
   std::uint32_t* cursor;
+  std::uint32_t chunk;
   std::uint32_t next = 0U, end = 0U;
   __device__ std::uint32_t get() {
     if (next == end) {
       std::uint32_t v = 0U;
-      if ((threadIdx.x & 63U) == 0U) v = atomicAdd(cursor, Chunk);
+      if ((threadIdx.x & 63U) == 0U) v = atomicAdd(cursor, chunk);
       next = static_cast<std::uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
-      end = next + Chunk;
+      end = next + chunk;
     }
     return next++;
   }
@@ -562,6 +562,7 @@ struct wave_lds {
   std::uint32_t stop[kChunkMax];
   std::uint16_t lb[kChunkMax][64];
   std::int16_t rt[kChunkMax][64];
+  std::int16_t tm[kChunkMax][64];
   std::uint16_t ev[kChunkEvBytes / 2U];  // [chunk stop][arr, dep][n_t]
 };
 constexpr std::uint32_t kWaveLdsBytes = sizeof(wave_lds);
@@ -635,16 +636,23 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
     std::uint16_t lbr[kChunkMax];
     std::int16_t rtr[kChunkMax];
     std::uint32_t mkr[kChunkMax];
+    // tmp at the chunk's stops for lanes that ride into the chunk: a ride
+    // whose arrival is not better than tmp leaves tmp as it is (tmp only
+    // ever improves, so an older value can only cause a redundant atomic)
+    std::int16_t tmr[kChunkMax];
+    bool const riding = !done && et.valid();
 #pragma unroll
     for (std::uint32_t j = 0U; j < kChunkMax; ++j) {
       lbr[j] = 0U;
       rtr[j] = D::kInvalid;
       mkr[j] = 0U;
+      tmr[j] = D::kInvalid;
       if (j < nc && !done) {
         auto const l = x.stop[j] & kLocMask;
         mkr[j] = mark_t32[mrow + 2U * l];
         lbr[j] = s.lb[col0 + l * 64U];
         rtr[j] = rt_q[static_cast<std::size_t>(l) * 64U];
+        if (riding) tmr[j] = static_cast<std::int16_t>(s.tmp[col0 + l * 64U]);
       }
     }
     // event blocks: for chunk stop j, [arr at stop_idx | dep at stop_idx]
@@ -674,6 +682,7 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
       if (j < nc) {
         x.lb[j][lane] = lbr[j];
         x.rt[j][lane] = rtr[j];
+        x.tm[j][lane] = tmr[j];
       }
     }
     wave_sync_lds();
@@ -699,10 +708,12 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
         auto const e = (Fwd ? ev_arr : ev_dep)[et.t_off];
         auto const by = to_delta<Fwd>(et.day, (e & 31) * 1440 + (e >> 5), base);
         if (D::better_eq(by, tad_k) && lbv != kUnreachableLb && D::better_eq(by + D::dir(lbv), tad_k)) {
-          if constexpr (Fwd) {
-            atomicMin(&s.tmp[off], static_cast<int>(by));
-          } else {
-            atomicMax(&s.tmp[off], static_cast<int>(by));
+          if (D::better(by, x.tm[j][lane])) {
+            if constexpr (Fwd) {
+              atomicMin(&s.tmp[off], static_cast<int>(by));
+            } else {
+              atomicMax(&s.tmp[off], static_cast<int>(by));
+            }
           }
           s.mark_rb[off] = 1U;
           s.flag_r[q.g * L + l] = 1U;
@@ -795,17 +806,21 @@ __global__ void k_route_pack(round_args a) {
     }
     auto const c = static_cast<std::uint32_t>(__popcll(mask));
     // greedy assignment in item order (uniform loop over the 64 entries)
-    std::uint32_t my_w = 0U, my_o = 0U, n_w = 0U, fill = 0U, cur_r = ~0U;
+    // (at most max_entries list entries, i.e. groups, per wavefront)
+    auto const max_entries = a.tune.pack_max_entries;
+    std::uint32_t my_w = 0U, my_o = 0U, n_w = 0U, fill = 0U, cur_r = ~0U, entries = 0U;
     for (std::uint32_t t = 0U; t < 64U; ++t) {
       auto const ct = static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(c), static_cast<int>(t)));
       if (ct == 0U) continue;
       auto const rt = static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(r), static_cast<int>(t)));
-      if (n_w == 0U || rt != cur_r || fill + ct > 64U) {
+      if (n_w == 0U || rt != cur_r || fill + ct > 64U || entries == max_entries) {
         if (n_w != 0U && lane == 0U) fills[n_w - 1U] = fill;
         ++n_w;
         fill = 0U;
+        entries = 0U;
         cur_r = rt;
       }
+      ++entries;
       if (lane == t) {
         my_w = n_w - 1U;
         my_o = fill;
@@ -852,7 +867,7 @@ __global__ void __launch_bounds__(256, 8) k_route_scan(round_args a) {
     std::uint32_t const lane = threadIdx.x & 63U;
     auto const n_items = clamp_count(s.counts[5], s.cap_route);
     std::uint32_t c_stops = 0U, c_tmp = 0U, c_ev = 0U, c_rd = 0U, c_tt = 0U, c_ws = 0U;
-    item_claim<1U> claim{&s.counts[6]};
+    item_claim claim{&s.counts[6], a.tune.route_claim};
     for (std::uint32_t item = claim.get(); item < n_items; item = claim.get()) {
       auto const info = s.packed_info[item];
       auto const r = uni(static_cast<std::uint32_t>(info >> 32));
@@ -1183,7 +1198,7 @@ __global__ void k_transfer_fp(round_args a) {
   auto const n_items = clamp_count(s.counts[1], s.cap_r);
   std::uint32_t const lane_id = W == 64 ? (threadIdx.x & 63U) : 0U;
   // batched engine: items claimed per wavefront; plain engine: grid stride
-  item_claim<16U> claim{&s.counts[7]};
+  item_claim claim{&s.counts[7], a.tune.transfer_claim};
   auto const next_item = [&](std::uint32_t i) {
     return W == 64 ? claim.get() : i + gridDim.x * blockDim.x;
   };
@@ -1382,7 +1397,6 @@ __global__ void k_lane_reset(epoch_args a) {
 // dijkstra.cc:18-67 produces.
 constexpr std::uint32_t kLbCut = 7201U;
 constexpr std::uint32_t kLbInf = 0xFFFFFFFFU;
-constexpr std::uint32_t kLbDelta = 8U;
 constexpr int kLbBlock = 1024;
 
 // Relaxes the lower-bound edges of u (tentative distance du < kLbCut): the
@@ -1466,7 +1480,7 @@ __global__ void __launch_bounds__(kLbBlock) k_lb_query(lb_args a) {
       auto const mn = s_min;
       if (mn == kLbInf) break;
       auto const t_old = T;
-      T = static_cast<std::int64_t>(mn) + kLbDelta - 1;
+      T = static_cast<std::int64_t>(mn) + a.delta - 1;
       ++phase;
       ++sweep;
       if (tid == 0U) {

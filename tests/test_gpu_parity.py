@@ -285,3 +285,34 @@ def test_gpu_graph_rounds_equal_eager_rounds():
     assert [r["journeys"] for r in c] == [r["journeys"] for r in a]
     st = graph.stats()
     assert st["route_scan_launches"] > 0 and st["route_scan_ms"] > 0.0
+
+
+def test_gpu_launch_knobs_do_not_change_results():
+    # every launch-shape knob away from its default: results stay bit-exact
+    tt, img = image_for("w3s", lambda: na.Timetable.generate(3, scale=0.03))
+    qs = na.Queries.generate(tt, 3, 128, seed=0x7172 + 43)
+    eng = na.Engine(img, lanes=64, groups=2, max_k=10)
+    eng.configure(grid_waves=512, route_claim=4, transfer_claim=3, pack_max_entries=1, lb_delta=64)
+    cfg = eng.get_config()
+    assert (cfg["grid_waves"], cfg["route_claim"], cfg["transfer_claim"], cfg["pack_max_entries"],
+            cfg["lb_delta"]) == (512, 4, 3, 1, 64)
+    say("knobs away from defaults, workload3 mix n=128")
+    compare(tt, eng, qs)
+    eng.configure(route_claim=2, transfer_claim=64, pack_max_entries=3, lb_delta=1)
+    say("knobs, second setting")
+    compare(tt, eng, qs)
+
+
+def test_gpu_autotune_picks_candidates_and_keeps_results():
+    tt, img = image_for("w3s", lambda: na.Timetable.generate(3, scale=0.03))
+    qs = na.Queries.generate(tt, 3, 64, seed=0x7172 + 47)
+    eng = na.Engine(img, lanes=64, groups=1, max_k=10)
+    say("autotune workload3 n=64")
+    chosen, trials = eng.autotune(qs)
+    assert len(trials) == 4 + 3 + 3 + 5 + 5
+    for knob, value in chosen.items():
+        tried = [(v, ms) for k, v, ms in trials if k == knob]
+        assert value in [v for v, _ in tried], (knob, value)
+        assert all(ms > 0.0 for _, ms in tried)
+    assert eng.get_config()["lb_delta"] == chosen["lb_delta"]
+    compare(tt, eng, qs)

@@ -2,10 +2,15 @@
 batched (64 queries per wavefront) against the plain one-query-per-launch
 GPU path, with the CPU oracle timed on the host cores as a baseline.
 
-A step is one batch of `--batch` queries of the workload, run through the
-serving loop with every query arriving at the start of the step (host
-preparation overlaps device rounds).  Inputs (timetable image, query state)
-are resident on the GPU before the timed region starts.
+A step is one batch of `--batch` queries of the workload.  By default the
+timed steps' batches go through one run of the serving loop: a batch is
+admitted whole (its queries start their rounds together) once the previous
+batch has finished, and its host preparation and device lower bounds
+overlap the previous batch's rounds.  Latency is admission -> completion;
+the wait for admission is reported separately.  `--batch-per-step` runs
+each step as its own serving call instead (latency from the step start).
+Inputs (timetable image, query state) are resident on the GPU before the
+timed region starts.
 """
 import argparse
 import json
@@ -134,11 +139,22 @@ def main():
                     help="serving measurement instead: Poisson arrivals at RATE queries/s "
                          "of the workload's query mix through the continuous-batching loop")
     ap.add_argument("--groups", type=int, default=16, help="64-query groups (serving mode)")
+    ap.add_argument("--continuous", action="store_true",
+                    help="the timed steps' batches go through one serving run back to back: the "
+                         "next batch's preparation and lower bounds overlap the current batch's "
+                         "rounds (continuous batching); latency = admission -> completion")
+    ap.add_argument("--batch-per-step", action="store_true",
+                    help="each step is its own serving run of one batch (nothing overlaps "
+                         "across steps); default: the timed steps' batches go through one "
+                         "serving run, each batch admitted whole once the previous one has "
+                         "finished, its preparation and lower bounds overlapping the previous "
+                         "batch's rounds")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes instead of autotuning them on the first batch")
     ap.add_argument("--eager", action="store_true",
                     help="launch each round's kernels one by one instead of replaying captured hipGraphs")
     args = ap.parse_args()
+    args.pipelined = not args.batch_per_step and not args.continuous
     if args.copy_calibration:
         import torch
         x = torch.ones(512 << 20, dtype=torch.uint8, device="cuda")
@@ -173,7 +189,8 @@ def main():
     eng.profile(True)
     # a step submits a whole batch at once: let preparation fill every slot
     # before the first epoch starts
-    eng.configure(admission_wait_ms=args.admission_wait_ms, graphs=not args.eager)
+    eng.configure(admission_wait_ms=args.admission_wait_ms, graphs=not args.eager,
+                  admit_when_idle=args.pipelined)
     B = groups * 64
     n_q = len(qs)
 
@@ -200,13 +217,28 @@ def main():
     # legs are written into the caller's arrays (sized for this workload:
     # under 8 journeys and 40 legs per query), not turned into Python objects
     caps = dict(journey_capacity=8 * B, leg_capacity=320 * B, legs="raw")
+    queue = []
     t_start = time.perf_counter()
-    for s in range(args.steps):
-        res, wall = eng.serve(batch(args.warmup + s), zeros, **caps)
+    if args.continuous or args.pipelined:
+        # all K batches queued at the start; slots take the next query as they free
+        idx = [((args.warmup + s) * B + j) % n_q for s in range(args.steps) for j in range(B)]
+        allq = qs.subset(idx)
+        caps_all = dict(journey_capacity=8 * B * args.steps, leg_capacity=320 * B * args.steps,
+                        legs="raw")
+        res, wall = eng.serve(allq, np.zeros(len(idx)), **caps_all)
         serve_wall += wall
-        lat += [r["latency_ms"] for r in res]
+        lat += [r["latency_ms"] - r["queue_ms"] for r in res]
+        queue += [r["queue_ms"] for r in res]
         n_journeys += sum(len(r["journeys"]) for r in res)
-        print(f"[bench] step {s}: {wall:.1f} ms", file=sys.stderr, flush=True)
+        print(f"[bench] {args.steps} batches: {wall:.1f} ms", file=sys.stderr, flush=True)
+    else:
+        for s in range(args.steps):
+            res, wall = eng.serve(batch(args.warmup + s), zeros, **caps)
+            serve_wall += wall
+            lat += [r["latency_ms"] for r in res]
+            queue += [r["queue_ms"] for r in res]
+            n_journeys += sum(len(r["journeys"]) for r in res)
+            print(f"[bench] step {s}: {wall:.1f} ms", file=sys.stderr, flush=True)
     sync()
     t_total = time.perf_counter() - t_start
     st = eng.stats()
@@ -263,10 +295,17 @@ def main():
         "dtype": "int16",
         "data": "synthetic (seeded generator)",
         "config": {"workload": f"config {cfg}: " + WORKLOADS[cfg], "scale": args.scale,
+                   "mode": ("continuous" if args.continuous else
+                            "pipelined batches" if args.pipelined else "batch per step"),
                    "batch": B, "lanes": 64, "groups": groups, "max_k": max_k,
                    "queries": n_q},
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "latency": ("admission -> completion (queries queued for a slot; queue wait separate)"
+                    if args.continuous or args.pipelined
+                    else "arrival -> completion (a batch arrives at step start)"),
+        "queue_p50_ms": round(float(np.percentile(queue, 50)), 3),
+        "queue_p99_ms": round(float(np.percentile(queue, 99)), 3),
         "plain_1query_qps": round(plain_qps, 2),
         "speedup_vs_plain": round(qps / plain_qps, 2) if plain_qps > 0 else None,
         "journeys": n_journeys,

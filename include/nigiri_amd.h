@@ -348,6 +348,10 @@ int nigiri_amd_engine_stats(nigiri_amd_engine* e, nigiri_engine_stats* out,
  * pack_max_entries: (route, group) entries, i.e. 64-query groups, that one
  *   packed route wavefront may mix (1-64; 64).
  * lb_delta: threshold step in minutes of the near-far lower-bound kernel (8).
+ * admit_when_idle: serving loop; 1 = new queries are admitted only when no
+ *   slot is busy, so that a batch starts together while the preparation and
+ *   lower bounds of the next batch overlap the current batch's rounds
+ *   (0 = a query takes any free slot at an epoch boundary; default).
  */
 typedef struct nigiri_engine_config {
   double admission_wait_ms;
@@ -357,7 +361,8 @@ typedef struct nigiri_engine_config {
   uint32_t transfer_claim;
   uint32_t pack_max_entries;
   uint32_t lb_delta;
-  uint32_t reserved[2];
+  uint32_t admit_when_idle;
+  uint32_t reserved[1];
 } nigiri_engine_config;
 
 int nigiri_amd_engine_configure(nigiri_amd_engine* e,

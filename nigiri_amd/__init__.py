@@ -127,7 +127,8 @@ class EngineConfig(C.Structure):
     _fields_ = [("admission_wait_ms", C.c_double), ("grid_waves", C.c_uint32),
                 ("eager_rounds", C.c_uint32), ("route_claim", C.c_uint32),
                 ("transfer_claim", C.c_uint32), ("pack_max_entries", C.c_uint32),
-                ("lb_delta", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+                ("lb_delta", C.c_uint32), ("admit_when_idle", C.c_uint32),
+                ("reserved", C.c_uint32 * 1)]
 
 
 class TuneTrial(C.Structure):
@@ -448,13 +449,15 @@ class Engine:
         _check(lib().nigiri_amd_engine_state_bytes(self.handle, C.byref(b)))
         return b.value
 
-    def configure(self, admission_wait_ms=2.0, grid_waves=0, graphs=True, **knobs):
+    def configure(self, admission_wait_ms=2.0, grid_waves=0, graphs=True, admit_when_idle=False,
+                  **knobs):
         """Sets the engine configuration; launch-shape knobs (route_claim,
         transfer_claim, pack_max_entries, lb_delta) left out keep their values."""
         c = EngineConfig()
         c.admission_wait_ms = admission_wait_ms
         c.grid_waves = grid_waves
         c.eager_rounds = 0 if graphs else 1
+        c.admit_when_idle = 1 if admit_when_idle else 0
         for k, v in knobs.items():
             if k not in TUNE_KNOBS[1:]:
                 raise TypeError(f"unknown engine knob {k}")

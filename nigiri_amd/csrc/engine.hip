@@ -271,6 +271,8 @@ struct nigiri_amd_engine {
   // launch-shape knobs (nigiri_amd_engine_autotune chooses them)
   round_tune tune{1U, 16U, 64U};
   std::uint32_t lb_delta = 8U;
+  // serving: admit new queries only when no slot is busy (whole batches)
+  bool admit_when_idle = false;
   round_graphs* graphs = round_graphs_create();
   // instrumentation
   bool profile = false;
@@ -826,7 +828,10 @@ struct runner {
       }
     }
     std::vector<std::int32_t> admitted;
-    while (admitted.size() < free_list.size()) {
+    // whole-batch admission: new queries only start together, once no slot
+    // is busy (their preparation and lower bounds overlap the rounds before)
+    bool const gate_closed = e.admit_when_idle && (!starting.empty() || any_running());
+    while (!gate_closed && admitted.size() < free_list.size()) {
       auto const q = next();
       if (q < 0) break;
       auto& p = pq[static_cast<std::size_t>(q)];
@@ -1011,6 +1016,16 @@ struct runner {
 
   bool any_running() const {
     return std::any_of(slots.begin(), slots.end(), [](slot_info const& si) { return si.running; });
+  }
+
+  // a slot runs an epoch or has further epochs of its range query to run
+  bool any_busy() const {
+    for (auto const& si : slots) {
+      if (si.running || (si.q >= 0 && si.epoch + 1U < pq[static_cast<std::size_t>(si.q)].epochs_.size())) {
+        return true;
+      }
+    }
+    return false;
   }
 
   // idle slots that take a new query at the next step
@@ -1431,6 +1446,7 @@ int nigiri_amd_engine_configure(nigiri_amd_engine* e, const nigiri_engine_config
   if (c->transfer_claim != 0U) e->tune.transfer_claim = c->transfer_claim;
   if (c->pack_max_entries != 0U) e->tune.pack_max_entries = std::min<std::uint32_t>(c->pack_max_entries, 64U);
   if (c->lb_delta != 0U) e->lb_delta = c->lb_delta;
+  e->admit_when_idle = c->admit_when_idle != 0U;
   return NIGIRI_AMD_OK;
 }
 
@@ -1478,6 +1494,7 @@ int nigiri_amd_engine_get_config(const nigiri_amd_engine* e, nigiri_engine_confi
   out->transfer_claim = e->tune.transfer_claim;
   out->pack_max_entries = e->tune.pack_max_entries;
   out->lb_delta = e->lb_delta;
+  out->admit_when_idle = e->admit_when_idle ? 1U : 0U;
   return NIGIRI_AMD_OK;
 }
 
@@ -1707,7 +1724,8 @@ int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs, const double*
         auto const free = r.free_slots();
         auto const deadline = clock::now() + std::chrono::duration_cast<clock::duration>(
                                                   std::chrono::duration<double, std::milli>(e->admission_wait_ms));
-        while (clock::now() < deadline) {
+        // (whole-batch admission: nothing is admitted while slots are busy)
+        while (!(e->admit_when_idle && r.any_busy()) && clock::now() < deadline) {
           std::size_t n_ready = 0U;
           std::uint32_t n_settled = 0U;  // arrived queries that cannot get more ready
           if (pipe) {

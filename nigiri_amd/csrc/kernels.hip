@@ -655,14 +655,21 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
         if (riding) tmr[j] = static_cast<std::int16_t>(s.tmp[col0 + l * 64U]);
       }
     }
-    // event blocks: for chunk stop j, [arr at stop_idx | dep at stop_idx]
-    for (auto e = lane; e < 2U * nc * n_t; e += 64U) {
-      auto const j = e / (2U * n_t), rem = e % (2U * n_t);
-      auto const which = rem / n_t, t = rem % n_t;  // 0 arrival, 1 departure
+    // event blocks: for chunk stop j, [arr at stop_idx | dep at stop_idx];
+    // the arrival row of stop_idx directly precedes its departure row in the
+    // route's event array (stop-pair layout), so both rows are one run of
+    // 2 n_t events starting at n_t (2 stop_idx - 1); a missing row (no
+    // arrival at the first stop, no departure at the last) is zeroed
+    for (std::uint32_t j = 0U; j < nc; ++j) {
       auto const si = c0 + j;
       auto const stop_idx = Fwd ? si : n_stops - si - 1U;
-      bool const exists = which == 0U ? stop_idx != 0U : stop_idx + 1U != n_stops;
-      x.ev[e] = exists ? ev_g[n_t * (2U * stop_idx - (which == 0U ? 1U : 0U)) + t] : 0U;
+      auto const lo = stop_idx == 0U ? n_t : 0U;                  // first valid element
+      auto const hi = stop_idx + 1U == n_stops ? n_t : 2U * n_t;  // end of valid elements
+      auto const* src = ev_g + static_cast<std::ptrdiff_t>(n_t) * (2 * static_cast<std::ptrdiff_t>(stop_idx) - 1);
+      auto* dst = x.ev + 2U * j * n_t;
+      for (auto e = lane; e < 2U * n_t; e += 64U) {
+        dst[e] = (e >= lo && e < hi) ? src[e] : std::uint16_t{0U};
+      }
     }
     std::uint32_t mbits = 0U;  // bit j: this query is marked at chunk stop j
 #pragma unroll

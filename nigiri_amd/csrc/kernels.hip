@@ -262,19 +262,18 @@ template <bool Fwd, int W>
 __global__ void k_best_sweep(round_args a, std::uint32_t const* sweep) {
   auto const& s = a.st;
   auto const n_groups = sweep[0];
-  auto const* groups = sweep + 1;
-  auto const per_group = static_cast<std::size_t>(s.L) * W;
-  auto const total = per_group * n_groups;
-  for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < total;
-       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
-    auto const gi = i / per_group;
-    auto const rem = i % per_group;
-    auto const g = groups[gi];
-    auto const l = static_cast<std::uint32_t>(rem / W), lane = static_cast<std::uint32_t>(rem % W);
-    if ((s.grp_active[g] & (1ULL << lane)) == 0ULL) continue;
-    auto const c = lay<W>::col(s, g, l, lane);
-    auto const v = s.round_times[lay<W>::rt(s, g, s.kslot[g * W + lane], l, lane)];
-    s.best[c] = dirop<Fwd>::best_of(v, s.best[c]);
+  auto const per_group = s.L * static_cast<std::uint32_t>(W);  // < 2^32 (checked at engine creation)
+  auto const stride = gridDim.x * blockDim.x;
+  for (std::uint32_t gi = 0U; gi < n_groups; ++gi) {  // uniform: no per-element division
+    auto const g = sweep[1U + gi];
+    auto const act = s.grp_active[g];
+    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < per_group; i += stride) {
+      auto const l = i / static_cast<std::uint32_t>(W), lane = i % static_cast<std::uint32_t>(W);
+      if ((act & (1ULL << lane)) == 0ULL) continue;
+      auto const c = lay<W>::col(s, g, l, lane);
+      auto const v = s.round_times[lay<W>::rt(s, g, s.kslot[g * W + lane], l, lane)];
+      if (dirop<Fwd>::better(v, s.best[c])) s.best[c] = v;
+    }
   }
 }
 
@@ -1361,33 +1360,39 @@ __global__ void k_hazards(round_args a) {
 template <int W>
 __global__ void k_lane_reset(epoch_args a) {
   auto const& s = a.st;
+  auto const stride = gridDim.x * blockDim.x;
+  auto const tid = blockIdx.x * blockDim.x + threadIdx.x;
   // station marks left by an epoch that ended at its last round are cleared
   // for the restarting lanes (the words may hold running lanes' marks)
-  auto const words = static_cast<std::size_t>(W == 64 ? s.L : s.LW);
-  for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x;
-       i < words * a.n_reset_groups; i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
-    auto const g = a.reset_groups[i / words];
+  auto const words = W == 64 ? s.L : s.LW;
+  for (std::uint32_t gi = 0U; gi < a.n_reset_groups; ++gi) {  // uniform
+    auto const g = a.reset_groups[gi];
     auto const m = W == 64 ? a.reset_bt_mask[g] : (a.reset_bt_mask[g] != 0ULL ? ~0ULL : 0ULL);
-    auto& w = s.mark_t[static_cast<std::size_t>(g) * words + i % words];
-    if ((w & m) != 0ULL) w &= ~m;
-  }
-  auto const per_group = static_cast<std::size_t>(s.L) * W;
-  auto const total = per_group * a.n_reset_groups;
-  for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < total;
-       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
-    auto const g = a.reset_groups[i / per_group];
-    auto const rem = i % per_group;
-    auto const l = static_cast<std::uint32_t>(rem / W), lane = static_cast<std::uint32_t>(rem % W);
-    auto const bit = 1ULL << lane;
-    if ((a.reset_rt_mask[g] & bit) != 0ULL) {
-      for (std::uint32_t k = 0; k < s.K; ++k) {
-        s.round_times[lay<W>::rt(s, g, k, l, lane)] = a.invalid;
-      }
+    if (m == 0ULL) continue;
+    auto* mt = s.mark_t + static_cast<std::size_t>(g) * words;
+    for (std::uint32_t i = tid; i < words; i += stride) {
+      auto const w = mt[i];
+      if ((w & m) != 0ULL) mt[i] = w & ~m;
     }
-    if ((a.reset_bt_mask[g] & bit) != 0ULL) {
-      auto const c = lay<W>::col(s, g, l, lane);
-      s.best[c] = a.invalid;
-      s.tmp[c] = a.invalid;
+  }
+  auto const per_group = s.L * static_cast<std::uint32_t>(W);
+  for (std::uint32_t gi = 0U; gi < a.n_reset_groups; ++gi) {
+    auto const g = a.reset_groups[gi];
+    auto const rtm = a.reset_rt_mask[g], btm = a.reset_bt_mask[g];
+    if ((rtm | btm) == 0ULL) continue;
+    for (std::uint32_t i = tid; i < per_group; i += stride) {
+      auto const l = i / static_cast<std::uint32_t>(W), lane = i % static_cast<std::uint32_t>(W);
+      auto const bit = 1ULL << lane;
+      if ((rtm & bit) != 0ULL) {
+        for (std::uint32_t k = 0; k < s.K; ++k) {
+          s.round_times[lay<W>::rt(s, g, k, l, lane)] = a.invalid;
+        }
+      }
+      if ((btm & bit) != 0ULL) {
+        auto const c = lay<W>::col(s, g, l, lane);
+        s.best[c] = a.invalid;
+        s.tmp[c] = a.invalid;
+      }
     }
   }
 }

@@ -316,3 +316,34 @@ def test_gpu_autotune_picks_candidates_and_keeps_results():
         assert all(ms > 0.0 for _, ms in tried)
     assert eng.get_config()["lb_delta"] == chosen["lb_delta"]
     compare(tt, eng, qs)
+
+
+def test_gpu_whole_batch_admission_serving():
+    # admit_when_idle: queries queued behind a running batch start together
+    # once it has finished; results equal the oracle's
+    tt, img = image_for("w1", lambda: na.Timetable.generate(1))
+    qs = na.Queries.generate(tt, 1, 128, seed=0x7172 + 53)
+    eng = na.Engine(img, lanes=64, groups=1, max_k=9)
+    eng.configure(admission_wait_ms=500.0, admit_when_idle=True)
+    assert eng.get_config()["admit_when_idle"] == 1
+    say("whole-batch admission, 128 queries through 64 slots")
+    res, wall = eng.serve(qs, np.zeros(len(qs)))
+    # queries without start labels complete at admission without a slot,
+    # so the batches are told apart by admission time, not by index: a
+    # batch starts together (one admission step), and the second only once
+    # every slot of the first is idle (its last queries complete, legs
+    # written, right after that admission)
+    queue = np.array([r["queue_ms"] for r in res])
+    lat = np.array([r["latency_ms"] for r in res])
+    t_second = queue.max()
+    second = queue > queue.min() + 0.5 * (t_second - queue.min())
+    first = ~second
+    assert 64 - 8 <= second.sum() <= 64, second.sum()
+    assert queue[second].max() - queue[second].min() < 1.0, queue[second]
+    assert queue[first].max() - queue[first].min() < 1.0, queue[first]
+    assert t_second >= lat[first].max() - 5.0, (t_second, lat[first].max())
+    assert t_second <= lat[first].max() + 1e-3, (t_second, lat[first].max())
+    for i in range(0, 128, 8):
+        ref = oracle.search(tt, qs[i])
+        assert res[i]["journeys"] == ref["journeys"], i
+        assert res[i]["legs"] == ref["legs"], i

@@ -325,6 +325,7 @@ def main():
                      "traffic": committed_traffic(cfg, args.scale, B), "kernel": "k_route_scan",
                      "alg_bytes_per_launch": int(alg_bytes / launches),
                      "avg_launch_ms": round(avg_ms, 4), "launches": launches},
+        "transfer_kernel_avg_ms": round(st["transfer_ms"] / max(1, st["transfer_launches"]), 4),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -1275,16 +1275,6 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     set_error("engine_create: n_groups * n_locations * 64 must stay below 2^32");
     return NIGIRI_AMD_EINVAL;
   }
-  if (lanes == 64U) {
-    auto const& rt = img->tt->route_transport_from;
-    for (std::size_t r = 0; r + 1U < rt.size(); ++r) {
-      if (rt[r + 1U] - rt[r] > kBatchedMaxRouteTransports) {
-        set_error("engine_create: the batched engine takes routes of at most " +
-                  std::to_string(kBatchedMaxRouteTransports) + " transports");
-        return NIGIRI_AMD_EUNSUPPORTED;
-      }
-    }
-  }
   return guarded([&]() {
     HIP_CHECK(hipSetDevice(img->device));
     auto e = std::make_unique<nigiri_amd_engine>();

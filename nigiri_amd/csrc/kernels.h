@@ -80,8 +80,9 @@ struct lb_args {
 // One workgroup per query, all sweeps in one launch.
 void launch_lb_queries(bool fwd, lb_args const& a, hipStream_t stream);
 
-// The batched route scan stages the events of at least one stop in LDS: it
-// takes routes of at most this many transports.
+// The batched route scan stages the events of a stop in LDS for routes of at
+// most this many transports; larger routes are scanned lane by lane from
+// global memory.
 constexpr std::uint32_t kBatchedMaxRouteTransports = 256U;
 
 // Launch helpers (defined in kernels.hip), all asynchronous on `stream`.

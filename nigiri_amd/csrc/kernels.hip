@@ -773,6 +773,90 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
   if (started) c_stops += my_stops;
 }
 
+// update_route (raptor.h:1181-1349) for routes whose stop events do not fit
+// the LDS staging of scan_route64 (more than kBatchedMaxRouteTransports
+// transports), batched engine: every lane scans the route for its own query,
+// reading the image and state from global memory; route-phase marks and
+
+// register allocation does not carry it.
+template <bool Fwd>
+__device__ inline void scan_route_lane64(dev_image const& im, dev_state const& s, std::uint32_t r,
+                                               std::uint32_t k, lane_query const q, std::uint16_t base,
+                                               int tad_k, std::uint32_t& c_stops, std::uint32_t& c_tmp,
+                                               std::uint32_t& c_ev, std::uint32_t& c_rd, std::uint32_t& c_tt,
+                                               bool& any, bool& started) {
+  using D = dirop<Fwd>;
+  auto const L = s.L;
+  auto const s0 = im.route_stop_offsets[r];
+  auto const n_stops = im.route_stop_offsets[r + 1U] - s0;
+  auto const t_from = im.route_transport_from[r];
+  auto const n_t = im.route_transport_from[r + 1U] - t_from;
+  auto const* stops = im.route_stops + s0;
+  auto const* ev = im.event_times + im.route_stop_time_from[r];
+  route_bits const bits{im.bitfields + static_cast<std::size_t>(im.route_traffic_days[r]) * 8U, nullptr,
+                        im.transport_traffic_days + t_from, im.bitfields, false};
+  auto const top = 1U << (31 - __clz(n_t));
+  std::uint32_t const col0 = q.g * L * 64U + q.ln;
+  std::uint32_t const mrow = 2U * q.g * L + (q.ln >> 5);
+  std::uint32_t const mbit = 1U << (q.ln & 31U);
+  auto const* mark_t32 = reinterpret_cast<std::uint32_t const*>(s.mark_t);
+  auto const* rt_q = s.round_times + (static_cast<std::size_t>(q.g) * s.K + (k - 1U)) * L * 64U + q.ln;
+  started = q.on;  // the packing admits only queries for which the route is marked
+  if (!q.on) return;
+  et_t<Fwd> et{~0U, 0xFFFFU};
+  et_counters ce;
+  for (std::uint32_t i = 0U; i < n_stops; ++i) {
+    auto const stop_idx = Fwd ? i : n_stops - i - 1U;
+    auto const stp = stops[stop_idx];
+    auto const l = stp & kLocMask;
+    auto const off = col0 + l * 64U;
+    bool const marked = (mark_t32[mrow + 2U * l] & mbit) != 0U;
+    int const lbv = s.lb[off];
+    auto const in_ok = ((stp >> 28) & 1U) != 0U, out_ok = ((stp >> 29) & 1U) != 0U;
+    bool const can_finish = Fwd ? out_ok : in_ok;
+    bool const can_start = Fwd ? in_ok : out_ok;
+    ++c_stops;
+    bool ride = false;
+    if (et.valid() && can_finish) {
+      ride = true;
+      auto const by = time_at_stop<Fwd>(ev, n_t, et, stop_idx, Fwd, base);
+      if (D::better_eq(by, tad_k) && lbv != kUnreachableLb && D::better_eq(by + D::dir(lbv), tad_k)) {
+        if constexpr (Fwd) {
+          atomicMin(&s.tmp[off], static_cast<int>(by));
+        } else {
+          atomicMax(&s.tmp[off], static_cast<int>(by));
+        }
+        s.mark_rb[off] = 1U;
+        s.flag_r[q.g * L + l] = 1U;
+        any = true;
+        ++c_tmp;
+      }
+    }
+    if (i == n_stops - 1U || !can_start || !marked) continue;
+    if (lbv == kUnreachableLb) break;  // raptor.h:1309-1311
+    auto const prev = rt_q[static_cast<std::size_t>(l) * 64U];
+    auto const et_dep = et.valid() ? time_at_stop<Fwd>(ev, n_t, et, stop_idx, !Fwd, base) : D::kInvalid;
+    if (prev != D::kInvalid && D::better_eq(prev, et_dep)) {
+      int const minutes = static_cast<int>(base) * 1440 + prev;
+      auto const day = static_cast<std::uint16_t>(minutes / 1440);
+      int const mam = static_cast<std::int16_t>(minutes % 1440);
+      auto const* ev_board = ev + n_t * (stop_idx * 2U - (Fwd ? 0U : 1U));
+      auto const new_et = earliest_transport<Fwd, false>(bits, ev_board, n_t, top, day, mam, lbv, tad_k, base, ce);
+      if (new_et.valid()) {
+        // current_best is only read when the new trip departs later (see
+        // scan_route64)
+        if (D::better_eq(time_at_stop<Fwd>(ev, n_t, new_et, stop_idx, !Fwd, base), et_dep) ||
+            !(ride || s.best[off] != D::kInvalid || s.tmp[off] != D::kInvalid)) {
+          et = new_et;
+        }
+      }
+    }
+  }
+  c_ev += static_cast<std::uint32_t>(ce.events);
+  c_rd += static_cast<std::uint32_t>(ce.route_days);
+  c_tt += static_cast<std::uint32_t>(ce.transport_tests);
+}
+
 // route_pack (batched engine): the flagged (route, group) pairs, compacted
 // in route-major order, become packed wavefronts that each hold up to 64
 // queries (group, lane) of one route.  A query takes part in a route when
@@ -891,8 +975,13 @@ __global__ void __launch_bounds__(256, 8) k_route_scan(round_args a) {
       int const tad_k = s.tad[static_cast<std::size_t>(slot) * kRowsMax + k];
       bool any = false, started = false;
       wave_sync_lds();  // the previous item's readers are done
-      scan_route64<Fwd>(x, im, s, r, k, lane, q, base, tad_k, c_stops, c_tmp, c_ev, c_rd, c_tt,
-                        any, started);
+      auto const n_t = uni(im.route_transport_from[r + 1U]) - uni(im.route_transport_from[r]);
+      if (n_t <= kBatchedMaxRouteTransports) {
+        scan_route64<Fwd>(x, im, s, r, k, lane, q, base, tad_k, c_stops, c_tmp, c_ev, c_rd, c_tt,
+                          any, started);
+      } else {
+        scan_route_lane64<Fwd>(im, s, r, k, q, base, tad_k, c_stops, c_tmp, c_ev, c_rd, c_tt, any, started);
+      }
       if (__ballot(started) != 0ULL) {
         c_ws += im.route_stop_offsets[r + 1U] - im.route_stop_offsets[r];
       }

@@ -187,22 +187,29 @@ def test_gpu_work_counters_equal_oracle_counters():
         assert w[k] == v, (k, w[k], v)
 
 
-def test_gpu_batched_engine_rejects_routes_beyond_its_lds_staging():
-    # a route of 300 transports: the plain engine runs it, the batched
-    # engine refuses it when it is created (no kernel is launched)
+def test_gpu_batched_engine_routes_beyond_its_lds_staging():
+    # routes of 300 and 257 transports (beyond the LDS event staging) next to
+    # a small one: the batched engine scans them lane by lane from global
+    # memory; both engines equal the oracle, state included
     from ttbuilder import Builder
-    b = Builder(20000, 3, 3)
+    b = Builder(20000, 6, 6)
     day = 20001
     b.add_line([9, 10, 11], [([i * 4, i * 4 + 5, i * 4 + 6, i * 4 + 12], [day]) for i in range(300)])
+    b.add_line([11, 12, 13], [([i * 5 + 3, i * 5 + 9, i * 5 + 10, i * 5 + 15], [day, day + 1])
+                              for i in range(257)])
+    b.add_line([9, 13], [([i * 30 + 1, i * 30 + 50], [day]) for i in range(20)])
     tt = b.build()
     img = na.Image(tt)
-    say("batched engine on a 300-transport route")
-    with pytest.raises(na.NigiriError) as ex:
-        na.Engine(img, lanes=64, groups=1, max_k=8)
-    assert ex.value.code == na.EUNSUPPORTED
-    plain = na.Engine(img, lanes=1, groups=1, max_k=8)
-    qs = na.Queries.build([dict(start=[(9, 0)], dest=[(11, 0)], time=day * 1440 + 100)])
-    compare(tt, plain, qs)
+    specs = [dict(start=[(9, 0)], dest=[(13, 0)], time=day * 1440 + 100 + 37 * i) for i in range(40)]
+    specs += [dict(start=[(13, 0)], dest=[(9, 0)], time=day * 1440 + 900 + 11 * i, dir=na.DIR_BWD)
+              for i in range(8)]
+    specs += [dict(start=[(10, 0)], dest=[(12, 0)], interval=(day * 1440 + 60, day * 1440 + 240))]
+    for lanes in (64, 1):
+        eng = na.Engine(img, lanes=lanes, groups=1 if lanes == 64 else 4, max_k=8)
+        for d in (na.DIR_FWD, na.DIR_BWD):
+            qs = na.Queries.build([q for q in specs if q.get("dir", na.DIR_FWD) == d])
+            say(f"routes of 300/257 transports, lanes={lanes} dir={d} n={len(qs)}")
+            compare(tt, eng, qs, check_state=len(qs) <= eng.slots)
 
 
 def test_gpu_algo_reconstruct_equals_search_legs():

@@ -561,7 +561,6 @@ struct wave_lds {
   std::uint32_t stop[kChunkMax];
   std::uint16_t lb[kChunkMax][64];
   std::int16_t rt[kChunkMax][64];
-  std::int16_t tm[kChunkMax][64];
   std::uint16_t ev[kChunkEvBytes / 2U];  // [chunk stop][arr, dep][n_t]
 };
 constexpr std::uint32_t kWaveLdsBytes = sizeof(wave_lds);
@@ -635,23 +634,16 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
     std::uint16_t lbr[kChunkMax];
     std::int16_t rtr[kChunkMax];
     std::uint32_t mkr[kChunkMax];
-    // tmp at the chunk's stops for lanes that ride into the chunk: a ride
-    // whose arrival is not better than tmp leaves tmp as it is (tmp only
-    // ever improves, so an older value can only cause a redundant atomic)
-    std::int16_t tmr[kChunkMax];
-    bool const riding = !done && et.valid();
 #pragma unroll
     for (std::uint32_t j = 0U; j < kChunkMax; ++j) {
       lbr[j] = 0U;
       rtr[j] = D::kInvalid;
       mkr[j] = 0U;
-      tmr[j] = D::kInvalid;
       if (j < nc && !done) {
         auto const l = x.stop[j] & kLocMask;
         mkr[j] = mark_t32[mrow + 2U * l];
         lbr[j] = s.lb[col0 + l * 64U];
         rtr[j] = rt_q[static_cast<std::size_t>(l) * 64U];
-        if (riding) tmr[j] = static_cast<std::int16_t>(s.tmp[col0 + l * 64U]);
       }
     }
     // event blocks: for chunk stop j, [arr at stop_idx | dep at stop_idx];
@@ -688,7 +680,6 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
       if (j < nc) {
         x.lb[j][lane] = lbr[j];
         x.rt[j][lane] = rtr[j];
-        x.tm[j][lane] = tmr[j];
       }
     }
     wave_sync_lds();
@@ -714,12 +705,10 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
         auto const e = (Fwd ? ev_arr : ev_dep)[et.t_off];
         auto const by = to_delta<Fwd>(et.day, (e & 31) * 1440 + (e >> 5), base);
         if (D::better_eq(by, tad_k) && lbv != kUnreachableLb && D::better_eq(by + D::dir(lbv), tad_k)) {
-          if (D::better(by, x.tm[j][lane])) {
-            if constexpr (Fwd) {
-              atomicMin(&s.tmp[off], static_cast<int>(by));
-            } else {
-              atomicMax(&s.tmp[off], static_cast<int>(by));
-            }
+          if constexpr (Fwd) {
+            atomicMin(&s.tmp[off], static_cast<int>(by));
+          } else {
+            atomicMax(&s.tmp[off], static_cast<int>(by));
           }
           s.mark_rb[off] = 1U;
           s.flag_r[q.g * L + l] = 1U;

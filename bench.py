@@ -317,6 +317,8 @@ def main():
         "speedup_vs_plain": round(qps / plain_qps, 2) if plain_qps > 0 else None,
         "journeys": n_journeys,
         "lane_utilization": round(st["work"]["route_stops"] / max(1, 64 * st["work"]["wave_stops"]), 4),
+        "transfer_lane_utilization": round(st["work"]["transfer_stations"]
+                                           / max(1, 64 * st["work"]["transfer_items"]), 4),
         "rounds": st["rounds"], "epochs": st["epochs"],
         "round_launch": "eager" if args.eager else "hipGraph",
         "launch_config": eng.get_config(), "autotune": tuned,

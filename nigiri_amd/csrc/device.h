@@ -29,6 +29,7 @@ enum : int {
   kWorkMarkedStations = 7,
   kWorkLocationRoutes = 8,
   kWorkWaveStops = 9,      // wavefront stop iterations of the batched route scan
+  kWorkTransferItems = 10,  // (group, station) items of the batched transfer kernel
   kWorkCount = 16
 };
 
@@ -129,6 +130,7 @@ struct dev_state {
   // algorithmic work counters (see kWork* indices)
   unsigned long long* work;
   std::size_t cap_t, cap_r, cap_route, cap_hz;
+  std::size_t cap_pack;  // packed route / transfer wavefronts (batched engine)
   // per slot
   slot_param* param;
   std::uint8_t* kslot;     // [S] round of the slot's running epoch

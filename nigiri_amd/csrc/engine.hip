@@ -1329,8 +1329,11 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     st.list_t = m.alloc<std::uint64_t>(st.cap_t);
     st.list_r = m.alloc<std::uint64_t>(st.cap_r);
     st.list_route = m.alloc<std::uint64_t>(st.cap_route);
-    st.packed_info = W == 64U ? m.alloc<std::uint64_t>(st.cap_route) : nullptr;
-    st.packed_lanes = W == 64U ? m.alloc<std::uint16_t>(st.cap_route * 64U) : nullptr;
+    // packed wavefronts: at most one per route list entry (route scan) or
+    // per station list entry (transfers)
+    st.cap_pack = W == 64U ? std::max(st.cap_route, st.cap_r) : 0U;
+    st.packed_info = W == 64U ? m.alloc<std::uint64_t>(st.cap_pack) : nullptr;
+    st.packed_lanes = W == 64U ? m.alloc<std::uint16_t>(st.cap_pack * 64U) : nullptr;
     st.list_hz = m.alloc<std::uint64_t>(st.cap_hz);
     st.counts = m.alloc<std::uint32_t>(8U);
     st.work = m.alloc<unsigned long long>(kWorkCount);

@@ -711,7 +711,7 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
             atomicMax(&s.tmp[off], static_cast<int>(by));
           }
           s.mark_rb[off] = 1U;
-          s.flag_r[q.g * L + l] = 1U;
+          s.flag_r[static_cast<std::size_t>(l) * s.G + q.g] = 1U;
           any = true;
           ++c_tmp;
         }
@@ -816,7 +816,7 @@ __device__ inline void scan_route_lane64(dev_image const& im, dev_state const& s
           atomicMax(&s.tmp[off], static_cast<int>(by));
         }
         s.mark_rb[off] = 1U;
-        s.flag_r[q.g * L + l] = 1U;
+        s.flag_r[static_cast<std::size_t>(l) * s.G + q.g] = 1U;
         any = true;
         ++c_tmp;
       }
@@ -913,7 +913,7 @@ __global__ void k_route_pack(round_args a) {
     base = static_cast<std::uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(base)));
     if (c != 0U) {
       auto const pw = static_cast<std::size_t>(base) + my_w;
-      if (pw < s.cap_route) {
+      if (pw < s.cap_pack) {
         if (my_o == 0U) s.packed_info[pw] = (static_cast<std::uint64_t>(r) << 32) | fills[my_w];
         auto* out = s.packed_lanes + pw * 64U + my_o;
         auto m = mask;
@@ -944,7 +944,7 @@ __global__ void __launch_bounds__(256, 8) k_route_scan(round_args a) {
     __shared__ wave_lds lds_all[kWavesPerBlock];
     auto& x = lds_all[threadIdx.x / 64U];
     std::uint32_t const lane = threadIdx.x & 63U;
-    auto const n_items = clamp_count(s.counts[5], s.cap_route);
+    auto const n_items = clamp_count(s.counts[5], s.cap_pack);
     std::uint32_t c_stops = 0U, c_tmp = 0U, c_ev = 0U, c_rd = 0U, c_tt = 0U, c_ws = 0U;
     item_claim claim{&s.counts[6], a.tune.route_claim};
     for (std::uint32_t item = claim.get(); item < n_items; item = claim.get()) {
@@ -1075,11 +1075,12 @@ __global__ void k_clear_marks(dev_state s, int which) {
   auto* marks = which == 0 ? s.mark_t : (which == 1 ? s.mark_r : s.hz_mask);
   auto const* list = which == 0 ? s.list_t : (which == 1 ? s.list_r : s.list_hz);
   if (W == 64 && which == 1) {
-    // batched route-phase marks: 64 bytes per (g, l), cleared as 8 words
+    // batched route-phase marks: 64 bytes per (g, l), cleared as 8 words;
+    // route-phase list entries are (l << 32 | g)
     auto* words = reinterpret_cast<std::uint64_t*>(s.mark_rb);
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n * 8U; i += gridDim.x * blockDim.x) {
       auto const e = list[i / 8U];
-      auto const gl = static_cast<std::size_t>(e >> 32) * s.L + static_cast<std::uint32_t>(e);
+      auto const gl = static_cast<std::size_t>(static_cast<std::uint32_t>(e)) * s.L + (e >> 32);
       words[gl * 8U + i % 8U] = 0ULL;
     }
     return;
@@ -1273,6 +1274,151 @@ __device__ inline int relax(dev_state const& s, std::uint32_t g, std::uint32_t l
   return tad_e != tad0 ? kRelaxHazard : 0;
 }
 
+// transfer_pack (batched engine): the (station, group) pairs flagged by the
+// route phase, compacted station by station, become packed wavefronts that
+// each hold up to 64 queries (group, lane) marked at one station; every
+// lane then relaxes that station's transfer and footpaths for its own query
+// (update_transfers / update_footpaths, raptor.h:680-845).  Packing as in
+// k_route_pack; the packed arrays are free once the route scan is done.
+__global__ void k_transfer_pack(round_args a) {
+  auto const& s = a.st;
+  __shared__ std::uint32_t fills_all[kWavesPerBlock][64];
+  auto* fills = fills_all[threadIdx.x / 64U];
+  auto const lane = threadIdx.x & 63U;
+  auto const n_items = clamp_count(s.counts[1], s.cap_r);
+  auto const wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64U;
+  for (std::uint32_t w0 = wave * 64U; w0 < n_items; w0 += gridDim.x * kWavesPerBlock * 64U) {
+    auto const item = w0 + lane;
+    std::uint32_t l = ~0U, g = 0U;
+    std::uint64_t mask = 0ULL;
+    if (item < n_items) {
+      auto const e = s.list_r[item];
+      l = static_cast<std::uint32_t>(e >> 32);
+      g = static_cast<std::uint32_t>(e);
+      // the group's 64 route-phase mark bytes (0 or 1) -> a lane mask: for
+      // bytes b_0..b_7 of a word, (w * 0x0102040810204080) >> 56 has bit k = b_k
+      auto const* mb = reinterpret_cast<std::uint64_t const*>(s.mark_rb + (static_cast<std::size_t>(g) * s.L + l) * 64U);
+#pragma unroll
+      for (std::uint32_t j = 0U; j < 8U; ++j) {
+        mask |= ((mb[j] * 0x0102040810204080ULL) >> 56) << (8U * j);
+      }
+      mask &= s.grp_active[g];
+    }
+    auto const c = static_cast<std::uint32_t>(__popcll(mask));
+    auto const max_entries = a.tune.pack_max_entries;
+    std::uint32_t my_w = 0U, my_o = 0U, n_w = 0U, fill = 0U, cur_l = ~0U, entries = 0U;
+    for (std::uint32_t t = 0U; t < 64U; ++t) {
+      auto const ct = static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(c), static_cast<int>(t)));
+      if (ct == 0U) continue;
+      auto const lt = static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(l), static_cast<int>(t)));
+      if (n_w == 0U || lt != cur_l || fill + ct > 64U || entries == max_entries) {
+        if (n_w != 0U && lane == 0U) fills[n_w - 1U] = fill;
+        ++n_w;
+        fill = 0U;
+        entries = 0U;
+        cur_l = lt;
+      }
+      ++entries;
+      if (lane == t) {
+        my_w = n_w - 1U;
+        my_o = fill;
+      }
+      fill += ct;
+    }
+    if (n_w != 0U && lane == 0U) fills[n_w - 1U] = fill;
+    wave_sync_lds();
+    std::uint32_t base = 0U;
+    if (lane == 0U && n_w != 0U) base = atomicAdd(&s.counts[5], n_w);
+    base = static_cast<std::uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(base)));
+    if (c != 0U) {
+      auto const pw = static_cast<std::size_t>(base) + my_w;
+      if (pw < s.cap_pack) {
+        if (my_o == 0U) s.packed_info[pw] = (static_cast<std::uint64_t>(l) << 32) | fills[my_w];
+        auto* out = s.packed_lanes + pw * 64U + my_o;
+        auto m = mask;
+        for (std::uint32_t j = 0U; m != 0ULL; ++j) {
+          auto const b = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1);
+          m &= m - 1ULL;
+          out[j] = static_cast<std::uint16_t>((g << 6) | b);
+        }
+      } else {
+        atomicOr(&s.counts[4], 1U);
+      }
+    }
+    wave_sync_lds();  // fills is reused by the next window
+  }
+}
+
+// update_transfers + update_footpaths (raptor.h:680-845), batched engine:
+// one packed wavefront per station, one query (group, lane) per lane.
+// Relaxations and their checks as in k_transfer_fp; marks are set per lane.
+template <bool Fwd>
+__global__ void k_transfer_fp64(round_args a) {
+  using D = dirop<Fwd>;
+  auto const& im = a.im;
+  auto const& s = a.st;
+  auto const n_items = clamp_count(s.counts[5], s.cap_pack);
+  auto const lane = threadIdx.x & 63U;
+  item_claim claim{&s.counts[7], a.tune.transfer_claim};
+  auto const* off = Fwd ? im.fp_out_offsets : im.fp_in_offsets;
+  auto const* fps = Fwd ? im.fp_out : im.fp_in;
+  unsigned long long c_st = 0ULL, c_fp = 0ULL, c_items = 0ULL;
+  for (std::uint32_t item = claim.get(); item < n_items; item = claim.get()) {
+    auto const info = s.packed_info[item];
+    auto const i = uni(static_cast<std::uint32_t>(info >> 32));
+    auto const n_lanes = uni(static_cast<std::uint32_t>(info));
+    if (lane == 0U) ++c_items;
+    std::uint32_t g = 0U, ln = 0U;
+    bool on = false;
+    if (lane < n_lanes) {
+      auto const d = s.packed_lanes[static_cast<std::size_t>(item) * 64U + lane];
+      g = static_cast<std::uint32_t>(d) >> 6;
+      ln = static_cast<std::uint32_t>(d) & 63U;
+      on = ((s.grp_active[g] >> ln) & 1ULL) != 0ULL;
+    }
+    auto const slot = g * 64U + ln;
+    auto const& p = s.param[slot];
+    auto const k = static_cast<std::uint32_t>(s.kslot[slot]);
+    auto const tmp_i = on ? s.tmp[lay<64>::col(s, g, i, ln)] : static_cast<int>(D::kInvalid);
+    bool const valid = on && tmp_i != D::kInvalid;
+    auto const tad0 = s.tad0[slot];
+    auto const fe0 = off[i], fe1 = off[i + 1U];
+    if (on) {
+      ++c_st;
+      c_fp += fe1 - fe0;
+    }
+    auto const bit = 1ULL << ln;
+    auto const emit = [&](std::uint32_t t, int code) {
+      auto const gt = static_cast<std::size_t>(g) * s.L + t;
+      if (code == kRelaxImproved) {
+        atomicOr(reinterpret_cast<unsigned long long*>(&s.mark_t[gt]), static_cast<unsigned long long>(bit));
+        s.flag_t[gt] = 1U;
+      } else if (code == kRelaxHazard) {
+        atomicOr(reinterpret_cast<unsigned long long*>(&s.hz_mask[gt]), static_cast<unsigned long long>(bit));
+        s.flag_hz[gt] = 1U;
+      }
+    };
+    // transfer at i (destination stations are handled by k_dest_pass)
+    if (valid && !lay<64>::test(s.dest_mask, s, g, i, ln)) {
+      auto const tt = adjusted<std::uint8_t>(p, im.transfer_time[i]);
+      emit(i, relax<Fwd, 64>(s, g, ln, slot, k, i, tmp_i + D::dir(tt), kPosTransfer | i, tad0));
+    }
+    // footpaths of i in list order
+    for (auto fe = fe0; fe != fe1; ++fe) {
+      auto const fp = fps[fe];
+      auto const t = fp & ((1U << 23) - 1U);
+      if (valid && !lay<64>::test(s.dest_mask, s, g, t, ln)) {
+        auto const dur = static_cast<std::int16_t>(fp >> 23);
+        emit(t, relax<Fwd, 64>(s, g, ln, slot, k, t, tmp_i + D::dir(adjusted<std::int16_t>(p, dur)),
+                               kPosFootpath | fe, tad0));
+      }
+    }
+  }
+  wave_count(&s.work[kWorkTransferStations], c_st);
+  wave_count(&s.work[kWorkFootpaths], c_fp);
+  wave_count(&s.work[kWorkTransferItems], c_items);
+}
+
 template <bool Fwd, int W>
 __global__ void k_transfer_fp(round_args a) {
   using D = dirop<Fwd>;
@@ -1289,7 +1435,7 @@ __global__ void k_transfer_fp(round_args a) {
   std::uint32_t item = W == 64 ? claim.get() : blockIdx.x * blockDim.x + threadIdx.x;
   auto const* off = Fwd ? im.fp_out_offsets : im.fp_in_offsets;
   auto const* fps = Fwd ? im.fp_out : im.fp_in;
-  unsigned long long c_st = 0ULL, c_fp = 0ULL;
+  unsigned long long c_st = 0ULL, c_fp = 0ULL, c_items = 0ULL;
   for (; item < n_items; item = next_item(item)) {
     auto const e = s.list_r[item];
     auto const g = static_cast<std::uint32_t>(e >> 32), i = static_cast<std::uint32_t>(e);
@@ -1297,6 +1443,7 @@ __global__ void k_transfer_fp(round_args a) {
     if constexpr (W == 64) {
       on = ((s.grp_active[g] >> lane_id) & 1ULL) != 0ULL && marked_r<W>(s, g, i, lane_id);
       if (__ballot(on) == 0ULL) continue;
+      if (lane_id == 0U) ++c_items;
     } else {
       on = (lay<W>::lanes_at(s.mark_r[lay<W>::lword(s, g, i)], i) & s.grp_active[g]) != 0ULL;
       if (!on) continue;
@@ -1352,6 +1499,7 @@ __global__ void k_transfer_fp(round_args a) {
     }
   }
   wave_count(&s.work[kWorkTransferStations], c_st);
+  wave_count(&s.work[kWorkTransferItems], c_items);
   wave_count(&s.work[kWorkFootpaths], c_fp);
 }
 
@@ -1807,12 +1955,19 @@ void round_segment(int seg, round_args const& a, std::uint32_t const* sweep, boo
       // list_t, routes, packed
       hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 1U | 4U | 32U);
       if constexpr (W == 64) {
-        compact(a.st.flag_r, a.st.L, a.st.G, a.st.list_r, &a.st.counts[1], a.st.cap_r, a.st, st);
+        // station-major flags: list entries (l << 32 | g), then packed
+        // transfer wavefronts (counts[5] was reset after the route scan)
+        compact(a.st.flag_r, a.st.G, a.st.L, a.st.list_r, &a.st.counts[1], a.st.cap_r, a.st, st);
+        hipLaunchKernelGGL(k_transfer_pack, dim3(item_blocks), dim3(kBlock), 0, st, a);
       }
       hipLaunchKernelGGL((k_dest_pass<Fwd, W>), dim3(grid_for(n_slots, 64, 1 << 20)), dim3(64), 0, st, a, n_slots);
       break;
     case 3:
-      hipLaunchKernelGGL((k_transfer_fp<Fwd, W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
+      if constexpr (W == 64) {
+        hipLaunchKernelGGL((k_transfer_fp64<Fwd>), dim3(item_blocks), dim3(kBlock), 0, st, a);
+      } else {
+        hipLaunchKernelGGL((k_transfer_fp<Fwd, W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
+      }
       break;
     default:
       if constexpr (W == 64) {
@@ -1823,8 +1978,8 @@ void round_segment(int seg, round_args const& a, std::uint32_t const* sweep, boo
       hipLaunchKernelGGL((k_hazards<Fwd, W>), dim3(1024), dim3(kBlock), 0, st, a);
       hipLaunchKernelGGL((k_clear_marks<W>), dim3(1024), dim3(kBlock), 0, st, a.st, 1);
       hipLaunchKernelGGL((k_clear_marks<W>), dim3(256), dim3(kBlock), 0, st, a.st, 2);
-      // list_r, hazards, work cursors of the route scan and transfers
-      hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 2U | 8U | 64U | 128U);
+      // list_r, hazards, packed wavefronts, work cursors of the route scan and transfers
+      hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 2U | 8U | 32U | 64U | 128U);
       hipLaunchKernelGGL((k_round_finish<W>), dim3(slot_blocks), dim3(kBlock), 0, st, a, n_slots);
       break;
   }

@@ -1351,7 +1351,7 @@ __global__ void k_transfer_pack(round_args a) {
 
 // update_transfers + update_footpaths (raptor.h:680-845), batched engine:
 // one packed wavefront per station, one query (group, lane) per lane.
-// Relaxations and their checks as in k_transfer_fp; marks are set per lane.
+// Relaxations and their checks as in k_transfer_fp1; marks are set per lane.
 template <bool Fwd>
 __global__ void k_transfer_fp64(round_args a) {
   using D = dirop<Fwd>;
@@ -1419,87 +1419,53 @@ __global__ void k_transfer_fp64(round_args a) {
   wave_count(&s.work[kWorkTransferItems], c_items);
 }
 
-template <bool Fwd, int W>
-__global__ void k_transfer_fp(round_args a) {
+// update_transfers + update_footpaths (raptor.h:680-845), plain engine:
+// one thread per (slot, station) marked by the route phase.
+template <bool Fwd>
+__global__ void k_transfer_fp1(round_args a) {
   using D = dirop<Fwd>;
+  constexpr int W = 1;
   auto const& im = a.im;
   auto const& s = a.st;
   auto* overflow = &s.counts[4];
   auto const n_items = clamp_count(s.counts[1], s.cap_r);
-  std::uint32_t const lane_id = W == 64 ? (threadIdx.x & 63U) : 0U;
-  // batched engine: items claimed per wavefront; plain engine: grid stride
-  item_claim claim{&s.counts[7], a.tune.transfer_claim};
-  auto const next_item = [&](std::uint32_t i) {
-    return W == 64 ? claim.get() : i + gridDim.x * blockDim.x;
-  };
-  std::uint32_t item = W == 64 ? claim.get() : blockIdx.x * blockDim.x + threadIdx.x;
   auto const* off = Fwd ? im.fp_out_offsets : im.fp_in_offsets;
   auto const* fps = Fwd ? im.fp_out : im.fp_in;
-  unsigned long long c_st = 0ULL, c_fp = 0ULL, c_items = 0ULL;
-  for (; item < n_items; item = next_item(item)) {
+  unsigned long long c_st = 0ULL, c_fp = 0ULL;
+  for (std::uint32_t item = blockIdx.x * blockDim.x + threadIdx.x; item < n_items;
+       item += gridDim.x * blockDim.x) {
     auto const e = s.list_r[item];
     auto const g = static_cast<std::uint32_t>(e >> 32), i = static_cast<std::uint32_t>(e);
-    bool on;
-    if constexpr (W == 64) {
-      on = ((s.grp_active[g] >> lane_id) & 1ULL) != 0ULL && marked_r<W>(s, g, i, lane_id);
-      if (__ballot(on) == 0ULL) continue;
-      if (lane_id == 0U) ++c_items;
-    } else {
-      on = (lay<W>::lanes_at(s.mark_r[lay<W>::lword(s, g, i)], i) & s.grp_active[g]) != 0ULL;
-      if (!on) continue;
-    }
-    auto const slot = g * W + lane_id;
+    if ((lay<W>::lanes_at(s.mark_r[lay<W>::lword(s, g, i)], i) & s.grp_active[g]) == 0ULL) continue;
+    auto const slot = g;
     auto const& p = s.param[slot];
     auto const k = static_cast<std::uint32_t>(s.kslot[slot]);
-    auto const tmp_i = s.tmp[lay<W>::col(s, g, i, lane_id)];
-    bool const valid = on && tmp_i != D::kInvalid;
+    auto const tmp_i = s.tmp[lay<W>::col(s, g, i, 0U)];
+    bool const valid = tmp_i != D::kInvalid;
     auto const tad0 = s.tad0[slot];
-    if (on) {
-      ++c_st;
-      c_fp += off[i + 1U] - off[i];
-    }
-
+    ++c_st;
+    c_fp += off[i + 1U] - off[i];
     auto const emit = [&](std::uint32_t t, int code) {
-      if constexpr (W == 64) {
-        auto const m = __ballot(code == kRelaxImproved);
-        auto const h = __ballot(code == kRelaxHazard);
-        if (m != 0ULL && lane_id == static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1)) {
-          mark_wave64(s.mark_t, s.flag_t, g, t, m, s.L);
-        }
-        if (h != 0ULL && lane_id == static_cast<std::uint32_t>(__ffsll(static_cast<long long>(h)) - 1)) {
-          mark_wave64(s.hz_mask, s.flag_hz, g, t, h, s.L);
-        }
-      } else {
-        if (code == kRelaxImproved) mark_one<W>(s.mark_t, s.list_t, &s.counts[0], s.cap_t, overflow, s, g, t, 0U);
-        if (code == kRelaxHazard) mark_one<W>(s.hz_mask, s.list_hz, &s.counts[3], s.cap_hz, overflow, s, g, t, 0U);
-      }
+      if (code == kRelaxImproved) mark_one<W>(s.mark_t, s.list_t, &s.counts[0], s.cap_t, overflow, s, g, t, 0U);
+      if (code == kRelaxHazard) mark_one<W>(s.hz_mask, s.list_hz, &s.counts[3], s.cap_hz, overflow, s, g, t, 0U);
     };
-
     // transfer at i (destination stations are handled by k_dest_pass)
-    {
-      bool const is_dest = lay<W>::test(s.dest_mask, s, g, i, lane_id);
-      int code = 0;
-      if (valid && !is_dest) {
-        auto const tt = adjusted<std::uint8_t>(p, im.transfer_time[i]);
-        code = relax<Fwd, W>(s, g, lane_id, slot, k, i, tmp_i + D::dir(tt), kPosTransfer | i, tad0);
-      }
-      emit(i, code);
+    if (valid && !lay<W>::test(s.dest_mask, s, g, i, 0U)) {
+      auto const tt = adjusted<std::uint8_t>(p, im.transfer_time[i]);
+      emit(i, relax<Fwd, W>(s, g, 0U, slot, k, i, tmp_i + D::dir(tt), kPosTransfer | i, tad0));
     }
     // footpaths of i in list order
     for (auto fe = off[i]; fe != off[i + 1U]; ++fe) {
       auto const fp = fps[fe];
       auto const t = fp & ((1U << 23) - 1U);
-      int code = 0;
-      if (valid && !lay<W>::test(s.dest_mask, s, g, t, lane_id)) {
+      if (valid && !lay<W>::test(s.dest_mask, s, g, t, 0U)) {
         auto const dur = static_cast<std::int16_t>(fp >> 23);
-        code = relax<Fwd, W>(s, g, lane_id, slot, k, t, tmp_i + D::dir(adjusted<std::int16_t>(p, dur)),
-                             kPosFootpath | fe, tad0);
+        emit(t, relax<Fwd, W>(s, g, 0U, slot, k, t, tmp_i + D::dir(adjusted<std::int16_t>(p, dur)),
+                              kPosFootpath | fe, tad0));
       }
-      emit(t, code);
     }
   }
   wave_count(&s.work[kWorkTransferStations], c_st);
-  wave_count(&s.work[kWorkTransferItems], c_items);
   wave_count(&s.work[kWorkFootpaths], c_fp);
 }
 
@@ -1966,7 +1932,7 @@ void round_segment(int seg, round_args const& a, std::uint32_t const* sweep, boo
       if constexpr (W == 64) {
         hipLaunchKernelGGL((k_transfer_fp64<Fwd>), dim3(item_blocks), dim3(kBlock), 0, st, a);
       } else {
-        hipLaunchKernelGGL((k_transfer_fp<Fwd, W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
+        hipLaunchKernelGGL((k_transfer_fp1<Fwd>), dim3(item_blocks), dim3(kBlock), 0, st, a);
       }
       break;
     default:

@@ -319,6 +319,7 @@ def main():
         "lane_utilization": round(st["work"]["route_stops"] / max(1, 64 * st["work"]["wave_stops"]), 4),
         "transfer_lane_utilization": round(st["work"]["transfer_stations"]
                                            / max(1, 64 * st["work"]["transfer_items"]), 4),
+        "hazard_lanes_per_item": round(st["work"]["hazard_lanes"] / max(1, st["work"]["hazard_items"]), 4),
         "rounds": st["rounds"], "epochs": st["epochs"],
         "round_launch": "eager" if args.eager else "hipGraph",
         "launch_config": eng.get_config(), "autotune": tuned,

@@ -30,6 +30,8 @@ enum : int {
   kWorkLocationRoutes = 8,
   kWorkWaveStops = 9,      // wavefront stop iterations of the batched route scan
   kWorkTransferItems = 10,  // (group, station) items of the batched transfer kernel
+  kWorkHazardItems = 11,    // (group, station) items of the hazard replay
+  kWorkHazardLanes = 12,    // queries replayed there
   kWorkCount = 16
 };
 

@@ -1490,60 +1490,89 @@ __global__ void k_rt_fixup(round_args a) {
 
 // Exact replay for hazard stations: last passing relaxation P is the first
 // eligible one with x == best; later pre-writes lower round_times below it.
+// Replay of one hazard station t for one query (group g, lane): the last
+// passing relaxation P is the first eligible one with x == best; the
+// pre-writes after it lower round_times[k][t] below best.
 template <bool Fwd, int W>
-__global__ void k_hazards(round_args a) {
+__device__ inline void replay_hazard(round_args const& a, std::uint32_t g, std::uint32_t t,
+                                     std::uint32_t lane) {
   using D = dirop<Fwd>;
   auto const& im = a.im;
   auto const& s = a.st;
+  auto const slot = g * W + lane;
+  auto const& p = s.param[slot];
+  auto const k = static_cast<std::uint32_t>(s.kslot[slot]);
+  auto const c = lay<W>::col(s, g, t, lane);
+  auto const bt = s.best[c];
+  int const lbv = s.lb[c];
+  auto const eligible = [&](std::int16_t x, std::uint32_t pos) {
+    auto const tad_e = tad_at(s, slot, pos);
+    return D::better_eq(x, tad_e) && lbv != kUnreachableLb && D::better_eq(x + D::dir(lbv), tad_e);
+  };
+  // enumerate events into t: (pos, x)
+  auto const* off = Fwd ? im.fp_in_offsets : im.fp_out_offsets;
+  auto const* fps = Fwd ? im.fp_in : im.fp_out;
+  auto const* map = Fwd ? im.fp_in_to_out : im.fp_out_to_in;
+  auto const for_each_event = [&](auto&& fn) {
+    if (marked_r<W>(s, g, t, lane)) {
+      auto const tmp_t = s.tmp[c];
+      if (tmp_t != D::kInvalid) {
+        fn(kPosTransfer | t, clamp16(tmp_t + D::dir(adjusted<std::uint8_t>(p, im.transfer_time[t]))));
+      }
+    }
+    for (auto fe = off[t]; fe != off[t + 1U]; ++fe) {
+      auto const src = fps[fe] & ((1U << 23) - 1U);
+      if (!marked_r<W>(s, g, src, lane)) continue;
+      auto const tmp_s = s.tmp[lay<W>::col(s, g, src, lane)];
+      if (tmp_s == D::kInvalid) continue;
+      auto const dur = static_cast<std::int16_t>(fps[fe] >> 23);
+      fn(kPosFootpath | map[fe], clamp16(tmp_s + D::dir(adjusted<std::int16_t>(p, dur))));
+    }
+  };
+  std::uint32_t pos_p = 0xFFFFFFFFU;
+  for_each_event([&](std::uint32_t pos, std::int16_t x) {
+    if (x == bt && eligible(x, pos) && pos < pos_p) pos_p = pos;
+  });
+  if (pos_p == 0xFFFFFFFFU) return;
+  auto v = bt;
+  for_each_event([&](std::uint32_t pos, std::int16_t x) {
+    if (pos > pos_p) v = D::best_of(x, v);
+  });
+  s.round_times[lay<W>::rt(s, g, k, t, lane)] = v;
+}
+
+// Exact replay for hazard stations (see replay_hazard).  Batched engine: one
+// thread per (group, station) item, looping over its hazard lanes (about one
+// per item); plain engine: one thread per item.
+template <bool Fwd, int W>
+__global__ void k_hazards(round_args a) {
+  auto const& s = a.st;
   auto const n = clamp_count(s.counts[3], s.cap_hz);
-  for (std::uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < n * W; h += gridDim.x * blockDim.x) {
-    auto const e = s.list_hz[h / W];
+  unsigned long long c_items = 0ULL, c_lanes = 0ULL;
+  for (std::uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < n; h += gridDim.x * blockDim.x) {
+    auto const e = s.list_hz[h];
     auto const g = static_cast<std::uint32_t>(e >> 32), t = static_cast<std::uint32_t>(e);
-    auto const lane = h % W, slot = g * W + lane;
-    if (!lay<W>::test(s.hz_mask, s, g, t, lane)) continue;
-    if ((s.grp_active[g] & (1ULL << lane)) == 0ULL) continue;
-    if (!lay<W>::test(s.mark_t, s, g, t, lane)) continue;  // no passing relaxation
-    if (lay<W>::test(s.dest_mask, s, g, t, lane)) continue;
-    auto const& p = s.param[slot];
-    auto const k = static_cast<std::uint32_t>(s.kslot[slot]);
-    auto const c = lay<W>::col(s, g, t, lane);
-    auto const bt = s.best[c];
-    int const lbv = s.lb[c];
-    auto const eligible = [&](std::int16_t x, std::uint32_t pos) {
-      auto const tad_e = tad_at(s, slot, pos);
-      return D::better_eq(x, tad_e) && lbv != kUnreachableLb && D::better_eq(x + D::dir(lbv), tad_e);
-    };
-    // enumerate events into t: (pos, x)
-    auto const* off = Fwd ? im.fp_in_offsets : im.fp_out_offsets;
-    auto const* fps = Fwd ? im.fp_in : im.fp_out;
-    auto const* map = Fwd ? im.fp_in_to_out : im.fp_out_to_in;
-    auto const for_each_event = [&](auto&& fn) {
-      if (marked_r<W>(s, g, t, lane)) {
-        auto const tmp_t = s.tmp[c];
-        if (tmp_t != D::kInvalid) {
-          fn(kPosTransfer | t, clamp16(tmp_t + D::dir(adjusted<std::uint8_t>(p, im.transfer_time[t]))));
-        }
-      }
-      for (auto fe = off[t]; fe != off[t + 1U]; ++fe) {
-        auto const src = fps[fe] & ((1U << 23) - 1U);
-        if (!marked_r<W>(s, g, src, lane)) continue;
-        auto const tmp_s = s.tmp[lay<W>::col(s, g, src, lane)];
-        if (tmp_s == D::kInvalid) continue;
-        auto const dur = static_cast<std::int16_t>(fps[fe] >> 23);
-        fn(kPosFootpath | map[fe], clamp16(tmp_s + D::dir(adjusted<std::int16_t>(p, dur))));
-      }
-    };
-    std::uint32_t pos_p = 0xFFFFFFFFU;
-    for_each_event([&](std::uint32_t pos, std::int16_t x) {
-      if (x == bt && eligible(x, pos) && pos < pos_p) pos_p = pos;
-    });
-    if (pos_p == 0xFFFFFFFFU) continue;
-    auto v = bt;
-    for_each_event([&](std::uint32_t pos, std::int16_t x) {
-      if (pos > pos_p) v = D::best_of(x, v);
-    });
-    s.round_times[lay<W>::rt(s, g, k, t, lane)] = v;
+    ++c_items;
+    std::uint64_t m;
+    if constexpr (W == 64) {
+      // lanes with a hazard, active, with a passing relaxation, not a destination
+      auto const gt = static_cast<std::size_t>(g) * s.L + t;
+      m = s.hz_mask[gt] & s.grp_active[g] & s.mark_t[gt] & ~s.dest_mask[gt];
+    } else {
+      m = (lay<W>::test(s.hz_mask, s, g, t, 0U) && (s.grp_active[g] & 1ULL) != 0ULL &&
+           lay<W>::test(s.mark_t, s, g, t, 0U) && !lay<W>::test(s.dest_mask, s, g, t, 0U))
+              ? 1ULL
+              : 0ULL;
+    }
+    while (m != 0ULL) {
+      auto const lane = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1);
+      m &= m - 1ULL;
+      ++c_lanes;
+      replay_hazard<Fwd, W>(a, g, t, lane);
+    }
   }
+  wave_count(&s.work[kWorkHazardItems], c_items);
+  wave_count(&s.work[kWorkHazardLanes], c_lanes);
 }
 
 // ======================================================================

@@ -1475,16 +1475,26 @@ template <bool Fwd, int W>
 __global__ void k_rt_fixup(round_args a) {
   auto const& s = a.st;
   auto const n = clamp_count(s.counts[0], s.cap_t);
-  for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n * W;
-       i += gridDim.x * blockDim.x) {
-    auto const item = i / W, lane = i % W;
-    auto const e = s.list_t[item];
+  // one thread per (group, station) item, over its improved lanes
+  for (std::uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < n; h += gridDim.x * blockDim.x) {
+    auto const e = s.list_t[h];
     auto const g = static_cast<std::uint32_t>(e >> 32), t = static_cast<std::uint32_t>(e);
-    if (!lay<W>::test(s.mark_t, s, g, t, lane)) continue;
-    if ((s.grp_active[g] & (1ULL << lane)) == 0ULL) continue;
-    if (lay<W>::test(s.dest_mask, s, g, t, lane)) continue;
-    auto const k = static_cast<std::uint32_t>(s.kslot[g * W + lane]);
-    s.round_times[lay<W>::rt(s, g, k, t, lane)] = s.best[lay<W>::col(s, g, t, lane)];
+    std::uint64_t m;
+    if constexpr (W == 64) {
+      auto const gt = static_cast<std::size_t>(g) * s.L + t;
+      m = s.mark_t[gt] & s.grp_active[g] & ~s.dest_mask[gt];
+    } else {
+      m = (lay<W>::test(s.mark_t, s, g, t, 0U) && (s.grp_active[g] & 1ULL) != 0ULL &&
+           !lay<W>::test(s.dest_mask, s, g, t, 0U))
+              ? 1ULL
+              : 0ULL;
+    }
+    while (m != 0ULL) {
+      auto const lane = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1);
+      m &= m - 1ULL;
+      auto const k = static_cast<std::uint32_t>(s.kslot[g * W + lane]);
+      s.round_times[lay<W>::rt(s, g, k, t, lane)] = s.best[lay<W>::col(s, g, t, lane)];
+    }
   }
 }
 

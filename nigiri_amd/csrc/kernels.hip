@@ -650,18 +650,25 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
     // the arrival row of stop_idx directly precedes its departure row in the
     // route's event array (stop-pair layout), so both rows are one run of
     // 2 n_t events starting at n_t (2 stop_idx - 1); a missing row (no
-    // arrival at the first stop, no departure at the last) is zeroed
-    for (std::uint32_t j = 0U; j < nc; ++j) {
-      auto const si = c0 + j;
-      auto const stop_idx = Fwd ? si : n_stops - si - 1U;
-      auto const lo = stop_idx == 0U ? n_t : 0U;                  // first valid element
-      auto const hi = stop_idx + 1U == n_stops ? n_t : 2U * n_t;  // end of valid elements
-      auto const* src = ev_g + static_cast<std::ptrdiff_t>(n_t) * (2 * static_cast<std::ptrdiff_t>(stop_idx) - 1);
-      auto* dst = x.ev + 2U * j * n_t;
-      for (auto e = lane; e < 2U * n_t; e += 64U) {
-        dst[e] = (e >= lo && e < hi) ? src[e] : std::uint16_t{0U};
+    // arrival at the first stop, no departure at the last) is zeroed.  While
+    // a lane rides, the chunk is scanned for sure and its events are loaded
+    // with the lane values; otherwise only once a mark shows the chunk is
+    // not skipped.
+    auto const stage_events = [&]() {
+      for (std::uint32_t j = 0U; j < nc; ++j) {
+        auto const si = c0 + j;
+        auto const stop_idx = Fwd ? si : n_stops - si - 1U;
+        auto const lo = stop_idx == 0U ? n_t : 0U;                  // first valid element
+        auto const hi = stop_idx + 1U == n_stops ? n_t : 2U * n_t;  // end of valid elements
+        auto const* src = ev_g + static_cast<std::ptrdiff_t>(n_t) * (2 * static_cast<std::ptrdiff_t>(stop_idx) - 1);
+        auto* dst = x.ev + 2U * j * n_t;
+        for (auto e = lane; e < 2U * n_t; e += 64U) {
+          dst[e] = (e >= lo && e < hi) ? src[e] : std::uint16_t{0U};
+        }
       }
-    }
+    };
+    bool const any_rider = __ballot(!done && et.valid()) != 0ULL;
+    if (any_rider) stage_events();
     std::uint32_t mbits = 0U;  // bit j: this query is marked at chunk stop j
 #pragma unroll
     for (std::uint32_t j = 0U; j < kChunkMax; ++j) {
@@ -675,6 +682,7 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
       my_stops += done ? 0U : nc;
       continue;
     }
+    if (!any_rider) stage_events();
 #pragma unroll
     for (std::uint32_t j = 0U; j < kChunkMax; ++j) {
       if (j < nc) {

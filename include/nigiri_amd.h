@@ -348,6 +348,8 @@ int nigiri_amd_engine_stats(nigiri_amd_engine* e, nigiri_engine_stats* out,
  * pack_max_entries: (route, group) entries, i.e. 64-query groups, that one
  *   packed route wavefront may mix (1-64; 64).
  * lb_delta: threshold step in minutes of the near-far lower-bound kernel (8).
+ * transfer_pack_max_entries: (station, group) entries, i.e. 64-query
+ *   groups, that one packed transfer wavefront may mix (1-64; 64).
  * admit_when_idle: serving loop; 1 = new queries are admitted only when no
  *   slot is busy, so that a batch starts together while the preparation and
  *   lower bounds of the next batch overlap the current batch's rounds
@@ -362,7 +364,7 @@ typedef struct nigiri_engine_config {
   uint32_t pack_max_entries;
   uint32_t lb_delta;
   uint32_t admit_when_idle;
-  uint32_t reserved[1];
+  uint32_t transfer_pack_max_entries;
 } nigiri_engine_config;
 
 int nigiri_amd_engine_configure(nigiri_amd_engine* e,
@@ -373,15 +375,17 @@ int nigiri_amd_engine_get_config(const nigiri_amd_engine* e,
 
 /* ---- autotuner (no counterpart in the reference) ------------------------
  * Chooses the launch-shape knobs of nigiri_engine_config (grid_waves,
- * route_claim, transfer_claim, pack_max_entries, lb_delta) for this engine
- * and timetable on a sample of queries (ideally one batch of the real
+ * route_claim, transfer_claim, pack_max_entries, lb_delta,
+ * transfer_pack_max_entries) for this engine and timetable on a sample of
+ * queries (ideally one batch of the real
  * workload): one pass of coordinate descent in which every candidate value
  * of a knob runs the sample through the engine `repeats` times and the value
  * with the smallest median engine time is kept.  The chosen values stay set
  * on the engine and are written to *out (may be NULL).  Each trial is
  * reported as (knob, value, ms) in trials[0 .. min(n, trial_capacity)),
  * with the total count in *n_trials (may be NULL).  Knob ids: 0 grid_waves,
- * 1 route_claim, 2 transfer_claim, 3 pack_max_entries, 4 lb_delta.
+ * 1 route_claim, 2 transfer_claim, 3 pack_max_entries, 4 lb_delta,
+ * 5 transfer_pack_max_entries.
  * Results never depend on the knobs. */
 typedef struct nigiri_tune_trial {
   uint32_t knob;

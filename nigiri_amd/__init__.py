@@ -128,14 +128,15 @@ class EngineConfig(C.Structure):
                 ("eager_rounds", C.c_uint32), ("route_claim", C.c_uint32),
                 ("transfer_claim", C.c_uint32), ("pack_max_entries", C.c_uint32),
                 ("lb_delta", C.c_uint32), ("admit_when_idle", C.c_uint32),
-                ("reserved", C.c_uint32 * 1)]
+                ("transfer_pack_max_entries", C.c_uint32)]
 
 
 class TuneTrial(C.Structure):
     _fields_ = [("knob", C.c_uint32), ("value", C.c_uint32), ("ms", C.c_double)]
 
 
-TUNE_KNOBS = ["grid_waves", "route_claim", "transfer_claim", "pack_max_entries", "lb_delta"]
+TUNE_KNOBS = ["grid_waves", "route_claim", "transfer_claim", "pack_max_entries", "lb_delta",
+              "transfer_pack_max_entries"]
 
 
 WORK_NAMES = ["route_stops", "events", "route_day_tests", "transport_day_tests", "tmp_writes",
@@ -452,7 +453,8 @@ class Engine:
     def configure(self, admission_wait_ms=2.0, grid_waves=0, graphs=True, admit_when_idle=False,
                   **knobs):
         """Sets the engine configuration; launch-shape knobs (route_claim,
-        transfer_claim, pack_max_entries, lb_delta) left out keep their values."""
+        transfer_claim, pack_max_entries, lb_delta, transfer_pack_max_entries)
+        left out keep their values."""
         c = EngineConfig()
         c.admission_wait_ms = admission_wait_ms
         c.grid_waves = grid_waves
@@ -467,7 +469,7 @@ class Engine:
     def get_config(self):
         c = EngineConfig()
         _check(lib().nigiri_amd_engine_get_config(self.handle, C.byref(c)))
-        return {n: getattr(c, n) for n, _ in EngineConfig._fields_ if n != "reserved"}
+        return {n: getattr(c, n) for n, _ in EngineConfig._fields_}
 
     def autotune(self, queries, repeats=1):
         """Chooses the launch-shape knobs on a sample of queries; returns the

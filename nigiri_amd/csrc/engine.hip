@@ -269,7 +269,7 @@ struct nigiri_amd_engine {
   // rounds replayed from captured hipGraphs (off: kernels launched one by one)
   bool use_graphs = true;
   // launch-shape knobs (nigiri_amd_engine_autotune chooses them)
-  round_tune tune{1U, 16U, 64U};
+  round_tune tune{1U, 16U, 64U, 64U};
   std::uint32_t lb_delta = 8U;
   // serving: admit new queries only when no slot is busy (whole batches)
   bool admit_when_idle = false;
@@ -1438,6 +1438,9 @@ int nigiri_amd_engine_configure(nigiri_amd_engine* e, const nigiri_engine_config
   if (c->route_claim != 0U) e->tune.route_claim = c->route_claim;
   if (c->transfer_claim != 0U) e->tune.transfer_claim = c->transfer_claim;
   if (c->pack_max_entries != 0U) e->tune.pack_max_entries = std::min<std::uint32_t>(c->pack_max_entries, 64U);
+  if (c->transfer_pack_max_entries != 0U) {
+    e->tune.transfer_pack_max_entries = std::min<std::uint32_t>(c->transfer_pack_max_entries, 64U);
+  }
   if (c->lb_delta != 0U) e->lb_delta = c->lb_delta;
   e->admit_when_idle = c->admit_when_idle != 0U;
   return NIGIRI_AMD_OK;
@@ -1486,6 +1489,7 @@ int nigiri_amd_engine_get_config(const nigiri_amd_engine* e, nigiri_engine_confi
   out->route_claim = e->tune.route_claim;
   out->transfer_claim = e->tune.transfer_claim;
   out->pack_max_entries = e->tune.pack_max_entries;
+  out->transfer_pack_max_entries = e->tune.transfer_pack_max_entries;
   out->lb_delta = e->lb_delta;
   out->admit_when_idle = e->admit_when_idle ? 1U : 0U;
   return NIGIRI_AMD_OK;
@@ -1509,13 +1513,15 @@ int nigiri_amd_engine_autotune(nigiri_amd_engine* e, const nigiri_query* qs, uin
       std::vector<std::uint32_t> values;
     };
     // knob ids as in nigiri_tune_trial: 0 grid_waves, 1 route_claim,
-    // 2 transfer_claim, 3 pack_max_entries, 4 lb_delta
+    // 2 transfer_claim, 3 pack_max_entries, 4 lb_delta,
+    // 5 transfer_pack_max_entries
     std::vector<knob> knobs{{0U, &e->grid_waves, {8U * c, 16U * c, 32U * c, 64U * c}},
                             {1U, &e->tune.route_claim, {1U, 2U, 4U}},
                             {2U, &e->tune.transfer_claim, {4U, 16U, 64U}}};
     if (e->W == 64U) {  // packing and device lower bounds: batched engine only
       knobs.push_back({3U, &e->tune.pack_max_entries, {1U, 2U, 4U, 8U, 64U}});
       knobs.push_back({4U, &e->lb_delta, {4U, 8U, 16U, 32U, 64U}});
+      knobs.push_back({5U, &e->tune.transfer_pack_max_entries, {1U, 2U, 4U, 8U, 64U}});
     }
     std::uint32_t nt = 0U;
     std::vector<prepared_query> pq;

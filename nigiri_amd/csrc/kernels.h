@@ -41,6 +41,7 @@ struct round_tune {
   std::uint32_t route_claim;       // packed route wavefronts claimed per atomic
   std::uint32_t transfer_claim;    // stations claimed per atomic (transfers)
   std::uint32_t pack_max_entries;  // (route, group) entries per packed wavefront
+  std::uint32_t transfer_pack_max_entries;  // (station, group) entries per packed transfer wavefront
 };
 
 struct round_args {

@@ -1313,7 +1313,7 @@ __global__ void k_transfer_pack(round_args a) {
       mask &= s.grp_active[g];
     }
     auto const c = static_cast<std::uint32_t>(__popcll(mask));
-    auto const max_entries = a.tune.pack_max_entries;
+    auto const max_entries = a.tune.transfer_pack_max_entries;
     std::uint32_t my_w = 0U, my_o = 0U, n_w = 0U, fill = 0U, cur_l = ~0U, entries = 0U;
     for (std::uint32_t t = 0U; t < 64U; ++t) {
       auto const ct = static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(c), static_cast<int>(t)));

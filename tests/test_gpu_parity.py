@@ -356,3 +356,34 @@ def test_gpu_whole_batch_admission_serving():
         ref = oracle.search(tt, qs[i])
         assert res[i]["journeys"] == ref["journeys"], i
         assert res[i]["legs"] == ref["legs"], i
+
+
+# ---- full configured sizes (samples small enough for the oracle) -----------
+
+def test_gpu_full_size_workload2_sample():
+    # 50k stops, 365-day bitfields: 64 batched queries with state, 8 plain
+    tt, img = image_for("w2", lambda: na.Timetable.generate(2))
+    qs = na.Queries.generate(tt, 2, 64, seed=0x7172 + 61)
+    say("full-size workload2 batched n=64")
+    compare(tt, na.Engine(img, lanes=64, groups=1, max_k=16), qs)
+    say("full-size workload2 plain n=8")
+    compare(tt, na.Engine(img, lanes=1, groups=1, max_k=16), qs.subset(list(range(8))), check_state=False)
+
+
+def test_gpu_full_size_workload3_mix_sample():
+    # 200k stops: point and range queries (1-4 h windows) and intermodal ones
+    tt, img = image_for("w3", lambda: na.Timetable.generate(3))
+    qs = na.Queries.generate(tt, 3, 24, seed=0x7172 + 67)
+    say("full-size workload3 point/range n=24")
+    compare(tt, na.Engine(img, lanes=64, groups=1, max_k=10), qs)
+    qi = na.Queries.generate(tt, 4, 16, seed=0x7172 + 71)
+    say("full-size workload4 intermodal n=16")
+    compare(tt, na.Engine(img, lanes=64, groups=1, max_k=10), qi)
+
+
+def test_gpu_full_size_workload5_mix_sample():
+    # 500k stops, the serving mix (point / range / intermodal)
+    tt, img = image_for("w5", lambda: na.Timetable.generate(5))
+    qs = na.Queries.generate(tt, 5, 12, seed=0x7172 + 73)
+    say("full-size workload5 mix n=12")
+    compare(tt, na.Engine(img, lanes=64, groups=1, max_k=10), qs)

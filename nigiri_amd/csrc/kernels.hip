@@ -554,6 +554,8 @@ constexpr std::uint32_t kChunkMax = 4U;
 constexpr std::uint32_t kStagedTransports = 16U;
 constexpr std::uint32_t kChunkEvBytes = 1024U;
 static_assert(kChunkEvBytes / 4U >= kBatchedMaxRouteTransports, "a chunk holds at least one stop");
+static_assert(kStagedTransports * 8U * 8U >= kBatchedMaxRouteTransports * 4U,
+              "the bitfield staging area holds the bitfield indices of a staged route");
 
 struct wave_lds {
   std::uint64_t rbits[8];
@@ -600,16 +602,22 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
   auto const* rt_q = s.round_times + (static_cast<std::size_t>(q.g) * s.K + (k - 1U)) * L * 64U + q.ln;
   auto const* mark_t32 = reinterpret_cast<std::uint32_t const*>(s.mark_t);
 
-  // route-level staging: traffic-day bitfields
+  // route-level staging: traffic-day bitfields; for routes of more than
+  // kStagedTransports transports, the same LDS area holds the transports'
+  // bitfield indices instead (at most 256 x 4 B), so that a traffic-day test
+  // makes one global load instead of two dependent ones
   auto const rbf = uni(im.route_traffic_days[r]);
   bool const staged_t = n_t <= kStagedTransports;
+  auto* tidx = reinterpret_cast<std::uint32_t*>(x.tbits);
   if (lane < 8U) x.rbits[lane] = im.bitfields[static_cast<std::size_t>(rbf) * 8U + lane];
   if (staged_t) {
     for (auto j = lane; j < 8U * n_t; j += 64U) {
       x.tbits[j] = im.bitfields[static_cast<std::size_t>(tbf_g[j / 8U]) * 8U + j % 8U];
     }
+  } else {
+    for (auto j = lane; j < n_t; j += 64U) tidx[j] = tbf_g[j];
   }
-  route_bits const bits{x.rbits, x.tbits, tbf_g, im.bitfields, staged_t};
+  route_bits const bits{x.rbits, x.tbits, tidx, im.bitfields, staged_t};
   auto const chunk = min(kChunkMax, kChunkEvBytes / (4U * n_t));
   auto const top = 1U << (31 - __clz(n_t));  // largest power of two <= n_t
 

@@ -356,8 +356,8 @@ struct et_counters {
 // indices into the global table.
 struct route_bits {
   std::uint64_t const* route;   // 8 words
-  std::uint64_t const* staged;  // n_t x 8 words (valid when use_staged)
-  std::uint32_t const* tbf;     // bitfield index per transport
+  std::uint64_t const* staged;  // n_t x 8 words (transport_active<true>)
+  std::uint32_t const* tbf;     // bitfield index per transport (LDS or global)
   std::uint64_t const* table;   // global bitfield table
   bool use_staged;
   __device__ bool route_active(std::uint32_t day) const {
@@ -370,9 +370,6 @@ struct route_bits {
     auto const w = Staged ? staged[t_off * 8U + day / 64U]
                           : table[static_cast<std::size_t>(tbf[t_off]) * 8U + day / 64U];
     return ((w >> (day % 64U)) & 1ULL) != 0ULL;
-  }
-  __device__ bool transport_active(std::uint32_t t_off, std::uint32_t day) const {
-    return use_staged ? transport_active<true>(t_off, day) : transport_active<false>(t_off, day);
   }
 };
 

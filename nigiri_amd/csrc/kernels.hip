@@ -286,26 +286,37 @@ __global__ void k_mark_routes(round_args a) {
   auto const n_items = clamp_count(s.counts[0], s.cap_t);
   if constexpr (W == 64) {
     // flag_route[r][g] = 1 for every route of a marked station (plain
-    // stores); route_pack rebuilds the lane masks from the station marks
+    // stores); route_pack rebuilds the lane masks from the station marks.
+    // A station serves a few routes: 8 lanes per (group, station) item,
+    // 8 items per wavefront.
+    constexpr std::uint32_t kSub = 8U;
     auto const lane = threadIdx.x & 63U;
+    auto const sl = lane % kSub;
     std::uint32_t c_st = 0U, c_lr = 0U;
-    for (std::uint32_t item = (blockIdx.x * blockDim.x + threadIdx.x) / 64U; item < n_items;
-         item += gridDim.x * kWavesPerBlock) {
+    auto const wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64U;
+    for (std::uint32_t base = wave * (64U / kSub); base < n_items;
+         base += gridDim.x * kWavesPerBlock * (64U / kSub)) {
+      auto const item = base + lane / kSub;
+      if (item >= n_items) continue;
       auto const e = s.list_t[item];
       auto const g = static_cast<std::uint32_t>(e >> 32), l = static_cast<std::uint32_t>(e);
       auto const word = s.mark_t[static_cast<std::size_t>(g) * s.L + l] & s.grp_active[g];
       if (word == 0ULL) continue;
       auto const r0 = im.location_route_offsets[l], r1 = im.location_route_offsets[l + 1U];
-      if (lane == 0U) {
+      if (sl == 0U) {
         auto const nq = static_cast<std::uint32_t>(__popcll(word));
         c_st += nq;
         c_lr += nq * (r1 - r0);
       }
       // route-major flags: [route][group]
-      for (auto j = r0 + lane; j < r1; j += 64U) {
+      for (auto j = r0 + sl; j < r1; j += kSub) {
         s.flag_route[static_cast<std::size_t>(im.location_routes[j]) * s.G + g] = 1U;
       }
-      if (r1 != r0 && ((word >> lane) & 1ULL) != 0ULL) s.any_route[g * 64U + lane] = 1U;
+      if (r1 != r0) {
+        for (auto b = sl; b < 64U; b += kSub) {
+          if (((word >> b) & 1ULL) != 0ULL) s.any_route[g * 64U + b] = 1U;
+        }
+      }
     }
     wave_count(&s.work[kWorkMarkedStations], c_st);
     wave_count(&s.work[kWorkLocationRoutes], c_lr);

@@ -291,18 +291,23 @@ struct pareto_set {
 struct raptor_state {
   std::uint32_t n_locations_{0U};
   std::vector<delta_t> tmp_, best_, round_times_;  // round_times [kRows][L]
+  std::vector<delta_t> bounds_;  // [kRows][L], written by fill_bounds
   std::vector<bool> station_mark_, prev_station_mark_, route_mark_;
   void resize(std::uint32_t n_locations, std::uint32_t n_routes) {
     n_locations_ = n_locations;
     tmp_.resize(n_locations);
     best_.resize(n_locations);
     round_times_.resize(static_cast<std::size_t>(n_locations) * kRows);
+    bounds_.resize(static_cast<std::size_t>(n_locations) * kRows);
     station_mark_.resize(n_locations);
     prev_station_mark_.resize(n_locations);
     route_mark_.resize(n_routes);
   }
   delta_t& rt(std::uint32_t k, std::uint32_t l) {
     return round_times_[static_cast<std::size_t>(k) * n_locations_ + l];
+  }
+  delta_t& bound(std::uint32_t k, std::uint32_t l) {
+    return bounds_[static_cast<std::size_t>(k) * n_locations_ + l];
   }
 };
 
@@ -371,6 +376,29 @@ struct raptor {
     state_.best_[l] = get_best(d, state_.best_[l]);
     state_.rt(0, l) = get_best(d, state_.rt(0, l));
     state_.station_mark_[l] = true;
+  }
+
+  // raptor.h:117-170 (Vias = 0, no td footpaths): bounds row k is the best
+  // round time of rounds 0..k
+  void fill_bounds(std::size_t n_rows) {
+    for (auto x = 0U; x != n_locations_; ++x) state_.bound(0, x) = state_.rt(0, x);
+    for (auto k = 1U; k < n_rows; ++k) {
+      for (auto x = 0U; x != n_locations_; ++x) {
+        state_.bound(k, x) = Fwd ? std::min(state_.rt(k, x), state_.bound(k - 1U, x))
+                                 : std::max(state_.rt(k, x), state_.bound(k - 1U, x));
+      }
+    }
+  }
+
+  // raptor.h:172
+  void set_bounds(unsigned last_round) { bounds_last_k_ = last_round; }
+
+  // raptor.h:458-489 (Vias = 0: no stays): t must not be better than the
+  // other direction's bound for the remaining rounds plus the transfer time
+  bool within_bounds(unsigned k, std::uint32_t l, int t) {
+    if (bounds_last_k_ == 0U) return true;
+    auto const transfer = dir(adjusted<int>(tts_, static_cast<int>(tt_.v_.transfer_time[l])));
+    return is_better_or_eq(t, static_cast<int>(state_.bound(bounds_last_k_ - k, l)) + transfer);
   }
 
   // raptor.h:205-414
@@ -469,7 +497,7 @@ struct raptor {
               ? 0
               : dir(adjusted<std::uint8_t>(tts_, tt_.v_.transfer_time[i]));
       auto const fp_target_time = clamp(static_cast<long long>(tmp_time) + transfer_time);
-      if (is_better(fp_target_time, best[i])) {
+      if (bounds_last_k_ == 0U && is_better(fp_target_time, best[i])) {
         state_.rt(k, i) = get_best(fp_target_time, state_.rt(k, i));
       }
       if (is_better(fp_target_time, best[i]) &&
@@ -478,6 +506,7 @@ struct raptor {
             !is_better_loose(fp_target_time + dir(lb_[i]), time_at_dest_[k])) {
           continue;
         }
+        if (!within_bounds(k, i, fp_target_time)) continue;
         state_.rt(k, i) = fp_target_time;
         best[i] = fp_target_time;
         c_.s.alg_bytes += 4U;
@@ -505,7 +534,7 @@ struct raptor {
         auto const fp_target_time = clamp(
             static_cast<long long>(tmp_time) +
             dir(adjusted<std::int16_t>(tts_, fp_duration(fp)) + 0));
-        if (is_better(fp_target_time, best[target])) {
+        if (bounds_last_k_ == 0U && is_better(fp_target_time, best[target])) {
           state_.rt(k, target) = get_best(fp_target_time, state_.rt(k, target));
         }
         if (is_better(fp_target_time, best[target]) &&
@@ -514,6 +543,7 @@ struct raptor {
               !is_better_loose(fp_target_time + dir(lb_[target]), time_at_dest_[k])) {
             continue;
           }
+          if (!within_bounds(k, target, fp_target_time)) continue;
           state_.rt(k, target) = fp_target_time;
           best[target] = fp_target_time;
           c_.s.alg_bytes += 4U;
@@ -568,7 +598,8 @@ struct raptor {
           current_best = get_best(get_best(state_.rt(k - 1, l_idx), tmp[l_idx]), best[l_idx]);
           if (is_better_loose(by_transport, time_at_dest_[k]) &&
               lb_reachable(l_idx) &&
-              is_better_loose(by_transport + dir(lb_[l_idx]), time_at_dest_[k])) {
+              is_better_loose(by_transport + dir(lb_[l_idx]), time_at_dest_[k]) &&
+              within_bounds(k, l_idx, by_transport)) {
             tmp[l_idx] = get_best(by_transport, tmp[l_idx]);
             c_.s.alg_bytes += 2U;
             state_.station_mark_[l_idx] = true;
@@ -682,6 +713,7 @@ struct raptor {
   std::vector<std::uint16_t> const& dist_to_end_;
   std::vector<std::uint16_t> const& lb_;
   delta_t time_at_dest_[kRows];
+  unsigned bounds_last_k_{0U};  // 0: no bounds (raptor.h:1527)
   std::uint16_t base_;
   tts_t tts_;
   counters& c_;
@@ -1651,6 +1683,162 @@ void run_search(tt_t const& tt, query_t q, raptor_state& rs, search_result& out,
   if (tad_out != nullptr) tad_out->assign(algo.time_at_dest_, algo.time_at_dest_ + kRows);
 }
 
+// query::flip_dir (query.cc:52-57; no td offsets, no vias)
+void flip_dir(query_t& q) {
+  std::swap(q.start_, q.destination_);
+  std::swap(q.start_intermodal_, q.dest_intermodal_);
+}
+
+constexpr std::int32_t kMinLookAhead = 1440;  // raptor/pong.h:11
+
+// pong<SearchDir, Rt = false, Vias = 0> (src/routing/raptor/pong.cc:62-428):
+// a ping search from each start time in the search direction, then for each
+// of its journeys (most transfers first) a pong search in the other
+// direction from the arrival, pruned by the ping's round times
+// (fill_bounds / set_bounds), which finds the latest departure; the next
+// ping starts one minute after the earliest of those departures.  Results
+// are the pong journeys with start and destination swapped.  Left out:
+// enrich_with_slow_direct (a no-op: query::slow_direct_ is false,
+// direct.cc:614-616) and the earliest-alternative rewrite of the legs
+// between three transit legs (pong.cc:430-470), which changes legs only.
+// Both raptors share one raptor_state, as in the reference.
+template <bool Fwd>
+void run_pong(tt_t const& tt, query_t q, raptor_state& rs, search_result& out, counters& c) {
+  sanitize(q);
+  auto const fastest_direct = get_fastest_direct(tt, q, Fwd);
+  auto const search_interval = q.is_interval_ ? interval_t{q.iv_from_, q.iv_to_}
+                                              : interval_t{q.start_time_, q.start_time_};
+  auto const mid = static_cast<std::int64_t>(search_interval.from_) +
+                   (search_interval.to_ - search_interval.from_) / 2;
+  auto const base_day = static_cast<std::uint16_t>(
+      (round_days(mid) * 1440 - tt.internal_from()) / 1440);
+
+  // ping (pong.cc:113-151): lower bounds from the destinations
+  std::vector<bool> ping_is_dest;
+  std::vector<std::uint16_t> ping_dist_to_dest, ping_lb;
+  collect_destinations(tt, q, ping_is_dest, ping_dist_to_dest);
+  dijkstra_lb(tt, q, Fwd ? tt.v_.lb_fwd_offsets : tt.v_.lb_bwd_offsets,
+              Fwd ? tt.v_.lb_fwd : tt.v_.lb_bwd, ping_lb);
+  raptor<Fwd> ping{tt, rs, ping_is_dest, ping_dist_to_dest, ping_lb, base_day, q.tts_, c};
+
+  // pong (pong.cc:153-206): the flipped query, lower bounds all zero
+  flip_dir(q);
+  std::vector<bool> pong_is_dest;
+  std::vector<std::uint16_t> pong_dist_to_dest;
+  collect_destinations(tt, q, pong_is_dest, pong_dist_to_dest);
+  std::vector<std::uint16_t> const pong_lb(tt.n_locations(), 0U);
+  raptor<!Fwd> pong{tt, rs, pong_is_dest, pong_dist_to_dest, pong_lb, base_day, q.tts_, c};
+  flip_dir(q);
+
+  // pong.cc:211-395
+  auto& results = out.results_;
+  results.els_.clear();
+  std::vector<start_t> starts;
+  auto start_time = Fwd ? search_interval.from_ : search_interval.to_ - 1;
+  auto const end_time = Fwd ? search_interval.to_ : search_interval.from_ - 1;
+  auto const is_better = [](std::int32_t a, std::int32_t b) { return Fwd ? a < b : a > b; };
+  auto const is_validated = [&](journey_t const& j) { return is_better(j.dest_time_, start_time); };
+  auto const result_count = [&](bool include_too_slow) {
+    return static_cast<unsigned>(std::count_if(
+        results.els_.begin(), results.els_.end(), [&](journey_t const& j) {
+          return is_validated(j) &&
+                 (include_too_slow ||
+                  (j.travel_time() < fastest_direct && j.travel_time() <= q.max_travel_time_));
+        }));
+  };
+  interval_t const external{tt.v_.date_range_from, tt.v_.date_range_to};
+  while ((is_better(start_time, end_time) ||
+          result_count(true) + result_count(false) < 2U * q.min_connection_count_) &&
+         external.contains(start_time)) {
+    starts.clear();
+    get_starts(Fwd, tt, false, start_time, search_interval, q, starts, false);
+    ping.reset_arrivals();
+    ping.next_start_time();
+    for (auto const& st : starts) ping.add_start(st.stop_, st.time_at_stop_);
+    auto const worst_time_at_dest =
+        start_time + (Fwd ? 1 : -1) * std::min<std::int32_t>(q.max_travel_time_ + kMinLookAhead,
+                                                             kMaxTravelTime);
+    pareto_set ping_results;
+    ++c.s.n_epochs;
+    ping.execute(start_time, q.max_transfers_, worst_time_at_dest, ping_results);
+    if (ping_results.els_.empty()) break;
+    auto& pr = ping_results.els_;
+    pr.erase(std::remove_if(pr.begin(), pr.end(),
+                            [&](journey_t const& x) {  // pareto_set.h:17-19
+                              return std::any_of(results.els_.begin(), results.els_.end(),
+                                                 [&](journey_t const& y) { return y.dominates(x); });
+                            }),
+             pr.end());
+    if (pr.empty()) break;
+    // transfers are unique among one ping's journeys (equal start times)
+    std::sort(pr.begin(), pr.end(),
+              [](journey_t const& a, journey_t const& b) { return a.transfers_ > b.transfers_; });
+
+    ping.fill_bounds(pr.front().transfers_ + std::size_t{1U});
+    flip_dir(q);
+    pong.reset_arrivals();
+    auto q_pong_starts = q;  // start footpaths unless the pong starts intermodal
+    q_pong_starts.use_start_footpaths_ = !q.start_intermodal_;
+    for (auto& ping_j : pr) {
+      pong.set_bounds(ping_j.transfers_ + 1U);
+      starts.clear();
+      get_starts(!Fwd, tt, false, ping_j.dest_time_, search_interval, q_pong_starts, starts, false);
+      pong.next_start_time();
+      for (auto const& st : starts) pong.add_start(st.stop_, st.time_at_stop_);
+      ++c.s.n_epochs;
+      pong.execute(ping_j.dest_time_, ping_j.transfers_, ping_j.start_time_, results);
+      auto const match = std::find_if(results.els_.begin(), results.els_.end(),
+                                      [&](journey_t const& pong_j) {
+                                        return pong_j.transfers_ == ping_j.transfers_ &&
+                                               pong_j.start_time_ == ping_j.dest_time_;
+                                      });
+      if (match == results.els_.end()) throw std::runtime_error("no pong");
+      if (!match->is_reconstructed_ && !match->error_) {
+        reconstructor<!Fwd>{tt, q, rs, base_day, *match}.run();
+      }
+      ping_j.start_time_ = match->dest_time_;
+    }
+    flip_dir(q);
+
+    auto const first = std::min_element(pr.begin(), pr.end(), [&](journey_t const& a, journey_t const& b) {
+      return is_better(a.start_time_, b.start_time_);
+    });
+    auto const next = first->start_time_ + (Fwd ? 1 : -1);
+    if (!is_better(start_time, next)) throw std::runtime_error("no pong progress");
+    start_time = next;
+  }
+
+  // pong.cc:397-428
+  auto& e = results.els_;
+  e.erase(std::remove_if(e.begin(), e.end(),
+                         [&](journey_t const& j) {
+                           auto const j_start_time = j.dest_time_;
+                           auto const out_of_interval =
+                               Fwd ? !q.extend_later_ && j_start_time >= search_interval.to_
+                                   : !q.extend_earlier_ && j_start_time < search_interval.from_;
+                           return !j.is_reconstructed_ || !is_validated(j) || out_of_interval ||
+                                  j.travel_time() >= fastest_direct ||
+                                  j.travel_time() > q.max_travel_time_;
+                         }),
+          e.end());
+  for (auto& x : e) std::swap(x.start_time_, x.dest_time_);
+  out.interval_ = {Fwd ? search_interval.from_ : start_time + 1,
+                   Fwd ? start_time : search_interval.to_};
+  out.base_ = base_day;
+  out.fastest_direct_ = fastest_direct;
+  auto const swap_special = [](std::uint32_t l) {
+    return l == kStart ? kEnd : (l == kEnd ? kStart : l);
+  };
+  for (auto& j : e) {
+    j.legs_.front().from_ = swap_special(j.legs_.front().from_);
+    j.legs_.back().to_ = swap_special(j.legs_.back().to_);
+  }
+  // (start_time, transfers) is unique in a Pareto set
+  std::sort(e.begin(), e.end(), [](journey_t const& a, journey_t const& b) {
+    return std::tie(a.start_time_, a.transfers_) < std::tie(b.start_time_, b.transfers_);
+  });
+}
+
 int run(nigiri_tt_view const& v, nigiri_query const& cq, raptor_state& rs,
         search_result& out, counters& c, std::vector<std::uint16_t>* lb,
         std::vector<delta_t>* tad) {
@@ -1709,6 +1897,49 @@ int oracle_search(const nigiri_tt_view* tt, const nigiri_query* q, nigiri_journe
       std::memcpy(time_at_dest, tad.data(), sizeof(int16_t) * kRows);
     }
     if (lb != nullptr && lbv.size() == L) std::memcpy(lb, lbv.data(), sizeof(uint16_t) * L);
+    if (info != nullptr) {
+      info[0] = res.base_;
+      info[1] = res.fastest_direct_;
+      info[2] = res.interval_.from_;
+      info[3] = res.interval_.to_;
+    }
+    if (stats != nullptr) *stats = c.s;
+    return 0;
+  } catch (std::exception const&) {
+    return -1;
+  }
+}
+
+int oracle_pong(const nigiri_tt_view* v, const nigiri_query* cq, nigiri_journey* out,
+                uint32_t cap, uint32_t* n_out, nigiri_leg* legs, uint32_t leg_cap,
+                uint32_t* n_legs, int32_t* info, oracle_stats* stats) {
+  try {
+    tt_t const tt{*v};
+    auto const q = from_c(*cq);
+    raptor_state rs;
+    search_result res;
+    counters c;
+    if (q.fwd_) {
+      run_pong<true>(tt, q, rs, res, c);
+    } else {
+      run_pong<false>(tt, q, rs, res, c);
+    }
+    auto const& e = res.results_.els_;
+    if (n_out != nullptr) *n_out = static_cast<uint32_t>(e.size());
+    std::uint32_t leg_pos = 0U;
+    for (std::size_t i = 0; i < e.size() && i < cap; ++i) {
+      out[i] = nigiri_journey{e[i].start_time_, e[i].dest_time_, e[i].dest_, e[i].transfers_, {},
+                              leg_pos, static_cast<uint32_t>(e[i].legs_.size())};
+      for (auto const& l : e[i].legs_) {
+        if (legs != nullptr && leg_pos < leg_cap) {
+          legs[leg_pos] = nigiri_leg{l.kind_, l.from_, l.to_, l.dep_time_, l.arr_time_,
+                                     l.transport_, l.route_, l.day_, l.stop_from_, l.stop_to_,
+                                     l.duration_, l.target_};
+        }
+        ++leg_pos;
+      }
+    }
+    if (n_legs != nullptr) *n_legs = leg_pos;
     if (info != nullptr) {
       info[0] = res.base_;
       info[1] = res.fastest_direct_;

@@ -52,6 +52,15 @@ int oracle_search(const nigiri_tt_view* tt, const nigiri_query* q,
                   int16_t* time_at_dest, uint16_t* lb, int32_t* info,
                   oracle_stats* stats);
 
+/* Range search by pong (routing::pong_search, raptor/pong.cc:62-428):
+ * ping/pong pairs of searches until the interval is covered.  Outputs as
+ * in oracle_search; info = {base_day, fastest_direct, interval_from,
+ * interval_to} of the result.  Returns -1 where the reference throws. */
+int oracle_pong(const nigiri_tt_view* tt, const nigiri_query* q,
+                nigiri_journey* out, uint32_t cap, uint32_t* n_out,
+                nigiri_leg* legs, uint32_t leg_cap, uint32_t* n_legs,
+                int32_t* info, oracle_stats* stats);
+
 /* optimize_footpaths (raptor/optimize_footpaths.cc:444-453) on n legs in
  * travel order, in place; q is the journey-direction query. */
 int oracle_optimize_footpaths(const nigiri_tt_view* tt, const nigiri_query* q,

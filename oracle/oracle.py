@@ -49,6 +49,9 @@ def lib():
                                     P(Leg), C.c_uint32, P(C.c_uint32),
                                     P(C.c_int16), P(C.c_int16), P(C.c_int16), P(C.c_int16),
                                     P(C.c_uint16), P(C.c_int32), P(Stats)]
+        h.oracle_pong.restype = C.c_int
+        h.oracle_pong.argtypes = [P(TtView), P(Query), P(Journey), C.c_uint32, P(C.c_uint32),
+                                  P(Leg), C.c_uint32, P(C.c_uint32), P(C.c_int32), P(Stats)]
         h.oracle_optimize_footpaths.restype = C.c_int
         h.oracle_optimize_footpaths.argtypes = [P(TtView), P(Query), P(Leg), C.c_uint32]
         h.oracle_lb.restype = C.c_int
@@ -105,6 +108,31 @@ def search(tt, q, state=False, cap=4096, leg_cap=1 << 15):
     if state:
         out.update(round_times=rt, best=best, tmp=tmp, time_at_dest=tad, lb=lb)
     return out
+
+
+def pong(tt, q, cap=4096, leg_cap=1 << 15):
+    """Range search by pong (ping/pong pairs); same result layout as search().
+    Raises RuntimeError where the reference throws."""
+    from nigiri_amd import Journey, Leg
+    jr = (Journey * cap)()
+    lg = (Leg * leg_cap)()
+    n = C.c_uint32()
+    n_legs = C.c_uint32()
+    info = np.zeros(4, np.int32)
+    stats = Stats()
+    rc = lib().oracle_pong(C.byref(tt.view), C.byref(q), jr, cap, C.byref(n), lg, leg_cap,
+                           C.byref(n_legs), _p(info, C.c_int32), C.byref(stats))
+    if rc != 0:
+        raise RuntimeError("oracle_pong failed")
+    if n.value > cap or n_legs.value > leg_cap:
+        raise RuntimeError("oracle_pong: output capacity exceeded")
+    k = n.value
+    js = [(jr[i].start_time, jr[i].dest_time, jr[i].dest, jr[i].transfers) for i in range(k)]
+    legs = [[lg[jr[i].leg_offset + x].as_tuple() for x in range(jr[i].n_legs)]
+            for i in range(k)]
+    return {"journeys": js, "legs": legs, "base_day": int(info[0]),
+            "fastest_direct": int(info[1]), "interval": (int(info[2]), int(info[3])),
+            "stats": stats.as_dict()}
 
 
 def optimize_footpaths(tt, q, legs):

@@ -257,6 +257,20 @@ int nigiri_amd_search(nigiri_amd_engine* e, const nigiri_query* qs,
                       uint32_t journey_capacity, nigiri_leg* legs,
                       uint32_t leg_capacity);
 
+/* Range (or ontrip) search by pong (replaces routing::pong_search,
+ * include/nigiri/routing/raptor/pong.h:22-30; src/routing/raptor/pong.cc:
+ * 62-428) for one query on a plain engine (lanes == 1): ping searches on
+ * slot ping_slot, pong searches on ping_slot + 1, bounded by the ping's
+ * round times.  Journeys (start and destination times in travel direction;
+ * dest is the pong's destination, i.e. the query's start location, as in
+ * the reference) and their legs go to journeys / legs as in
+ * nigiri_amd_search, res->n_epochs counts executes.  Not done: the leg
+ * rewrite by earlier alternatives (pong.cc:430-470). */
+int nigiri_amd_pong(nigiri_amd_engine* e, const nigiri_query* q,
+                    uint32_t ping_slot, nigiri_result* res,
+                    nigiri_journey* journeys, uint32_t journey_capacity,
+                    nigiri_leg* legs, uint32_t leg_capacity);
+
 /* ---- Algo-level entry points for one slot: the method set of
  * raptor<> (raptor.h:174-414) / gpu_raptor<> (gpu/raptor.h:83-99) that
  * search<Dir, Algo> drives.  `slot` < lanes * n_groups. ---------------- */
@@ -289,6 +303,19 @@ int nigiri_amd_algo_execute(nigiri_amd_engine* e, uint32_t slot,
                             int32_t start_time, uint8_t max_transfers,
                             int32_t worst_time_at_dest, int16_t* dest_times,
                             uint32_t* n_rounds);
+/* Bounds of a pong search (raptor<>::fill_bounds / set_bounds,
+ * raptor.h:117-172, with within_bounds, raptor.h:458-489).  fill_bounds
+ * writes bounds rows 0 .. n_rows-1 of slot dst_slot from the round times of
+ * slot src_slot as its last execute left them (row k: the best of rounds
+ * 0..k); in the reference ping and pong share one state, here the pong's
+ * slot holds the bounds.  set_bounds(slot, last_round) makes the slot's
+ * next executes prune labels by bounds row last_round - k (0 switches
+ * bounds off; an execute with more rounds than last_round is refused).
+ * Plain engine only (lanes == 1): NIGIRI_AMD_EUNSUPPORTED otherwise. */
+int nigiri_amd_algo_fill_bounds(nigiri_amd_engine* e, uint32_t src_slot,
+                                uint32_t dst_slot, uint32_t n_rows);
+int nigiri_amd_algo_set_bounds(nigiri_amd_engine* e, uint32_t slot,
+                               uint32_t last_round);
 /* Reconstructs the legs of journey *j (its start_time, dest_time, dest and
  * transfers) from the slot's state as the last execute left it (replaces
  * raptor<>::reconstruct, raptor.h:416-423, i.e. reconstruct_journey,

@@ -172,6 +172,10 @@ SIGNATURES = {
                                           C.POINTER(C.c_int16), C.POINTER(C.c_uint32)]),
     "nigiri_amd_algo_reconstruct": (C.c_int, [_P, C.c_uint32, C.POINTER(Query),
                                               C.POINTER(Journey), C.POINTER(Leg), C.c_uint32]),
+    "nigiri_amd_algo_fill_bounds": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "nigiri_amd_algo_set_bounds": (C.c_int, [_P, C.c_uint32, C.c_uint32]),
+    "nigiri_amd_pong": (C.c_int, [_P, C.POINTER(Query), C.c_uint32, C.POINTER(Result),
+                                  C.POINTER(Journey), C.c_uint32, C.POINTER(Leg), C.c_uint32]),
     "nigiri_amd_algo_download": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_int16),
                                            C.POINTER(C.c_int16), C.POINTER(C.c_int16),
                                            C.POINTER(C.c_int16), C.POINTER(C.c_uint16)]),
@@ -518,6 +522,18 @@ class Engine:
         _check(lib().nigiri_amd_serve(self.handle, queries.q, ptr(arr, C.c_double), n, res, jr,
                                       cap, lg, leg_cap, C.byref(wall)))
         return _collect(res, jr, n, lg if legs is True else None), wall.value
+
+    def pong(self, query, ping_slot=0, journey_capacity=4096, leg_capacity=1 << 15):
+        """Range search of one query by pong (nigiri's pong_search) on a plain engine
+        (lanes == 1); query is a Query struct.  Returns the result dict of search()."""
+        res = (Result * 1)()
+        jr = (Journey * journey_capacity)()
+        lg = (Leg * leg_capacity)()
+        _check(lib().nigiri_amd_pong(self.handle, C.byref(query), ping_slot, res, jr,
+                                     journey_capacity, lg, leg_capacity))
+        out = _collect(res, jr, 1, lg)[0]
+        out["interval"] = (res[0].interval_from, res[0].interval_to)
+        return out
 
     def reconstruct(self, slot, query, journey, leg_capacity=64):
         """Legs of journey (start, dest_time, dest, transfers) from the slot's state as its

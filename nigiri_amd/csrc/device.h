@@ -78,7 +78,7 @@ struct slot_param {
   std::uint8_t intermodal;
   std::uint8_t n_dest;   // non-intermodal destination bits
   std::uint8_t n_dist;   // intermodal egress offsets
-  std::uint8_t pad;
+  std::uint8_t bounds_last_k;  // plain engine: pong bounds (0 = none)
 };
 
 // Device query state of an engine.  `W` lanes share one group: 64 for the
@@ -96,6 +96,8 @@ struct dev_state {
   std::uint32_t W, G, K, L, R;
   std::uint32_t LW, RW;  // words per group for W == 1 bitsets
   std::int16_t* round_times;
+  std::int16_t* bounds;    // plain engine: [G][K][L], the other search's
+                           // best round times of rounds 0..k (else null)
   std::int16_t* best;
   std::int32_t* tmp;       // int32 so that route scans update it with
                            // non-returning atomicMin / atomicMax

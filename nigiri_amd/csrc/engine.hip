@@ -30,6 +30,7 @@
 #include "handles.h"
 #include "device.h"
 #include "journey_legs.h"
+#include "engine_access.h"
 #include "kernels.h"
 #include "search_host.h"
 #include "timetable.h"
@@ -705,6 +706,7 @@ struct runner {
         sp.base = p.base_;
         sp.end_k = static_cast<std::uint8_t>(std::min<std::uint32_t>(p.end_k_, K));
         sp.intermodal = p.intermodal_dest_ ? 1U : 0U;
+        sp.bounds_last_k = 0U;
         if (p.dests_.size() > kMaxDestBits || p.dist_loc_.size() > kMaxDist) {
           throw status_error{NIGIRI_AMD_EUNSUPPORTED, "too many destination offsets"};
         }
@@ -1227,6 +1229,13 @@ void run_batched(nigiri_amd_engine& e, bool fwd, std::vector<prepared_query>& pq
 
 }  // namespace nigiri_amd
 
+namespace nigiri_amd {
+timetable const& engine_timetable(nigiri_amd_engine const* e) { return *e->img->tt; }
+std::uint32_t engine_lanes(nigiri_amd_engine const* e) { return e->W; }
+std::uint32_t engine_slots(nigiri_amd_engine const* e) { return e->S; }
+std::uint32_t engine_max_k(nigiri_amd_engine const* e) { return e->K; }
+}  // namespace nigiri_amd
+
 extern "C" {
 
 int nigiri_amd_image_create(const nigiri_amd_tt* tt, int device, nigiri_amd_image** out) {
@@ -1302,6 +1311,7 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     st.RW = (R + 63U) / 64U;
     auto const cols = static_cast<std::size_t>(G) * L * W;
     st.round_times = m.alloc<std::int16_t>(cols * K);
+    st.bounds = W == 64U ? nullptr : m.alloc<std::int16_t>(cols * K);
     st.best = m.alloc<std::int16_t>(cols);
     st.tmp = m.alloc<std::int32_t>(cols);
     st.lb = m.alloc<std::uint16_t>(cols);
@@ -1785,6 +1795,7 @@ int nigiri_amd_algo_init(nigiri_amd_engine* e, uint32_t slot, const nigiri_algo_
     sp.tts_add = init->tts_additional_time;
     sp.tts_factor = init->tts_factor;
     sp.end_k = static_cast<std::uint8_t>(e->K);
+    sp.bounds_last_k = 0U;
     for (std::uint32_t j = 0; j != init->n_dest; ++j) e->h_dest[slot * kMaxDestBits + j] = init->dest[j];
     for (std::uint32_t j = 0; j != init->n_dist; ++j) {
       e->h_dist_loc[slot * kMaxDist + j] = init->dist_loc[j];
@@ -1867,6 +1878,39 @@ int nigiri_amd_algo_add_start(nigiri_amd_engine* e, uint32_t slot, uint32_t loca
   return NIGIRI_AMD_OK;
 }
 
+int nigiri_amd_algo_fill_bounds(nigiri_amd_engine* e, uint32_t src_slot, uint32_t dst_slot,
+                                uint32_t n_rows) {
+  if (e == nullptr || src_slot >= e->S || dst_slot >= e->S || n_rows == 0U || n_rows > e->K) {
+    set_error("algo_fill_bounds: bad argument");
+    return NIGIRI_AMD_EINVAL;
+  }
+  if (e->W != 1U) {
+    set_error("algo_fill_bounds: bounded searches run on the plain engine (lanes == 1)");
+    return NIGIRI_AMD_EUNSUPPORTED;
+  }
+  return guarded([&]() {
+    HIP_CHECK(hipSetDevice(e->img->device));
+    launch_fill_bounds(e->algo_dir[src_slot] == NIGIRI_AMD_DIR_FWD, e->st, src_slot, dst_slot, n_rows,
+                       e->stream);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(e->stream));
+    return NIGIRI_AMD_OK;
+  });
+}
+
+int nigiri_amd_algo_set_bounds(nigiri_amd_engine* e, uint32_t slot, uint32_t last_round) {
+  if (e == nullptr || slot >= e->S || last_round >= e->K) {
+    set_error("algo_set_bounds: bad argument");
+    return NIGIRI_AMD_EINVAL;
+  }
+  if (e->W != 1U && last_round != 0U) {
+    set_error("algo_set_bounds: bounded searches run on the plain engine (lanes == 1)");
+    return NIGIRI_AMD_EUNSUPPORTED;
+  }
+  e->h_param[slot].bounds_last_k = static_cast<std::uint8_t>(last_round);
+  return NIGIRI_AMD_OK;
+}
+
 int nigiri_amd_algo_execute(nigiri_amd_engine* e, uint32_t slot, int32_t start_time,
                             uint8_t max_transfers, int32_t worst_time_at_dest, int16_t* dest_times,
                             uint32_t* n_rounds) {
@@ -1877,6 +1921,10 @@ int nigiri_amd_algo_execute(nigiri_amd_engine* e, uint32_t slot, int32_t start_t
     auto const fwd = e->algo_dir[slot] == NIGIRI_AMD_DIR_FWD;
     auto& sp = e->h_param[slot];
     auto const end_k = std::min<std::uint32_t>(std::min<std::uint32_t>(max_transfers, 14U) + 2U, e->K);
+    if (sp.bounds_last_k != 0U && end_k - 1U > sp.bounds_last_k) {
+      // within_bounds reads bounds row last_k - k (raptor.h:466-469)
+      throw status_error{NIGIRI_AMD_EINVAL, "algo_execute: rounds beyond the bounds' last round"};
+    }
     sp.end_k = static_cast<std::uint8_t>(end_k);
     auto const base = tt.internal_from + static_cast<std::int32_t>(sp.base) * 1440;
     (void)start_time;

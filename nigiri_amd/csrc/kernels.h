@@ -148,6 +148,10 @@ void launch_gather_slot(std::uint32_t W, dev_state const& st, std::uint32_t slot
                         hipStream_t stream);
 // round_times[k][dest bit] of the listed slots -> st.dest_rows (journey
 // extraction input, raptor.h:393-413)
+// Plain engine: bounds of slot dst from the round times of slot src (rows
+// 0 .. n_rows-1, n_rows <= K); fwd is the direction of src's search.
+void launch_fill_bounds(bool fwd, dev_state const& st, std::uint32_t src, std::uint32_t dst,
+                        std::uint32_t n_rows, hipStream_t stream);
 void launch_extract(std::uint32_t W, dev_state const& st, std::uint32_t const* slots,
                     std::uint32_t n_slots, hipStream_t stream);
 

@@ -202,6 +202,24 @@ __device__ inline std::int16_t tad_at(dev_state const& s, std::uint32_t slot, st
   return v;
 }
 
+// within_bounds (raptor.h:458-489, Vias = 0): with bounds set (a pong
+// search), a new label must not be better than the bound of the other
+// direction's search for the remaining rounds plus the transfer time.
+// Plain engine only; the batched engine runs no bounded searches.
+template <bool Fwd, int W>
+__device__ inline bool within_bounds(dev_image const& im, dev_state const& s, std::uint32_t slot,
+                                     std::uint32_t k, std::uint32_t l, int t) {
+  if constexpr (W != 1) {
+    return true;
+  } else {
+    auto const& p = s.param[slot];
+    if (p.bounds_last_k == 0U) return true;
+    int const b = s.bounds[lay<1>::rt(s, slot, p.bounds_last_k - k, l, 0U)];
+    return dirop<Fwd>::better_eq(
+        t, b + dirop<Fwd>::dir(adjusted<int>(p, static_cast<int>(im.transfer_time[l]))));
+  }
+}
+
 // ======================================================================
 // round_end_check: slots whose rounds are exhausted (k == end_k) stop
 template <int W>
@@ -511,7 +529,8 @@ __device__ inline void scan_route(route_view const& rv, dev_image const& im, dev
     if (!done && et.valid() && can_finish) {
       ride = true;
       auto const by = time_at_stop<Fwd>(rv.ev, n_t, et, stop_idx, Fwd, base);
-      if (D::better_eq(by, tad_k) && lbv != kUnreachableLb && D::better_eq(by + D::dir(lbv), tad_k)) {
+      if (D::better_eq(by, tad_k) && lbv != kUnreachableLb && D::better_eq(by + D::dir(lbv), tad_k) &&
+          within_bounds<Fwd, W>(im, s, g * W + lane_id, k, l, by)) {
         if constexpr (Fwd) {
           atomicMin(&s.tmp[c], static_cast<int>(by));
         } else {
@@ -1241,12 +1260,13 @@ __global__ void k_dest_pass(round_args a, std::uint32_t n_slots) {
       }
     } else {
       // update_transfers / update_footpaths for a destination target
-      if (D::better(e.x, s.best[c])) {
+      if (p.bounds_last_k == 0U && D::better(e.x, s.best[c])) {
         s.round_times[rti] = D::best_of(e.x, s.round_times[rti]);
       }
       if (D::better(e.x, s.best[c]) && D::better_eq(e.x, tad[k])) {
         int const lbv = s.lb[c];
         if (lbv == kUnreachableLb || !D::better_eq(e.x + D::dir(lbv), tad[k])) continue;
+        if (!within_bounds<Fwd, W>(im, s, slot, k, e.target, e.x)) continue;
         s.round_times[rti] = e.x;
         s.best[c] = e.x;
         if constexpr (W == 64) {
@@ -1279,9 +1299,9 @@ constexpr int kRelaxImproved = 1;
 constexpr int kRelaxHazard = 2;
 
 template <bool Fwd, int W>
-__device__ inline int relax(dev_state const& s, std::uint32_t g, std::uint32_t lane,
-                            std::uint32_t slot, std::uint32_t k, std::uint32_t t, int x,
-                            std::uint32_t pos, std::int16_t tad0) {
+__device__ inline int relax(dev_image const& im, dev_state const& s, std::uint32_t g,
+                            std::uint32_t lane, std::uint32_t slot, std::uint32_t k,
+                            std::uint32_t t, int x, std::uint32_t pos, std::int16_t tad0) {
   using D = dirop<Fwd>;
   auto const c = lay<W>::col(s, g, t, lane);
   auto const xv = clamp16(x);
@@ -1289,11 +1309,14 @@ __device__ inline int relax(dev_state const& s, std::uint32_t g, std::uint32_t l
   auto const tad_e = tad_at(s, slot, pos);
   int const lbv = s.lb[c];
   bool const elig = D::better_eq(xv, tad_e) && lbv != kUnreachableLb &&
-                    D::better_eq(xv + D::dir(lbv), tad_e);
+                    D::better_eq(xv + D::dir(lbv), tad_e) &&
+                    within_bounds<Fwd, W>(im, s, slot, k, t, xv);
   if (elig) {
     auto const old = atomic_best16<Fwd>(&s.best[c], xv);
     return D::better(xv, old) ? kRelaxImproved : 0;
   }
+  // no pre-write while bounds are set (raptor.h:709, 790)
+  if (W == 1 && s.param[slot].bounds_last_k != 0U) return 0;
   atomic_best16<Fwd>(&s.round_times[lay<W>::rt(s, g, k, t, lane)], xv);  // pre-write
   return tad_e != tad0 ? kRelaxHazard : 0;
 }
@@ -1425,7 +1448,7 @@ __global__ void k_transfer_fp64(round_args a) {
     // transfer at i (destination stations are handled by k_dest_pass)
     if (valid && !lay<64>::test(s.dest_mask, s, g, i, ln)) {
       auto const tt = adjusted<std::uint8_t>(p, im.transfer_time[i]);
-      emit(i, relax<Fwd, 64>(s, g, ln, slot, k, i, tmp_i + D::dir(tt), kPosTransfer | i, tad0));
+      emit(i, relax<Fwd, 64>(a.im, s, g, ln, slot, k, i, tmp_i + D::dir(tt), kPosTransfer | i, tad0));
     }
     // footpaths of i in list order
     for (auto fe = fe0; fe != fe1; ++fe) {
@@ -1433,7 +1456,7 @@ __global__ void k_transfer_fp64(round_args a) {
       auto const t = fp & ((1U << 23) - 1U);
       if (valid && !lay<64>::test(s.dest_mask, s, g, t, ln)) {
         auto const dur = static_cast<std::int16_t>(fp >> 23);
-        emit(t, relax<Fwd, 64>(s, g, ln, slot, k, t, tmp_i + D::dir(adjusted<std::int16_t>(p, dur)),
+        emit(t, relax<Fwd, 64>(a.im, s, g, ln, slot, k, t, tmp_i + D::dir(adjusted<std::int16_t>(p, dur)),
                                kPosFootpath | fe, tad0));
       }
     }
@@ -1476,7 +1499,7 @@ __global__ void k_transfer_fp1(round_args a) {
     // transfer at i (destination stations are handled by k_dest_pass)
     if (valid && !lay<W>::test(s.dest_mask, s, g, i, 0U)) {
       auto const tt = adjusted<std::uint8_t>(p, im.transfer_time[i]);
-      emit(i, relax<Fwd, W>(s, g, 0U, slot, k, i, tmp_i + D::dir(tt), kPosTransfer | i, tad0));
+      emit(i, relax<Fwd, W>(a.im, s, g, 0U, slot, k, i, tmp_i + D::dir(tt), kPosTransfer | i, tad0));
     }
     // footpaths of i in list order
     for (auto fe = off[i]; fe != off[i + 1U]; ++fe) {
@@ -1484,7 +1507,7 @@ __global__ void k_transfer_fp1(round_args a) {
       auto const t = fp & ((1U << 23) - 1U);
       if (valid && !lay<W>::test(s.dest_mask, s, g, t, 0U)) {
         auto const dur = static_cast<std::int16_t>(fp >> 23);
-        emit(t, relax<Fwd, W>(s, g, 0U, slot, k, t, tmp_i + D::dir(adjusted<std::int16_t>(p, dur)),
+        emit(t, relax<Fwd, W>(a.im, s, g, 0U, slot, k, t, tmp_i + D::dir(adjusted<std::int16_t>(p, dur)),
                               kPosFootpath | fe, tad0));
       }
     }
@@ -1539,9 +1562,11 @@ __device__ inline void replay_hazard(round_args const& a, std::uint32_t g, std::
   auto const c = lay<W>::col(s, g, t, lane);
   auto const bt = s.best[c];
   int const lbv = s.lb[c];
+  // (a slot with bounds makes no pre-writes, so it has no hazards)
   auto const eligible = [&](std::int16_t x, std::uint32_t pos) {
     auto const tad_e = tad_at(s, slot, pos);
-    return D::better_eq(x, tad_e) && lbv != kUnreachableLb && D::better_eq(x + D::dir(lbv), tad_e);
+    return D::better_eq(x, tad_e) && lbv != kUnreachableLb && D::better_eq(x + D::dir(lbv), tad_e) &&
+           within_bounds<Fwd, W>(im, s, slot, k, t, x);
   };
   // enumerate events into t: (pos, x)
   auto const* off = Fwd ? im.fp_in_offsets : im.fp_out_offsets;
@@ -1918,6 +1943,21 @@ __global__ void k_extract(dev_state s, std::uint32_t const* slots, std::uint32_t
 }
 
 // copies one slot's state into contiguous rows (state download)
+// fill_bounds (raptor.h:117-170, Vias = 0, no td footpaths) of the plain
+// engine: bounds row k of slot dst is the best round time of slot src over
+// rounds 0..k; Fwd is the direction of src's search.
+template <bool Fwd>
+__global__ void k_fill_bounds(dev_state s, std::uint32_t src, std::uint32_t dst, std::uint32_t n_rows) {
+  for (std::uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < s.L; x += gridDim.x * blockDim.x) {
+    auto v = s.round_times[lay<1>::rt(s, src, 0U, x, 0U)];
+    s.bounds[lay<1>::rt(s, dst, 0U, x, 0U)] = v;
+    for (std::uint32_t k = 1U; k < n_rows; ++k) {
+      v = dirop<Fwd>::best_of(s.round_times[lay<1>::rt(s, src, k, x, 0U)], v);
+      s.bounds[lay<1>::rt(s, dst, k, x, 0U)] = v;
+    }
+  }
+}
+
 template <int W>
 __global__ void k_gather_slot(dev_state s, std::uint32_t slot, std::int16_t* rt, std::int16_t* best,
                               std::int16_t* tmp, std::uint16_t* lb) {
@@ -2396,6 +2436,16 @@ void launch_extract(std::uint32_t W, dev_state const& st, std::uint32_t const* s
     hipLaunchKernelGGL((k_extract<64>), dim3(grid_for(work, kBlock, 4096)), dim3(kBlock), 0, stream, st, slots, n_slots);
   } else {
     hipLaunchKernelGGL((k_extract<1>), dim3(grid_for(work, kBlock, 4096)), dim3(kBlock), 0, stream, st, slots, n_slots);
+  }
+}
+
+void launch_fill_bounds(bool fwd, dev_state const& st, std::uint32_t src, std::uint32_t dst,
+                        std::uint32_t n_rows, hipStream_t stream) {
+  auto const grid = dim3(grid_for(st.L, kBlock, 4096));
+  if (fwd) {
+    hipLaunchKernelGGL((k_fill_bounds<true>), grid, dim3(kBlock), 0, stream, st, src, dst, n_rows);
+  } else {
+    hipLaunchKernelGGL((k_fill_bounds<false>), grid, dim3(kBlock), 0, stream, st, src, dst, n_rows);
   }
 }
 

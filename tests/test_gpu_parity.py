@@ -387,3 +387,91 @@ def test_gpu_full_size_workload5_mix_sample():
     qs = na.Queries.generate(tt, 5, 12, seed=0x7172 + 73)
     say("full-size workload5 mix n=12")
     compare(tt, na.Engine(img, lanes=64, groups=1, max_k=10), qs)
+
+
+def compare_pong(tt, eng, q, check_work=False):
+    """The plain engine's pong against the oracle's: journeys, legs, interval and
+    the number of executes; with check_work also the route-scan work counters,
+    which the ping bounds prune."""
+    if check_work:
+        eng.stats(reset=True)
+    got = eng.pong(q)
+    ref = oracle.pong(tt, q)
+    assert got["status"] == 0
+    assert got["journeys"] == ref["journeys"], (got["journeys"], ref["journeys"])
+    assert got["legs"] == ref["legs"]
+    assert got["interval"] == ref["interval"]
+    assert got["n_epochs"] == ref["stats"]["n_epochs"]
+    if check_work:
+        w = eng.stats()["work"]
+        s = ref["stats"]
+        assert (w["route_stops"], w["events"], w["route_day_tests"], w["transport_day_tests"]) == (
+            s["n_route_stops_visited"], s["n_events_scanned"], s["n_route_day_tests"],
+            s["n_transport_day_tests"])
+    return got
+
+
+def test_gpu_pong_golden():
+    # pong_minlookahead_test.cc and pong_gap_truncation_test.cc on the plain
+    # engine, plus the golden range queries of the A-B-C network
+    from golden import pong as P
+    from test_oracle_pong import line_tt
+    say("pong golden: min lookahead")
+    tt, loc, day0 = line_tt(P.MINLOOKAHEAD)
+    eng = na.Engine(na.Image(tt), lanes=1, groups=2, max_k=16)
+    for when, max_tt, want in P.MINLOOKAHEAD["CASES"]:
+        spec = dict(start=[(loc["S0"], 0)], dest=[(loc["S2"], 0)], max_travel_time=max_tt,
+                    min_connection_count=1, extend_interval_later=1)
+        if isinstance(when, tuple):
+            spec["interval"] = (day0 + hm(when[0]), day0 + hm(when[1]))
+        else:
+            spec["time"] = day0 + hm(when)
+        got = compare_pong(tt, eng, na.Queries.build([spec])[0])
+        assert [j[:2] for j in got["journeys"]] == [(day0 + hm(d), day0 + hm(a)) for d, a, _ in want]
+    say("pong golden: gap truncation")
+    tt, loc, day0 = line_tt(P.GAP)
+    eng = na.Engine(na.Image(tt), lanes=1, groups=2, max_k=16)
+    iv = (day0 + hm(P.GAP["INTERVAL"][0]), day0 + hm(P.GAP["INTERVAL"][1]))
+    q = na.Queries.build([dict(start=[(loc["A"], 0)], dest=[(loc["B"], 0)], interval=iv,
+                               min_connection_count=P.GAP["MIN_CONNECTIONS"],
+                               extend_interval_later=1)])[0]
+    got = compare_pong(tt, eng, q)
+    assert sorted(j[0] for j in got["journeys"]) == [day0 + hm(t) for t in P.GAP["EXPECTED_DEPARTURES"]]
+    say("pong golden: abc range and intermodal, both directions")
+    tt, loc, day0 = abc_timetable()
+    eng = na.Engine(na.Image(tt), lanes=1, groups=2, max_k=16)
+    iv = (day0 + hm(G.INTERVAL[0]), day0 + hm(G.INTERVAL[1]))
+    (s, sd), (d, dd) = G.INTERMODAL_START, G.INTERMODAL_DEST
+    specs = [dict(start=[(loc["A"], 0)], dest=[(loc["C"], 0)], interval=iv),
+             dict(start=[(loc["C"], 0)], dest=[(loc["A"], 0)], interval=iv, dir=na.DIR_BWD),
+             dict(start=[(loc[s], sd)], dest=[(loc[d], dd)], interval=iv,
+                  start_match_mode=na.MATCH_INTERMODAL, dest_match_mode=na.MATCH_INTERMODAL)]
+    qs = na.Queries.build(specs)
+    for i in range(len(qs)):
+        assert len(compare_pong(tt, eng, qs[i])["journeys"]) > 0
+
+
+@pytest.mark.parametrize("direction", [na.DIR_FWD, na.DIR_BWD])
+def test_gpu_pong_generated(direction):
+    # range queries of the workload-3 generator (2k stops): pong on the plain
+    # engine equals the oracle's pong, work counters included
+    tt, img = image_for("w3p", lambda: na.Timetable.generate(3, scale=0.01))
+    eng = na.Engine(img, lanes=1, groups=2, max_k=10)
+    qs = na.Queries.generate(tt, 3, 48, seed=0x7172 + 61)
+    idx = [i for i in range(len(qs)) if qs[i].is_interval][:10]
+    n = 0
+    for i in idx:
+        qs[i].dir = direction
+        say(f"pong workload3 small dir={direction} query {i}")
+        n += len(compare_pong(tt, eng, qs[i], check_work=True)["journeys"])
+    assert n > 0
+
+
+def test_gpu_pong_needs_the_plain_engine():
+    tt, loc, day0 = abc_timetable()
+    eng = na.Engine(na.Image(tt), lanes=64, groups=1, max_k=16)
+    q = na.Queries.build([dict(start=[(loc["A"], 0)], dest=[(loc["C"], 0)],
+                               interval=(day0 + hm(G.INTERVAL[0]), day0 + hm(G.INTERVAL[1])))])[0]
+    with pytest.raises(na.NigiriError) as e:
+        eng.pong(q)
+    assert e.value.code == na.EUNSUPPORTED

@@ -121,6 +121,81 @@ def serve_main(args):
     }), flush=True)
 
 
+def pong_main(args):
+    """Range queries by pong (nigiri's pong_search: ping/pong pairs, the pong
+    pruned by the ping's round times) on the plain engine, beside the range
+    search of the same queries (plain and batched engine) and the oracle's
+    pong and range search on one CPU thread; checks that both algorithms find
+    the same journeys in each query's interval."""
+    import nigiri_amd as na
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    cfg = args.workload
+    tt = na.Timetable.generate(cfg, scale=args.scale)
+    pool = na.Queries.generate(tt, cfg, 8 * args.pong)
+    idx = [i for i in range(len(pool)) if pool[i].is_interval][:args.pong]
+    qs = pool.subset(idx)
+    n = len(qs)
+    img = na.Image(tt)
+    max_k = 16 if cfg in (1, 2) else 10
+    plain = na.Engine(img, lanes=1, groups=2, max_k=max_k)
+    plain.pong(qs[0])  # warm-up
+    print(f"[bench] pong: {n} range queries, warm-up done", file=sys.stderr, flush=True)
+
+    def in_window(q, journeys):
+        return sorted((min(j[:2]), max(j[:2]), j[3]) for j in journeys
+                      if q.interval_from <= (min(j[:2]) if q.dir == na.DIR_FWD else max(j[:2])) < q.interval_to)
+
+    pong_ms, pong_exec, pong_res = [], [], []
+    for i in range(n):
+        t = time.perf_counter()
+        r = plain.pong(qs[i])
+        pong_ms.append((time.perf_counter() - t) * 1e3)
+        pong_exec.append(r["n_epochs"])
+        pong_res.append(r["journeys"])
+    print("[bench] pong on the plain engine done", file=sys.stderr, flush=True)
+    range_ms, range_res = [], []
+    for i in range(n):
+        t = time.perf_counter()
+        r = plain.search(qs.subset([i]))[0]
+        range_ms.append((time.perf_counter() - t) * 1e3)
+        range_res.append(r["journeys"])
+    print("[bench] range search on the plain engine done", file=sys.stderr, flush=True)
+    batched = na.Engine(img, lanes=64, groups=1, max_k=max_k)
+    batched.search(qs.subset([0]))
+    t = time.perf_counter()
+    batched.search(qs)
+    batched_ms = (time.perf_counter() - t) * 1e3
+    agree = sum(in_window(qs[i], pong_res[i]) == in_window(qs[i], range_res[i]) for i in range(n))
+    cpu = {}
+    if not args.no_cpu:
+        m = min(n, args.pong_cpu)
+        t = time.perf_counter()
+        for i in range(m):
+            oracle.pong(tt, qs[i])
+        cpu["pong_qps"] = round(m / (time.perf_counter() - t), 2)
+        t = time.perf_counter()
+        for i in range(m):
+            oracle.search(tt, qs[i])
+        cpu["range_qps"] = round(m / (time.perf_counter() - t), 2)
+        cpu.update(cores=1, kind="port", sample=f"first {m} of the {n} range queries, one thread")
+
+    def stat(ms):
+        a = np.array(ms)
+        return {"qps": round(len(a) / (a.sum() / 1e3), 2), "p50_ms": round(float(np.percentile(a, 50)), 1),
+                "p99_ms": round(float(np.percentile(a, 99)), 1)}
+
+    print(json.dumps({
+        "metric": "pong range queries/s (ping/pong searches, one query at a time), 1 MI355X",
+        "workload": f"config {cfg}: " + WORKLOADS[cfg] + " (its range queries)", "scale": args.scale,
+        "queries": n, "pong_plain_gpu": {**stat(pong_ms), "executes_per_query": round(float(np.mean(pong_exec)), 1)},
+        "range_plain_gpu": stat(range_ms),
+        "range_batched_gpu": {"qps": round(n / (batched_ms / 1e3), 2), "batch": n},
+        "cpu_oracle": cpu or None,
+        "pong_equals_range_in_interval": f"{agree}/{n}",
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +228,10 @@ def main():
                          "batch's rounds")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes instead of autotuning them on the first batch")
+    ap.add_argument("--pong", type=int, default=0, metavar="N",
+                    help="measure pong range search on N range queries of the workload instead")
+    ap.add_argument("--pong-cpu", type=int, default=4, metavar="M",
+                    help="range queries timed on the CPU oracle in --pong mode")
     ap.add_argument("--eager", action="store_true",
                     help="launch each round's kernels one by one instead of replaying captured hipGraphs")
     args = ap.parse_args()
@@ -179,6 +258,9 @@ def main():
 
     if args.serve > 0.0:
         serve_main(args)
+        return
+    if args.pong > 0:
+        pong_main(args)
         return
 
     import nigiri_amd as na

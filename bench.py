@@ -154,6 +154,36 @@ def pong_main(args):
         pong_exec.append(r["n_epochs"])
         pong_res.append(r["journeys"])
     print("[bench] pong on the plain engine done", file=sys.stderr, flush=True)
+    concurrent = None
+    if args.pong_threads > 1:
+        # independent plain engines (own state and stream) driven from host
+        # threads, as nigiri's benchmark runs several GPU states at once
+        import threading
+        engines = [na.Engine(img, lanes=1, groups=2, max_k=max_k) for _ in range(args.pong_threads)]
+        for e in engines:
+            e.pong(qs[0])
+        got = [None] * n
+        nxt = iter(range(n))
+        lock = threading.Lock()
+
+        def work(e):
+            while True:
+                with lock:
+                    i = next(nxt, None)
+                if i is None:
+                    return
+                got[i] = e.pong(qs[i])["journeys"]
+
+        ths = [threading.Thread(target=work, args=(e,)) for e in engines]
+        t = time.perf_counter()
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        wall = time.perf_counter() - t
+        concurrent = {"threads": args.pong_threads, "qps": round(n / wall, 2),
+                      "results_equal_sequential": got == pong_res}
+        print("[bench] concurrent pong done", file=sys.stderr, flush=True)
     range_ms, range_res = [], []
     for i in range(n):
         t = time.perf_counter()
@@ -189,6 +219,7 @@ def pong_main(args):
         "metric": "pong range queries/s (ping/pong searches, one query at a time), 1 MI355X",
         "workload": f"config {cfg}: " + WORKLOADS[cfg] + " (its range queries)", "scale": args.scale,
         "queries": n, "pong_plain_gpu": {**stat(pong_ms), "executes_per_query": round(float(np.mean(pong_exec)), 1)},
+        "pong_plain_gpu_concurrent": concurrent,
         "range_plain_gpu": stat(range_ms),
         "range_batched_gpu": {"qps": round(n / (batched_ms / 1e3), 2), "batch": n},
         "cpu_oracle": cpu or None,
@@ -230,6 +261,8 @@ def main():
                     help="keep the default launch shapes instead of autotuning them on the first batch")
     ap.add_argument("--pong", type=int, default=0, metavar="N",
                     help="measure pong range search on N range queries of the workload instead")
+    ap.add_argument("--pong-threads", type=int, default=8, metavar="T",
+                    help="--pong mode: also run the queries on T plain engines from T host threads")
     ap.add_argument("--pong-cpu", type=int, default=4, metavar="M",
                     help="range queries timed on the CPU oracle in --pong mode")
     ap.add_argument("--eager", action="store_true",

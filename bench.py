@@ -139,6 +139,7 @@ def pong_main(args):
     img = na.Image(tt)
     max_k = 16 if cfg in (1, 2) else 10
     plain = na.Engine(img, lanes=1, groups=2, max_k=max_k)
+    plain.configure(graphs=not args.eager)  # the range search below replays graphs otherwise
     plain.pong(qs[0])  # warm-up
     print(f"[bench] pong: {n} range queries, warm-up done", file=sys.stderr, flush=True)
 
@@ -192,6 +193,7 @@ def pong_main(args):
         range_res.append(r["journeys"])
     print("[bench] range search on the plain engine done", file=sys.stderr, flush=True)
     batched = na.Engine(img, lanes=64, groups=1, max_k=max_k)
+    batched.configure(graphs=not args.eager)
     batched.search(qs.subset([0]))
     t = time.perf_counter()
     batched.search(qs)

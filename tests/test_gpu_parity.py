@@ -475,3 +475,34 @@ def test_gpu_pong_needs_the_plain_engine():
     with pytest.raises(na.NigiriError) as e:
         eng.pong(q)
     assert e.value.code == na.EUNSUPPORTED
+
+
+def test_gpu_pong_on_concurrent_engines():
+    # plain engines driven from several host threads at once (each with its
+    # own state and stream) give each query the oracle's pong result
+    import threading
+    tt, img = image_for("w3p", lambda: na.Timetable.generate(3, scale=0.01))
+    qs = na.Queries.generate(tt, 3, 96, seed=0x7172 + 71)
+    idx = [i for i in range(len(qs)) if qs[i].is_interval][:12]
+    engines = [na.Engine(img, lanes=1, groups=2, max_k=10) for _ in range(4)]
+    got = {}
+    errors = []
+
+    def work(e, part):
+        try:
+            for i in part:
+                got[i] = e.pong(qs[i])
+        except Exception as ex:  # reported below
+            errors.append(ex)
+
+    say(f"pong on 4 engines from 4 threads, {len(idx)} queries")
+    ths = [threading.Thread(target=work, args=(e, idx[k::4])) for k, e in enumerate(engines)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    assert not errors, errors
+    for i in idx:
+        ref = oracle.pong(tt, qs[i])
+        assert got[i]["journeys"] == ref["journeys"]
+        assert got[i]["legs"] == ref["legs"]

@@ -127,6 +127,10 @@ def pong_main(args):
     search of the same queries (plain and batched engine) and the oracle's
     pong and range search on one CPU thread; checks that both algorithms find
     the same journeys in each query's interval."""
+    # HIP maps streams onto 4 hardware queues by default, which caps how many
+    # engines' kernels overlap; read once when HIP initialises (below)
+    if args.pong_threads > 4 and int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
+        os.environ["GPU_MAX_HW_QUEUES"] = "16"
     import nigiri_amd as na
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -183,6 +187,7 @@ def pong_main(args):
             th.join()
         wall = time.perf_counter() - t
         concurrent = {"threads": args.pong_threads, "qps": round(n / wall, 2),
+                      "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
                       "results_equal_sequential": got == pong_res}
         print("[bench] concurrent pong done", file=sys.stderr, flush=True)
     range_ms, range_res = [], []
@@ -263,7 +268,7 @@ def main():
                     help="keep the default launch shapes instead of autotuning them on the first batch")
     ap.add_argument("--pong", type=int, default=0, metavar="N",
                     help="measure pong range search on N range queries of the workload instead")
-    ap.add_argument("--pong-threads", type=int, default=8, metavar="T",
+    ap.add_argument("--pong-threads", type=int, default=16, metavar="T",
                     help="--pong mode: also run the queries on T plain engines from T host threads")
     ap.add_argument("--pong-cpu", type=int, default=4, metavar="M",
                     help="range queries timed on the CPU oracle in --pong mode")

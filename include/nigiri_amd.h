@@ -123,7 +123,7 @@ typedef struct nigiri_query {
   int16_t max_travel_time;   /* default 7200                                */
   uint8_t max_transfers;     /* default 14 (kMaxTransfers)                  */
   uint8_t pad_[3];
-  uint32_t min_connection_count; /* only 0 is supported                     */
+  uint32_t min_connection_count; /* widens the interval (search.h:237-317) */
   int32_t extend_interval_earlier;
   int32_t extend_interval_later;
   int32_t prf_idx;               /* only 0 is supported                     */

@@ -135,6 +135,19 @@ typedef struct nigiri_query {
   int16_t fastest_direct;
   int16_t pad2_;
   double fastest_direct_factor;  /* default 1.0                             */
+  /* query::max_interval_ (query.h:89, optional): the search interval, as
+   * widened for min_connection_count, is clamped into [from, to)
+   * (interval_estimate.h:132-145)                                           */
+  int32_t has_max_interval;
+  int32_t max_interval_from;     /* unix minutes                            */
+  int32_t max_interval_to;
+  /* the timeout argument of raptor_search / pong_search (search.h:179,
+   * 227-244; pong.cc:68, 235-245), optional, in seconds: once it has passed
+   * since the first start time began, the interval is not extended further
+   * (search) or no further ping starts (pong)                               */
+  int32_t has_timeout;
+  int32_t timeout_s;
+  int32_t pad3_;
 } nigiri_query;
 
 /* A journey of the result Pareto set (journey.h:104-111).  Its legs, in
@@ -210,6 +223,31 @@ int nigiri_amd_tt_generate(int cfg, uint64_t seed, double scale,
  * routes, route traffic days, footpaths_in and the lb graphs are rebuilt as
  * nigiri's finalize() does (src/loader/init_finish.cc:200-234). */
 int nigiri_amd_tt_from_arrays(const nigiri_tt_view* v, nigiri_amd_tt** out);
+/* Loads a GTFS feed given as files in memory, as nigiri's loader::mem_dir
+ * holds them (file_names[i] such as "stops.txt", file_contents[i] its text),
+ * for the date range [date_from_day, date_to_day) in days since 1970-01-01
+ * (timetable::date_range_), the way nigiri's GTFS loader and finalize() do
+ * (replaces loader::gtfs::load_timetable, src/loader/gtfs/load_timetable.cc:
+ * 84-545, and loader::finalize, src/loader/init_finish.cc:200-234).
+ * tz_dir: directory of TZif time-zone files (NULL or "": $TZDIR, else
+ * /usr/share/zoneinfo).  Feeds with frequencies, stay-seated transfers,
+ * block_id trips, flex or stop times that need interpolation are refused
+ * (NIGIRI_AMD_EUNSUPPORTED).  Footpaths come from transfers.txt only. */
+int nigiri_amd_tt_load_gtfs(const char* const* file_names,
+                            const char* const* file_contents, uint32_t n_files,
+                            int32_t date_from_day, int32_t date_to_day,
+                            const char* tz_dir, nigiri_amd_tt** out);
+/* Feed ids of a loaded feed: the stop_id of location l (special stations:
+ * their nigiri names, special_stations.h:7-22) and the trip_id of transport
+ * t, NUL-terminated into buf[0 .. cap); the location of a stop_id
+ * (timetable::locations_.location_id_to_idx_).  NIGIRI_AMD_EINVAL for a
+ * timetable without ids or an unknown id. */
+int nigiri_amd_tt_location_id(const nigiri_amd_tt* tt, uint32_t l, char* buf,
+                              uint32_t cap);
+int nigiri_amd_tt_find_location(const nigiri_amd_tt* tt, const char* stop_id,
+                                uint32_t* l);
+int nigiri_amd_tt_transport_trip_id(const nigiri_amd_tt* tt, uint32_t t,
+                                    char* buf, uint32_t cap);
 int nigiri_amd_tt_view(const nigiri_amd_tt* tt, nigiri_tt_view* out);
 void nigiri_amd_tt_free(nigiri_amd_tt* tt);
 

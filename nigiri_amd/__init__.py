@@ -68,6 +68,9 @@ class Query(C.Structure):
         ("tts_additional_time", C.c_int16), ("tts_factor", C.c_float),
         ("has_fastest_direct", C.c_int32), ("fastest_direct", C.c_int16), ("pad2_", C.c_int16),
         ("fastest_direct_factor", C.c_double),
+        ("has_max_interval", C.c_int32), ("max_interval_from", C.c_int32),
+        ("max_interval_to", C.c_int32), ("has_timeout", C.c_int32), ("timeout_s", C.c_int32),
+        ("pad3_", C.c_int32),
     ]
 
 
@@ -152,6 +155,11 @@ SIGNATURES = {
     "nigiri_amd_tt_generate": (C.c_int, [C.c_int, C.c_uint64, C.c_double, C.POINTER(_P)]),
     "nigiri_amd_tt_from_arrays": (C.c_int, [C.POINTER(TtView), C.POINTER(_P)]),
     "nigiri_amd_tt_view": (C.c_int, [_P, C.POINTER(TtView)]),
+    "nigiri_amd_tt_load_gtfs": (C.c_int, [C.POINTER(C.c_char_p), C.POINTER(C.c_char_p), C.c_uint32,
+                                          C.c_int32, C.c_int32, C.c_char_p, C.POINTER(_P)]),
+    "nigiri_amd_tt_location_id": (C.c_int, [_P, C.c_uint32, C.c_char_p, C.c_uint32]),
+    "nigiri_amd_tt_find_location": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_uint32)]),
+    "nigiri_amd_tt_transport_trip_id": (C.c_int, [_P, C.c_uint32, C.c_char_p, C.c_uint32]),
     "nigiri_amd_tt_free": (None, [_P]),
     "nigiri_amd_queries_generate": (C.c_int, [_P, C.c_int, C.c_uint64, C.c_uint32,
                                               C.POINTER(Query), C.POINTER(C.c_uint32),
@@ -242,6 +250,19 @@ _VIEW_ARRAYS = {
 }
 
 
+def parse_mem_dir(text):
+    """Splits text in the format of nigiri's loader::mem_dir::read into {file name: content}:
+    a line "# name" starts a file, the lines up to the next such line are its content."""
+    files, cur = {}, None
+    for line in text.splitlines():
+        if line.startswith("# "):
+            cur = line[2:].strip()
+            files[cur] = []
+        elif cur is not None:
+            files[cur].append(line)
+    return {k: "\n".join(v).strip("\n") + "\n" for k, v in files.items()}
+
+
 class Timetable:
     """Host timetable (nigiri's RAPTOR arrays as CSR)."""
 
@@ -292,6 +313,49 @@ class Timetable:
         h = C.c_void_p()
         _check(lib().nigiri_amd_tt_from_arrays(C.byref(v), C.byref(h)))
         return cls(h)
+
+    @classmethod
+    def load_gtfs(cls, files, date_from, date_to, tz_dir=None):
+        """Loads a GTFS feed (see nigiri_amd_tt_load_gtfs).  files: dict of file name ->
+        text, or a string in the format of nigiri's loader::mem_dir::read ("# name.txt"
+        lines starting each file).  date_from / date_to: (y, m, d) of
+        timetable::date_range_ ([from, to)).  tz_dir defaults to the tzdata package's
+        zoneinfo directory when it is installed."""
+        import datetime
+        if isinstance(files, str):
+            files = parse_mem_dir(files)
+        if tz_dir is None:
+            try:
+                import tzdata
+                tz_dir = os.path.join(os.path.dirname(tzdata.__file__), "zoneinfo")
+            except ImportError:
+                tz_dir = ""
+        names = (C.c_char_p * max(len(files), 1))(*[k.encode() for k in files])
+        texts = (C.c_char_p * max(len(files), 1))(*[v.encode() for v in files.values()])
+        epoch = datetime.date(1970, 1, 1)
+        d0 = (datetime.date(*date_from) - epoch).days
+        d1 = (datetime.date(*date_to) - epoch).days
+        h = C.c_void_p()
+        _check(lib().nigiri_amd_tt_load_gtfs(names, texts, len(files), d0, d1, tz_dir.encode(),
+                                             C.byref(h)))
+        return cls(h)
+
+    def location(self, stop_id):
+        """Location index of a GTFS stop_id (loaded feeds only)."""
+        out = C.c_uint32()
+        _check(lib().nigiri_amd_tt_find_location(self.handle, stop_id.encode(), C.byref(out)))
+        return out.value
+
+    def location_id(self, l):
+        buf = C.create_string_buffer(256)
+        _check(lib().nigiri_amd_tt_location_id(self.handle, l, buf, 256))
+        return buf.value.decode()
+
+    def trip_id(self, t):
+        """GTFS trip_id of transport t (loaded feeds only)."""
+        buf = C.create_string_buffer(256)
+        _check(lib().nigiri_amd_tt_transport_trip_id(self.handle, t, buf, 256))
+        return buf.value.decode()
 
     def array(self, name):
         dtype, n = _VIEW_ARRAYS[name]
@@ -371,6 +435,12 @@ class Queries:
             if "fastest_direct" in s:
                 q.has_fastest_direct = 1
                 q.fastest_direct = s["fastest_direct"]
+            if "max_interval" in s:
+                q.has_max_interval = 1
+                q.max_interval_from, q.max_interval_to = s["max_interval"]
+            if "timeout_s" in s:
+                q.has_timeout = 1
+                q.timeout_s = s["timeout_s"]
             if "interval" in s:
                 q.is_interval = 1
                 q.interval_from, q.interval_to = s["interval"]
@@ -431,7 +501,8 @@ def _collect(res, jr, n, lg=None):
                 legs.append([lg[j.leg_offset + y].as_tuple() for y in range(j.n_legs)])
         out.append({"status": r.status, "journeys": js, "legs": legs, "n_epochs": r.n_epochs,
                     "base_day": r.base_day, "fastest_direct": r.fastest_direct, "slot": r.slot,
-                    "latency_ms": r.latency_ms, "queue_ms": r.queue_ms})
+                    "latency_ms": r.latency_ms, "queue_ms": r.queue_ms,
+                    "interval_from": r.interval_from, "interval_to": r.interval_to})
     return out
 
 

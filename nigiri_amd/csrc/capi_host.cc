@@ -5,6 +5,7 @@
 
 #include "capi_util.h"
 #include "generator.h"
+#include "gtfs.h"
 #include "handles.h"
 #include "timetable.h"
 
@@ -90,6 +91,65 @@ int nigiri_amd_tt_from_arrays(const nigiri_tt_view* v, nigiri_amd_tt** out) {
     *out = h.release();
     return NIGIRI_AMD_OK;
   });
+}
+
+int nigiri_amd_tt_load_gtfs(const char* const* file_names, const char* const* file_contents, uint32_t n_files,
+                            int32_t date_from_day, int32_t date_to_day, const char* tz_dir, nigiri_amd_tt** out) {
+  if (out == nullptr || (n_files != 0U && (file_names == nullptr || file_contents == nullptr))) {
+    set_error("tt_load_gtfs: null argument");
+    return NIGIRI_AMD_EINVAL;
+  }
+  return guarded([&]() {
+    std::vector<std::pair<std::string, std::string>> files;
+    for (uint32_t i = 0; i != n_files; ++i) {
+      if (file_names[i] == nullptr || file_contents[i] == nullptr) {
+        throw status_error{NIGIRI_AMD_EINVAL, "tt_load_gtfs: null file"};
+      }
+      files.emplace_back(file_names[i], file_contents[i]);
+    }
+    auto h = std::make_unique<nigiri_amd_tt>();
+    gtfs_names names;
+    h->tt = load_gtfs(files, date_from_day, date_to_day, tz_dir == nullptr ? "" : tz_dir, names);
+    auto const err = validate(h->tt);
+    if (!err.empty()) throw status_error{NIGIRI_AMD_EINVAL, "tt_load_gtfs: timetable invalid: " + err};
+    h->location_ids = std::move(names.location_ids);
+    h->transport_trip_ids = std::move(names.transport_trip);
+    *out = h.release();
+    return NIGIRI_AMD_OK;
+  });
+}
+
+namespace {
+int copy_id(std::vector<std::string> const& ids, uint32_t i, char* buf, uint32_t cap) {
+  if (i >= ids.size() || buf == nullptr || cap <= ids[i].size()) {
+    set_error("id lookup: index out of range or buffer too small");
+    return NIGIRI_AMD_EINVAL;
+  }
+  std::memcpy(buf, ids[i].c_str(), ids[i].size() + 1U);
+  return NIGIRI_AMD_OK;
+}
+}  // namespace
+
+int nigiri_amd_tt_location_id(const nigiri_amd_tt* tt, uint32_t l, char* buf, uint32_t cap) {
+  if (tt == nullptr) return NIGIRI_AMD_EINVAL;
+  return copy_id(tt->location_ids, l, buf, cap);
+}
+
+int nigiri_amd_tt_find_location(const nigiri_amd_tt* tt, const char* stop_id, uint32_t* l) {
+  if (tt == nullptr || stop_id == nullptr || l == nullptr) return NIGIRI_AMD_EINVAL;
+  for (std::size_t i = kNSpecialStations; i < tt->location_ids.size(); ++i) {
+    if (tt->location_ids[i] == stop_id) {
+      *l = static_cast<uint32_t>(i);
+      return NIGIRI_AMD_OK;
+    }
+  }
+  set_error(std::string{"find_location: unknown stop_id "} + stop_id);
+  return NIGIRI_AMD_EINVAL;
+}
+
+int nigiri_amd_tt_transport_trip_id(const nigiri_amd_tt* tt, uint32_t t, char* buf, uint32_t cap) {
+  if (tt == nullptr) return NIGIRI_AMD_EINVAL;
+  return copy_id(tt->transport_trip_ids, t, buf, cap);
 }
 
 int nigiri_amd_tt_view(const nigiri_amd_tt* tt, nigiri_tt_view* out) {

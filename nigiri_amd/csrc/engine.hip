@@ -697,6 +697,7 @@ struct runner {
       if (si.is_new) {
         rt_mask[g] |= 1ULL << lane;
         new_slots.push_back(s);
+        mark_execute_start(p);
         // slot parameters
         auto& sp = e.h_param[s];
         sp.tts_min = p.tts_.min_transfer_time_;

@@ -216,10 +216,16 @@ int nigiri_amd_pong(nigiri_amd_engine* e, const nigiri_query* q, uint32_t ping_s
                (include_too_slow || (j.travel_time() < fastest_direct && j.travel_time() <= max_travel_time));
       }));
     };
+    // is_timeout_reached (pong.cc:235-241), measured from the call's start
+    // (pong.cc:83)
+    auto const timeout_reached = [&]() {
+      return q->has_timeout != 0 &&
+             std::chrono::steady_clock::now() - t0 >= std::chrono::seconds{q->timeout_s};
+    };
     std::vector<nigiri_leg> leg_buf(256U);
     while ((is_better(start_time, end_time) ||
             result_count(true) + result_count(false) < 2U * q->min_connection_count) &&
-           start_time >= tt.date_range_from && start_time < tt.date_range_to) {
+           start_time >= tt.date_range_from && start_time < tt.date_range_to && !timeout_reached()) {
       check(nigiri_amd_algo_reset_arrivals(e, ping.slot));
       auto const starts = ontrip_starts(tt, fwd, q->start_loc, q->start_dur, q->n_start,
                                         q->use_start_footpaths != 0, raw, start_time);

@@ -240,6 +240,10 @@ bool can_extend_later(prepared_query const& p, std::int32_t to) {
 void clamp_interval(prepared_query const& p, std::int32_t& from, std::int32_t& to) {
   from = std::clamp(std::clamp(from, p.ext_from_, p.ext_to_), p.max_from_, p.max_to_);
   to = std::clamp(std::clamp(to, p.ext_from_, p.ext_to_), p.max_from_, p.max_to_);
+  if (p.has_max_interval_) {  // interval_estimate.h:142-145
+    from = std::clamp(from, p.max_iv_from_, p.max_iv_to_);
+    to = std::clamp(to, p.max_iv_from_, p.max_iv_to_);
+  }
 }
 
 unsigned results_in_interval(prepared_query const& p) {  // search.h:382-391
@@ -266,8 +270,21 @@ void early_termination(prepared_query& p, std::size_t epoch_idx) {
   p.epochs_.resize(epoch_idx + 1U);
 }
 
+void mark_execute_start(prepared_query& p) {
+  if (p.exec_started_) return;
+  p.exec_started_ = true;
+  p.exec_start_ = std::chrono::steady_clock::now();
+}
+
+bool timeout_reached(prepared_query const& p) {
+  return p.has_timeout_ &&
+         (!p.exec_started_ ||
+          std::chrono::steady_clock::now() - p.exec_start_ >= std::chrono::seconds{p.timeout_s_});
+}
+
 bool wants_extension(prepared_query const& p) {
-  return !p.results_.empty() && !p.ontrip_ && results_in_interval(p) < p.min_connection_count_;
+  return !p.results_.empty() && !p.ontrip_ && results_in_interval(p) < p.min_connection_count_ &&
+         !timeout_reached(p);
 }
 
 bool extend_interval(timetable const& tt, prepared_query& p) {
@@ -465,6 +482,11 @@ void prepare(timetable const& tt, nigiri_query const& q, prepared_query& p,
   p.extend_later_ = q.extend_interval_later != 0;
   p.ext_from_ = tt.date_range_from;
   p.ext_to_ = tt.date_range_to;
+  p.has_max_interval_ = q.has_max_interval != 0;
+  p.max_iv_from_ = q.max_interval_from;
+  p.max_iv_to_ = q.max_interval_to;
+  p.has_timeout_ = q.has_timeout != 0;
+  p.timeout_s_ = q.timeout_s;
   {
     constexpr std::int64_t kMaxSearchIntervalSize = (32767 / 1440 - 7) * 1440;  // limits.h:11-13
     auto const from = static_cast<std::int64_t>(p.iv_from_), to = static_cast<std::int64_t>(p.iv_to_);

@@ -5,6 +5,7 @@
 // destination round times of each epoch into the final Pareto set.
 #pragma once
 
+#include <chrono>
 #include <cstdint>
 #include <utility>
 #include <vector>
@@ -70,6 +71,13 @@ struct prepared_query {
   bool extend_earlier_{false}, extend_later_{false};
   std::int32_t ext_from_{0}, ext_to_{0};  // timetable's external interval
   std::int32_t max_from_{0}, max_to_{0};  // interval the int16 deltas can hold
+  bool has_max_interval_{false};          // query::max_interval_ (query.h:89)
+  std::int32_t max_iv_from_{0}, max_iv_to_{0};
+  // timeout (search.h:179, 227-244): measured from the first epoch's start
+  bool has_timeout_{false};
+  std::int32_t timeout_s_{0};
+  bool exec_started_{false};
+  std::chrono::steady_clock::time_point exec_start_{};
   std::int16_t max_start_offset_{7200};
   // shortest start offset per location (get_starts, start_times.cc:251-271)
   std::vector<std::pair<std::uint32_t, std::int16_t>> start_map_;
@@ -127,8 +135,16 @@ void reconstruction_candidates(prepared_query const& p, std::vector<std::size_t>
 // epochs are dropped and the search interval shrinks to the epoch's start.
 void early_termination(prepared_query& p, std::size_t epoch_idx);
 
+// Marks the start of search::execute's processing (search.h:226): the
+// point the timeout is measured from.  Only the first call counts.
+void mark_execute_start(prepared_query& p);
+
+// is_timeout_reached (search.h:227-235).
+bool timeout_reached(prepared_query const& p);
+
 // Whether execute() (search.h:240-270) goes on after the last epoch:
-// a pretrip search with fewer than min_connection_count journeys in it.
+// a pretrip search with fewer than min_connection_count journeys in it,
+// before its timeout.
 bool wants_extension(prepared_query const& p);
 
 // The interval extension step of execute() (search.h:282-317): widens the

@@ -174,6 +174,10 @@ struct query_t {
   bool has_fastest_direct_{false};
   std::int16_t fastest_direct_{0};
   double fastest_direct_factor_{1.0};
+  bool has_max_interval_{false};  // query::max_interval_ (query.h:89)
+  std::int32_t max_iv_from_{0}, max_iv_to_{0};
+  bool has_timeout_{false};       // raptor_search / pong_search timeout
+  std::int32_t timeout_s_{0};
 };
 
 query_t from_c(nigiri_query const& c) {
@@ -201,6 +205,11 @@ query_t from_c(nigiri_query const& c) {
   q.has_fastest_direct_ = c.has_fastest_direct != 0;
   q.fastest_direct_ = c.fastest_direct;
   q.fastest_direct_factor_ = c.fastest_direct_factor;
+  q.has_max_interval_ = c.has_max_interval != 0;
+  q.max_iv_from_ = c.max_interval_from;
+  q.max_iv_to_ = c.max_interval_to;
+  q.has_timeout_ = c.has_timeout != 0;
+  q.timeout_s_ = c.timeout_s;
   return q;
 }
 
@@ -1511,6 +1520,10 @@ struct interval_estimator {
     itv.to_ = std::clamp(itv.to_, ext_.from_, ext_.to_);
     itv.from_ = std::clamp(itv.from_, max_.from_, max_.to_);
     itv.to_ = std::clamp(itv.to_, max_.from_, max_.to_);
+    if (q_.has_max_interval_) {  // interval_estimate.h:142-145
+      itv.from_ = std::clamp(itv.from_, q_.max_iv_from_, q_.max_iv_to_);
+      itv.to_ = std::clamp(itv.to_, q_.max_iv_from_, q_.max_iv_to_);
+    }
   }
 
   query_t const& q_;
@@ -1591,6 +1604,13 @@ void run_search(tt_t const& tt, query_t q, raptor_state& rs, search_result& out,
         [&](journey_t const& j) { return search_interval.contains(j.start_time_); }));
   };
   auto const min_count = static_cast<unsigned>(q.min_connection_count_);
+  // is_timeout_reached (search.h:227-235): measured from here, after the
+  // first start labels
+  auto const processing_start_time = std::chrono::steady_clock::now();
+  auto const is_timeout_reached = [&]() {
+    return q.has_timeout_ && std::chrono::steady_clock::now() - processing_start_time >=
+                                 std::chrono::seconds{q.timeout_s_};
+  };
 
   // execute() (search.h:204-320)
   while (true) {
@@ -1634,7 +1654,10 @@ void run_search(tt_t const& tt, query_t q, raptor_state& rs, search_result& out,
       from = to;
     }
 
-    if (results.els_.empty() || is_ontrip || n_results_in_interval() >= min_count) break;
+    if (results.els_.empty() || is_ontrip || n_results_in_interval() >= min_count ||
+        is_timeout_reached()) {
+      break;
+    }
 
     auto const new_interval = est.extension(search_interval, min_count - n_results_in_interval());
     if (new_interval == search_interval) break;
@@ -1704,6 +1727,11 @@ constexpr std::int32_t kMinLookAhead = 1440;  // raptor/pong.h:11
 // Both raptors share one raptor_state, as in the reference.
 template <bool Fwd>
 void run_pong(tt_t const& tt, query_t q, raptor_state& rs, search_result& out, counters& c) {
+  auto const processing_start_time = std::chrono::steady_clock::now();  // pong.cc:83
+  auto const is_timeout_reached = [&]() {  // pong.cc:235-241
+    return q.has_timeout_ && std::chrono::steady_clock::now() - processing_start_time >=
+                                 std::chrono::seconds{q.timeout_s_};
+  };
   sanitize(q);
   auto const fastest_direct = get_fastest_direct(tt, q, Fwd);
   auto const search_interval = q.is_interval_ ? interval_t{q.iv_from_, q.iv_to_}
@@ -1749,7 +1777,7 @@ void run_pong(tt_t const& tt, query_t q, raptor_state& rs, search_result& out, c
   interval_t const external{tt.v_.date_range_from, tt.v_.date_range_to};
   while ((is_better(start_time, end_time) ||
           result_count(true) + result_count(false) < 2U * q.min_connection_count_) &&
-         external.contains(start_time)) {
+         external.contains(start_time) && !is_timeout_reached()) {
     starts.clear();
     get_starts(Fwd, tt, false, start_time, search_interval, q, starts, false);
     ping.reset_arrivals();

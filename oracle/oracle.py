@@ -104,7 +104,8 @@ def search(tt, q, state=False, cap=4096, leg_cap=1 << 15):
     legs = [[lg[jr[i].leg_offset + x].as_tuple() for x in range(jr[i].n_legs)]
             for i in range(k)]
     out = {"journeys": js, "legs": legs, "base_day": int(info[0]),
-           "fastest_direct": int(info[1]), "stats": stats.as_dict()}
+           "fastest_direct": int(info[1]), "interval": (int(info[2]), int(info[3])),
+           "stats": stats.as_dict()}
     if state:
         out.update(round_times=rt, best=best, tmp=tmp, time_at_dest=tad, lb=lb)
     return out

@@ -2,15 +2,21 @@
 batched (64 queries per wavefront) against the plain one-query-per-launch
 GPU path, with the CPU oracle timed on the host cores as a baseline.
 
-A step is one batch of `--batch` queries of the workload.  By default the
-timed steps' batches go through one run of the serving loop.  For
-point-query workloads a batch is admitted whole (its queries start their
-rounds together) once the previous batch has finished, and its host
-preparation and device lower bounds overlap the previous batch's rounds;
-for workloads with range queries (3, 5) queries take free slots as they
-appear (`--continuous`).  Latency is admission -> completion; the wait for
-admission is reported separately.  `--batch-per-step` runs
-each step as its own serving call instead (latency from the step start).
+Default (workload 5, the 500k-stop serving configuration): queries of the
+workload's 60/25/15 point/range/intermodal mix arrive as a Poisson stream at
+--rate queries/s (1000 by default) and go through the continuous-batching
+serving loop.  A step is one batch of --batch queries of that stream; the
+K timed batches arrive back to back as one stream.  value = queries / wall
+time of the stream, latency = arrival -> completion per query (p50/p99
+overall and per query kind).  The plain engine runs a sample of the same
+stream one query at a time.
+
+Other workloads (--workload 1..4) keep the batch modes: a step is one batch
+of `--batch` queries; point-query batches are admitted whole once the
+previous batch has finished (their preparation overlapping its rounds),
+mixes with range queries take free slots as they appear (`--continuous`);
+latency is admission -> completion there, the wait for admission reported
+separately.  `--batch-per-step` runs each step as its own serving call.
 Inputs (timetable image, query state) are resident on the GPU before the
 timed region starts.
 """
@@ -55,70 +61,195 @@ def route_scan_bytes(work):
 
 def committed_traffic(workload, scale, batch):
     """HBM bytes per route-scan launch from the committed rocprof PMC summary
-    (FETCH_SIZE + WRITE_SIZE, each scaled by a device copy of known size).
-    PMC counters need their own profiler runs, so the bench line carries the
-    figure of the last profiled build for the default workload only."""
-    path = os.path.join(ROOT, "profiles", "k_route_scan_workload2_pmc.md")
-    if (workload, scale, batch) != (2, 1.0, 2048) or not os.path.exists(path):
+    of this workload (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by a
+    device copy of known size).  PMC counters need their own profiler runs, so
+    the bench line carries the figure of the last profiled build, for the
+    workload, scale and batch that summary names (else null)."""
+    path = os.path.join(ROOT, "profiles", f"k_route_scan_workload{workload}_pmc.md")
+    if not os.path.exists(path):
         return None
+    fields = {}
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":")[1])
-    return None
+        if ":" in line and not line.startswith(("#", "|", " ")):
+            k, v = line.split(":", 1)
+            fields[k.strip()] = v.strip()
+    try:
+        if (float(fields.get("scale", "nan")) != scale or int(fields.get("batch", "-1")) != batch):
+            return None
+        return float(fields["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
 
 
-def serve_main(args):
-    """Serving: queries of the workload's mix arrive as a Poisson process;
-    reports latency percentiles (arrival -> completion) per query kind and
-    the single-query latency of the plain and the batched engine."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(tt, qs, seconds):
+    """The oracle (a C++ restatement of nigiri's CPU raptor, built -O3
+    -march=native without byte counters) timed on this host's cores in the
+    style of exe/benchmark.cc:210-270 (one state per thread, an atomic query
+    counter), at nproc threads and at one thread, on a bounded sample of the
+    workload's queries sized to about `seconds` of work each."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    n_q = len(qs)
+    nproc = os.cpu_count() or 1
+    probe = qs.subset(list(range(min(8, n_q))))
+    r = oracle.search_many(tt, probe, threads=1, baseline=True)
+    per_q = max(1e-4, r["wall_ms"] / 1e3 / len(probe))
+    out = {}
+    for threads in (nproc, 1):
+        n = int(min(n_q, max(threads * 2, seconds * threads / per_q)))
+        r = oracle.search_many(tt, qs.subset(list(range(n))), threads=threads, baseline=True)
+        out[threads] = (n, n / (r["wall_ms"] / 1e3), r)
+    n, qps, r = out[nproc]
+    n1, qps1, r1 = out[1]
+    return {"value": round(qps, 2), "unit": "queries/s", "cores": nproc, "kind": "port",
+            "sample": f"first {n} queries of the workload's stream, the oracle (C++ restatement of "
+                      f"nigiri's raptor search incl. reconstruction, -O3 -march=native) on {nproc} "
+                      f"threads: p50 {np.percentile(r['latency_ms'], 50):.1f} ms, p99 "
+                      f"{np.percentile(r['latency_ms'], 99):.1f} ms per query",
+            "one_thread": {"value": round(qps1, 2), "queries": n1,
+                           "p50_ms": round(float(np.percentile(r1["latency_ms"], 50)), 2),
+                           "p99_ms": round(float(np.percentile(r1["latency_ms"], 99)), 2)},
+            "cpu_model": cpu_model()}
+
+
+def pct(a, p):
+    return round(float(np.percentile(a, p)), 2) if len(a) else None
+
+
+def served_main(args):
+    """The metric on the serving workload (config 5 by default): queries of the
+    workload's mix arrive as a Poisson stream at --rate per second; a step is
+    one batch of --batch queries of that stream; the K timed batches arrive
+    back to back as one stream through the continuous-batching loop.  value =
+    queries completed / wall time of the stream; latency = arrival ->
+    completion."""
     import nigiri_amd as na
     cfg = args.workload
+    t0 = time.time()
     tt = na.Timetable.generate(cfg, scale=args.scale)
-    qs = na.Queries.generate(tt, cfg, args.queries)
-    n = len(qs)
+    B = args.batch
+    n_total = (args.warmup + args.steps) * B
+    qs = na.Queries.generate(tt, cfg, n_total)
+    print(f"[bench] workload {cfg}: {tt.view.n_locations} locations, {tt.view.n_routes} routes, "
+          f"{tt.view.n_transports} transports ({time.time() - t0:.1f}s incl. contraction hierarchy)",
+          file=sys.stderr, flush=True)
     img = na.Image(tt)
-    max_k = 16 if cfg in (1, 2) else 10
+    max_k = 10
     eng = na.Engine(img, lanes=64, groups=args.groups, max_k=max_k)
+    eng.profile(True)
     eng.configure(admission_wait_ms=1.0, graphs=not args.eager)
     kinds = np.array(["intermodal" if qs[i].dest_match_mode == na.MATCH_INTERMODAL
-                      else ("range" if qs[i].is_interval else "point") for i in range(n)])
-    eng.serve(qs.subset(list(range(min(64, n)))), np.zeros(min(64, n)))  # warm-up
-    print("[bench] serving warm-up done", file=sys.stderr, flush=True)
-    arrivals = np.cumsum(np.random.default_rng(7).exponential(1000.0 / args.serve, n))
+                      else ("range" if qs[i].is_interval else "point") for i in range(n_total)])
+    rng = np.random.default_rng(7)
+    caps = dict(journey_capacity=16 * B * max(1, args.steps), leg_capacity=400 * B * max(1, args.steps),
+                legs="raw")
+    for w in range(args.warmup):
+        arr = np.cumsum(rng.exponential(1000.0 / args.rate, B))
+        eng.serve(qs.subset(list(range(w * B, (w + 1) * B))), arr - arr[0], **caps)
+        print(f"[bench] warmup batch {w} done", file=sys.stderr, flush=True)
     eng.stats(reset=True)
-    res, wall = eng.serve(qs, arrivals)
-    lat = np.array([r["latency_ms"] for r in res])
+    first = args.warmup * B
+    idx = list(range(first, n_total))
+    arr = np.cumsum(rng.exponential(1000.0 / args.rate, len(idx)))
+    arr -= arr[0]
+    sync()
+    t_start = time.perf_counter()
+    res, wall = eng.serve(qs.subset(idx), arr, **caps)
+    sync()
+    t_total = time.perf_counter() - t_start
     st = eng.stats()
-
-    def pct(a, p):
-        return round(float(np.percentile(a, p)), 2) if len(a) else None
-
+    n = len(idx)
+    value = n / t_total
+    lat = np.array([r["latency_ms"] for r in res])
     queue = np.array([r["queue_ms"] for r in res])
-    kinds_out = {}
+    kd = kinds[first:]
+    ok = sum(1 for r in res if r["status"] == 0)
+    by_kind = {}
     for k in ("point", "range", "intermodal"):
-        m = kinds == k
-        kinds_out[k] = {"n": int(m.sum()), "p50_ms": pct(lat[m], 50), "p99_ms": pct(lat[m], 99),
-                        "queue_p50_ms": pct(queue[m], 50), "queue_p99_ms": pct(queue[m], 99)}
+        m = kd == k
+        by_kind[k] = {"n": int(m.sum()), "p50_ms": pct(lat[m], 50), "p99_ms": pct(lat[m], 99),
+                      "queue_p50_ms": pct(queue[m], 50)}
+    print(f"[bench] {n} queries served in {t_total:.1f} s", file=sys.stderr, flush=True)
+
+    # a lone point query on an idle engine, batched and plain
+    pts = [i for i in range(n_total) if kinds[i] == "point"][:5]
     plain = na.Engine(img, lanes=1, groups=1, max_k=max_k)
-    pts = [i for i in range(n) if kinds[i] == "point"][:6]
-    single_plain, single_batched = [], []
+    plain.configure(graphs=not args.eager)
+    plain.search(qs.subset([pts[0]]))  # warm-up
+    lone_b, lone_p = [], []
     for i in pts:
         t = time.perf_counter()
-        plain.search(qs.subset([i]))
-        single_plain.append((time.perf_counter() - t) * 1e3)
-        t = time.perf_counter()
         eng.search(qs.subset([i]))
-        single_batched.append((time.perf_counter() - t) * 1e3)
-    print(json.dumps({
-        "metric": "serving latency (arrival -> completion), Poisson arrivals, 1 MI355X",
-        "workload": f"config {cfg}: " + WORKLOADS[cfg], "scale": args.scale,
-        "offered_qps": args.serve, "queries": n, "slots": 64 * args.groups,
-        "wall_ms": round(wall, 1), "completed_qps": round(n / (wall / 1e3), 1),
-        "p50_ms": pct(lat, 50), "p90_ms": pct(lat, 90), "p99_ms": pct(lat, 99),
-        "by_kind": kinds_out, "epochs": st["epochs"], "rounds": st["rounds"],
-        "single_point_ms": {"plain": round(float(np.median(single_plain)), 1),
-                            "batched": round(float(np.median(single_batched)), 1)},
-    }), flush=True)
+        lone_b.append((time.perf_counter() - t) * 1e3)
+        t = time.perf_counter()
+        plain.search(qs.subset([i]))
+        lone_p.append((time.perf_counter() - t) * 1e3)
+    # the plain one-query-per-launch path on a sample of the same stream
+    ps = qs.subset(idx[:args.plain_sample])
+    sync()
+    tp = time.perf_counter()
+    plain.search(ps)
+    sync()
+    plain_qps = len(ps) / (time.perf_counter() - tp)
+    plain.close()
+    print(f"[bench] plain path: {plain_qps:.2f} q/s", file=sys.stderr, flush=True)
+
+    launches = max(1, st["route_scan_launches"])
+    alg_bytes = route_scan_bytes(st["work"])
+    avg_ms = st["route_scan_ms"] / launches
+    achieved = (alg_bytes / launches) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    cpu = None if args.no_cpu else cpu_baseline(tt, qs.subset(idx), args.cpu_seconds)
+    line = {
+        "metric": "earliest-arrival queries/s + p50/p99 latency, batched vs 1-query GPU, 1 MI355X",
+        "value": round(value, 2),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_total * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int16",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": f"config {cfg}: " + WORKLOADS[cfg], "scale": args.scale,
+                   "mode": "served: Poisson arrivals through the continuous-batching loop",
+                   "offered_qps": args.rate, "batch": B, "queries": n, "slots": 64 * args.groups,
+                   "lanes": 64, "groups": args.groups, "max_k": max_k},
+        "completed_ok": ok,
+        "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+        "latency": "arrival -> completion (queue, preparation, lower bounds, rounds, legs)",
+        "by_kind": by_kind,
+        "lone_point_query_ms": {"batched": round(float(np.median(lone_b)), 2),
+                                "plain": round(float(np.median(lone_p)), 2)},
+        "plain_1query_qps": round(plain_qps, 2),
+        "plain_sample": len(ps),
+        "speedup_vs_plain": round(value / plain_qps, 2) if plain_qps > 0 else None,
+        "rounds": st["rounds"], "epochs": st["epochs"],
+        "round_launch": "eager" if args.eager else "hipGraph",
+        "lane_utilization": round(st["work"]["route_stops"] / max(1, 64 * st["work"]["wave_stops"]), 4),
+        "work": {k: int(v) for k, v in st["work"].items()},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": committed_traffic(cfg, args.scale, B), "kernel": "k_route_scan",
+                     "alg_bytes_per_launch": int(alg_bytes / launches),
+                     "avg_launch_ms": round(avg_ms, 4), "launches": launches},
+        "transfer_kernel_avg_ms": round(st["transfer_ms"] / max(1, st["transfer_launches"]), 4),
+        "cpu_baseline": cpu,
+    }
+    eng.close()
+    img.close()
+    print(json.dumps(line), flush=True)
 
 
 def pong_main(args):
@@ -239,11 +370,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", type=int, default=2)
+    ap.add_argument("--workload", type=int, default=5)
     ap.add_argument("--scale", type=float, default=1.0)
-    ap.add_argument("--batch", type=int, default=2048)
+    ap.add_argument("--batch", type=int, default=0,
+                    help="queries per step (default: 512 served, 2048 in batch modes)")
+    ap.add_argument("--rate", type=float, default=1000.0,
+                    help="served mode: Poisson arrival rate, queries/s (config 5: 1k-50k)")
     ap.add_argument("--queries", type=int, default=10000)
-    ap.add_argument("--plain-sample", type=int, default=64)
+    ap.add_argument("--plain-sample", type=int, default=24)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -251,8 +385,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 512 MiB (scales rocprof FETCH/WRITE_SIZE)")
     ap.add_argument("--serve", type=float, default=0.0, metavar="RATE",
-                    help="serving measurement instead: Poisson arrivals at RATE queries/s "
-                         "of the workload's query mix through the continuous-batching loop")
+                    help="served measurement (the default for workload 5) at RATE queries/s "
+                         "for any workload")
     ap.add_argument("--groups", type=int, default=16, help="64-query groups (serving mode)")
     ap.add_argument("--continuous", action="store_true",
                     help="the timed steps' batches go through one serving run back to back: the "
@@ -297,8 +431,12 @@ def main():
         sys.exit(2)
 
     if args.serve > 0.0:
-        serve_main(args)
+        args.rate = args.serve
+    if args.pong == 0 and (args.serve > 0.0 or args.workload == 5):
+        args.batch = args.batch or 512
+        served_main(args)
         return
+    args.batch = args.batch or 2048
     if args.pong > 0:
         pong_main(args)
         return

@@ -249,6 +249,18 @@ int nigiri_amd_tt_find_location(const nigiri_amd_tt* tt, const char* stop_id,
 int nigiri_amd_tt_transport_trip_id(const nigiri_amd_tt* tt, uint32_t t,
                                     char* buf, uint32_t cap);
 int nigiri_amd_tt_view(const nigiri_amd_tt* tt, nigiri_tt_view* out);
+
+/* Lower bounds of a search on the host (replaces routing::dijkstra,
+ * src/routing/dijkstra.cc:18-67, as search::init calls it, search.h:
+ * 114-131): the shortest travel time from every location to the seeds
+ * (location, minutes: the destination offsets) on the lower-bound graph of
+ * direction `dir`, 0xFFFF where it is 7201 minutes or more.  Computed with
+ * the timetable's contraction hierarchy (upward search, then a sweep over
+ * all locations); the values equal the dial-bucket Dijkstra's.
+ * lb: [n_locations]. */
+int nigiri_amd_tt_lower_bounds(const nigiri_amd_tt* tt, int32_t dir,
+                               uint32_t n_seeds, const uint32_t* seed_loc,
+                               const uint16_t* seed_min, uint16_t* lb);
 void nigiri_amd_tt_free(nigiri_amd_tt* tt);
 
 /* Seeded query workload for `cfg` (point / range / intermodal mix as the
@@ -266,6 +278,18 @@ int nigiri_amd_image_create(const nigiri_amd_tt* tt, int device,
                             nigiri_amd_image** out);
 int nigiri_amd_image_bytes(const nigiri_amd_image* img, uint64_t* bytes);
 void nigiri_amd_image_destroy(nigiri_amd_image* img);
+
+/* Lower bounds of a search on the device (replaces routing::dijkstra,
+ * src/routing/dijkstra.cc:18-67, as search::init calls it, search.h:
+ * 114-131): lb[l] = shortest travel time from location l to the seeds
+ * (location, minutes) on the lower-bound graph of `dir`, 0xFFFF where it is
+ * 7201 minutes or more; the values equal the dial-bucket Dijkstra's.  An
+ * upward search of the image's contraction hierarchy on the host, then the
+ * PHAST sweep over all locations on the device.  lb: device memory,
+ * [n_locations] uint16, written before the call returns. */
+int nigiri_amd_lower_bounds(nigiri_amd_image* img, int32_t dir,
+                            uint32_t n_seeds, const uint32_t* seed_loc,
+                            const uint16_t* seed_min, uint16_t* lb);
 
 /* ---- engine: device query state (replaces gpu_raptor_state,
  * gpu/raptor.h:47-53, and raptor_state, raptor/raptor_state.h:22-111) ----
@@ -412,7 +436,6 @@ int nigiri_amd_engine_stats(nigiri_amd_engine* e, nigiri_engine_stats* out,
  *   and footpath kernel (16).
  * pack_max_entries: (route, group) entries, i.e. 64-query groups, that one
  *   packed route wavefront may mix (1-64; 64).
- * lb_delta: threshold step in minutes of the near-far lower-bound kernel (8).
  * transfer_pack_max_entries: (station, group) entries, i.e. 64-query
  *   groups, that one packed transfer wavefront may mix (1-64; 64).
  * admit_when_idle: serving loop; 1 = new queries are admitted only when no
@@ -427,7 +450,6 @@ typedef struct nigiri_engine_config {
   uint32_t route_claim;
   uint32_t transfer_claim;
   uint32_t pack_max_entries;
-  uint32_t lb_delta;
   uint32_t admit_when_idle;
   uint32_t transfer_pack_max_entries;
 } nigiri_engine_config;
@@ -440,7 +462,7 @@ int nigiri_amd_engine_get_config(const nigiri_amd_engine* e,
 
 /* ---- autotuner (no counterpart in the reference) ------------------------
  * Chooses the launch-shape knobs of nigiri_engine_config (grid_waves,
- * route_claim, transfer_claim, pack_max_entries, lb_delta,
+ * route_claim, transfer_claim, pack_max_entries,
  * transfer_pack_max_entries) for this engine and timetable on a sample of
  * queries (ideally one batch of the real
  * workload): one pass of coordinate descent in which every candidate value
@@ -449,8 +471,8 @@ int nigiri_amd_engine_get_config(const nigiri_amd_engine* e,
  * on the engine and are written to *out (may be NULL).  Each trial is
  * reported as (knob, value, ms) in trials[0 .. min(n, trial_capacity)),
  * with the total count in *n_trials (may be NULL).  Knob ids: 0 grid_waves,
- * 1 route_claim, 2 transfer_claim, 3 pack_max_entries, 4 lb_delta,
- * 5 transfer_pack_max_entries.
+ * 1 route_claim, 2 transfer_claim, 3 pack_max_entries,
+ * 4 transfer_pack_max_entries.
  * Results never depend on the knobs. */
 typedef struct nigiri_tune_trial {
   uint32_t knob;

@@ -130,7 +130,7 @@ class EngineConfig(C.Structure):
     _fields_ = [("admission_wait_ms", C.c_double), ("grid_waves", C.c_uint32),
                 ("eager_rounds", C.c_uint32), ("route_claim", C.c_uint32),
                 ("transfer_claim", C.c_uint32), ("pack_max_entries", C.c_uint32),
-                ("lb_delta", C.c_uint32), ("admit_when_idle", C.c_uint32),
+                ("admit_when_idle", C.c_uint32),
                 ("transfer_pack_max_entries", C.c_uint32)]
 
 
@@ -138,7 +138,7 @@ class TuneTrial(C.Structure):
     _fields_ = [("knob", C.c_uint32), ("value", C.c_uint32), ("ms", C.c_double)]
 
 
-TUNE_KNOBS = ["grid_waves", "route_claim", "transfer_claim", "pack_max_entries", "lb_delta",
+TUNE_KNOBS = ["grid_waves", "route_claim", "transfer_claim", "pack_max_entries",
               "transfer_pack_max_entries"]
 
 
@@ -158,6 +158,8 @@ SIGNATURES = {
     "nigiri_amd_tt_load_gtfs": (C.c_int, [C.POINTER(C.c_char_p), C.POINTER(C.c_char_p), C.c_uint32,
                                           C.c_int32, C.c_int32, C.c_char_p, C.POINTER(_P)]),
     "nigiri_amd_tt_location_id": (C.c_int, [_P, C.c_uint32, C.c_char_p, C.c_uint32]),
+    "nigiri_amd_tt_lower_bounds": (C.c_int, [_P, C.c_int32, C.c_uint32, C.POINTER(C.c_uint32),
+                                             C.POINTER(C.c_uint16), C.POINTER(C.c_uint16)]),
     "nigiri_amd_tt_find_location": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_uint32)]),
     "nigiri_amd_tt_transport_trip_id": (C.c_int, [_P, C.c_uint32, C.c_char_p, C.c_uint32]),
     "nigiri_amd_tt_free": (None, [_P]),
@@ -167,6 +169,8 @@ SIGNATURES = {
     "nigiri_amd_image_create": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
     "nigiri_amd_image_bytes": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "nigiri_amd_image_destroy": (None, [_P]),
+    "nigiri_amd_lower_bounds": (C.c_int, [_P, C.c_int32, C.c_uint32, C.POINTER(C.c_uint32),
+                                          C.POINTER(C.c_uint16), C.c_void_p]),
     "nigiri_amd_engine_create": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
     "nigiri_amd_engine_state_bytes": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "nigiri_amd_engine_destroy": (None, [_P]),
@@ -340,6 +344,15 @@ class Timetable:
                                              C.byref(h)))
         return cls(h)
 
+    def lower_bounds(self, seeds, direction=0):
+        """Lower bounds (nigiri_amd_tt_lower_bounds) for seeds [(location, minutes)]."""
+        loc = np.array([l for l, _ in seeds], np.uint32)
+        mins = np.array([d for _, d in seeds], np.uint16)
+        out = np.zeros(self.n_locations, np.uint16)
+        _check(lib().nigiri_amd_tt_lower_bounds(self.handle, direction, len(seeds), ptr(loc, C.c_uint32),
+                                                ptr(mins, C.c_uint16), ptr(out, C.c_uint16)))
+        return out
+
     def location(self, stop_id):
         """Location index of a GTFS stop_id (loaded feeds only)."""
         out = C.c_uint32()
@@ -467,6 +480,14 @@ class Queries:
 class Image:
     """Device-resident timetable (one per GPU)."""
 
+    def lower_bounds(self, seeds, direction, lb_ptr):
+        """Lower bounds for seeds [(location, minutes)] into the device buffer at
+        lb_ptr (n_locations uint16): nigiri_amd_lower_bounds."""
+        loc = np.array([l for l, _ in seeds], np.uint32)
+        mins = np.array([d for _, d in seeds], np.uint16)
+        _check(lib().nigiri_amd_lower_bounds(self.handle, direction, len(seeds), ptr(loc, C.c_uint32),
+                                             ptr(mins, C.c_uint16), C.c_void_p(lb_ptr)))
+
     def __init__(self, tt, device=0):
         self.tt = tt  # the image borrows the host timetable
         self.handle = C.c_void_p()
@@ -528,7 +549,7 @@ class Engine:
     def configure(self, admission_wait_ms=2.0, grid_waves=0, graphs=True, admit_when_idle=False,
                   **knobs):
         """Sets the engine configuration; launch-shape knobs (route_claim,
-        transfer_claim, pack_max_entries, lb_delta, transfer_pack_max_entries)
+        transfer_claim, pack_max_entries, transfer_pack_max_entries)
         left out keep their values."""
         c = EngineConfig()
         c.admission_wait_ms = admission_wait_ms

@@ -18,7 +18,7 @@ COMMON = ["-O3", "-std=c++20", "-fPIC", "-Wall", "-Wno-unused-parameter"]
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-munsafe-fp-atomics"]
 HOST_SOURCES = ["timetable.cc", "generator.cc", "search_host.cc", "journey_legs.cc", "capi_host.cc",
-                "pong_host.cc", "tz.cc", "gtfs.cc"]
+                "pong_host.cc", "tz.cc", "gtfs.cc", "ch.cc"]
 HIP_SOURCES = ["kernels.hip", "engine.hip"]
 
 

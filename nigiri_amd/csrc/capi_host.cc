@@ -1,4 +1,5 @@
 // C ABI: host timetable and query-workload entry points.
+#include <algorithm>
 #include <cstring>
 #include <exception>
 #include <string>
@@ -150,6 +151,27 @@ int nigiri_amd_tt_find_location(const nigiri_amd_tt* tt, const char* stop_id, ui
 int nigiri_amd_tt_transport_trip_id(const nigiri_amd_tt* tt, uint32_t t, char* buf, uint32_t cap) {
   if (tt == nullptr) return NIGIRI_AMD_EINVAL;
   return copy_id(tt->transport_trip_ids, t, buf, cap);
+}
+
+int nigiri_amd_tt_lower_bounds(const nigiri_amd_tt* tt, int32_t dir, uint32_t n_seeds, const uint32_t* seed_loc,
+                               const uint16_t* seed_min, uint16_t* lb) {
+  if (tt == nullptr || lb == nullptr || (n_seeds != 0U && (seed_loc == nullptr || seed_min == nullptr)) ||
+      (dir != NIGIRI_AMD_DIR_FWD && dir != NIGIRI_AMD_DIR_BWD)) {
+    set_error("tt_lower_bounds: bad argument");
+    return NIGIRI_AMD_EINVAL;
+  }
+  return guarded([&]() {
+    std::vector<std::pair<std::uint32_t, std::uint16_t>> seeds;
+    for (uint32_t i = 0; i != n_seeds; ++i) {
+      if (seed_loc[i] >= tt->tt.n_locations) throw status_error{NIGIRI_AMD_EINVAL, "tt_lower_bounds: seed out of range"};
+      seeds.emplace_back(seed_loc[i], seed_min[i]);
+    }
+    ch_search_scratch s;
+    std::vector<std::uint16_t> out;
+    ch_lower_bounds(tt->tt.ch, dir == NIGIRI_AMD_DIR_FWD, seeds, s, out);
+    std::copy(out.begin(), out.end(), lb);
+    return NIGIRI_AMD_OK;
+  });
 }
 
 int nigiri_amd_tt_view(const nigiri_amd_tt* tt, nigiri_tt_view* out) {

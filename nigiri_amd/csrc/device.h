@@ -35,6 +35,17 @@ enum : int {
   kWorkCount = 16
 };
 
+// Downward sweep of PHAST for one search direction (the contraction
+// hierarchy of the lower-bound graph): locations in level order, each with
+// its edges from higher-ranked locations.
+struct dev_sweep {
+  std::uint32_t n_nodes;
+  std::uint32_t const* order;
+  std::uint32_t const* edge_off;  // [n_nodes + 1]
+  std::uint32_t const* src;
+  std::uint16_t const* w;         // minutes, at most 7201
+};
+
 // Read-only timetable image in HBM (one allocation per array).
 struct dev_image {
   std::uint32_t L, R, T, NB;
@@ -66,6 +77,7 @@ struct dev_image {
   std::uint32_t const* lb_fwd;
   std::uint32_t const* lb_bwd_offsets;
   std::uint32_t const* lb_bwd;
+  dev_sweep sweep[2];  // [0]: forward search's lower bounds, [1]: backward
 };
 
 // Per-slot parameters written by the host at admission.

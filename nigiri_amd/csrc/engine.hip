@@ -99,6 +99,17 @@ struct nigiri_amd_image {
   int device = 0;
   dev_image d{};
   dev_alloc mem;
+  // nigiri_amd_lower_bounds: the upward search's labels and the selection
+  // of one row, grown on demand (calls serialised by the mutex)
+  std::mutex lb_mu;
+  std::uint32_t* d_lb_sel = nullptr;  // [1]
+  std::uint32_t* d_lb_seed_idx = nullptr;
+  std::uint16_t* d_lb_seed_min = nullptr;
+  std::size_t lb_seed_cap = 0U;
+  ~nigiri_amd_image() {
+    if (d_lb_seed_idx != nullptr) (void)hipFree(d_lb_seed_idx);
+    if (d_lb_seed_min != nullptr) (void)hipFree(d_lb_seed_min);
+  }
 };
 
 // host threads that write the legs of a step's journeys (with the calling
@@ -210,32 +221,14 @@ struct nigiri_amd_engine {
   std::int16_t* d_worst = nullptr;
   std::uint32_t* d_sweep_groups = nullptr;
   std::uint32_t* d_slot_list = nullptr;  // [S] slots whose epoch ended
-  std::uint16_t* d_lb_staging = nullptr;
-  std::uint32_t* d_lb_slot = nullptr;
-  std::int32_t* d_lb_row = nullptr;
-  std::uint32_t* d_lb_groups = nullptr;
-  std::uint32_t lb_rows_cap = 64U;
-  // device lower bounds (batched engine): kLbWorkers threads, each with
-  // its own stream and scratch, compute them ahead of admission into a
-  // pool of S rows (one workgroup per query, up to `chunk` per launch)
-  struct lb_worker_res {
-    hipStream_t stream = nullptr;
-    std::uint32_t chunk = 0U;
-    std::uint32_t* out_row = nullptr;
-    std::uint32_t* dist = nullptr;
-    std::uint32_t* stamp_near = nullptr;
-    std::uint32_t* stamp_far = nullptr;
-    std::uint32_t* near0 = nullptr;
-    std::uint32_t* near1 = nullptr;
-    std::uint32_t* far0 = nullptr;
-    std::uint32_t* far1 = nullptr;
-    std::uint32_t* seed_off = nullptr;
-    std::uint32_t* seed_loc = nullptr;
-    std::uint16_t* seed_dur = nullptr;
-    std::size_t seed_cap = 0U;
-  };
-  std::vector<lb_worker_res> lbw;
-  std::uint16_t* d_lb_pool = nullptr;
+  // lower bounds of admitted slots (PHAST): selected groups / rows, their
+  // lane masks, and the upward searches' labels (growable)
+  std::uint16_t* d_lb_row = nullptr;   // [L] lower bounds given by the caller
+  std::uint32_t* d_ph_sel = nullptr;   // [S]
+  std::uint64_t* d_ph_masks = nullptr; // [S]
+  std::uint32_t* d_ph_seed_idx = nullptr;
+  std::uint16_t* d_ph_seed_min = nullptr;
+  std::size_t ph_seed_cap = 0U;
   // growable device buffers for starts / dest ops
   std::uint32_t* d_start_slot = nullptr;
   std::uint32_t* d_start_loc = nullptr;
@@ -255,7 +248,6 @@ struct nigiri_amd_engine {
   std::unique_ptr<host_pool> pool;  // writes the legs of a step's journeys
   std::vector<void*> grow_ptrs;
   // pinned host staging
-  pinned<std::uint16_t> h_lb;
   pinned<std::uint64_t> h_active;
   pinned<std::int16_t> h_rows;
   pinned<std::uint32_t> h_counts;
@@ -271,7 +263,6 @@ struct nigiri_amd_engine {
   bool use_graphs = true;
   // launch-shape knobs (nigiri_amd_engine_autotune chooses them)
   round_tune tune{1U, 16U, 64U, 64U};
-  std::uint32_t lb_delta = 8U;
   // serving: admit new queries only when no slot is busy (whole batches)
   bool admit_when_idle = false;
   round_graphs* graphs = round_graphs_create();
@@ -285,9 +276,6 @@ struct nigiri_amd_engine {
     for (auto ev : events) (void)hipEventDestroy(ev);
     for (auto* p : grow_ptrs) (void)hipFree(p);
     if (stream != nullptr) (void)hipStreamDestroy(stream);
-    for (auto& w : lbw) {
-      if (w.stream != nullptr) (void)hipStreamDestroy(w.stream);
-    }
   }
 
   template <typename T>
@@ -392,6 +380,16 @@ void build_image(nigiri_amd_image& img, timetable const& tt, hipStream_t s) {
   d.lb_fwd = m.upload(tt.lb_fwd, s);
   d.lb_bwd_offsets = m.upload(tt.lb_bwd_offsets, s);
   d.lb_bwd = m.upload(tt.lb_bwd, s);
+  // PHAST sweeps of the lower-bound graph's contraction hierarchy
+  for (int dir = 0; dir != 2; ++dir) {
+    auto const& sw = dir == 0 ? tt.ch.sweep_fwd : tt.ch.sweep_bwd;
+    auto& ds = d.sweep[dir];
+    ds.n_nodes = static_cast<std::uint32_t>(sw.order.size());
+    ds.order = m.upload(sw.order, s);
+    ds.edge_off = m.upload(sw.edge_off, s);
+    ds.src = m.upload(sw.src, s);
+    ds.w = m.upload(sw.w, s);
+  }
   HIP_CHECK(hipStreamSynchronize(s));  // host vectors above go out of scope
 }
 
@@ -409,175 +407,8 @@ struct slot_info {
   bool is_new = false;
 };
 
-// Lower bounds of queued queries (batched engine), computed ahead of
-// admission by kLbWorkers worker threads, each on its own stream: queries
-// are submitted once prepared, become ready when their row of the pool
-// holds their bounds, and give the row back after admission has transposed
-// it.  A query waits at most for one launch of a free worker.
-// threads (and streams) computing lower bounds ahead of admission
-constexpr std::size_t kLbWorkers = 3U;
-
 static_assert(kMaxQueryDests == kMaxDestBits && kMaxQueryEgress == kMaxDist,
               "query limits of the host preparation and the device state agree");
-
-struct lb_pipeline {
-  nigiri_amd_engine& e;
-  bool fwd;
-  std::vector<prepared_query>& pq;
-  std::mutex mu;
-  std::condition_variable cv;
-  std::deque<std::int32_t> waiting, ready;
-  std::vector<std::uint32_t> free_rows;
-  std::size_t in_flight = 0U;
-  std::uint32_t n_finished = 0U;
-  bool stop = false;
-  std::exception_ptr err;
-  std::vector<std::thread> workers;
-
-  lb_pipeline(nigiri_amd_engine& eng, bool f, std::vector<prepared_query>& p) : e{eng}, fwd{f}, pq{p} {
-    for (std::uint32_t r = 0; r != e.S; ++r) free_rows.push_back(e.S - 1U - r);
-    for (std::size_t w = 0; w != e.lbw.size(); ++w) {
-      workers.emplace_back([this, w]() { run(e.lbw[w]); });
-    }
-  }
-  ~lb_pipeline() {
-    {
-      std::lock_guard<std::mutex> lock{mu};
-      stop = true;
-    }
-    cv.notify_all();
-    for (auto& t : workers) t.join();
-  }
-  void submit(std::int32_t q) {
-    {
-      std::lock_guard<std::mutex> lock{mu};
-      waiting.push_back(q);
-    }
-    cv.notify_all();
-  }
-  void check() {
-    if (err) std::rethrow_exception(err);
-  }
-  std::int32_t pop_ready() {
-    std::lock_guard<std::mutex> lock{mu};
-    check();
-    if (ready.empty()) return -1;
-    auto const q = ready.front();
-    ready.pop_front();
-    return q;
-  }
-  std::size_t n_ready() {
-    std::lock_guard<std::mutex> lock{mu};
-    check();
-    return ready.size();
-  }
-  std::uint32_t finished() {
-    std::lock_guard<std::mutex> lock{mu};
-    check();
-    return n_finished;
-  }
-  // blocks until `want` queries are ready or nothing more is coming
-  void wait_ready(std::size_t want) {
-    std::unique_lock<std::mutex> lock{mu};
-    cv.wait(lock, [&]() {
-      return err || ready.size() >= want || (waiting.empty() && in_flight == 0U);
-    });
-    check();
-  }
-  void release(std::uint32_t row) {
-    {
-      std::lock_guard<std::mutex> lock{mu};
-      free_rows.push_back(row);
-    }
-    cv.notify_all();
-  }
-
-  void run(nigiri_amd_engine::lb_worker_res const& w) {
-    std::size_t taken = 0U;
-    try {
-      HIP_CHECK(hipSetDevice(e.img->device));
-      while (true) {
-        std::vector<std::int32_t> batch;
-        std::vector<std::uint32_t> rows;
-        {
-          std::unique_lock<std::mutex> lock{mu};
-          cv.wait(lock, [&]() { return stop || err || (!waiting.empty() && !free_rows.empty()); });
-          if (stop || err) return;
-          while (!waiting.empty() && !free_rows.empty() && batch.size() < w.chunk) {
-            auto const q = waiting.front();
-            waiting.pop_front();
-            auto const row = free_rows.back();
-            free_rows.pop_back();
-            pq[static_cast<std::size_t>(q)].lb_row_ = static_cast<std::int32_t>(row);
-            batch.push_back(q);
-            rows.push_back(row);
-          }
-          taken = batch.size();
-          in_flight += taken;
-        }
-        compute(w, batch, rows);
-        {
-          std::lock_guard<std::mutex> lock{mu};
-          for (auto const q : batch) ready.push_back(q);
-          n_finished += static_cast<std::uint32_t>(batch.size());
-          in_flight -= taken;
-          taken = 0U;
-        }
-        cv.notify_all();
-      }
-    } catch (...) {
-      {
-        std::lock_guard<std::mutex> lock{mu};
-        if (!err) err = std::current_exception();
-        in_flight -= taken;
-      }
-      cv.notify_all();
-    }
-  }
-
-  void compute(nigiri_amd_engine::lb_worker_res const& w, std::vector<std::int32_t> const& batch,
-               std::vector<std::uint32_t> const& rows) {
-    auto const stream = w.stream;
-    std::vector<std::uint32_t> seed_off{0U}, seed_loc;
-    std::vector<std::uint16_t> seed_dur;
-    for (auto const q : batch) {
-      for (auto const& [l, d] : pq[static_cast<std::size_t>(q)].lb_seeds_) {
-        seed_loc.push_back(l);
-        seed_dur.push_back(d);
-      }
-      seed_off.push_back(static_cast<std::uint32_t>(seed_loc.size()));
-    }
-    if (seed_loc.size() > w.seed_cap) {
-      throw status_error{NIGIRI_AMD_EUNSUPPORTED, "too many destination offsets"};
-    }
-    HIP_CHECK(hipMemcpyAsync(w.seed_off, seed_off.data(), 4U * seed_off.size(), hipMemcpyHostToDevice, stream));
-    HIP_CHECK(hipMemcpyAsync(w.out_row, rows.data(), 4U * rows.size(), hipMemcpyHostToDevice, stream));
-    if (!seed_loc.empty()) {
-      HIP_CHECK(hipMemcpyAsync(w.seed_loc, seed_loc.data(), 4U * seed_loc.size(), hipMemcpyHostToDevice, stream));
-      HIP_CHECK(hipMemcpyAsync(w.seed_dur, seed_dur.data(), 2U * seed_dur.size(), hipMemcpyHostToDevice, stream));
-    }
-    lb_args q{};
-    q.im = e.img->d;
-    q.L = e.st.L;
-    q.delta = e.lb_delta;
-    q.n_q = static_cast<std::uint32_t>(batch.size());
-    q.seed_off = w.seed_off;
-    q.seed_loc = w.seed_loc;
-    q.seed_dur = w.seed_dur;
-    q.dist = w.dist;
-    q.stamp_near = w.stamp_near;
-    q.stamp_far = w.stamp_far;
-    q.near0 = w.near0;
-    q.near1 = w.near1;
-    q.far0 = w.far0;
-    q.far1 = w.far1;
-    q.out_row = w.out_row;
-    q.rows = e.d_lb_pool;
-    launch_lb_queries(fwd, q, stream);
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipStreamSynchronize(stream));
-  }
-};
 
 // Runs the device journey reconstruction of `jobs` on the engine's stream;
 // their records are in e.h_rc_out afterwards.
@@ -628,7 +459,6 @@ struct runner {
   std::vector<prepared_query>& pq;
   std::function<std::int32_t()> next;            // next query id or -1
   std::function<void(std::int32_t)> on_done;      // query finished
-  std::function<void(std::uint32_t)> release_row; // lower-bound row consumed
   std::function<void(std::int32_t)> on_admit;     // query takes a slot
   std::vector<slot_info> slots;
   std::uint64_t rounds = 0U, epochs = 0U;
@@ -795,13 +625,9 @@ struct runner {
     a.slot_flags = e.d_flags;
     a.slot_worst = e.d_worst;
     a.n_slots = S;
-    // lower bounds of new slots: computed on the device by the batched
-    // engine, uploaded from the host preparation by the plain engine
-    if (W == 64U) {
-      scatter_pool_rows(new_slots);
-    } else {
-      upload_lbs(new_slots, a);
-    }
+    // lower bounds of the new slots on the device (PHAST sweep from the
+    // upward searches of their preparation)
+    phast_lbs(new_slots);
     launch_epoch_begin(fwd, W, a, stream);
     HIP_CHECK(hipGetLastError());
     // the host vectors above go out of scope: wait for the copies
@@ -839,10 +665,6 @@ struct runner {
       if (q < 0) break;
       auto& p = pq[static_cast<std::size_t>(q)];
       if (p.epochs_.empty()) {  // overlap or no start labels
-        if (p.lb_row_ >= 0 && release_row) {
-          release_row(static_cast<std::uint32_t>(p.lb_row_));
-          p.lb_row_ = -1;
-        }
         finish(p);
         on_done(q);
         continue;
@@ -1040,72 +862,57 @@ struct runner {
     return n;
   }
 
-  // Lower bounds of the new slots (batched engine): their rows were
-  // computed ahead of admission (lb_pipeline); transpose them into the
-  // lane-interleaved lb columns, then hand the rows back.
-  void scatter_pool_rows(std::vector<std::uint32_t> const& new_slots) {
+  // Lower bounds of the new slots (search.h:114-131): PHAST on the device,
+  // seeded with the upward searches of their host preparation.  Batched
+  // engine: the lanes of each touched group are swept together.
+  void phast_lbs(std::vector<std::uint32_t> const& new_slots) {
     if (new_slots.empty()) return;
-    auto const stream = e.stream;
-    std::vector<std::int32_t> row_of(e.S, -1);
-    std::vector<std::uint32_t> groups, rows;
+    auto const L = e.st.L, W = e.W;
+    std::vector<std::uint32_t> sel, seed_idx;
+    std::vector<std::uint64_t> masks;
+    std::vector<std::uint16_t> seed_min;
+    std::vector<std::int32_t> pos_of(W == 64U ? e.G : e.S, -1);
     for (auto const s : new_slots) {
-      auto const row = pq[static_cast<std::size_t>(slots[s].q)].lb_row_;
-      if (row < 0) throw status_error{NIGIRI_AMD_EDEVICE, "query admitted without lower bounds"};
-      row_of[s] = row;
-      rows.push_back(static_cast<std::uint32_t>(row));
-      auto const g = s / e.W;
-      if (groups.empty() || groups.back() != g) groups.push_back(g);
-    }
-    HIP_CHECK(hipMemcpyAsync(e.d_lb_row, row_of.data(), 4U * e.S, hipMemcpyHostToDevice, stream));
-    HIP_CHECK(hipMemcpyAsync(e.d_lb_groups, groups.data(), 4U * groups.size(), hipMemcpyHostToDevice, stream));
-    epoch_args a{};
-    a.st = e.st;
-    a.lb_staging = e.d_lb_pool;
-    a.n_lb = static_cast<std::uint32_t>(new_slots.size());
-    a.lb_row_of_slot = e.d_lb_row;
-    a.lb_groups = e.d_lb_groups;
-    a.n_lb_groups = static_cast<std::uint32_t>(groups.size());
-    launch_lb_scatter(e.W, a, stream);
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipStreamSynchronize(stream));  // rows are free again, host vectors go out of scope
-    for (auto const s : new_slots) pq[static_cast<std::size_t>(slots[s].q)].lb_row_ = -1;
-    for (auto const row : rows) release_row(row);
-  }
-
-  void upload_lbs(std::vector<std::uint32_t> const& new_slots, epoch_args& a) {
-    if (new_slots.empty()) return;
-    auto const L = e.st.L;
-    auto const stream = e.stream;
-    std::vector<std::int32_t> row_of(e.S, -1);
-    for (std::size_t from = 0; from < new_slots.size(); from += e.lb_rows_cap) {
-      auto const to = std::min<std::size_t>(new_slots.size(), from + e.lb_rows_cap);
-      HIP_CHECK(hipStreamSynchronize(stream));  // staging buffer reuse
-      std::vector<std::uint32_t> slots_chunk(new_slots.begin() + static_cast<std::ptrdiff_t>(from),
-                                             new_slots.begin() + static_cast<std::ptrdiff_t>(to));
-      std::vector<std::uint32_t> groups;
-      std::fill(row_of.begin(), row_of.end(), -1);
-      for (std::size_t j = 0; j != slots_chunk.size(); ++j) {
-        auto const s = slots_chunk[j];
-        auto const& lb = pq[static_cast<std::size_t>(slots[s].q)].lb_;
-        std::memcpy(e.h_lb.p + j * L, lb.data(), 2U * L);
-        row_of[s] = static_cast<std::int32_t>(j);
-        auto const g = s / e.W;
-        if (groups.empty() || groups.back() != g) groups.push_back(g);
+      auto const key = W == 64U ? s / 64U : s;
+      if (pos_of[key] < 0) {
+        pos_of[key] = static_cast<std::int32_t>(sel.size());
+        sel.push_back(key);
+        masks.push_back(0U);
       }
-      HIP_CHECK(hipMemcpyAsync(e.d_lb_staging, e.h_lb.p, 2U * L * slots_chunk.size(), hipMemcpyHostToDevice, stream));
-      HIP_CHECK(hipMemcpyAsync(e.d_lb_slot, slots_chunk.data(), 4U * slots_chunk.size(), hipMemcpyHostToDevice, stream));
-      HIP_CHECK(hipMemcpyAsync(e.d_lb_row, row_of.data(), 4U * e.S, hipMemcpyHostToDevice, stream));
-      HIP_CHECK(hipMemcpyAsync(e.d_lb_groups, groups.data(), 4U * groups.size(), hipMemcpyHostToDevice, stream));
-      a.lb_staging = e.d_lb_staging;
-      a.lb_slot = e.d_lb_slot;
-      a.n_lb = static_cast<std::uint32_t>(slots_chunk.size());
-      a.lb_row_of_slot = e.d_lb_row;
-      a.lb_groups = e.d_lb_groups;
-      a.n_lb_groups = static_cast<std::uint32_t>(groups.size());
-      launch_lb_scatter(e.W, a, stream);
-      HIP_CHECK(hipGetLastError());
-      HIP_CHECK(hipStreamSynchronize(stream));  // host vectors go out of scope
+      masks[static_cast<std::size_t>(pos_of[key])] |= W == 64U ? 1ULL << (s % 64U) : 1ULL;
+      for (auto const& [l, d] : pq[static_cast<std::size_t>(slots[s].q)].lb_up_) {
+        seed_idx.push_back(W == 64U ? ((s / 64U) * L + l) * 64U + s % 64U : s * L + l);
+        seed_min.push_back(d);
+      }
     }
+    auto const stream = e.stream;
+    if (seed_idx.size() > e.ph_seed_cap) {
+      e.ph_seed_cap = seed_idx.size() * 2U;
+      e.d_ph_seed_idx = e.grow(e.d_ph_seed_idx, e.ph_seed_cap);
+      e.d_ph_seed_min = e.grow(e.d_ph_seed_min, e.ph_seed_cap);
+    }
+    HIP_CHECK(hipMemcpyAsync(e.d_ph_sel, sel.data(), 4U * sel.size(), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(e.d_ph_masks, masks.data(), 8U * masks.size(), hipMemcpyHostToDevice, stream));
+    if (!seed_idx.empty()) {
+      HIP_CHECK(hipMemcpyAsync(e.d_ph_seed_idx, seed_idx.data(), 4U * seed_idx.size(), hipMemcpyHostToDevice,
+                               stream));
+      HIP_CHECK(hipMemcpyAsync(e.d_ph_seed_min, seed_min.data(), 2U * seed_min.size(), hipMemcpyHostToDevice,
+                               stream));
+    }
+    lb_phast_args a{};
+    a.sw = e.img->d.sweep[fwd ? 0 : 1];
+    a.dist = e.st.lb;
+    a.L = L;
+    a.sel = e.d_ph_sel;
+    a.masks = e.d_ph_masks;
+    a.n_sel = static_cast<std::uint32_t>(sel.size());
+    a.seed_idx = e.d_ph_seed_idx;
+    a.seed_min = e.d_ph_seed_min;
+    a.n_seeds = static_cast<std::uint32_t>(seed_idx.size());
+    auto const& lv = fwd ? tt.ch.sweep_fwd.level_off : tt.ch.sweep_bwd.level_off;
+    launch_lb_phast(W, a, lv.data(), static_cast<std::uint32_t>(lv.size() - 1U), stream);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(stream));  // host vectors go out of scope
   }
 };
 
@@ -1202,32 +1009,6 @@ void engine_run_dir(nigiri_amd_engine& e, bool fwd, std::vector<prepared_query>&
   }
 }
 
-// Batched engine, all queries known: lower bounds are computed ahead of
-// admission; queries are admitted as slots free up.
-void run_batched(nigiri_amd_engine& e, bool fwd, std::vector<prepared_query>& pq,
-                 std::vector<std::int32_t> const& queue) {
-  lb_pipeline pipe{e, fwd, pq};
-  for (auto const q : queue) pipe.submit(q);
-  runner r{e, *e.img->tt, fwd, pq};
-  std::size_t popped = 0U, done = 0U;
-  r.next = [&]() {
-    auto const q = pipe.pop_ready();
-    if (q >= 0) ++popped;
-    return q;
-  };
-  r.on_done = [&](std::int32_t) { ++done; };
-  r.release_row = [&](std::uint32_t row) { pipe.release(row); };
-  while (done < queue.size()) {
-    // with nothing running, wait until the free slots can be filled (or
-    // no more queries are coming); otherwise admit what is ready
-    if (!r.any_running()) pipe.wait_ready(std::min<std::size_t>(r.free_slots(), queue.size() - popped));
-    if (!r.step() && done < queue.size()) {
-      if (popped == queue.size()) throw status_error{NIGIRI_AMD_EDEVICE, "batched search made no progress"};
-      pipe.wait_ready(1U);
-    }
-  }
-}
-
 }  // namespace nigiri_amd
 
 namespace nigiri_amd {
@@ -1272,6 +1053,93 @@ int nigiri_amd_image_bytes(const nigiri_amd_image* img, uint64_t* bytes) {
 
 void nigiri_amd_image_destroy(nigiri_amd_image* img) { delete img; }
 
+namespace nigiri_amd {
+namespace {
+
+// The upward search (host) of a lower-bound query; throws on bad seeds.
+void upward_labels(nigiri_amd_image const& img, int32_t dir, uint32_t n_seeds, const uint32_t* seed_loc,
+                   const uint16_t* seed_min, std::vector<std::pair<std::uint32_t, std::uint16_t>>& up) {
+  thread_local ch_search_scratch s;
+  thread_local std::vector<std::pair<std::uint32_t, std::uint16_t>> seeds;
+  seeds.clear();
+  for (uint32_t i = 0; i != n_seeds; ++i) {
+    if (seed_loc[i] >= img.d.L) throw status_error{NIGIRI_AMD_EINVAL, "lower_bounds: seed out of range"};
+    seeds.emplace_back(seed_loc[i], seed_min[i]);
+  }
+  ch_upward(img.tt->ch, dir == NIGIRI_AMD_DIR_FWD, seeds, s, up);
+}
+
+// Stages the labels (host pinned-free copies into sel / seed arrays at the
+// given device addresses) and runs the sweep into the row lb of [L].
+void lb_row_sweep(nigiri_amd_image const& img, int32_t dir, std::vector<std::pair<std::uint32_t, std::uint16_t>> const& up,
+                  std::uint32_t* d_sel, std::uint32_t* d_idx, std::uint16_t* d_min, std::uint16_t* lb) {
+  thread_local std::vector<std::uint32_t> idx;
+  thread_local std::vector<std::uint16_t> mins;
+  idx.clear();
+  mins.clear();
+  for (auto const& [l, d] : up) {
+    idx.push_back(l);
+    mins.push_back(d);
+  }
+  std::uint32_t const zero = 0U;
+  HIP_CHECK(hipMemcpy(d_sel, &zero, 4U, hipMemcpyHostToDevice));
+  if (!idx.empty()) {
+    HIP_CHECK(hipMemcpy(d_idx, idx.data(), 4U * idx.size(), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(d_min, mins.data(), 2U * mins.size(), hipMemcpyHostToDevice));
+  }
+  lb_phast_args a{};
+  a.sw = img.d.sweep[dir == NIGIRI_AMD_DIR_FWD ? 0 : 1];
+  a.dist = lb;
+  a.L = img.d.L;
+  a.sel = d_sel;
+  a.masks = nullptr;
+  a.n_sel = 1U;
+  a.seed_idx = d_idx;
+  a.seed_min = d_min;
+  a.n_seeds = static_cast<std::uint32_t>(idx.size());
+  auto const& lv = dir == NIGIRI_AMD_DIR_FWD ? img.tt->ch.sweep_fwd.level_off : img.tt->ch.sweep_bwd.level_off;
+  launch_lb_phast(1U, a, lv.data(), static_cast<std::uint32_t>(lv.size() - 1U), nullptr);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(nullptr));
+}
+
+bool lb_args_ok(nigiri_amd_image const* img, int32_t dir, uint32_t n_seeds, const uint32_t* seed_loc,
+                const uint16_t* seed_min, uint16_t const* lb) {
+  return img != nullptr && lb != nullptr && (n_seeds == 0U || (seed_loc != nullptr && seed_min != nullptr)) &&
+         (dir == NIGIRI_AMD_DIR_FWD || dir == NIGIRI_AMD_DIR_BWD);
+}
+
+}  // namespace
+}  // namespace nigiri_amd
+
+int nigiri_amd_lower_bounds(nigiri_amd_image* img, int32_t dir, uint32_t n_seeds, const uint32_t* seed_loc,
+                            const uint16_t* seed_min, uint16_t* lb) {
+  if (!lb_args_ok(img, dir, n_seeds, seed_loc, seed_min, lb)) {
+    set_error("lower_bounds: bad argument");
+    return NIGIRI_AMD_EINVAL;
+  }
+  return guarded([&]() {
+    HIP_CHECK(hipSetDevice(img->device));
+    std::lock_guard<std::mutex> lock{img->lb_mu};
+    std::vector<std::pair<std::uint32_t, std::uint16_t>> up;
+    upward_labels(*img, dir, n_seeds, seed_loc, seed_min, up);
+    if (img->d_lb_sel == nullptr) img->d_lb_sel = img->mem.alloc<std::uint32_t>(1U);
+    if (up.size() > img->lb_seed_cap) {
+      if (img->d_lb_seed_idx != nullptr) (void)hipFree(img->d_lb_seed_idx);
+      if (img->d_lb_seed_min != nullptr) (void)hipFree(img->d_lb_seed_min);
+      img->d_lb_seed_idx = nullptr;
+      img->d_lb_seed_min = nullptr;
+      img->lb_seed_cap = 0U;
+      auto const cap = std::max<std::size_t>(up.size() * 2U, 1024U);
+      HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&img->d_lb_seed_idx), 4U * cap));
+      HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&img->d_lb_seed_min), 2U * cap));
+      img->lb_seed_cap = cap;
+    }
+    lb_row_sweep(*img, dir, up, img->d_lb_sel, img->d_lb_seed_idx, img->d_lb_seed_min, lb);
+    return NIGIRI_AMD_OK;
+  });
+}
+
 int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_groups,
                              uint32_t max_k, nigiri_amd_engine** out) {
   if (img == nullptr || out == nullptr || (lanes != 64U && lanes != 1U) || n_groups == 0U ||
@@ -1281,8 +1149,8 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
               "2 <= max_k <= 16");
     return NIGIRI_AMD_EINVAL;
   }
-  if (lanes == 64U && static_cast<std::uint64_t>(n_groups) * img->d.L * 64U >= (1ULL << 32)) {
-    set_error("engine_create: n_groups * n_locations * 64 must stay below 2^32");
+  if (static_cast<std::uint64_t>(n_groups) * img->d.L * lanes >= (1ULL << 32)) {
+    set_error("engine_create: n_groups * n_locations * lanes must stay below 2^32");
     return NIGIRI_AMD_EINVAL;
   }
   return guarded([&]() {
@@ -1389,37 +1257,9 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     e->d_worst = m.alloc<std::int16_t>(S);
     e->d_sweep_groups = m.alloc<std::uint32_t>(G + 1U);
     e->d_slot_list = m.alloc<std::uint32_t>(S);
-    e->lb_rows_cap = std::min<std::uint32_t>(S, 64U);
-    e->d_lb_staging = m.alloc<std::uint16_t>(static_cast<std::size_t>(e->lb_rows_cap) * L);
-    e->d_lb_slot = m.alloc<std::uint32_t>(e->lb_rows_cap);
-    e->d_lb_row = m.alloc<std::int32_t>(S);
-    e->d_lb_groups = m.alloc<std::uint32_t>(G);
-    if (W == 64U) {
-      // device lower-bound scratch: 28 bytes per location and query and
-      // worker, about 4 GiB in all, at least one query per launch
-      auto const per_q = 28ULL * L;
-      e->d_lb_pool = m.alloc<std::uint16_t>(static_cast<std::size_t>(S) * L);
-      e->lbw.resize(kLbWorkers);
-      for (auto& w : e->lbw) {
-        w.chunk = static_cast<std::uint32_t>(std::max<std::uint64_t>(
-            1U, std::min<std::uint64_t>(S, (4ULL << 30) / (kLbWorkers * per_q))));
-        auto const n = static_cast<std::size_t>(w.chunk) * L;
-        w.dist = m.alloc<std::uint32_t>(n);
-        w.stamp_near = m.alloc<std::uint32_t>(n);
-        w.stamp_far = m.alloc<std::uint32_t>(n);
-        w.near0 = m.alloc<std::uint32_t>(n);
-        w.near1 = m.alloc<std::uint32_t>(n);
-        w.far0 = m.alloc<std::uint32_t>(n);
-        w.far1 = m.alloc<std::uint32_t>(n);
-        w.out_row = m.alloc<std::uint32_t>(w.chunk);
-        w.seed_off = m.alloc<std::uint32_t>(static_cast<std::size_t>(w.chunk) + 1U);
-        w.seed_cap = static_cast<std::size_t>(w.chunk) * (kMaxDist + kMaxDestBits);
-        w.seed_loc = m.alloc<std::uint32_t>(w.seed_cap);
-        w.seed_dur = m.alloc<std::uint16_t>(w.seed_cap);
-        HIP_CHECK(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
-      }
-    }
-    e->h_lb.reserve(static_cast<std::size_t>(e->lb_rows_cap) * L);
+    e->d_lb_row = m.alloc<std::uint16_t>(L);
+    e->d_ph_sel = m.alloc<std::uint32_t>(S);
+    e->d_ph_masks = m.alloc<std::uint64_t>(S);
     e->h_active.reserve(G);
     e->h_rows.reserve(static_cast<std::size_t>(S) * kMaxDestBits * K);
     e->h_counts.reserve(8U);
@@ -1452,7 +1292,6 @@ int nigiri_amd_engine_configure(nigiri_amd_engine* e, const nigiri_engine_config
   if (c->transfer_pack_max_entries != 0U) {
     e->tune.transfer_pack_max_entries = std::min<std::uint32_t>(c->transfer_pack_max_entries, 64U);
   }
-  if (c->lb_delta != 0U) e->lb_delta = c->lb_delta;
   e->admit_when_idle = c->admit_when_idle != 0U;
   return NIGIRI_AMD_OK;
 }
@@ -1466,8 +1305,8 @@ namespace {
 double run_search(nigiri_amd_engine& e, const nigiri_query* qs, std::uint32_t n, std::vector<prepared_query>& pq,
                   std::vector<int>& status, std::vector<std::string>& errors) {
   auto const& tt = *e.img->tt;
-  // the batched engine computes lower bounds on the device
-  prepare_all(tt, qs, n, pq, status, errors, e.W != 64U);
+  // lower bounds: upward searches on the host, sweeps on the device
+  prepare_all(tt, qs, n, pq, status, errors, false);
   auto const t0 = std::chrono::steady_clock::now();
   for (int dir = 0; dir != 2; ++dir) {
     std::vector<std::int32_t> queue;
@@ -1477,10 +1316,6 @@ double run_search(nigiri_amd_engine& e, const nigiri_query* qs, std::uint32_t n,
       }
     }
     if (queue.empty()) continue;
-    if (e.W == 64U) {
-      run_batched(e, dir == 0, pq, queue);
-      continue;
-    }
     std::size_t head = 0U;
     engine_run_dir(
         e, dir == 0, pq, [&]() { return head < queue.size() ? queue[head++] : -1; }, [](std::int32_t) {}, {});
@@ -1501,7 +1336,6 @@ int nigiri_amd_engine_get_config(const nigiri_amd_engine* e, nigiri_engine_confi
   out->transfer_claim = e->tune.transfer_claim;
   out->pack_max_entries = e->tune.pack_max_entries;
   out->transfer_pack_max_entries = e->tune.transfer_pack_max_entries;
-  out->lb_delta = e->lb_delta;
   out->admit_when_idle = e->admit_when_idle ? 1U : 0U;
   return NIGIRI_AMD_OK;
 }
@@ -1524,15 +1358,13 @@ int nigiri_amd_engine_autotune(nigiri_amd_engine* e, const nigiri_query* qs, uin
       std::vector<std::uint32_t> values;
     };
     // knob ids as in nigiri_tune_trial: 0 grid_waves, 1 route_claim,
-    // 2 transfer_claim, 3 pack_max_entries, 4 lb_delta,
-    // 5 transfer_pack_max_entries
+    // 2 transfer_claim, 3 pack_max_entries, 4 transfer_pack_max_entries
     std::vector<knob> knobs{{0U, &e->grid_waves, {8U * c, 16U * c, 32U * c, 64U * c}},
                             {1U, &e->tune.route_claim, {1U, 2U, 4U}},
                             {2U, &e->tune.transfer_claim, {4U, 16U, 64U}}};
     if (e->W == 64U) {  // packing and device lower bounds: batched engine only
       knobs.push_back({3U, &e->tune.pack_max_entries, {1U, 2U, 4U, 8U, 64U}});
-      knobs.push_back({4U, &e->lb_delta, {4U, 8U, 16U, 32U, 64U}});
-      knobs.push_back({5U, &e->tune.transfer_pack_max_entries, {1U, 2U, 4U, 8U, 64U}});
+      knobs.push_back({4U, &e->tune.transfer_pack_max_entries, {1U, 2U, 4U, 8U, 64U}});
     }
     std::uint32_t nt = 0U;
     std::vector<prepared_query> pq;
@@ -1664,9 +1496,6 @@ int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs, const double*
     std::atomic<std::uint32_t> n_prepared_ctr{0U};
     std::atomic<std::uint32_t> n_failed{0U};
     bool const fwd = qs[0].dir == NIGIRI_AMD_DIR_FWD;
-    // batched engine: lower bounds are computed ahead of admission
-    std::unique_ptr<lb_pipeline> pipe;
-    if (e->W == 64U) pipe = std::make_unique<lb_pipeline>(*e, fwd, pq);
     auto const t0 = clock::now();
     auto const ms_since = [&](clock::time_point t) {
       return std::chrono::duration<double, std::milli>(t - t0).count();
@@ -1684,7 +1513,7 @@ int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs, const double*
                                     std::chrono::duration<double, std::milli>(arrival_ms[i]));
           std::this_thread::sleep_until(due);
           try {
-            prepare(tt, qs[i], pq[i], scratch, e->W != 64U);
+            prepare(tt, qs[i], pq[i], scratch, false);
           } catch (status_error const& ex) {
             status[i] = ex.code_;
           } catch (std::exception const&) {
@@ -1697,9 +1526,7 @@ int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs, const double*
             n_prepared_ctr.fetch_add(1U);
             continue;
           }
-          if (pipe) {
-            pipe->submit(static_cast<std::int32_t>(i));
-          } else {
+          {
             std::lock_guard<std::mutex> lock{mu};
             ready.push_back(static_cast<std::int32_t>(i));
           }
@@ -1709,15 +1536,11 @@ int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs, const double*
     }
     runner r{*e, tt, fwd, pq};
     r.next = [&]() -> std::int32_t {
-      if (pipe) return pipe->pop_ready();
       std::lock_guard<std::mutex> lock{mu};
       if (ready.empty()) return -1;
       auto const q = ready.front();
       ready.pop_front();
       return q;
-    };
-    r.release_row = [&](std::uint32_t row) {
-      if (pipe) pipe->release(row);
     };
     r.on_done = [&](std::int32_t q) {
       latency[static_cast<std::size_t>(q)] = ms_since(clock::now()) - arrival_ms[q];
@@ -1738,10 +1561,7 @@ int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs, const double*
         while (!(e->admit_when_idle && r.any_busy()) && clock::now() < deadline) {
           std::size_t n_ready = 0U;
           std::uint32_t n_settled = 0U;  // arrived queries that cannot get more ready
-          if (pipe) {
-            n_ready = pipe->n_ready();
-            n_settled = n_failed.load() + pipe->finished();
-          } else {
+          {
             std::lock_guard<std::mutex> lock{mu};
             n_ready = ready.size();
             n_settled = n_prepared.load();
@@ -1825,27 +1645,14 @@ int nigiri_amd_algo_init(nigiri_amd_engine* e, uint32_t slot, const nigiri_algo_
       e->d_dest_ops = e->grow(e->d_dest_ops, e->dest_ops_cap);
       e->d_dest_set = e->grow(e->d_dest_set, e->dest_ops_cap);
     }
-    auto const L = e->st.L;
-    std::memcpy(e->h_lb.p, init->lb, 2U * L);
-    std::vector<std::int32_t> row_of(e->S, -1);
-    row_of[slot] = 0;
-    std::uint32_t const g = slot / e->W;
-    HIP_CHECK(hipMemcpyAsync(e->d_lb_staging, e->h_lb.p, 2U * L, hipMemcpyHostToDevice, e->stream));
-    HIP_CHECK(hipMemcpyAsync(e->d_lb_slot, &slot, 4U, hipMemcpyHostToDevice, e->stream));
-    HIP_CHECK(hipMemcpyAsync(e->d_lb_row, row_of.data(), 4U * e->S, hipMemcpyHostToDevice, e->stream));
-    HIP_CHECK(hipMemcpyAsync(e->d_lb_groups, &g, 4U, hipMemcpyHostToDevice, e->stream));
+    // the caller's lower bounds (search.h:114-131 computes them) into the
+    // slot's column
+    HIP_CHECK(hipMemcpyAsync(e->d_lb_row, init->lb, 2U * e->st.L, hipMemcpyHostToDevice, e->stream));
+    launch_lb_put(e->W, e->st, slot, e->d_lb_row, e->stream);
     if (!ops.empty()) {
       HIP_CHECK(hipMemcpyAsync(e->d_dest_ops, ops.data(), 8U * ops.size(), hipMemcpyHostToDevice, e->stream));
       HIP_CHECK(hipMemcpyAsync(e->d_dest_set, set.data(), set.size(), hipMemcpyHostToDevice, e->stream));
     }
-    a.lb_staging = e->d_lb_staging;
-    a.lb_slot = e->d_lb_slot;
-    a.n_lb = 1U;
-    a.lb_row_of_slot = e->d_lb_row;
-    a.lb_groups = e->d_lb_groups;
-    a.n_lb_groups = 1U;
-    launch_lb_scatter(e->W, a, e->stream);
-    a.n_lb = 0U;
     a.dest_ops = e->d_dest_ops;
     a.dest_set = e->d_dest_set;
     a.n_dest_ops = static_cast<std::uint32_t>(ops.size());

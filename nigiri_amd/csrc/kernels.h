@@ -53,33 +53,31 @@ struct round_args {
   hipEvent_t ev[4];
 };
 
-// Device lower bounds: exact shortest travel times to the destination
-// offsets on the lower-bound graph with the 7201-minute cut-off of nigiri's
-// dial-bucket Dijkstra (dijkstra.cc:18-67), one workgroup per query.  Query
-// q's seeds are seed_loc/seed_dur[seed_off[q] .. seed_off[q+1]); its result
-// row is rows[out_row[q]][L] (0xFFFF = unreachable).  Scratch arrays are
-// [n_q][L].
-struct lb_args {
-  dev_image im;
+// Device lower bounds by PHAST (the exact shortest travel times of
+// dijkstra.cc:18-67, cut off at 7201 minutes; see ch.h): every selected
+// bound is set to unreachable, the upward search's labels are written, and
+// the downward sweep runs level by level.  W = 64: the selected bounds are
+// lanes masks[i] of group sel[i] in dist[G][L][64] (the 64 queries of a
+// group share each location's 128 B row); W = 1: rows sel[i] of dist[*][L].
+// A seed is an index into dist and its minutes.
+struct lb_phast_args {
+  dev_sweep sw;
+  std::uint16_t* dist;
   std::uint32_t L;
-  std::uint32_t n_q;
-  std::uint32_t delta;  // near-far threshold step in minutes (any value >= 1 is exact)
-  std::uint32_t const* seed_off;
-  std::uint32_t const* seed_loc;
-  std::uint16_t const* seed_dur;
-  std::uint32_t* dist;
-  std::uint32_t* stamp_near;
-  std::uint32_t* stamp_far;
-  std::uint32_t* near0;
-  std::uint32_t* near1;
-  std::uint32_t* far0;
-  std::uint32_t* far1;
-  std::uint32_t const* out_row;
-  std::uint16_t* rows;
+  std::uint32_t const* sel;
+  std::uint64_t const* masks;
+  std::uint32_t n_sel;
+  std::uint32_t const* seed_idx;
+  std::uint16_t const* seed_min;
+  std::uint32_t n_seeds;
 };
 
-// One workgroup per query, all sweeps in one launch.
-void launch_lb_queries(bool fwd, lb_args const& a, hipStream_t stream);
+// level_off: the sweep's level boundaries (host copy), one launch per level
+void launch_lb_phast(std::uint32_t W, lb_phast_args const& a, std::uint32_t const* level_off,
+                     std::uint32_t n_levels, hipStream_t stream);
+// lower bounds given as a row [L] (device) into the column of one slot
+void launch_lb_put(std::uint32_t W, dev_state const& st, std::uint32_t slot, std::uint16_t const* row,
+                   hipStream_t stream);
 
 // The batched route scan stages the events of a stop in LDS for routes of at
 // most this many transports; larger routes are scanned lane by lane from
@@ -114,14 +112,6 @@ struct epoch_args {
   std::uint32_t n_reset_groups;
   std::uint64_t const* reset_rt_mask;  // [G] lanes whose round_times reset
   std::uint64_t const* reset_bt_mask;  // [G] lanes whose best/tmp reset
-  // lower bounds: staging rows [n_lb][L] for slots lb_slot[i];
-  // lb_row_of_slot[slot] = staging row or -1; lb_groups = groups with rows
-  std::uint16_t const* lb_staging;
-  std::uint32_t const* lb_slot;
-  std::uint32_t n_lb;
-  std::int32_t const* lb_row_of_slot;
-  std::uint32_t const* lb_groups;
-  std::uint32_t n_lb_groups;
   // destination mask ops: (slot << 32 | location); dest_set[i] != 0 sets
   std::uint64_t const* dest_ops;
   std::uint8_t const* dest_set;
@@ -138,10 +128,9 @@ struct epoch_args {
 };
 
 // lane resets, destination masks, slot init, starts (lower bounds go
-// through launch_lb_scatter)
+// through launch_lb_phast)
 void launch_epoch_begin(bool fwd, std::uint32_t W, epoch_args const& a,
                         hipStream_t stream);
-void launch_lb_scatter(std::uint32_t W, epoch_args const& a, hipStream_t stream);
 // state download: round_times [K][L], best [L], tmp [L] of one slot
 void launch_gather_slot(std::uint32_t W, dev_state const& st, std::uint32_t slot,
                         std::int16_t* rt, std::int16_t* best, std::int16_t* tmp, std::uint16_t* lb,

@@ -1677,199 +1677,76 @@ __global__ void k_lane_reset(epoch_args a) {
   }
 }
 
-// ---- device lower bounds -------------------------------------------------
-// One workgroup per query: near-far label-correcting shortest paths on the
-// lower-bound graph from the query's destination offsets.  Locations whose
-// tentative distance is at most the threshold T are relaxed in sweeps (near
-// list); improvements above T wait in the far list until T advances to the
-// smallest waiting distance + delta - 1.  Distances are only accepted below
-// the cut-off of nigiri's dial (7201 = kMaxTravelTime + 1 buckets), so the
-// result is the exact shortest distance when it is below the cut-off and
-// unreachable (0xFFFF) otherwise: the values the bucket Dijkstra of
-// dijkstra.cc:18-67 produces.
-constexpr std::uint32_t kLbCut = 7201U;
-constexpr std::uint32_t kLbInf = 0xFFFFFFFFU;
-constexpr int kLbBlock = 1024;
+// ---- device lower bounds (PHAST) ---------------------------------------------
+constexpr std::uint32_t kLbCutoff = 7201U;  // dial buckets of dijkstra.h:52
 
-// Relaxes the lower-bound edges of u (tentative distance du < kLbCut): the
-// edge words and the targets' distances are loaded 8 at a time so that
-// their latencies overlap; push(t, nd) is called for every improvement.
-template <typename Push>
-__device__ inline void lb_relax_node(std::uint32_t u, std::uint32_t du, std::uint32_t const* off,
-                                     std::uint32_t const* edges, std::uint32_t* dist, Push&& push) {
-  auto const e0 = off[u], e1 = off[u + 1U];
-  for (auto b = e0; b < e1; b += 8U) {
-    std::uint32_t ed[8], dt[8];
-#pragma unroll
-    for (std::uint32_t j = 0U; j < 8U; ++j) ed[j] = b + j < e1 ? edges[b + j] : 0U;
-#pragma unroll
-    for (std::uint32_t j = 0U; j < 8U; ++j) dt[j] = b + j < e1 ? dist[ed[j] & ((1U << 23) - 1U)] : 0U;
-#pragma unroll
-    for (std::uint32_t j = 0U; j < 8U; ++j) {
-      if (b + j < e1) {
-        auto const t = ed[j] & ((1U << 23) - 1U);
-        auto const nd = du + (ed[j] >> 23);
-        if (nd < kLbCut && nd < dt[j]) {
-          auto const old = atomicMin(&dist[t], nd);
-          if (nd < old) push(t, nd);
-        }
-      }
-    }
-  }
-}
-
-template <bool Fwd>
-__global__ void __launch_bounds__(kLbBlock) k_lb_query(lb_args a) {
-  auto const q = blockIdx.x;
-  if (q >= a.n_q) return;
-  auto const L = a.L;
-  auto const tid = threadIdx.x;
-  auto const* off = Fwd ? a.im.lb_fwd_offsets : a.im.lb_bwd_offsets;
-  auto const* edges = Fwd ? a.im.lb_fwd : a.im.lb_bwd;
-  auto const base = static_cast<std::size_t>(q) * L;
-  auto* dist = a.dist + base;
-  auto* stamp_near = a.stamp_near + base;
-  auto* stamp_far = a.stamp_far + base;
-  std::uint32_t* near = a.near0 + base;
-  std::uint32_t* near_next = a.near1 + base;
-  std::uint32_t* far = a.far0 + base;
-  std::uint32_t* far_next = a.far1 + base;
-  __shared__ std::uint32_t s_near, s_far, s_min;
-
-  for (auto l = tid; l < L; l += kLbBlock) {
-    dist[l] = kLbInf;
-    stamp_near[l] = 0U;
-    stamp_far[l] = 0U;
-  }
-  if (tid == 0U) s_far = 0U;
-  __syncthreads();
-  auto const s0 = a.seed_off[q], s1 = a.seed_off[q + 1U];
-  for (auto i = s0 + tid; i < s1; i += kLbBlock) {
-    atomicMin(&dist[a.seed_loc[i]], static_cast<std::uint32_t>(a.seed_dur[i]));
-  }
-  __syncthreads();
-  // every seed starts in the far list of phase 1
-  for (auto i = s0 + tid; i < s1; i += kLbBlock) {
-    auto const l = a.seed_loc[i];
-    if (atomicExch(&stamp_far[l], 1U) != 1U) far[atomicAdd(&s_far, 1U)] = l;
-  }
-  __syncthreads();
-  std::uint32_t n_far = s_far, n_near = 0U;
-  std::uint32_t phase = 1U, sweep = 1U;
-  std::int64_t T = -1;  // every distance <= T has been relaxed
-  while (true) {
-    if (n_near == 0U) {
-      if (n_far == 0U) break;
-      // advance T past the smallest waiting distance, then split the far
-      // list (entries whose distance fell to <= T were relaxed already)
-      if (tid == 0U) s_min = kLbInf;
-      __syncthreads();
-      for (auto i = tid; i < n_far; i += kLbBlock) {
-        auto const d = dist[far[i]];
-        if (static_cast<std::int64_t>(d) > T) atomicMin(&s_min, d);
-      }
-      __syncthreads();
-      auto const mn = s_min;
-      if (mn == kLbInf) break;
-      auto const t_old = T;
-      T = static_cast<std::int64_t>(mn) + a.delta - 1;
-      ++phase;
-      ++sweep;
-      if (tid == 0U) {
-        s_near = 0U;
-        s_far = 0U;
-      }
-      __syncthreads();
-      for (auto i = tid; i < n_far; i += kLbBlock) {
-        auto const l = far[i];
-        auto const d = static_cast<std::int64_t>(dist[l]);
-        if (d <= t_old) continue;
-        if (d <= T) {
-          if (atomicExch(&stamp_near[l], sweep) != sweep) near_next[atomicAdd(&s_near, 1U)] = l;
-        } else if (atomicExch(&stamp_far[l], phase) != phase) {
-          far_next[atomicAdd(&s_far, 1U)] = l;
-        }
-      }
-      __syncthreads();
-      n_near = s_near;
-      n_far = s_far;
-      auto* tn = near;
-      near = near_next;
-      near_next = tn;
-      auto* tf = far;
-      far = far_next;
-      far_next = tf;
-      __syncthreads();
-      continue;
-    }
-    // sweep: relax every near location; improvements <= T join the next
-    // near list, larger ones the far list of this phase
-    ++sweep;
-    if (tid == 0U) {
-      s_near = 0U;
-      s_far = n_far;
-    }
-    __syncthreads();
-    for (auto i = tid; i < n_near; i += kLbBlock) {
-      auto const u = near[i];
-      lb_relax_node(u, dist[u], off, edges, dist, [&](std::uint32_t t, std::uint32_t nd) {
-        if (static_cast<std::int64_t>(nd) <= T) {
-          if (atomicExch(&stamp_near[t], sweep) != sweep) near_next[atomicAdd(&s_near, 1U)] = t;
-        } else if (atomicExch(&stamp_far[t], phase) != phase) {
-          far[atomicAdd(&s_far, 1U)] = t;
-        }
-      });
-    }
-    __syncthreads();
-    n_near = s_near;
-    n_far = s_far;
-    auto* tn = near;
-    near = near_next;
-    near_next = tn;
-    __syncthreads();
-  }
-  auto* row = a.rows + static_cast<std::size_t>(a.out_row[q]) * L;
-  for (auto l = tid; l < L; l += kLbBlock) {
-    auto const d = dist[l];
-    row[l] = d < kLbCut ? static_cast<std::uint16_t>(d) : kUnreachableLb;
-  }
-}
-
-// lower bounds: staging rows -> interleaved columns.  W == 64: LDS-tiled
-// transpose, one block per (group, 64-location tile), both sides coalesced.
+// every selected bound = unreachable
 template <int W>
-__global__ void k_lb_scatter(epoch_args a) {
-  auto const& s = a.st;
+__global__ void k_lb_fill(lb_phast_args a) {
+  auto const per = static_cast<std::size_t>(a.L) * W;
+  auto const total = per * a.n_sel;
+  for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < total;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    auto const k = static_cast<std::uint32_t>(i / per);
+    auto const r = i % per;
+    if (W == 64 && ((a.masks[k] >> (r % 64U)) & 1ULL) == 0ULL) continue;
+    a.dist[static_cast<std::size_t>(a.sel[k]) * per + r] = kUnreachableLb;
+  }
+}
+
+// the upward searches' labels
+__global__ void k_lb_seed(lb_phast_args a) {
+  for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n_seeds; i += gridDim.x * blockDim.x) {
+    a.dist[a.seed_idx[i]] = a.seed_min[i];
+  }
+}
+
+// one level of the downward sweep: bound(v) = min(bound(v), bound(u) + w)
+// over v's edges from higher-ranked u; values of kLbCutoff or more are
+// unreachable.  W = 64: one wavefront per (location, group), one query per
+// lane, each edge one 128 B row load; W = 1: one thread per (location, row).
+template <int W>
+__global__ void k_lb_level(lb_phast_args a, std::uint32_t b, std::uint32_t e) {
+  auto const& sw = a.sw;
+  auto const n_items = (e - b) * a.n_sel;
   if constexpr (W == 64) {
-    __shared__ std::uint16_t tile[64][65];
-    auto const n_tiles = (s.L + 63U) / 64U;
-    for (std::uint32_t b = blockIdx.x; b < n_tiles * a.n_lb_groups; b += gridDim.x) {
-      auto const g = a.lb_groups[b / n_tiles];
-      auto const l0 = (b % n_tiles) * 64U;
-      __syncthreads();
-      for (std::uint32_t v = threadIdx.x; v < 64U * 64U; v += blockDim.x) {
-        auto const lane = v / 64U, x = v % 64U;
-        auto const row = a.lb_row_of_slot[g * 64U + lane];
-        if (row >= 0 && l0 + x < s.L) {
-          tile[x][lane] = a.lb_staging[static_cast<std::size_t>(row) * s.L + l0 + x];
-        }
+    auto const lane = threadIdx.x & 63U;
+    auto const n_waves = gridDim.x * (blockDim.x / 64U);
+    for (std::uint32_t item = (blockIdx.x * blockDim.x + threadIdx.x) / 64U; item < n_items; item += n_waves) {
+      auto const p = b + item / a.n_sel;
+      auto const k = item % a.n_sel;
+      auto const v = uni(sw.order[p]);
+      auto const e0 = uni(sw.edge_off[p]), e1 = uni(sw.edge_off[p + 1U]);
+      auto* col = a.dist + static_cast<std::size_t>(uni(a.sel[k])) * a.L * 64U + lane;
+      std::uint32_t d = col[static_cast<std::size_t>(v) * 64U];
+#pragma unroll 4
+      for (auto x = e0; x < e1; ++x) {
+        auto const u = uni(sw.src[x]);
+        d = min(d, static_cast<std::uint32_t>(col[static_cast<std::size_t>(u) * 64U]) + sw.w[x]);
       }
-      __syncthreads();
-      for (std::uint32_t v = threadIdx.x; v < 64U * 64U; v += blockDim.x) {
-        auto const x = v / 64U, lane = v % 64U;
-        if (a.lb_row_of_slot[g * 64U + lane] >= 0 && l0 + x < s.L) {
-          s.lb[lay<W>::col(s, g, l0 + x, lane)] = tile[x][lane];
-        }
+      if (((a.masks[k] >> lane) & 1ULL) != 0ULL) {
+        col[static_cast<std::size_t>(v) * 64U] = static_cast<std::uint16_t>(d < kLbCutoff ? d : kUnreachableLb);
       }
     }
   } else {
-    auto const total = static_cast<std::size_t>(a.n_lb) * s.L;
-    for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < total;
-         i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
-      auto const row = i / s.L;
-      auto const l = static_cast<std::uint32_t>(i % s.L);
-      s.lb[lay<W>::col(s, a.lb_slot[row], l, 0U)] = a.lb_staging[i];
+    for (std::uint32_t item = blockIdx.x * blockDim.x + threadIdx.x; item < n_items;
+         item += gridDim.x * blockDim.x) {
+      auto const p = b + item / a.n_sel;
+      auto const v = sw.order[p];
+      auto* row = a.dist + static_cast<std::size_t>(a.sel[item % a.n_sel]) * a.L;
+      std::uint32_t d = row[v];
+      for (auto x = sw.edge_off[p]; x < sw.edge_off[p + 1U]; ++x) {
+        d = min(d, static_cast<std::uint32_t>(row[sw.src[x]]) + sw.w[x]);
+      }
+      row[v] = static_cast<std::uint16_t>(d < kLbCutoff ? d : kUnreachableLb);
     }
+  }
+}
+
+template <int W>
+__global__ void k_lb_put(dev_state s, std::uint32_t slot, std::uint16_t const* row) {
+  for (std::uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < s.L; l += gridDim.x * blockDim.x) {
+    s.lb[lay<W>::col(s, slot / W, l, slot % W)] = row[l];
   }
 }
 
@@ -2393,21 +2270,38 @@ hipError_t launch_round_graph(round_graphs& g, bool fwd, std::uint32_t W, round_
   return hipSuccess;
 }
 
-void launch_lb_queries(bool fwd, lb_args const& a, hipStream_t stream) {
-  if (a.n_q == 0U) return;
-  if (fwd) {
-    hipLaunchKernelGGL((k_lb_query<true>), dim3(a.n_q), dim3(kLbBlock), 0, stream, a);
+void launch_lb_phast(std::uint32_t W, lb_phast_args const& a, std::uint32_t const* level_off,
+                     std::uint32_t n_levels, hipStream_t stream) {
+  if (a.n_sel == 0U) return;
+  auto const fill = static_cast<std::size_t>(a.n_sel) * a.L * W;
+  if (W == 64U) {
+    hipLaunchKernelGGL((k_lb_fill<64>), dim3(grid_for(fill, kBlock, 8192)), dim3(kBlock), 0, stream, a);
   } else {
-    hipLaunchKernelGGL((k_lb_query<false>), dim3(a.n_q), dim3(kLbBlock), 0, stream, a);
+    hipLaunchKernelGGL((k_lb_fill<1>), dim3(grid_for(fill, kBlock, 8192)), dim3(kBlock), 0, stream, a);
+  }
+  if (a.n_seeds != 0U) {
+    hipLaunchKernelGGL(k_lb_seed, dim3(grid_for(a.n_seeds, kBlock, 1024)), dim3(kBlock), 0, stream, a);
+  }
+  for (std::uint32_t l = 0U; l != n_levels; ++l) {
+    auto const b = level_off[l], e = level_off[l + 1U];
+    if (b == e) continue;
+    auto const items = static_cast<std::size_t>(e - b) * a.n_sel;
+    if (W == 64U) {
+      hipLaunchKernelGGL((k_lb_level<64>), dim3(grid_for(items * 64U, kBlock, 8192)), dim3(kBlock), 0, stream,
+                         a, b, e);
+    } else {
+      hipLaunchKernelGGL((k_lb_level<1>), dim3(grid_for(items, kBlock, 8192)), dim3(kBlock), 0, stream, a, b, e);
+    }
   }
 }
 
-void launch_lb_scatter(std::uint32_t W, epoch_args const& a, hipStream_t stream) {
-  if (a.n_lb == 0U) return;
+void launch_lb_put(std::uint32_t W, dev_state const& st, std::uint32_t slot, std::uint16_t const* row,
+                   hipStream_t stream) {
+  auto const grid = dim3(grid_for(st.L, kBlock, 4096));
   if (W == 64U) {
-    hipLaunchKernelGGL((k_lb_scatter<64>), dim3(4096), dim3(kBlock), 0, stream, a);
+    hipLaunchKernelGGL((k_lb_put<64>), grid, dim3(kBlock), 0, stream, st, slot, row);
   } else {
-    hipLaunchKernelGGL((k_lb_scatter<1>), dim3(4096), dim3(kBlock), 0, stream, a);
+    hipLaunchKernelGGL((k_lb_put<1>), grid, dim3(kBlock), 0, stream, st, slot, row);
   }
 }
 

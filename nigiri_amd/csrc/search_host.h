@@ -91,6 +91,8 @@ struct prepared_query {
   std::vector<std::uint16_t> lb_;  // [n_locations] (host lower bounds only)
   // shortest-path sources of the lower bounds: (location, offset minutes)
   std::vector<std::pair<std::uint32_t, std::uint16_t>> lb_seeds_;
+  // PHAST upward search from them: (location, minutes), for the device sweep
+  std::vector<std::pair<std::uint32_t, std::uint16_t>> lb_up_;
   std::vector<epoch> epochs_;
   std::uint64_t locality_key_{0U};  // Morton code of the first start stop
   std::int32_t slot_{-1};           // engine slot that ran the query
@@ -109,14 +111,17 @@ struct prepared_query {
   }
 };
 
-// Scratch reused across queries by one host worker (Dijkstra buckets).
+// Scratch reused across queries by one host worker (Dijkstra buckets,
+// upward searches).
 struct host_scratch {
   std::vector<std::vector<std::uint32_t>> buckets_;
+  ch_search_scratch ch_;
 };
 
 // Throws status_error for unsupported or malformed queries.  With
-// host_lb == false the lower bounds are left to the device (lb_seeds_ holds
-// their sources either way).
+// host_lb == true the lower bounds are computed on the host (lb_); else the
+// device computes them from the upward search lb_up_ (lb_seeds_ holds their
+// sources either way).
 void prepare(timetable const& tt, nigiri_query const& q, prepared_query& p,
              host_scratch& s, bool host_lb = true);
 

@@ -263,6 +263,7 @@ void finalize(timetable& tt) {
   build_footpaths_in(tt);
   build_lb_graph(tt, true);
   build_lb_graph(tt, false);
+  tt.ch = build_ch(tt.n_locations, tt.lb_fwd_offsets, tt.lb_fwd);
 }
 
 std::string validate(timetable const& tt) {

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/nigiri_amd.h"
+#include "ch.h"
 
 namespace nigiri_amd {
 
@@ -71,6 +72,8 @@ struct timetable {
   // class here: every trip counts with the priority of class kOther (1).
   std::vector<std::uint32_t> location_importance;
   std::uint32_t max_importance = 0U;
+  // contraction hierarchy of lb_fwd (exact lower bounds by PHAST)
+  contraction_hierarchy ch;
 
   std::uint32_t n_routes() const {
     return static_cast<std::uint32_t>(route_stop_offsets.size() - 1U);
@@ -86,7 +89,8 @@ struct timetable {
 // (src/loader/init_finish.cc:200-234): location_routes_, route_traffic_days_
 // (init_finish.cc:168-198, appending one bitfield per route), footpaths_in_
 // as the duration-sorted transpose of footpaths_out_ (build_footpaths.cc:
-// 404-436), and both lower-bound graphs (loader/build_lb_graph.h:11-99).
+// 404-436), and both lower-bound graphs (loader/build_lb_graph.h:11-99);
+// then the contraction hierarchy of the lower-bound graph.
 void finalize(timetable& tt);
 
 // Checks structural invariants (CSR bounds, FIFO order modulo 1440 inside

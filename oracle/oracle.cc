@@ -324,6 +324,15 @@ struct counters {
   oracle_stats s{};
 };
 
+// The algorithmic-byte counter of the measurement model; compiled out for
+// the timed CPU baseline (-DORACLE_NO_BYTE_COUNTERS), which then keeps only
+// the work counters nigiri's raptor_stats keeps too (raptor_stats.h:9-52).
+#ifdef ORACLE_NO_BYTE_COUNTERS
+#define ORACLE_ALG_BYTES(x) ((void)0)
+#else
+#define ORACLE_ALG_BYTES(x) (c_.s.alg_bytes += (x))
+#endif
+
 // ---------------------------------------------------------------------------
 // raptor.h:25-1535 with Rt = false, Vias = 0, SearchMode = kOneToOne.
 template <bool Fwd>
@@ -431,11 +440,11 @@ struct raptor {
       for (auto i = 0U; i != n_locations_; ++i) {
         if (!state_.station_mark_[i]) continue;
         ++c_.s.n_marked_stations;
-        c_.s.alg_bytes += 8U;
+        ORACLE_ALG_BYTES(8U);
         for (auto j = tt_.v_.location_route_offsets[i];
              j != tt_.v_.location_route_offsets[i + 1]; ++j) {
           ++c_.s.n_location_routes_read;
-          c_.s.alg_bytes += 4U;
+          ORACLE_ALG_BYTES(4U);
           any_marked = true;
           state_.route_mark_[tt_.v_.location_routes[j]] = true;
         }
@@ -497,7 +506,7 @@ struct raptor {
     for (auto i = 0U; i != n_locations_; ++i) {
       if (!state_.prev_station_mark_[i]) continue;
       ++c_.s.n_transfer_stations;
-      c_.s.alg_bytes += 2U + 1U + 8U + (dist_to_end_.empty() ? 0U : 2U);
+      ORACLE_ALG_BYTES(2U + 1U + 8U + (dist_to_end_.empty() ? 0U : 2U));
       auto const tmp_time = tmp[i];
       if (tmp_time == kInvalid) continue;
       auto const is_dest = is_dest_[i];
@@ -518,7 +527,7 @@ struct raptor {
         if (!within_bounds(k, i, fp_target_time)) continue;
         state_.rt(k, i) = fp_target_time;
         best[i] = fp_target_time;
-        c_.s.alg_bytes += 4U;
+        ORACLE_ALG_BYTES(4U);
         state_.station_mark_[i] = true;
         if (is_dest) update_time_at_dest(k, fp_target_time);
       }
@@ -535,7 +544,7 @@ struct raptor {
       if (!state_.prev_station_mark_[i]) continue;
       for (auto e = off[i]; e != off[i + 1]; ++e) {
         ++c_.s.n_footpaths_visited;
-        c_.s.alg_bytes += 6U;
+        ORACLE_ALG_BYTES(6U);
         auto const fp = fps[e];
         auto const target = fp_target(fp);
         auto const tmp_time = tmp[i];
@@ -555,7 +564,7 @@ struct raptor {
           if (!within_bounds(k, target, fp_target_time)) continue;
           state_.rt(k, target) = fp_target_time;
           best[target] = fp_target_time;
-          c_.s.alg_bytes += 4U;
+          ORACLE_ALG_BYTES(4U);
           state_.station_mark_[target] = true;
           if (is_dest_[target]) update_time_at_dest(k, fp_target_time);
         }
@@ -578,7 +587,7 @@ struct raptor {
         if (is_better_loose(end_time, time_at_dest_[k])) {
           state_.rt(k, kEnd) = end_time;
           best[kEnd] = end_time;
-          c_.s.alg_bytes += 4U;
+          ORACLE_ALG_BYTES(4U);
           update_time_at_dest(k, end_time);
         }
       }
@@ -598,7 +607,7 @@ struct raptor {
       auto const l_idx = stop_loc(stp);
       auto const is_last = i == n_stops - 1U;
       ++c_.s.n_route_stops_visited;
-      c_.s.alg_bytes += 8U;
+      ORACLE_ALG_BYTES(8U);
 
       delta_t current_best = kInvalid;
       if (et.is_valid()) {
@@ -610,7 +619,7 @@ struct raptor {
               is_better_loose(by_transport + dir(lb_[l_idx]), time_at_dest_[k]) &&
               within_bounds(k, l_idx, by_transport)) {
             tmp[l_idx] = get_best(by_transport, tmp[l_idx]);
-            c_.s.alg_bytes += 2U;
+            ORACLE_ALG_BYTES(2U);
             state_.station_mark_[l_idx] = true;
             if (is_better(by_transport, current_best)) current_best = by_transport;
             any_marked = true;
@@ -658,7 +667,7 @@ struct raptor {
       auto pos = 0U;
       for (; pos != n_t; ++pos) {
         ++c_.s.n_events_scanned;
-        c_.s.alg_bytes += 2U;
+        ORACLE_ALG_BYTES(2U);
         if (!is_better(delta_mam(ev_at(pos).second), mam_at_stop)) break;
       }
       return pos;
@@ -668,14 +677,14 @@ struct raptor {
     for (auto i = 0U; i != static_cast<unsigned>(n_days_to_iterate); ++i) {
       auto const day = static_cast<std::uint16_t>(Fwd ? day_at_stop + i : day_at_stop - i);
       ++c_.s.n_route_day_tests;
-      c_.s.alg_bytes += 8U;
+      ORACLE_ALG_BYTES(8U);
       if (!tt_.is_route_active(r, day)) continue;
       auto const begin = i == 0U ? seek_first_day() : 0U;
       if (begin == n_t) continue;
       for (auto pos = begin; pos != n_t; ++pos) {
         auto const [t_offset, ev] = ev_at(pos);
         ++c_.s.n_events_scanned;
-        c_.s.alg_bytes += 2U;
+        ORACLE_ALG_BYTES(2U);
         auto const ev_mam = delta_mam(ev);
         if (!is_better_loose(to_delta(day, ev_mam) + dir(lb_[l]), time_at_dest_[k])) {
           return {~0U, 0xFFFFU};
@@ -685,7 +694,7 @@ struct raptor {
         auto const ev_day_offset = delta_days(ev);
         auto const start_day = static_cast<std::uint16_t>(static_cast<int>(day) - ev_day_offset);
         ++c_.s.n_transport_day_tests;
-        c_.s.alg_bytes += 8U + 4U;
+        ORACLE_ALG_BYTES(8U + 4U);
         if (!tt_.is_transport_active(t, start_day)) continue;
         return {t, start_day};
       }

@@ -33,7 +33,29 @@ def build(arch_flags=""):
     return LIB_PATH
 
 
+BASELINE_PATH = os.path.join(HERE, "_build", "liboracle_baseline.so")
+
+
+def build_baseline():
+    """Compiles the timed CPU baseline (-O3 -march=native, no byte counters) for
+    the machine it runs on."""
+    subprocess.run(["make", "-C", HERE, "baseline"], check=True, stdout=subprocess.PIPE,
+                   stderr=subprocess.STDOUT)
+    return BASELINE_PATH
+
+
 _lib = None
+_baseline = None
+
+
+def baseline_lib():
+    """The baseline build (search_many only is meant to be called on it)."""
+    global _baseline
+    if _baseline is None:
+        h = C.CDLL(build_baseline())
+        _bind(h)
+        _baseline = h
+    return _baseline
 
 
 def lib():
@@ -41,31 +63,35 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        from nigiri_amd import TtView, Query, Journey, Leg  # struct layouts only
-        h = C.CDLL(LIB_PATH)
-        P = C.POINTER
-        h.oracle_search.restype = C.c_int
-        h.oracle_search.argtypes = [P(TtView), P(Query), P(Journey), C.c_uint32, P(C.c_uint32),
-                                    P(Leg), C.c_uint32, P(C.c_uint32),
-                                    P(C.c_int16), P(C.c_int16), P(C.c_int16), P(C.c_int16),
-                                    P(C.c_uint16), P(C.c_int32), P(Stats)]
-        h.oracle_pong.restype = C.c_int
-        h.oracle_pong.argtypes = [P(TtView), P(Query), P(Journey), C.c_uint32, P(C.c_uint32),
-                                  P(Leg), C.c_uint32, P(C.c_uint32), P(C.c_int32), P(Stats)]
-        h.oracle_optimize_footpaths.restype = C.c_int
-        h.oracle_optimize_footpaths.argtypes = [P(TtView), P(Query), P(Leg), C.c_uint32]
-        h.oracle_lb.restype = C.c_int
-        h.oracle_lb.argtypes = [P(TtView), P(Query), P(C.c_uint16)]
-        h.oracle_fastest_direct.restype = C.c_int
-        h.oracle_fastest_direct.argtypes = [P(TtView), P(Query), P(C.c_int16)]
-        h.oracle_starts.restype = C.c_int
-        h.oracle_starts.argtypes = [P(TtView), P(Query), P(C.c_int32), P(C.c_int32),
-                                    P(C.c_uint32), C.c_uint32, P(C.c_uint32)]
-        h.oracle_search_many.restype = C.c_int
-        h.oracle_search_many.argtypes = [P(TtView), P(Query), C.c_uint32, C.c_int,
-                                         P(C.c_double), P(C.c_double), P(C.c_uint32), P(Stats)]
-        _lib = h
+        _lib = _bind(C.CDLL(LIB_PATH))
     return _lib
+
+
+def _bind(h):
+    """Declares the entry points' signatures on a loaded library."""
+    from nigiri_amd import TtView, Query, Journey, Leg  # struct layouts only
+    P = C.POINTER
+    h.oracle_search.restype = C.c_int
+    h.oracle_search.argtypes = [P(TtView), P(Query), P(Journey), C.c_uint32, P(C.c_uint32),
+                                P(Leg), C.c_uint32, P(C.c_uint32),
+                                P(C.c_int16), P(C.c_int16), P(C.c_int16), P(C.c_int16),
+                                P(C.c_uint16), P(C.c_int32), P(Stats)]
+    h.oracle_pong.restype = C.c_int
+    h.oracle_pong.argtypes = [P(TtView), P(Query), P(Journey), C.c_uint32, P(C.c_uint32),
+                              P(Leg), C.c_uint32, P(C.c_uint32), P(C.c_int32), P(Stats)]
+    h.oracle_optimize_footpaths.restype = C.c_int
+    h.oracle_optimize_footpaths.argtypes = [P(TtView), P(Query), P(Leg), C.c_uint32]
+    h.oracle_lb.restype = C.c_int
+    h.oracle_lb.argtypes = [P(TtView), P(Query), P(C.c_uint16)]
+    h.oracle_fastest_direct.restype = C.c_int
+    h.oracle_fastest_direct.argtypes = [P(TtView), P(Query), P(C.c_int16)]
+    h.oracle_starts.restype = C.c_int
+    h.oracle_starts.argtypes = [P(TtView), P(Query), P(C.c_int32), P(C.c_int32),
+                                P(C.c_uint32), C.c_uint32, P(C.c_uint32)]
+    h.oracle_search_many.restype = C.c_int
+    h.oracle_search_many.argtypes = [P(TtView), P(Query), C.c_uint32, C.c_int,
+                                     P(C.c_double), P(C.c_double), P(C.c_uint32), P(Stats)]
+    return h
 
 
 def _p(a, t):
@@ -172,13 +198,16 @@ def starts(tt, q, cap=1 << 16):
     return list(zip(a[:k].tolist(), b[:k].tolist(), s[:k].tolist()))
 
 
-def search_many(tt, queries, threads=1):
+def search_many(tt, queries, threads=1, baseline=False):
+    """Throughput run (benchmark.cc:210-270 style); baseline=True times the
+    -O3 -march=native build without byte counters."""
     n = len(queries)
     lat = np.zeros(max(n, 1), np.float64)
     nj = np.zeros(max(n, 1), np.uint32)
     wall = C.c_double()
     stats = Stats()
-    rc = lib().oracle_search_many(C.byref(tt.view), queries.q, n, threads, _p(lat, C.c_double),
+    h = baseline_lib() if baseline else lib()
+    rc = h.oracle_search_many(C.byref(tt.view), queries.q, n, threads, _p(lat, C.c_double),
                                   C.byref(wall), _p(nj, C.c_uint32), C.byref(stats))
     if rc != 0:
         raise RuntimeError("oracle_search_many failed")

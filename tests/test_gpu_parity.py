@@ -93,8 +93,7 @@ def test_gpu_point_queries_workload1(lanes):
 
 
 def test_gpu_large_batch_workload1():
-    # a batch larger than the multi-workgroup lower-bound sweep takes: its
-    # lower bounds come from the one-workgroup-per-query kernel
+    # a batch of four groups: the lower-bound sweeps of all four run together
     tt, img = image_for("w1", lambda: na.Timetable.generate(1))
     eng = na.Engine(img, lanes=64, groups=4, max_k=16)
     qs = na.Queries.generate(tt, 1, 256, seed=0x7172 + 11)
@@ -299,14 +298,14 @@ def test_gpu_launch_knobs_do_not_change_results():
     tt, img = image_for("w3s", lambda: na.Timetable.generate(3, scale=0.03))
     qs = na.Queries.generate(tt, 3, 128, seed=0x7172 + 43)
     eng = na.Engine(img, lanes=64, groups=2, max_k=10)
-    eng.configure(grid_waves=512, route_claim=4, transfer_claim=3, pack_max_entries=1, lb_delta=64,
+    eng.configure(grid_waves=512, route_claim=4, transfer_claim=3, pack_max_entries=1,
                   transfer_pack_max_entries=2)
     cfg = eng.get_config()
     assert (cfg["grid_waves"], cfg["route_claim"], cfg["transfer_claim"], cfg["pack_max_entries"],
-            cfg["lb_delta"], cfg["transfer_pack_max_entries"]) == (512, 4, 3, 1, 64, 2)
+            cfg["transfer_pack_max_entries"]) == (512, 4, 3, 1, 2)
     say("knobs away from defaults, workload3 mix n=128")
     compare(tt, eng, qs)
-    eng.configure(route_claim=2, transfer_claim=64, pack_max_entries=3, lb_delta=1,
+    eng.configure(route_claim=2, transfer_claim=64, pack_max_entries=3,
                   transfer_pack_max_entries=1)
     say("knobs, second setting")
     compare(tt, eng, qs)
@@ -318,12 +317,12 @@ def test_gpu_autotune_picks_candidates_and_keeps_results():
     eng = na.Engine(img, lanes=64, groups=1, max_k=10)
     say("autotune workload3 n=64")
     chosen, trials = eng.autotune(qs)
-    assert len(trials) == 4 + 3 + 3 + 5 + 5 + 5
+    assert len(trials) == 4 + 3 + 3 + 5 + 5
     for knob, value in chosen.items():
         tried = [(v, ms) for k, v, ms in trials if k == knob]
         assert value in [v for v, _ in tried], (knob, value)
         assert all(ms > 0.0 for _, ms in tried)
-    assert eng.get_config()["lb_delta"] == chosen["lb_delta"]
+    assert eng.get_config()["pack_max_entries"] == chosen["pack_max_entries"]
     compare(tt, eng, qs)
 
 

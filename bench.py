@@ -45,12 +45,14 @@ WORKLOADS = {
 
 
 def sync():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
+    """Device-wide synchronisation through the HIP runtime the library links
+    (a second runtime, such as torch's bundled one, must not be initialised
+    next to it)."""
+    import ctypes
+    import nigiri_amd as na
+    na.lib()
+    if ctypes.CDLL("libamdhip64.so.7").hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipDeviceSynchronize failed")
 
 
 def route_scan_bytes(work):

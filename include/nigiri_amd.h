@@ -442,6 +442,13 @@ int nigiri_amd_engine_stats(nigiri_amd_engine* e, nigiri_engine_stats* out,
  *   slot is busy, so that a batch starts together while the preparation and
  *   lower bounds of the next batch overlap the current batch's rounds
  *   (0 = a query takes any free slot at an epoch boundary; default).
+ * state_budget_bytes: admission by device state memory.  A query takes
+ *   n_locations * (2 * end_k + 9) bytes while it runs (end_k rows of int16
+ *   round_times plus its best, tmp, lower-bound and mark columns) and is
+ *   admitted only while the admitted queries' total stays within the
+ *   budget; it waits otherwise.  A query that can never fit (more than the
+ *   budget, or end_k above the engine's max_k) fails alone with
+ *   NIGIRI_AMD_ECAPACITY.  0 = the memory of all slots at max_k rows.
  */
 typedef struct nigiri_engine_config {
   double admission_wait_ms;
@@ -452,6 +459,7 @@ typedef struct nigiri_engine_config {
   uint32_t pack_max_entries;
   uint32_t admit_when_idle;
   uint32_t transfer_pack_max_entries;
+  uint64_t state_budget_bytes;
 } nigiri_engine_config;
 
 int nigiri_amd_engine_configure(nigiri_amd_engine* e,

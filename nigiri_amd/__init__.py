@@ -131,7 +131,7 @@ class EngineConfig(C.Structure):
                 ("eager_rounds", C.c_uint32), ("route_claim", C.c_uint32),
                 ("transfer_claim", C.c_uint32), ("pack_max_entries", C.c_uint32),
                 ("admit_when_idle", C.c_uint32),
-                ("transfer_pack_max_entries", C.c_uint32)]
+                ("transfer_pack_max_entries", C.c_uint32), ("state_budget_bytes", C.c_uint64)]
 
 
 class TuneTrial(C.Structure):
@@ -547,7 +547,7 @@ class Engine:
         return b.value
 
     def configure(self, admission_wait_ms=2.0, grid_waves=0, graphs=True, admit_when_idle=False,
-                  **knobs):
+                  state_budget_bytes=0, **knobs):
         """Sets the engine configuration; launch-shape knobs (route_claim,
         transfer_claim, pack_max_entries, transfer_pack_max_entries)
         left out keep their values."""
@@ -556,6 +556,7 @@ class Engine:
         c.grid_waves = grid_waves
         c.eager_rounds = 0 if graphs else 1
         c.admit_when_idle = 1 if admit_when_idle else 0
+        c.state_budget_bytes = state_budget_bytes
         for k, v in knobs.items():
             if k not in TUNE_KNOBS[1:]:
                 raise TypeError(f"unknown engine knob {k}")

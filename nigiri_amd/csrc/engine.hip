@@ -265,6 +265,8 @@ struct nigiri_amd_engine {
   round_tune tune{1U, 16U, 64U, 64U};
   // serving: admit new queries only when no slot is busy (whole batches)
   bool admit_when_idle = false;
+  // device state budget of the admitted queries (0: every slot's full state)
+  std::uint64_t state_budget = 0U;
   round_graphs* graphs = round_graphs_create();
   // instrumentation
   bool profile = false;
@@ -460,6 +462,13 @@ struct runner {
   std::function<std::int32_t()> next;            // next query id or -1
   std::function<void(std::int32_t)> on_done;      // query finished
   std::function<void(std::int32_t)> on_admit;     // query takes a slot
+  // query refused (status code, message); it is not handed to on_done
+  std::function<void(std::int32_t, int, std::string const&)> on_fail;
+  // admission by device state memory: what the admitted queries' rows of
+  // round_times and their best / tmp / lb / mark columns take, against the
+  // budget; a query that does not fit now waits in `held`
+  std::uint64_t used_bytes = 0U;
+  std::int32_t held = -1;
   std::vector<slot_info> slots;
   std::uint64_t rounds = 0U, epochs = 0U;
   // legs being written on the host pool, and the queries waiting for them
@@ -478,6 +487,16 @@ struct runner {
       e.pool->wait();
     } catch (...) {
     }
+  }
+
+  std::uint64_t budget() const {
+    return e.state_budget != 0U ? e.state_budget
+                                : static_cast<std::uint64_t>(e.S) * query_state_bytes(e.st.L, e.K);
+  }
+
+  void fail(std::int32_t q, int code, std::string const& msg) {
+    if (!on_fail) throw status_error{code, msg};
+    on_fail(q, code, msg);
   }
 
   round_args rargs() const {
@@ -535,7 +554,7 @@ struct runner {
         sp.tts_factor = p.tts_.factor_;
         sp.tts_default = p.tts_.default_ ? 1U : 0U;
         sp.base = p.base_;
-        sp.end_k = static_cast<std::uint8_t>(std::min<std::uint32_t>(p.end_k_, K));
+        sp.end_k = p.end_k_;  // <= K: checked at admission
         sp.intermodal = p.intermodal_dest_ ? 1U : 0U;
         sp.bounds_last_k = 0U;
         if (p.dests_.size() > kMaxDestBits || p.dist_loc_.size() > kMaxDist) {
@@ -661,7 +680,9 @@ struct runner {
     // is busy (their preparation and lower bounds overlap the rounds before)
     bool const gate_closed = e.admit_when_idle && (!starting.empty() || any_running());
     while (!gate_closed && admitted.size() < free_list.size()) {
-      auto const q = next();
+      std::int32_t q = held;
+      held = -1;
+      if (q < 0) q = next();
       if (q < 0) break;
       auto& p = pq[static_cast<std::size_t>(q)];
       if (p.epochs_.empty()) {  // overlap or no start labels
@@ -669,6 +690,23 @@ struct runner {
         on_done(q);
         continue;
       }
+      // admission by the device memory the query's state takes: its
+      // end_k rows of round_times over all locations, plus best, tmp, lb and
+      // marks; refused alone when it can never fit
+      auto const need = query_state_bytes(e.st.L, p.end_k_);
+      if (p.end_k_ > K || need > budget()) {
+        fail(q, NIGIRI_AMD_ECAPACITY,
+             "query needs " + std::to_string(p.end_k_) + " rows of round_times (" + std::to_string(need) +
+                 " B of state); the engine holds " + std::to_string(K) + " rows per slot, budget " +
+                 std::to_string(budget()) + " B");
+        continue;
+      }
+      if (used_bytes + need > budget()) {
+        held = q;  // admitted once running queries free their state
+        break;
+      }
+      used_bytes += need;
+      p.state_bytes_ = need;
       admitted.push_back(q);
     }
     std::stable_sort(admitted.begin(), admitted.end(), [&](std::int32_t a, std::int32_t b) {
@@ -766,6 +804,10 @@ struct runner {
         join_legs();
         if (extend_interval(tt, p)) continue;  // the slot runs the new epochs
       }
+      // the query's device state is free once its last epoch's journeys
+      // are reconstructed (the legs are written on the host from records)
+      used_bytes -= p.state_bytes_;
+      p.state_bytes_ = 0U;
       completing.push_back(si.q);
     }
     if (!legs_pending) settle();
@@ -1000,12 +1042,12 @@ std::uint32_t write_results(std::vector<prepared_query> const& pq, std::vector<i
 // used by the serving loop (serve.cc)
 void engine_run_dir(nigiri_amd_engine& e, bool fwd, std::vector<prepared_query>& pq,
                     std::function<std::int32_t()> next, std::function<void(std::int32_t)> done,
-                    std::function<void()> after_step) {
+                    std::function<void(std::int32_t, int, std::string const&)> fail) {
   runner r{e, *e.img->tt, fwd, pq};
   r.next = std::move(next);
   r.on_done = std::move(done);
-  while (r.step()) {
-    if (after_step) after_step();
+  r.on_fail = std::move(fail);
+  while (r.step() || r.held >= 0) {
   }
 }
 
@@ -1293,6 +1335,7 @@ int nigiri_amd_engine_configure(nigiri_amd_engine* e, const nigiri_engine_config
     e->tune.transfer_pack_max_entries = std::min<std::uint32_t>(c->transfer_pack_max_entries, 64U);
   }
   e->admit_when_idle = c->admit_when_idle != 0U;
+  e->state_budget = c->state_budget_bytes;
   return NIGIRI_AMD_OK;
 }
 
@@ -1318,7 +1361,11 @@ double run_search(nigiri_amd_engine& e, const nigiri_query* qs, std::uint32_t n,
     if (queue.empty()) continue;
     std::size_t head = 0U;
     engine_run_dir(
-        e, dir == 0, pq, [&]() { return head < queue.size() ? queue[head++] : -1; }, [](std::int32_t) {}, {});
+        e, dir == 0, pq, [&]() { return head < queue.size() ? queue[head++] : -1; }, [](std::int32_t) {},
+        [&](std::int32_t q, int code, std::string const& msg) {
+          status[static_cast<std::size_t>(q)] = code;
+          errors[static_cast<std::size_t>(q)] = msg;
+        });
   }
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -1337,6 +1384,7 @@ int nigiri_amd_engine_get_config(const nigiri_amd_engine* e, nigiri_engine_confi
   out->pack_max_entries = e->tune.pack_max_entries;
   out->transfer_pack_max_entries = e->tune.transfer_pack_max_entries;
   out->admit_when_idle = e->admit_when_idle ? 1U : 0U;
+  out->state_budget_bytes = e->state_budget;
   return NIGIRI_AMD_OK;
 }
 
@@ -1543,6 +1591,11 @@ int nigiri_amd_serve(nigiri_amd_engine* e, const nigiri_query* qs, const double*
       return q;
     };
     r.on_done = [&](std::int32_t q) {
+      latency[static_cast<std::size_t>(q)] = ms_since(clock::now()) - arrival_ms[q];
+      n_done.fetch_add(1U);
+    };
+    r.on_fail = [&](std::int32_t q, int code, std::string const&) {
+      status[static_cast<std::size_t>(q)] = code;
       latency[static_cast<std::size_t>(q)] = ms_since(clock::now()) - arrival_ms[q];
       n_done.fetch_add(1U);
     };

@@ -96,7 +96,7 @@ struct prepared_query {
   std::vector<epoch> epochs_;
   std::uint64_t locality_key_{0U};  // Morton code of the first start stop
   std::int32_t slot_{-1};           // engine slot that ran the query
-  std::int32_t lb_row_{-1};         // device lower-bound row (batched engine)
+  std::uint64_t state_bytes_{0U};   // device state charged at admission
   std::vector<host_journey> results_;  // running Pareto set
 
   std::int32_t base_minutes(timetable const& tt) const {
@@ -110,6 +110,13 @@ struct prepared_query {
     return intermodal_dest_ ? kEnd : dests_[j];
   }
 };
+
+// Device state memory of one query: end_k rows of round_times (int16) over
+// all locations, plus its best (int16), tmp (int32), lower-bound (uint16)
+// and route-phase mark (one byte) columns.
+inline std::uint64_t query_state_bytes(std::uint32_t n_locations, std::uint32_t end_k) {
+  return static_cast<std::uint64_t>(n_locations) * (2U * end_k + 9U);
+}
 
 // Scratch reused across queries by one host worker (Dijkstra buckets,
 // upward searches).

@@ -154,10 +154,12 @@ def test_gpu_served_results_equal_alone_and_in_mix():
     arrivals = np.cumsum(np.random.default_rng(1).exponential(2.0, len(qs)))
     served, wall = eng.serve(qs, arrivals)
     alone = na.Engine(img, lanes=64, groups=1, max_k=10)
-    for i in range(0, 96, 12):
+    for i in range(len(qs)):
         one = alone.search(qs.subset([i]))[0]
         assert served[i]["journeys"] == one["journeys"], i
-        assert served[i]["journeys"] == oracle.search(tt, qs[i])["journeys"], i
+        assert served[i]["legs"] == one["legs"], i
+        ref = oracle.search(tt, qs[i])
+        assert served[i]["journeys"] == ref["journeys"] and served[i]["legs"] == ref["legs"], i
         assert served[i]["latency_ms"] >= 0.0
 
 
@@ -199,10 +201,13 @@ def test_gpu_batched_engine_routes_beyond_its_lds_staging():
     b.add_line([9, 13], [([i * 30 + 1, i * 30 + 50], [day]) for i in range(20)])
     tt = b.build()
     img = na.Image(tt)
-    specs = [dict(start=[(9, 0)], dest=[(13, 0)], time=day * 1440 + 100 + 37 * i) for i in range(40)]
-    specs += [dict(start=[(13, 0)], dest=[(9, 0)], time=day * 1440 + 900 + 11 * i, dir=na.DIR_BWD)
-              for i in range(8)]
-    specs += [dict(start=[(10, 0)], dest=[(12, 0)], interval=(day * 1440 + 60, day * 1440 + 240))]
+    # max_transfers 6: end_k 8 rows, what the engines below hold
+    specs = [dict(start=[(9, 0)], dest=[(13, 0)], time=day * 1440 + 100 + 37 * i, max_transfers=6)
+             for i in range(40)]
+    specs += [dict(start=[(13, 0)], dest=[(9, 0)], time=day * 1440 + 900 + 11 * i, dir=na.DIR_BWD,
+                   max_transfers=6) for i in range(8)]
+    specs += [dict(start=[(10, 0)], dest=[(12, 0)], interval=(day * 1440 + 60, day * 1440 + 240),
+                   max_transfers=6)]
     for lanes in (64, 1):
         eng = na.Engine(img, lanes=lanes, groups=1 if lanes == 64 else 4, max_k=8)
         for d in (na.DIR_FWD, na.DIR_BWD):

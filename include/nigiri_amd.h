@@ -290,6 +290,18 @@ void nigiri_amd_image_destroy(nigiri_amd_image* img);
 int nigiri_amd_lower_bounds(nigiri_amd_image* img, int32_t dir,
                             uint32_t n_seeds, const uint32_t* seed_loc,
                             const uint16_t* seed_min, uint16_t* lb);
+/* nigiri_amd_lower_bounds without any device allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * graph capture set-up, tight serving loops, harnesses that forbid
+ * hipMalloc).  The upward search's labels are staged in a static device
+ * array of 6 MiB + 256 B per GPU (held until the process ends), so a call
+ * takes at most 2^20 labels (NIGIRI_AMD_ECAPACITY beyond; the searches here
+ * settle a few thousand).  All calls on one GPU share that array: calls that
+ * overlap on one GPU race.  Same arguments and results as
+ * nigiri_amd_lower_bounds. */
+int nigiri_amd_lower_bounds_noalloc(nigiri_amd_image* img, int32_t dir,
+                                    uint32_t n_seeds, const uint32_t* seed_loc,
+                                    const uint16_t* seed_min, uint16_t* lb);
 
 /* ---- engine: device query state (replaces gpu_raptor_state,
  * gpu/raptor.h:47-53, and raptor_state, raptor/raptor_state.h:22-111) ----

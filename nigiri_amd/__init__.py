@@ -171,6 +171,8 @@ SIGNATURES = {
     "nigiri_amd_image_destroy": (None, [_P]),
     "nigiri_amd_lower_bounds": (C.c_int, [_P, C.c_int32, C.c_uint32, C.POINTER(C.c_uint32),
                                           C.POINTER(C.c_uint16), C.c_void_p]),
+    "nigiri_amd_lower_bounds_noalloc": (C.c_int, [_P, C.c_int32, C.c_uint32, C.POINTER(C.c_uint32),
+                                                  C.POINTER(C.c_uint16), C.c_void_p]),
     "nigiri_amd_engine_create": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
     "nigiri_amd_engine_state_bytes": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "nigiri_amd_engine_destroy": (None, [_P]),
@@ -480,13 +482,15 @@ class Queries:
 class Image:
     """Device-resident timetable (one per GPU)."""
 
-    def lower_bounds(self, seeds, direction, lb_ptr):
+    def lower_bounds(self, seeds, direction, lb_ptr, noalloc=False):
         """Lower bounds for seeds [(location, minutes)] into the device buffer at
-        lb_ptr (n_locations uint16): nigiri_amd_lower_bounds."""
+        lb_ptr (n_locations uint16): nigiri_amd_lower_bounds, or its variant that
+        allocates nothing (noalloc=True)."""
         loc = np.array([l for l, _ in seeds], np.uint32)
         mins = np.array([d for _, d in seeds], np.uint16)
-        _check(lib().nigiri_amd_lower_bounds(self.handle, direction, len(seeds), ptr(loc, C.c_uint32),
-                                             ptr(mins, C.c_uint16), C.c_void_p(lb_ptr)))
+        fn = lib().nigiri_amd_lower_bounds_noalloc if noalloc else lib().nigiri_amd_lower_bounds
+        _check(fn(self.handle, direction, len(seeds), ptr(loc, C.c_uint32), ptr(mins, C.c_uint16),
+                  C.c_void_p(lb_ptr)))
 
     def __init__(self, tt, device=0):
         self.tt = tt  # the image borrows the host timetable

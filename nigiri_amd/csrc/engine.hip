@@ -1154,6 +1154,65 @@ bool lb_args_ok(nigiri_amd_image const* img, int32_t dir, uint32_t n_seeds, cons
 }  // namespace
 }  // namespace nigiri_amd
 
+// Scratch of nigiri_amd_lower_bounds_noalloc: the selected row index (first
+// 256 bytes), then up to kNoAllocMaxLabels upward-search labels (location
+// indices, then minutes).  Each GPU holds its own copy until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress
+// gives on that device.
+constexpr std::size_t kNoAllocMaxLabels = std::size_t{1} << 20;
+constexpr std::size_t kNoAllocScratchBytes = 256U + kNoAllocMaxLabels * 6U;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace nigiri_amd {
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch_ptr;  // per device, set up on first use
+
+unsigned char* scratch_on(int device) {
+  std::lock_guard<std::mutex> lock{g_scratch_mu};
+  if (g_scratch_ptr.size() <= static_cast<std::size_t>(device)) {
+    g_scratch_ptr.resize(static_cast<std::size_t>(device) + 1U, nullptr);
+  }
+  auto& p = g_scratch_ptr[static_cast<std::size_t>(device)];
+  if (p == nullptr) {  // the device is current (hipSetDevice by the caller)
+    void* addr = nullptr;
+    auto const err = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || addr == nullptr) {
+      throw status_error{NIGIRI_AMD_EDEVICE, std::string{"hipGetSymbolAddress(scratch): "} + hipGetErrorString(err)};
+    }
+    p = static_cast<unsigned char*>(addr);
+  }
+  return p;
+}
+
+}  // namespace
+}  // namespace nigiri_amd
+
+int nigiri_amd_lower_bounds_noalloc(nigiri_amd_image* img, int32_t dir, uint32_t n_seeds, const uint32_t* seed_loc,
+                                    const uint16_t* seed_min, uint16_t* lb) {
+  if (!lb_args_ok(img, dir, n_seeds, seed_loc, seed_min, lb)) {
+    set_error("lower_bounds_noalloc: bad argument");
+    return NIGIRI_AMD_EINVAL;
+  }
+  return guarded([&]() {
+    HIP_CHECK(hipSetDevice(img->device));
+    auto* base = scratch_on(img->device);
+    thread_local std::vector<std::pair<std::uint32_t, std::uint16_t>> up;
+    upward_labels(*img, dir, n_seeds, seed_loc, seed_min, up);
+    if (up.size() > kNoAllocMaxLabels) {
+      throw status_error{NIGIRI_AMD_ECAPACITY, "lower_bounds_noalloc: " + std::to_string(up.size()) +
+                                                   " upward-search labels exceed the scratch capacity of " +
+                                                   std::to_string(kNoAllocMaxLabels)};
+    }
+    auto* d_sel = reinterpret_cast<std::uint32_t*>(base);
+    auto* d_idx = reinterpret_cast<std::uint32_t*>(base + 256U);
+    auto* d_min = reinterpret_cast<std::uint16_t*>(base + 256U + 4U * kNoAllocMaxLabels);
+    lb_row_sweep(*img, dir, up, d_sel, d_idx, d_min, lb);
+    return NIGIRI_AMD_OK;
+  });
+}
+
 int nigiri_amd_lower_bounds(nigiri_amd_image* img, int32_t dir, uint32_t n_seeds, const uint32_t* seed_loc,
                             const uint16_t* seed_min, uint16_t* lb) {
   if (!lb_args_ok(img, dir, n_seeds, seed_loc, seed_min, lb)) {

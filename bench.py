@@ -159,7 +159,7 @@ def served_main(args):
         arr = np.cumsum(rng.exponential(1000.0 / args.rate, B))
         eng.serve(qs.subset(list(range(w * B, (w + 1) * B))), arr - arr[0], **caps)
         print(f"[bench] warmup batch {w} done", file=sys.stderr, flush=True)
-    eng.stats(reset=True)
+    warm_launches = eng.stats(reset=True)["route_scan_launches"]
     first = args.warmup * B
     idx = list(range(first, n_total))
     arr = np.cumsum(rng.exponential(1000.0 / args.rate, len(idx)))
@@ -169,7 +169,7 @@ def served_main(args):
     res, wall = eng.serve(qs.subset(idx), arr, **caps)
     sync()
     t_total = time.perf_counter() - t_start
-    st = eng.stats()
+    st = eng.stats(reset=True)
     n = len(idx)
     value = n / t_total
     lat = np.array([r["latency_ms"] for r in res])
@@ -196,6 +196,11 @@ def served_main(args):
         t = time.perf_counter()
         plain.search(qs.subset([i]))
         lone_p.append((time.perf_counter() - t) * 1e3)
+    # batched route-scan dispatches in launch order (for matching a profiler's
+    # per-dispatch records to the timed region)
+    print(f"[bench] batched route-scan dispatches: warm-up {warm_launches}, timed "
+          f"{st['route_scan_launches']}, after {eng.stats()['route_scan_launches']}",
+          file=sys.stderr, flush=True)
     # the plain one-query-per-launch path on a sample of the same stream
     ps = qs.subset(idx[:args.plain_sample])
     sync()

@@ -450,6 +450,9 @@ int nigiri_amd_engine_stats(nigiri_amd_engine* e, nigiri_engine_stats* out,
  *   packed route wavefront may mix (1-64; 64).
  * transfer_pack_max_entries: (station, group) entries, i.e. 64-query
  *   groups, that one packed transfer wavefront may mix (1-64; 64).
+ * loose_lanes: packed route wavefronts of fewer queries than this are
+ *   scanned loose instead, one (route, query) per lane, so that sparse
+ *   routes do not each hold a whole wavefront (1 = none loose; 1-33; 17).
  * admit_when_idle: serving loop; 1 = new queries are admitted only when no
  *   slot is busy, so that a batch starts together while the preparation and
  *   lower bounds of the next batch overlap the current batch's rounds
@@ -472,6 +475,8 @@ typedef struct nigiri_engine_config {
   uint32_t admit_when_idle;
   uint32_t transfer_pack_max_entries;
   uint64_t state_budget_bytes;
+  uint32_t loose_lanes;
+  uint32_t pad_;
 } nigiri_engine_config;
 
 int nigiri_amd_engine_configure(nigiri_amd_engine* e,
@@ -483,7 +488,7 @@ int nigiri_amd_engine_get_config(const nigiri_amd_engine* e,
 /* ---- autotuner (no counterpart in the reference) ------------------------
  * Chooses the launch-shape knobs of nigiri_engine_config (grid_waves,
  * route_claim, transfer_claim, pack_max_entries,
- * transfer_pack_max_entries) for this engine and timetable on a sample of
+ * transfer_pack_max_entries, loose_lanes) for this engine and timetable on a sample of
  * queries (ideally one batch of the real
  * workload): one pass of coordinate descent in which every candidate value
  * of a knob runs the sample through the engine `repeats` times and the value
@@ -492,7 +497,7 @@ int nigiri_amd_engine_get_config(const nigiri_amd_engine* e,
  * reported as (knob, value, ms) in trials[0 .. min(n, trial_capacity)),
  * with the total count in *n_trials (may be NULL).  Knob ids: 0 grid_waves,
  * 1 route_claim, 2 transfer_claim, 3 pack_max_entries,
- * 4 transfer_pack_max_entries.
+ * 4 transfer_pack_max_entries, 5 loose_lanes.
  * Results never depend on the knobs. */
 typedef struct nigiri_tune_trial {
   uint32_t knob;

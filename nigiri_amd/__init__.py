@@ -131,7 +131,8 @@ class EngineConfig(C.Structure):
                 ("eager_rounds", C.c_uint32), ("route_claim", C.c_uint32),
                 ("transfer_claim", C.c_uint32), ("pack_max_entries", C.c_uint32),
                 ("admit_when_idle", C.c_uint32),
-                ("transfer_pack_max_entries", C.c_uint32), ("state_budget_bytes", C.c_uint64)]
+                ("transfer_pack_max_entries", C.c_uint32), ("state_budget_bytes", C.c_uint64),
+                ("loose_lanes", C.c_uint32), ("pad_", C.c_uint32)]
 
 
 class TuneTrial(C.Structure):
@@ -139,7 +140,7 @@ class TuneTrial(C.Structure):
 
 
 TUNE_KNOBS = ["grid_waves", "route_claim", "transfer_claim", "pack_max_entries",
-              "transfer_pack_max_entries"]
+              "transfer_pack_max_entries", "loose_lanes"]
 
 
 WORK_NAMES = ["route_stops", "events", "route_day_tests", "transport_day_tests", "tmp_writes",
@@ -553,8 +554,8 @@ class Engine:
     def configure(self, admission_wait_ms=2.0, grid_waves=0, graphs=True, admit_when_idle=False,
                   state_budget_bytes=0, **knobs):
         """Sets the engine configuration; launch-shape knobs (route_claim,
-        transfer_claim, pack_max_entries, transfer_pack_max_entries)
-        left out keep their values."""
+        transfer_claim, pack_max_entries, transfer_pack_max_entries,
+        loose_lanes) left out keep their values."""
         c = EngineConfig()
         c.admission_wait_ms = admission_wait_ms
         c.grid_waves = grid_waves

@@ -142,7 +142,13 @@ struct dev_state {
   std::uint16_t* packed_lanes;
   std::uint32_t* counts;   // [0] list_t, [1] list_r, [2] list_route, [3] hz, [4] overflow flags,
                            // [5] packed route wavefronts, [6] / [7] work cursors of
-                           // the route scan / transfer kernels (batched engine)
+                           // the route scan / transfer kernels, [8] loose route
+                           // items, [9] their cursor (batched engine); 16 entries
+  // batched engine: loose route items (route << 32 | g << 6 | lane): the
+  // queries of packed wavefronts too sparse to share a route's staging,
+  // scanned one (route, query) per lane
+  std::uint64_t* loose;
+  std::size_t cap_loose;
   // algorithmic work counters (see kWork* indices)
   unsigned long long* work;
   std::size_t cap_t, cap_r, cap_route, cap_hz;

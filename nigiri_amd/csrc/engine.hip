@@ -262,7 +262,7 @@ struct nigiri_amd_engine {
   // rounds replayed from captured hipGraphs (off: kernels launched one by one)
   bool use_graphs = true;
   // launch-shape knobs (nigiri_amd_engine_autotune chooses them)
-  round_tune tune{1U, 16U, 64U, 64U};
+  round_tune tune{1U, 16U, 64U, 64U, 17U};
   // serving: admit new queries only when no slot is busy (whole batches)
   bool admit_when_idle = false;
   // device state budget of the admitted queries (0: every slot's full state)
@@ -1314,8 +1314,12 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     st.cap_pack = W == 64U ? std::max(st.cap_route, st.cap_r) : 0U;
     st.packed_info = W == 64U ? m.alloc<std::uint64_t>(st.cap_pack) : nullptr;
     st.packed_lanes = W == 64U ? m.alloc<std::uint16_t>(st.cap_pack * 64U) : nullptr;
+    // loose route items: only packed wavefronts of fewer than 33 queries go
+    // loose, and each of their list entries adds fewer than 33 items
+    st.cap_loose = W == 64U ? 32U * st.cap_route : 0U;
+    st.loose = W == 64U ? m.alloc<std::uint64_t>(st.cap_loose) : nullptr;
     st.list_hz = m.alloc<std::uint64_t>(st.cap_hz);
-    st.counts = m.alloc<std::uint32_t>(8U);
+    st.counts = m.alloc<std::uint32_t>(16U);
     st.work = m.alloc<unsigned long long>(kWorkCount);
     st.param = m.alloc<slot_param>(S);
     st.kslot = m.alloc<std::uint8_t>(S);
@@ -1342,7 +1346,7 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     HIP_CHECK(hipMemsetAsync(st.grp_active, 0, 8U * G, e->stream));
     HIP_CHECK(hipMemsetAsync(st.any_route, 0, static_cast<std::size_t>(G) * W, e->stream));
     HIP_CHECK(hipMemsetAsync(st.any_tmp, 0, static_cast<std::size_t>(G) * W, e->stream));
-    HIP_CHECK(hipMemsetAsync(st.counts, 0, 32U, e->stream));
+    HIP_CHECK(hipMemsetAsync(st.counts, 0, 64U, e->stream));
     HIP_CHECK(hipMemsetAsync(st.work, 0, 8U * kWorkCount, e->stream));
     HIP_CHECK(hipMemsetAsync(st.n_chg, 0, 4U * S, e->stream));
     HIP_CHECK(hipMemsetAsync(st.kslot, 0, S, e->stream));
@@ -1393,6 +1397,7 @@ int nigiri_amd_engine_configure(nigiri_amd_engine* e, const nigiri_engine_config
   if (c->transfer_pack_max_entries != 0U) {
     e->tune.transfer_pack_max_entries = std::min<std::uint32_t>(c->transfer_pack_max_entries, 64U);
   }
+  if (c->loose_lanes != 0U) e->tune.loose_lanes = std::min<std::uint32_t>(c->loose_lanes, 33U);
   e->admit_when_idle = c->admit_when_idle != 0U;
   e->state_budget = c->state_budget_bytes;
   return NIGIRI_AMD_OK;
@@ -1442,6 +1447,7 @@ int nigiri_amd_engine_get_config(const nigiri_amd_engine* e, nigiri_engine_confi
   out->transfer_claim = e->tune.transfer_claim;
   out->pack_max_entries = e->tune.pack_max_entries;
   out->transfer_pack_max_entries = e->tune.transfer_pack_max_entries;
+  out->loose_lanes = e->tune.loose_lanes;
   out->admit_when_idle = e->admit_when_idle ? 1U : 0U;
   out->state_budget_bytes = e->state_budget;
   return NIGIRI_AMD_OK;
@@ -1465,13 +1471,15 @@ int nigiri_amd_engine_autotune(nigiri_amd_engine* e, const nigiri_query* qs, uin
       std::vector<std::uint32_t> values;
     };
     // knob ids as in nigiri_tune_trial: 0 grid_waves, 1 route_claim,
-    // 2 transfer_claim, 3 pack_max_entries, 4 transfer_pack_max_entries
+    // 2 transfer_claim, 3 pack_max_entries, 4 transfer_pack_max_entries,
+    // 5 loose_lanes
     std::vector<knob> knobs{{0U, &e->grid_waves, {8U * c, 16U * c, 32U * c, 64U * c}},
                             {1U, &e->tune.route_claim, {1U, 2U, 4U}},
                             {2U, &e->tune.transfer_claim, {4U, 16U, 64U}}};
     if (e->W == 64U) {  // packing and device lower bounds: batched engine only
       knobs.push_back({3U, &e->tune.pack_max_entries, {1U, 2U, 4U, 8U, 64U}});
       knobs.push_back({4U, &e->tune.transfer_pack_max_entries, {1U, 2U, 4U, 8U, 64U}});
+      knobs.push_back({5U, &e->tune.loose_lanes, {1U, 9U, 17U, 33U}});
     }
     std::uint32_t nt = 0U;
     std::vector<prepared_query> pq;

@@ -42,6 +42,7 @@ struct round_tune {
   std::uint32_t transfer_claim;    // stations claimed per atomic (transfers)
   std::uint32_t pack_max_entries;  // (route, group) entries per packed wavefront
   std::uint32_t transfer_pack_max_entries;  // (station, group) entries per packed transfer wavefront
+  std::uint32_t loose_lanes;  // packed route wavefronts of fewer lanes go loose (1: none; <= 33)
 };
 
 struct round_args {

@@ -900,7 +900,15 @@ __global__ void k_route_pack(round_args a) {
   auto const& im = a.im;
   auto const& s = a.st;
   __shared__ std::uint32_t fills_all[kWavesPerBlock][64];
+  // per packed wavefront of the window: its slot among the kept ones (or
+  // ~0 when it goes loose) / its offset among the loose lanes; the window's
+  // two bases
+  __shared__ std::uint32_t kidx_all[kWavesPerBlock][64], loff_all[kWavesPerBlock][64];
+  __shared__ std::uint32_t bases_all[kWavesPerBlock][2];
   auto* fills = fills_all[threadIdx.x / 64U];
+  auto* kidx = kidx_all[threadIdx.x / 64U];
+  auto* loff = loff_all[threadIdx.x / 64U];
+  auto* bases = bases_all[threadIdx.x / 64U];
   auto const lane = threadIdx.x & 63U;
   auto const n_items = clamp_count(s.counts[2], s.cap_route);
   auto const wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64U;
@@ -951,22 +959,52 @@ __global__ void k_route_pack(round_args a) {
     }
     if (n_w != 0U && lane == 0U) fills[n_w - 1U] = fill;
     wave_sync_lds();
-    std::uint32_t base = 0U;
-    if (lane == 0U && n_w != 0U) base = atomicAdd(&s.counts[5], n_w);
-    base = static_cast<std::uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(base)));
+    // wavefronts of fewer than loose_lanes queries go loose: their queries
+    // are scanned one (route, query) per lane; the kept ones take packed
+    // slots without gaps
+    if (lane == 0U) {
+      std::uint32_t n_keep = 0U, n_loose = 0U;
+      for (std::uint32_t w = 0U; w < n_w; ++w) {
+        if (fills[w] < a.tune.loose_lanes) {
+          kidx[w] = ~0U;
+          loff[w] = n_loose;
+          n_loose += fills[w];
+        } else {
+          kidx[w] = n_keep++;
+        }
+      }
+      bases[0] = n_keep != 0U ? atomicAdd(&s.counts[5], n_keep) : 0U;
+      bases[1] = n_loose != 0U ? atomicAdd(&s.counts[8], n_loose) : 0U;
+    }
+    wave_sync_lds();
     if (c != 0U) {
-      auto const pw = static_cast<std::size_t>(base) + my_w;
-      if (pw < s.cap_pack) {
-        if (my_o == 0U) s.packed_info[pw] = (static_cast<std::uint64_t>(r) << 32) | fills[my_w];
-        auto* out = s.packed_lanes + pw * 64U + my_o;
-        auto m = mask;
-        for (std::uint32_t j = 0U; m != 0ULL; ++j) {
-          auto const b = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1);
-          m &= m - 1ULL;
-          out[j] = static_cast<std::uint16_t>((g << 6) | b);
+      if (kidx[my_w] != ~0U) {
+        auto const pw = static_cast<std::size_t>(bases[0]) + kidx[my_w];
+        if (pw < s.cap_pack) {
+          if (my_o == 0U) s.packed_info[pw] = (static_cast<std::uint64_t>(r) << 32) | fills[my_w];
+          auto* out = s.packed_lanes + pw * 64U + my_o;
+          auto m = mask;
+          for (std::uint32_t j = 0U; m != 0ULL; ++j) {
+            auto const b = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1);
+            m &= m - 1ULL;
+            out[j] = static_cast<std::uint16_t>((g << 6) | b);
+          }
+        } else {
+          atomicOr(&s.counts[4], 1U);
         }
       } else {
-        atomicOr(&s.counts[4], 1U);
+        auto const at = static_cast<std::size_t>(bases[1]) + loff[my_w] + my_o;
+        if (at + c <= s.cap_loose) {
+          auto* out = s.loose + at;
+          auto m = mask;
+          for (std::uint32_t j = 0U; m != 0ULL; ++j) {
+            auto const b = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1);
+            m &= m - 1ULL;
+            out[j] = (static_cast<std::uint64_t>(r) << 32) | (g << 6) | b;
+          }
+        } else {
+          atomicOr(&s.counts[4], 1U);
+        }
       }
     }
     wave_sync_lds();  // fills is reused by the next window
@@ -1017,6 +1055,39 @@ __global__ void __launch_bounds__(256, 8) k_route_scan(round_args a) {
       if (__ballot(started) != 0ULL) {
         c_ws += im.route_stop_offsets[r + 1U] - im.route_stop_offsets[r];
       }
+      if (any) s.any_tmp[slot] = 1U;
+    }
+    // loose items: 64 per wavefront, each lane scans its own route for its
+    // own query from global memory
+    auto const n_loose = clamp_count(s.counts[8], s.cap_loose);
+    while (true) {
+      std::uint32_t b = 0U;
+      if (lane == 0U) b = atomicAdd(&s.counts[9], 64U);
+      b = uni(b);
+      if (b >= n_loose) break;
+      auto const it = b + lane;
+      lane_query q{0U, 0U, false};
+      std::uint32_t r = 0U, n_stops = 0U;
+      if (it < n_loose) {
+        auto const v = s.loose[it];
+        r = static_cast<std::uint32_t>(v >> 32);
+        q.g = static_cast<std::uint32_t>(v >> 6) & ((1U << 26) - 1U);
+        q.ln = static_cast<std::uint32_t>(v) & 63U;
+        q.on = ((s.grp_active[q.g] >> q.ln) & 1ULL) != 0ULL;
+        n_stops = im.route_stop_offsets[r + 1U] - im.route_stop_offsets[r];
+      }
+      auto const slot = q.g * 64U + q.ln;
+      auto const k = static_cast<std::uint32_t>(s.kslot[slot]);
+      int const tad_k = s.tad[static_cast<std::size_t>(slot) * kRowsMax + k];
+      bool any = false, started = false;
+      if (it < n_loose) {
+        scan_route_lane64<Fwd>(im, s, r, k, q, s.param[slot].base, tad_k, c_stops, c_tmp, c_ev, c_rd, c_tt,
+                               any, started);
+      }
+      // wavefront iterations: the longest route of the 64
+      auto ws = started ? n_stops : 0U;
+      for (int off = 32; off > 0; off >>= 1) ws = max(ws, static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(ws), off, 64)));
+      c_ws += ws;
       if (any) s.any_tmp[slot] = 1U;
     }
     wave_count(&s.work[kWorkRouteStops], c_stops);
@@ -1168,7 +1239,7 @@ __global__ void k_deactivate_all(dev_state s) {
 // the overflow flags, is never cleared here)
 __global__ void k_reset_counts(dev_state s, std::uint32_t mask) {
   if (threadIdx.x == 0U && blockIdx.x == 0U) {
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < 16; ++i) {
       if (i != 4 && ((mask >> i) & 1U) != 0U) s.counts[i] = 0U;
     }
   }
@@ -1898,8 +1969,8 @@ void round_segment(int seg, round_args const& a, std::uint32_t const* sweep, boo
     case 2:
       hipLaunchKernelGGL((k_clear_marks<W>), dim3(1024), dim3(kBlock), 0, st, a.st, 0);
       hipLaunchKernelGGL((k_group_update), dim3(grid_for(n_slots, kBlock, 1 << 20)), dim3(kBlock), 0, st, a.st);
-      // list_t, routes, packed
-      hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 1U | 4U | 32U);
+      // list_t, routes, packed, loose items and their cursor
+      hipLaunchKernelGGL((k_reset_counts), dim3(1), dim3(64), 0, st, a.st, 1U | 4U | 32U | 256U | 512U);
       if constexpr (W == 64) {
         // station-major flags: list entries (l << 32 | g), then packed
         // transfer wavefronts (counts[5] was reset after the route scan)

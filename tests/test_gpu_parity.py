@@ -304,14 +304,14 @@ def test_gpu_launch_knobs_do_not_change_results():
     qs = na.Queries.generate(tt, 3, 128, seed=0x7172 + 43)
     eng = na.Engine(img, lanes=64, groups=2, max_k=10)
     eng.configure(grid_waves=512, route_claim=4, transfer_claim=3, pack_max_entries=1,
-                  transfer_pack_max_entries=2)
+                  transfer_pack_max_entries=2, loose_lanes=17)
     cfg = eng.get_config()
     assert (cfg["grid_waves"], cfg["route_claim"], cfg["transfer_claim"], cfg["pack_max_entries"],
-            cfg["transfer_pack_max_entries"]) == (512, 4, 3, 1, 2)
+            cfg["transfer_pack_max_entries"], cfg["loose_lanes"]) == (512, 4, 3, 1, 2, 17)
     say("knobs away from defaults, workload3 mix n=128")
     compare(tt, eng, qs)
     eng.configure(route_claim=2, transfer_claim=64, pack_max_entries=3,
-                  transfer_pack_max_entries=1)
+                  transfer_pack_max_entries=1, loose_lanes=1)
     say("knobs, second setting")
     compare(tt, eng, qs)
 
@@ -322,7 +322,7 @@ def test_gpu_autotune_picks_candidates_and_keeps_results():
     eng = na.Engine(img, lanes=64, groups=1, max_k=10)
     say("autotune workload3 n=64")
     chosen, trials = eng.autotune(qs)
-    assert len(trials) == 4 + 3 + 3 + 5 + 5
+    assert len(trials) == 4 + 3 + 3 + 5 + 5 + 4
     for knob, value in chosen.items():
         tried = [(v, ms) for k, v, ms in trials if k == knob]
         assert value in [v for v, _ in tried], (knob, value)

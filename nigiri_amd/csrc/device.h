@@ -136,6 +136,14 @@ struct dev_state {
   std::uint8_t* flag_hz;
   // batched engine: route-phase marks, one byte per (g, l, lane) [G][L][64]
   std::uint8_t* mark_rb;
+  // batched engine: lanes that may hold a label, so that the best sweep and
+  // the lane resets touch only those.  rt_valid [G][K][L]: lanes whose
+  // round_times[k][l] may be valid; bt_valid [G][L]: lanes whose best[l] or
+  // tmp[l] may be valid, apart from the current round's station marks
+  // (mark_t, mark_rb), which are folded in when they are cleared.  A bit is
+  // set wherever a label is written and cleared only by a reset.
+  std::uint64_t* rt_valid;
+  std::uint64_t* bt_valid;
   // batched engine: packed route wavefronts (route << 32 | n_lanes) and
   // their lanes (g << 6 | lane, 64 per wavefront)
   std::uint64_t* packed_info;

@@ -256,6 +256,12 @@ struct nigiri_amd_engine {
   std::vector<std::uint8_t> algo_new;    // reset_arrivals requested
   std::vector<std::uint8_t> algo_fresh;  // next_start_time requested
   std::vector<std::uint8_t> algo_dir;
+  // batched engine, per slot: direction (0 fwd, 1 bwd) whose invalid value
+  // fills every element of the slot's state that the validity bitmaps do
+  // not list; -1 until a dense reset (first use, direction change, run ended
+  // by an error)
+  std::vector<std::int8_t> slot_dir;
+  std::uint64_t* d_full_mask = nullptr;
   // configuration
   double admission_wait_ms = 2.0;
   std::uint32_t grid_waves = 8192U;
@@ -454,6 +460,23 @@ void run_reconstruct(nigiri_amd_engine& e, bool fwd, std::vector<rc_job> const& 
   HIP_CHECK(hipStreamSynchronize(stream));
 }
 
+// Batched engine: lanes among `resetting` whose state elements outside the
+// validity bitmaps do not hold this direction's invalid value (first use,
+// the other direction before, or a run ended by an error) get a dense reset
+// (per group: a lane mask); the plain engine's resets are always dense.
+std::vector<std::uint64_t> dense_resets(nigiri_amd_engine& e, std::vector<std::uint32_t> const& resetting, bool fwd) {
+  std::vector<std::uint64_t> full(e.G, 0U);
+  if (e.W != 64U) return full;
+  auto const dir = static_cast<std::int8_t>(fwd ? 0 : 1);
+  for (auto const s : resetting) {
+    if (e.slot_dir[s] != dir) {
+      full[s / 64U] |= 1ULL << (s % 64U);
+      e.slot_dir[s] = dir;
+    }
+  }
+  return full;
+}
+
 struct runner {
   nigiri_amd_engine& e;
   timetable const& tt;
@@ -487,6 +510,9 @@ struct runner {
       e.pool->wait();
     } catch (...) {
     }
+    // a run ended by an error may leave marks that were never folded into
+    // the validity bitmaps: the next use of every group starts dense
+    if (std::uncaught_exceptions() > 0) std::fill(e.slot_dir.begin(), e.slot_dir.end(), std::int8_t{-1});
   }
 
   std::uint64_t budget() const {
@@ -605,6 +631,9 @@ struct runner {
     for (std::uint32_t g = 0; g != G; ++g) {
       if (bt_mask[g] != 0U) reset_groups.push_back(g);
     }
+    auto const full_mask = dense_resets(e, starting, fwd);
+    HIP_CHECK(hipMemcpyAsync(e.d_full_mask, full_mask.data(), sizeof(std::uint64_t) * G, hipMemcpyHostToDevice, stream));
+    a.reset_full_mask = e.d_full_mask;
     HIP_CHECK(hipMemcpyAsync(e.d_reset_groups, reset_groups.data(), sizeof(std::uint32_t) * reset_groups.size(), hipMemcpyHostToDevice, stream));
     HIP_CHECK(hipMemcpyAsync(e.d_rt_mask, rt_mask.data(), sizeof(std::uint64_t) * G, hipMemcpyHostToDevice, stream));
     HIP_CHECK(hipMemcpyAsync(e.d_bt_mask, bt_mask.data(), sizeof(std::uint64_t) * G, hipMemcpyHostToDevice, stream));
@@ -1299,6 +1328,8 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     st.flag_route = m.alloc<std::uint8_t>(rflag);
     st.route_mark = W == 64U ? nullptr : m.alloc<std::uint64_t>(rwords);
     st.mark_rb = W == 64U ? m.alloc<std::uint8_t>(cols) : nullptr;
+    st.rt_valid = W == 64U ? m.alloc<std::uint64_t>(static_cast<std::size_t>(G) * K * L) : nullptr;
+    st.bt_valid = W == 64U ? m.alloc<std::uint64_t>(static_cast<std::size_t>(G) * L) : nullptr;
     st.grp_active = m.alloc<std::uint64_t>(G);
     st.any_route = m.alloc<std::uint8_t>(static_cast<std::size_t>(G) * W);
     st.any_tmp = m.alloc<std::uint8_t>(static_cast<std::size_t>(G) * W);
@@ -1343,6 +1374,10 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     HIP_CHECK(hipMemsetAsync(st.flag_route, 0, rflag, e->stream));
     if (st.route_mark != nullptr) HIP_CHECK(hipMemsetAsync(st.route_mark, 0, 8U * rwords, e->stream));
     if (st.mark_rb != nullptr) HIP_CHECK(hipMemsetAsync(st.mark_rb, 0, cols, e->stream));
+    if (st.rt_valid != nullptr) {
+      HIP_CHECK(hipMemsetAsync(st.rt_valid, 0, 8U * static_cast<std::size_t>(G) * K * L, e->stream));
+      HIP_CHECK(hipMemsetAsync(st.bt_valid, 0, 8U * static_cast<std::size_t>(G) * L, e->stream));
+    }
     HIP_CHECK(hipMemsetAsync(st.grp_active, 0, 8U * G, e->stream));
     HIP_CHECK(hipMemsetAsync(st.any_route, 0, static_cast<std::size_t>(G) * W, e->stream));
     HIP_CHECK(hipMemsetAsync(st.any_tmp, 0, static_cast<std::size_t>(G) * W, e->stream));
@@ -1372,6 +1407,8 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     e->algo_new.assign(S, 0U);
     e->algo_fresh.assign(S, 0U);
     e->algo_dir.assign(S, 0U);
+    e->slot_dir.assign(S, std::int8_t{-1});
+    e->d_full_mask = m.alloc<std::uint64_t>(G);
     HIP_CHECK(hipStreamSynchronize(e->stream));
     *out = e.release();
     return NIGIRI_AMD_OK;
@@ -1886,6 +1923,8 @@ int nigiri_amd_algo_execute(nigiri_amd_engine* e, uint32_t slot, int32_t start_t
     HIP_CHECK(hipMemcpyAsync(tr.dist_min, e->h_dist_min.data(), 2U * e->h_dist_min.size(), hipMemcpyHostToDevice, stream));
     std::uint32_t const gg = g;
     HIP_CHECK(hipMemcpyAsync(e->d_reset_groups, &gg, 4U, hipMemcpyHostToDevice, stream));
+    auto const full_mask = dense_resets(*e, {slot}, fwd);
+    HIP_CHECK(hipMemcpyAsync(e->d_full_mask, full_mask.data(), 8U * G, hipMemcpyHostToDevice, stream));
     HIP_CHECK(hipMemcpyAsync(e->d_rt_mask, rt_mask.data(), 8U * G, hipMemcpyHostToDevice, stream));
     HIP_CHECK(hipMemcpyAsync(e->d_bt_mask, bt_mask.data(), 8U * G, hipMemcpyHostToDevice, stream));
     HIP_CHECK(hipMemcpyAsync(e->d_flags, flags.data(), S, hipMemcpyHostToDevice, stream));
@@ -1899,9 +1938,10 @@ int nigiri_amd_algo_execute(nigiri_amd_engine* e, uint32_t slot, int32_t start_t
     a.st = tr;
     a.invalid = fwd ? SHRT_MAX : SHRT_MIN;
     a.reset_groups = e->d_reset_groups;
-    a.n_reset_groups = (rt_mask[g] | bt_mask[g]) != 0U ? 1U : 0U;
+    a.n_reset_groups = (rt_mask[g] | bt_mask[g] | full_mask[g]) != 0U ? 1U : 0U;
     a.reset_rt_mask = e->d_rt_mask;
     a.reset_bt_mask = e->d_bt_mask;
+    a.reset_full_mask = e->d_full_mask;
     a.start_slot = e->d_start_slot;
     a.start_loc = e->d_start_loc;
     a.start_delta = e->d_start_delta;

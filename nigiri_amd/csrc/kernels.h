@@ -113,6 +113,9 @@ struct epoch_args {
   std::uint32_t n_reset_groups;
   std::uint64_t const* reset_rt_mask;  // [G] lanes whose round_times reset
   std::uint64_t const* reset_bt_mask;  // [G] lanes whose best/tmp reset
+  // [G] batched engine: lanes reset densely (every element of their state,
+  // not only those the validity bitmaps list)
+  std::uint64_t const* reset_full_mask;
   // destination mask ops: (slot << 32 | location); dest_set[i] != 0 sets
   std::uint64_t const* dest_ops;
   std::uint8_t const* dest_set;

@@ -135,6 +135,24 @@ struct lay {
   }
 };
 
+// Batched engine: round_times[k][l] / best[l] of (g, lane) may now hold a
+// label (the validity bitmaps that let resets and the best sweep skip the
+// rest); no-ops for the plain engine, whose resets are dense.
+template <int W>
+__device__ inline void note_rt(dev_state const& s, std::uint32_t g, std::uint32_t k, std::uint32_t l,
+                               std::uint32_t lane) {
+  if constexpr (W == 64) {
+    atomicOr(reinterpret_cast<unsigned long long*>(&s.rt_valid[(static_cast<std::size_t>(g) * s.K + k) * s.L + l]),
+             1ULL << lane);
+  }
+}
+template <int W>
+__device__ inline void note_bt(dev_state const& s, std::uint32_t g, std::uint32_t l, std::uint32_t lane) {
+  if constexpr (W == 64) {
+    atomicOr(reinterpret_cast<unsigned long long*>(&s.bt_valid[static_cast<std::size_t>(g) * s.L + l]), 1ULL << lane);
+  }
+}
+
 // Sets the mark of (g, l, lane); appends (g, l) to the list on the first
 // mark of the location word.
 template <int W>
@@ -273,13 +291,44 @@ __global__ void k_round_begin(round_args a, std::uint32_t n_slots) {
   }
 }
 
-// best = min(round_times[k], best) for groups holding later-epoch slots;
-// sweep[0] is the number of groups, sweep[1..] their indices (in device
-// memory, so that a captured round replays with the current list)
+// best = min(round_times[k], best) for groups holding later-epoch slots
+// (raptor.h:219-223); sweep[0] is the number of groups, sweep[1..] their
+// indices (in device memory, so that a captured round replays with the
+// current list).  Batched engine: one wavefront per (group, station), each
+// lane reading the rt_valid word of its own round, so that only labels
+// written by earlier epochs are read; the lanes that lower best are added
+// to bt_valid.
 template <bool Fwd, int W>
 __global__ void k_best_sweep(round_args a, std::uint32_t const* sweep) {
   auto const& s = a.st;
   auto const n_groups = sweep[0];
+  if constexpr (W == 64) {
+    auto const lane = threadIdx.x & 63U;
+    auto const wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64U;
+    auto const n_waves = gridDim.x * (blockDim.x / 64U);
+    for (std::uint32_t gi = 0U; gi < n_groups; ++gi) {  // uniform
+      auto const g = sweep[1U + gi];
+      bool const on = ((s.grp_active[g] >> lane) & 1ULL) != 0ULL;
+      auto const k = static_cast<std::uint32_t>(s.kslot[g * 64U + lane]);
+      auto const* valid = s.rt_valid + (static_cast<std::size_t>(g) * s.K + k) * s.L;
+      for (std::uint32_t l = wave; l < s.L; l += n_waves) {  // uniform per wavefront
+        bool const need = on && ((valid[l] >> lane) & 1ULL) != 0ULL;
+        if (__ballot(need) == 0ULL) continue;
+        bool wrote = false;
+        if (need) {
+          auto const v = s.round_times[lay<64>::rt(s, g, k, l, lane)];
+          auto const c = lay<64>::col(s, g, l, lane);
+          if (dirop<Fwd>::better(v, s.best[c])) {
+            s.best[c] = v;
+            wrote = true;
+          }
+        }
+        auto const m = __ballot(wrote);
+        if (lane == 0U && m != 0ULL) s.bt_valid[static_cast<std::size_t>(g) * s.L + l] |= m;
+      }
+    }
+    return;
+  }
   auto const per_group = s.L * static_cast<std::uint32_t>(W);  // < 2^32 (checked at engine creation)
   auto const stride = gridDim.x * blockDim.x;
   for (std::uint32_t gi = 0U; gi < n_groups; ++gi) {  // uniform: no per-element division
@@ -1189,19 +1238,48 @@ __global__ void k_clear_marks(dev_state s, int which) {
   auto* marks = which == 0 ? s.mark_t : (which == 1 ? s.mark_r : s.hz_mask);
   auto const* list = which == 0 ? s.list_t : (which == 1 ? s.list_r : s.list_hz);
   if (W == 64 && which == 1) {
-    // batched route-phase marks: 64 bytes per (g, l), cleared as 8 words;
-    // route-phase list entries are (l << 32 | g)
+    // batched route-phase marks: 64 bytes per (g, l), cleared as 8 words by
+    // 8 consecutive lanes; route-phase list entries are (l << 32 | g).  The
+    // marked lanes (their tmp was written) are added to bt_valid.
     auto* words = reinterpret_cast<std::uint64_t*>(s.mark_rb);
-    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n * 8U; i += gridDim.x * blockDim.x) {
-      auto const e = list[i / 8U];
-      auto const gl = static_cast<std::size_t>(static_cast<std::uint32_t>(e)) * s.L + (e >> 32);
-      words[gl * 8U + i % 8U] = 0ULL;
+    auto const lane = threadIdx.x & 63U;
+    auto const total = n * 8U;
+    // wavefront-uniform loop: the stride is a multiple of 64, so i % 8 == lane % 8
+    for (std::uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63U); base < total;
+         base += gridDim.x * blockDim.x) {
+      auto const i = base + lane;
+      std::uint64_t m = 0ULL;
+      std::size_t gl = 0U;
+      if (i < total) {
+        auto const e = list[i / 8U];
+        gl = static_cast<std::size_t>(static_cast<std::uint32_t>(e)) * s.L + (e >> 32);
+        auto const w = words[gl * 8U + lane % 8U];
+        if (w != 0ULL) {
+          words[gl * 8U + lane % 8U] = 0ULL;
+          for (std::uint32_t b = 0U; b < 8U; ++b) {
+            if (((w >> (8U * b)) & 0xFFULL) != 0ULL) m |= 1ULL << (8U * (lane % 8U) + b);
+          }
+        }
+      }
+      for (int off = 1; off < 8; off <<= 1) {
+        m |= static_cast<std::uint64_t>(__shfl_xor(static_cast<unsigned long long>(m), off, 64));
+      }
+      if (i < total && lane % 8U == 0U && m != 0ULL) s.bt_valid[gl] |= m;
     }
     return;
   }
   for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     auto const e = list[i];
-    marks[lay<W>::lword(s, static_cast<std::uint32_t>(e >> 32), static_cast<std::uint32_t>(e))] = 0ULL;
+    auto const w = lay<W>::lword(s, static_cast<std::uint32_t>(e >> 32), static_cast<std::uint32_t>(e));
+    if constexpr (W == 64) {
+      // station marks of the transfer phase (best was written for the marked
+      // lanes) and of the starts: added to bt_valid
+      if (which == 0) {
+        auto const m = marks[w];
+        if (m != 0ULL) s.bt_valid[w] |= m;
+      }
+    }
+    marks[w] = 0ULL;
   }
 }
 
@@ -1327,12 +1405,15 @@ __global__ void k_dest_pass(round_args a, std::uint32_t n_slots) {
       if (D::better_eq(e.x, tad[k])) {
         s.round_times[rti] = e.x;
         s.best[c] = e.x;
+        note_rt<W>(s, g, k, e.target, lane);
+        note_bt<W>(s, g, e.target, lane);
         update_tad<Fwd>(tad, k, e.x);
       }
     } else {
       // update_transfers / update_footpaths for a destination target
       if (p.bounds_last_k == 0U && D::better(e.x, s.best[c])) {
         s.round_times[rti] = D::best_of(e.x, s.round_times[rti]);
+        note_rt<W>(s, g, k, e.target, lane);
       }
       if (D::better(e.x, s.best[c]) && D::better_eq(e.x, tad[k])) {
         int const lbv = s.lb[c];
@@ -1340,6 +1421,7 @@ __global__ void k_dest_pass(round_args a, std::uint32_t n_slots) {
         if (!within_bounds<Fwd, W>(im, s, slot, k, e.target, e.x)) continue;
         s.round_times[rti] = e.x;
         s.best[c] = e.x;
+        note_rt<W>(s, g, k, e.target, lane);  // best: through the station mark
         if constexpr (W == 64) {
           mark_wave64(s.mark_t, s.flag_t, g, e.target, 1ULL << lane, s.L);
         } else {
@@ -1388,7 +1470,8 @@ __device__ inline int relax(dev_image const& im, dev_state const& s, std::uint32
   }
   // no pre-write while bounds are set (raptor.h:709, 790)
   if (W == 1 && s.param[slot].bounds_last_k != 0U) return 0;
-  atomic_best16<Fwd>(&s.round_times[lay<W>::rt(s, g, k, t, lane)], xv);  // pre-write
+  auto const old = atomic_best16<Fwd>(&s.round_times[lay<W>::rt(s, g, k, t, lane)], xv);  // pre-write
+  if (D::better(xv, old)) note_rt<W>(s, g, k, t, lane);
   return tad_e != tad0 ? kRelaxHazard : 0;
 }
 
@@ -1612,6 +1695,10 @@ __global__ void k_rt_fixup(round_args a) {
       m &= m - 1ULL;
       auto const k = static_cast<std::uint32_t>(s.kslot[g * W + lane]);
       s.round_times[lay<W>::rt(s, g, k, t, lane)] = s.best[lay<W>::col(s, g, t, lane)];
+      if constexpr (W == 64) {
+        // this thread owns the (g, t) items of this round: plain update
+        s.rt_valid[(static_cast<std::size_t>(g) * s.K + k) * s.L + t] |= 1ULL << lane;
+      }
     }
   }
 }
@@ -1708,11 +1795,72 @@ __global__ void k_hazards(round_args a) {
 // ======================================================================
 // epoch begin / end
 
+// reset_arrivals / next_start_time (raptor.h:174-188) for the lanes of the
+// reset masks.  Batched engine: only the labels the validity bitmaps list
+// are reset, since every other element already holds the direction's
+// invalid value, except for the lanes of reset_full_mask (first use of the
+// slot in this direction), which are reset densely; the station marks left
+// by an epoch that ended at its last round count as written best labels.
 template <int W>
 __global__ void k_lane_reset(epoch_args a) {
   auto const& s = a.st;
   auto const stride = gridDim.x * blockDim.x;
   auto const tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if constexpr (W == 64) {
+    for (std::uint32_t gi = 0U; gi < a.n_reset_groups; ++gi) {  // uniform
+      auto const g = a.reset_groups[gi];
+      auto const full = a.reset_full_mask[g];
+      auto const rtm = a.reset_rt_mask[g] & ~full, btm = a.reset_bt_mask[g] & ~full;
+      if ((rtm | btm | full) == 0ULL) continue;
+      auto const gl0 = static_cast<std::size_t>(g) * s.L;
+      if (full != 0ULL) {
+        // dense: every element of the lanes' columns
+        auto const per_group = s.L * 64U;
+        for (std::uint32_t i = tid; i < per_group; i += stride) {
+          auto const l = i / 64U, lane = i % 64U;
+          if (((full >> lane) & 1ULL) == 0ULL) continue;
+          auto const c = lay<64>::col(s, g, l, lane);
+          s.best[c] = a.invalid;
+          s.tmp[c] = a.invalid;
+          for (std::uint32_t k = 0; k < s.K; ++k) s.round_times[lay<64>::rt(s, g, k, l, lane)] = a.invalid;
+        }
+      }
+      if ((btm | full) != 0ULL) {
+        auto const clr = btm | full;
+        for (std::uint32_t l = tid; l < s.L; l += stride) {
+          auto const mk = s.mark_t[gl0 + l], v = s.bt_valid[gl0 + l];
+          if ((mk & clr) != 0ULL) s.mark_t[gl0 + l] = mk & ~clr;
+          if ((v & clr) != 0ULL) s.bt_valid[gl0 + l] = v & ~clr;
+          auto x = (mk | v) & btm;
+          while (x != 0ULL) {
+            auto const lane = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(x)) - 1);
+            x &= x - 1ULL;
+            auto const c = lay<64>::col(s, g, l, lane);
+            s.best[c] = a.invalid;
+            s.tmp[c] = a.invalid;
+          }
+        }
+      }
+      if ((rtm | full) != 0ULL) {
+        auto const clr = rtm | full;
+        auto const n = static_cast<std::size_t>(s.K) * s.L;
+        auto* valid = s.rt_valid + gl0 * s.K;
+        for (std::size_t i = tid; i < n; i += stride) {
+          auto const v = valid[i];
+          if ((v & clr) == 0ULL) continue;
+          valid[i] = v & ~clr;
+          auto x = v & rtm;
+          auto const k = static_cast<std::uint32_t>(i / s.L), l = static_cast<std::uint32_t>(i % s.L);
+          while (x != 0ULL) {
+            auto const lane = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(x)) - 1);
+            x &= x - 1ULL;
+            s.round_times[lay<64>::rt(s, g, k, l, lane)] = a.invalid;
+          }
+        }
+      }
+    }
+    return;
+  }
   // station marks left by an epoch that ended at its last round are cleared
   // for the restarting lanes (the words may hold running lanes' marks)
   auto const words = W == 64 ? s.L : s.LW;
@@ -1866,6 +2014,8 @@ __global__ void k_add_starts(epoch_args a) {
     auto const g = slot / W, lane = slot % W;
     atomic_best16<Fwd>(&s.best[lay<W>::col(s, g, l, lane)], d);
     atomic_best16<Fwd>(&s.round_times[lay<W>::rt(s, g, 0U, l, lane)], d);
+    note_rt<W>(s, g, 0U, l, lane);
+    note_bt<W>(s, g, l, lane);
     // appended to the station work list of the next round unless a lane
     // already marked the word there (other slots may be mid-epoch)
     mark_one<W>(s.mark_t, s.list_t, &s.counts[0], s.cap_t, &s.counts[4], s, g, l, lane);

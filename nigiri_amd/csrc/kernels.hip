@@ -294,37 +294,46 @@ __global__ void k_round_begin(round_args a, std::uint32_t n_slots) {
 // best = min(round_times[k], best) for groups holding later-epoch slots
 // (raptor.h:219-223); sweep[0] is the number of groups, sweep[1..] their
 // indices (in device memory, so that a captured round replays with the
-// current list).  Batched engine: one wavefront per (group, station), each
-// lane reading the rt_valid word of its own round, so that only labels
-// written by earlier epochs are read; the lanes that lower best are added
-// to bt_valid.
+// current list).  Batched engine: one thread per (group, station); the
+// group's active lanes are split by their round k, and for each such round
+// the thread reads the station's rt_valid word, so that only the labels of
+// earlier epochs are read (the word loads are coalesced, the label accesses
+// sparse); the lanes that lower best are added to bt_valid.
 template <bool Fwd, int W>
 __global__ void k_best_sweep(round_args a, std::uint32_t const* sweep) {
   auto const& s = a.st;
   auto const n_groups = sweep[0];
   if constexpr (W == 64) {
-    auto const lane = threadIdx.x & 63U;
-    auto const wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64U;
-    auto const n_waves = gridDim.x * (blockDim.x / 64U);
+    auto const tid = blockIdx.x * blockDim.x + threadIdx.x;
+    auto const stride = gridDim.x * blockDim.x;
     for (std::uint32_t gi = 0U; gi < n_groups; ++gi) {  // uniform
       auto const g = sweep[1U + gi];
-      bool const on = ((s.grp_active[g] >> lane) & 1ULL) != 0ULL;
-      auto const k = static_cast<std::uint32_t>(s.kslot[g * 64U + lane]);
-      auto const* valid = s.rt_valid + (static_cast<std::size_t>(g) * s.K + k) * s.L;
-      for (std::uint32_t l = wave; l < s.L; l += n_waves) {  // uniform per wavefront
-        bool const need = on && ((valid[l] >> lane) & 1ULL) != 0ULL;
-        if (__ballot(need) == 0ULL) continue;
-        bool wrote = false;
-        if (need) {
-          auto const v = s.round_times[lay<64>::rt(s, g, k, l, lane)];
-          auto const c = lay<64>::col(s, g, l, lane);
-          if (dirop<Fwd>::better(v, s.best[c])) {
-            s.best[c] = v;
-            wrote = true;
+      // the group's active lanes per round k: lane i of the wavefront looks
+      // at slot lane i (the same masks in every wavefront)
+      auto const me = threadIdx.x & 63U;
+      bool const on = ((s.grp_active[g] >> me) & 1ULL) != 0ULL;
+      auto const my_k = static_cast<std::uint32_t>(s.kslot[g * 64U + me]);
+      std::uint64_t at_k[kRowsMax];
+#pragma unroll
+      for (std::uint32_t k = 0U; k < kRowsMax; ++k) at_k[k] = __ballot(on && my_k == k);
+      for (std::uint32_t l = tid; l < s.L; l += stride) {
+        std::uint64_t wrote = 0ULL;
+#pragma unroll
+        for (std::uint32_t k = 1U; k < kRowsMax; ++k) {
+          if (k >= s.K || at_k[k] == 0ULL) continue;  // uniform
+          auto x = s.rt_valid[(static_cast<std::size_t>(g) * s.K + k) * s.L + l] & at_k[k];
+          while (x != 0ULL) {
+            auto const lane = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(x)) - 1);
+            x &= x - 1ULL;
+            auto const v = s.round_times[lay<64>::rt(s, g, k, l, lane)];
+            auto const c = lay<64>::col(s, g, l, lane);
+            if (dirop<Fwd>::better(v, s.best[c])) {
+              s.best[c] = v;
+              wrote |= 1ULL << lane;
+            }
           }
         }
-        auto const m = __ballot(wrote);
-        if (lane == 0U && m != 0ULL) s.bt_valid[static_cast<std::size_t>(g) * s.L + l] |= m;
+        if (wrote != 0ULL) s.bt_valid[static_cast<std::size_t>(g) * s.L + l] |= wrote;
       }
     }
     return;

@@ -32,6 +32,8 @@ enum : int {
   kWorkTransferItems = 10,  // (group, station) items of the batched transfer kernel
   kWorkHazardItems = 11,    // (group, station) items of the hazard replay
   kWorkHazardLanes = 12,    // queries replayed there
+  kWorkLooseWaveStops = 13,  // the part of kWorkWaveStops spent on loose route items
+  kWorkLooseItems = 14,      // loose (route, query) items scanned
   kWorkCount = 16
 };
 
@@ -44,6 +46,7 @@ struct dev_sweep {
   std::uint32_t const* edge_off;  // [n_nodes + 1]
   std::uint32_t const* src;
   std::uint16_t const* w;         // minutes, at most 7201
+  std::uint32_t const* level_off;  // [n_levels + 1] into order
 };
 
 // Read-only timetable image in HBM (one allocation per array).

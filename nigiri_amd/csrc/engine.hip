@@ -397,6 +397,7 @@ void build_image(nigiri_amd_image& img, timetable const& tt, hipStream_t s) {
     ds.edge_off = m.upload(sw.edge_off, s);
     ds.src = m.upload(sw.src, s);
     ds.w = m.upload(sw.w, s);
+    ds.level_off = m.upload(sw.level_off, s);
   }
   HIP_CHECK(hipStreamSynchronize(s));  // host vectors above go out of scope
 }

@@ -722,8 +722,13 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
       x.stop[lane] = stops_g[Fwd ? si : n_stops - si - 1U];
     }
     wave_sync_lds();
-    // this lane's station marks, lower bounds and round_times[k-1] at the
-    // chunk's stops, and the chunk's event blocks, all in flight together
+    // this lane's station marks at the chunk's stops, with its lower bounds
+    // there if it rides into the chunk (a rider needs them at every stop),
+    // and the chunk's event blocks, in flight together; a lane that does
+    // not ride needs lower bounds only from its first marked stop on, and
+    // round_times[k-1] only at marked stops: those are loaded once the
+    // marks are known (and not at all for a chunk that is skipped)
+    bool const rides_in = !done && et.valid();
     std::uint16_t lbr[kChunkMax];
     std::int16_t rtr[kChunkMax];
     std::uint32_t mkr[kChunkMax];
@@ -735,8 +740,7 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
       if (j < nc && !done) {
         auto const l = x.stop[j] & kLocMask;
         mkr[j] = mark_t32[mrow + 2U * l];
-        lbr[j] = s.lb[col0 + l * 64U];
-        rtr[j] = rt_q[static_cast<std::size_t>(l) * 64U];
+        if (rides_in) lbr[j] = s.lb[col0 + l * 64U];
       }
     }
     // event blocks: for chunk stop j, [arr at stop_idx | dep at stop_idx];
@@ -776,6 +780,17 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
       continue;
     }
     if (!any_rider) stage_events();
+    {
+      auto const first = mbits != 0U ? static_cast<std::uint32_t>(__builtin_ctz(mbits)) : kChunkMax;
+#pragma unroll
+      for (std::uint32_t j = 0U; j < kChunkMax; ++j) {
+        if (j < nc && !done) {
+          auto const l = x.stop[j] & kLocMask;
+          if (!rides_in && j >= first) lbr[j] = s.lb[col0 + l * 64U];
+          if (((mbits >> j) & 1U) != 0U) rtr[j] = rt_q[static_cast<std::size_t>(l) * 64U];
+        }
+      }
+    }
 #pragma unroll
     for (std::uint32_t j = 0U; j < kChunkMax; ++j) {
       if (j < nc) {
@@ -863,12 +878,11 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
   if (started) c_stops += my_stops;
 }
 
-// update_route (raptor.h:1181-1349) for routes whose stop events do not fit
-// the LDS staging of scan_route64 (more than kBatchedMaxRouteTransports
-// transports), batched engine: every lane scans the route for its own query,
-// reading the image and state from global memory; route-phase marks and
-
-// register allocation does not carry it.
+// update_route (raptor.h:1181-1349), one lane per (route, query): for routes
+// whose stop events do not fit the LDS staging of scan_route64 (more than
+// kBatchedMaxRouteTransports transports) and for the loose route items.
+// Every lane scans its route for its own query, reading the image and the
+// state from global memory stop by stop.
 template <bool Fwd>
 __device__ inline void scan_route_lane64(dev_image const& im, dev_state const& s, std::uint32_t r,
                                                std::uint32_t k, lane_query const q, std::uint16_t base,
@@ -901,11 +915,14 @@ __device__ inline void scan_route_lane64(dev_image const& im, dev_state const& s
     auto const l = stp & kLocMask;
     auto const off = col0 + l * 64U;
     bool const marked = (mark_t32[mrow + 2U * l] & mbit) != 0U;
+    ++c_stops;
+    // without a trip and a mark nothing happens at this stop (its lower
+    // bound is not needed)
+    if (!et.valid() && !marked) continue;
     int const lbv = s.lb[off];
     auto const in_ok = ((stp >> 28) & 1U) != 0U, out_ok = ((stp >> 29) & 1U) != 0U;
     bool const can_finish = Fwd ? out_ok : in_ok;
     bool const can_start = Fwd ? in_ok : out_ok;
-    ++c_stops;
     bool ride = false;
     if (et.valid() && can_finish) {
       ride = true;
@@ -1084,7 +1101,7 @@ __global__ void __launch_bounds__(256, 8) k_route_scan(round_args a) {
     auto& x = lds_all[threadIdx.x / 64U];
     std::uint32_t const lane = threadIdx.x & 63U;
     auto const n_items = clamp_count(s.counts[5], s.cap_pack);
-    std::uint32_t c_stops = 0U, c_tmp = 0U, c_ev = 0U, c_rd = 0U, c_tt = 0U, c_ws = 0U;
+    std::uint32_t c_stops = 0U, c_tmp = 0U, c_ev = 0U, c_rd = 0U, c_tt = 0U, c_ws = 0U, c_lws = 0U, c_li = 0U;
     item_claim claim{&s.counts[6], a.tune.route_claim};
     for (std::uint32_t item = claim.get(); item < n_items; item = claim.get()) {
       auto const info = s.packed_info[item];
@@ -1146,6 +1163,8 @@ __global__ void __launch_bounds__(256, 8) k_route_scan(round_args a) {
       auto ws = started ? n_stops : 0U;
       for (int off = 32; off > 0; off >>= 1) ws = max(ws, static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(ws), off, 64)));
       c_ws += ws;
+      c_lws += ws;
+      c_li += min(64U, n_loose - b);
       if (any) s.any_tmp[slot] = 1U;
     }
     wave_count(&s.work[kWorkRouteStops], c_stops);
@@ -1154,6 +1173,10 @@ __global__ void __launch_bounds__(256, 8) k_route_scan(round_args a) {
     wave_count(&s.work[kWorkRouteDayTests], c_rd);
     wave_count(&s.work[kWorkTransportTests], c_tt);
     if (lane == 0U && c_ws != 0U) atomicAdd(&s.work[kWorkWaveStops], static_cast<unsigned long long>(c_ws));
+    if (lane == 0U && c_lws != 0U) {
+      atomicAdd(&s.work[kWorkLooseWaveStops], static_cast<unsigned long long>(c_lws));
+      atomicAdd(&s.work[kWorkLooseItems], static_cast<unsigned long long>(c_li));
+    }
   } else {
     auto const n_items = clamp_count(s.counts[2], s.cap_route);
     unsigned long long c_stops = 0ULL, c_tmp = 0ULL;
@@ -1933,14 +1956,16 @@ __global__ void k_lb_seed(lb_phast_args a) {
 // over v's edges from higher-ranked u; values of kLbCutoff or more are
 // unreachable.  W = 64: one wavefront per (location, group), one query per
 // lane, each edge one 128 B row load; W = 1: one thread per (location, row).
+// `first` / `stride`: this thread's first item and the item stride, in
+// wavefronts (W = 64) or threads (W = 1).
 template <int W>
-__global__ void k_lb_level(lb_phast_args a, std::uint32_t b, std::uint32_t e) {
+__device__ inline void lb_level_items(lb_phast_args const& a, std::uint32_t b, std::uint32_t e,
+                                      std::uint32_t first, std::uint32_t stride) {
   auto const& sw = a.sw;
   auto const n_items = (e - b) * a.n_sel;
   if constexpr (W == 64) {
     auto const lane = threadIdx.x & 63U;
-    auto const n_waves = gridDim.x * (blockDim.x / 64U);
-    for (std::uint32_t item = (blockIdx.x * blockDim.x + threadIdx.x) / 64U; item < n_items; item += n_waves) {
+    for (std::uint32_t item = first; item < n_items; item += stride) {
       auto const p = b + item / a.n_sel;
       auto const k = item % a.n_sel;
       auto const v = uni(sw.order[p]);
@@ -1957,8 +1982,7 @@ __global__ void k_lb_level(lb_phast_args a, std::uint32_t b, std::uint32_t e) {
       }
     }
   } else {
-    for (std::uint32_t item = blockIdx.x * blockDim.x + threadIdx.x; item < n_items;
-         item += gridDim.x * blockDim.x) {
+    for (std::uint32_t item = first; item < n_items; item += stride) {
       auto const p = b + item / a.n_sel;
       auto const v = sw.order[p];
       auto* row = a.dist + static_cast<std::size_t>(a.sel[item % a.n_sel]) * a.L;
@@ -1968,6 +1992,32 @@ __global__ void k_lb_level(lb_phast_args a, std::uint32_t b, std::uint32_t e) {
       }
       row[v] = static_cast<std::uint16_t>(d < kLbCutoff ? d : kUnreachableLb);
     }
+  }
+}
+
+template <int W>
+__global__ void k_lb_level(lb_phast_args a, std::uint32_t b, std::uint32_t e) {
+  auto const t = blockIdx.x * blockDim.x + threadIdx.x;
+  if constexpr (W == 64) {
+    lb_level_items<64>(a, b, e, t / 64U, gridDim.x * (blockDim.x / 64U));
+  } else {
+    lb_level_items<1>(a, b, e, t, gridDim.x * blockDim.x);
+  }
+}
+
+// consecutive small levels [l0, l1) of the sweep in one workgroup, a barrier
+// between levels (a level's bounds are final before the next level reads
+// them): saves a launch per level near the top of the hierarchy
+template <int W>
+__global__ void __launch_bounds__(1024) k_lb_levels(lb_phast_args a, std::uint32_t l0, std::uint32_t l1) {
+  for (auto lv = l0; lv < l1; ++lv) {
+    auto const b = a.sw.level_off[lv], e = a.sw.level_off[lv + 1U];
+    if constexpr (W == 64) {
+      lb_level_items<64>(a, b, e, threadIdx.x / 64U, blockDim.x / 64U);
+    } else {
+      lb_level_items<1>(a, b, e, threadIdx.x, blockDim.x);
+    }
+    __syncthreads();
   }
 }
 
@@ -2512,10 +2562,24 @@ void launch_lb_phast(std::uint32_t W, lb_phast_args const& a, std::uint32_t cons
   if (a.n_seeds != 0U) {
     hipLaunchKernelGGL(k_lb_seed, dim3(grid_for(a.n_seeds, kBlock, 1024)), dim3(kBlock), 0, stream, a);
   }
+  // levels of at most kSmall items (wavefronts for W = 64, threads for
+  // W = 1) run in one 1024-thread workgroup, as many in a row as there are
+  std::size_t const kSmall = W == 64U ? 64U : 4096U;
   for (std::uint32_t l = 0U; l != n_levels; ++l) {
     auto const b = level_off[l], e = level_off[l + 1U];
     if (b == e) continue;
     auto const items = static_cast<std::size_t>(e - b) * a.n_sel;
+    if (items <= kSmall) {
+      auto l1 = l + 1U;
+      while (l1 != n_levels && static_cast<std::size_t>(level_off[l1 + 1U] - level_off[l1]) * a.n_sel <= kSmall) ++l1;
+      if (W == 64U) {
+        hipLaunchKernelGGL((k_lb_levels<64>), dim3(1), dim3(1024), 0, stream, a, l, l1);
+      } else {
+        hipLaunchKernelGGL((k_lb_levels<1>), dim3(1), dim3(1024), 0, stream, a, l, l1);
+      }
+      l = l1 - 1U;
+      continue;
+    }
     if (W == 64U) {
       hipLaunchKernelGGL((k_lb_level<64>), dim3(grid_for(items * 64U, kBlock, 8192)), dim3(kBlock), 0, stream,
                          a, b, e);

@@ -324,6 +324,8 @@ def pong_main(args):
         for th in ths:
             th.join()
         wall = time.perf_counter() - t
+        for e in engines:
+            e.close()
         concurrent = {"threads": args.pong_threads, "qps": round(n / wall, 2),
                       "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
                       "results_equal_sequential": got == pong_res}
@@ -341,6 +343,11 @@ def pong_main(args):
     t = time.perf_counter()
     batched.search(qs)
     batched_ms = (time.perf_counter() - t) * 1e3
+    # device objects are released here, not at interpreter exit (a profiler
+    # may have finalised the HIP runtime by then)
+    batched.close()
+    plain.close()
+    img.close()
     agree = sum(in_window(qs[i], pong_res[i]) == in_window(qs[i], range_res[i]) for i in range(n))
     cpu = {}
     if not args.no_cpu:
@@ -529,7 +536,7 @@ def main():
     plain.search(ps)
     sync()
     plain_qps = len(ps) / (time.perf_counter() - tp)
-    del plain
+    plain.close()
 
     # roofline of the dominant kernel (route scan), live from HIP events
     launches = max(1, st["route_scan_launches"])
@@ -605,6 +612,8 @@ def main():
         "transfer_kernel_avg_ms": round(st["transfer_ms"] / max(1, st["transfer_launches"]), 4),
         "cpu_baseline": cpu,
     }
+    eng.close()
+    img.close()
     print(json.dumps(line), flush=True)
 
 

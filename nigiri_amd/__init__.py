@@ -5,8 +5,10 @@ path is libnigiri_amd.so (HIP kernels for gfx950 + host C++); this module
 only marshals arguments.  Importing it does not touch the GPU; creating an
 Image does.  If the shared library is missing every entry point raises.
 """
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -481,6 +483,22 @@ class Queries:
         return out
 
 
+# device objects still open at interpreter exit are closed by an exit hook,
+# engines before images, while the HIP runtime is still whole (objects left
+# to garbage collection at exit would be released in no particular order,
+# possibly after a profiler or the runtime has shut down)
+_open_engines = weakref.WeakSet()
+_open_images = weakref.WeakSet()
+
+
+@atexit.register
+def _close_open_objects():
+    for e in list(_open_engines):
+        e.close()
+    for i in list(_open_images):
+        i.close()
+
+
 class Image:
     """Device-resident timetable (one per GPU)."""
 
@@ -498,6 +516,7 @@ class Image:
         self.tt = tt  # the image borrows the host timetable
         self.handle = C.c_void_p()
         _check(lib().nigiri_amd_image_create(tt.handle, device, C.byref(self.handle)))
+        _open_images.add(self)
 
     def bytes(self):
         b = C.c_uint64()
@@ -542,6 +561,7 @@ class Engine:
         self.handle = C.c_void_p()
         _check(lib().nigiri_amd_engine_create(image.handle, lanes, groups, max_k,
                                               C.byref(self.handle)))
+        _open_engines.add(self)
 
     @property
     def slots(self):

@@ -280,6 +280,10 @@ struct nigiri_amd_engine {
   nigiri_engine_stats stats{};
 
   ~nigiri_amd_engine() {
+    // the legs pool's threads are joined before the stream and the device
+    // buffers they may have used go away; then the stream drains
+    pool.reset();
+    if (stream != nullptr) (void)hipStreamSynchronize(stream);
     round_graphs_destroy(graphs);
     for (auto ev : events) (void)hipEventDestroy(ev);
     for (auto* p : grow_ptrs) (void)hipFree(p);

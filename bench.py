@@ -393,7 +393,6 @@ def main():
     ap.add_argument("--queries", type=int, default=10000)
     ap.add_argument("--plain-sample", type=int, default=24)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--admission-wait-ms", type=float, default=2000.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -544,23 +543,7 @@ def main():
     avg_ms = st["route_scan_ms"] / launches
     achieved = (alg_bytes / launches) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
 
-    cpu = None
-    if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        probe = qs.subset(list(range(min(16, n_q))))
-        r = oracle.search_many(tt, probe, threads=1)
-        per_q = max(1e-4, r["wall_ms"] / 1e3 / len(probe))
-        n_cpu = int(min(n_q, max(threads * 4, args.cpu_seconds * threads / per_q)))
-        sample = qs.subset(list(range(n_cpu)))
-        r = oracle.search_many(tt, sample, threads=threads)
-        cpu = {"value": round(n_cpu / (r["wall_ms"] / 1e3), 2), "unit": "queries/s",
-               "cores": threads, "kind": "port",
-               "sample": f"first {n_cpu} queries of the workload, CPU oracle (C++ restatement "
-                         f"of nigiri's raptor), {threads} threads, "
-                         f"p50 {np.percentile(r['latency_ms'], 50):.2f} ms, "
-                         f"p99 {np.percentile(r['latency_ms'], 99):.2f} ms per query"}
+    cpu = None if args.no_cpu else cpu_baseline(tt, qs, args.cpu_seconds)
 
     line = {
         "metric": "earliest-arrival queries/s + p50/p99 latency, batched vs 1-query GPU, 1 MI355X",

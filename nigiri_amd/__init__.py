@@ -148,7 +148,7 @@ TUNE_KNOBS = ["grid_waves", "route_claim", "transfer_claim", "pack_max_entries",
 WORK_NAMES = ["route_stops", "events", "route_day_tests", "transport_day_tests", "tmp_writes",
               "transfer_stations", "footpaths", "marked_stations", "location_routes",
               "wave_stops", "transfer_items", "hazard_items", "hazard_lanes", "loose_wave_stops",
-              "loose_items"]
+              "loose_items", "skipped_stops"]
 
 
 # every entry point of include/nigiri_amd.h: name -> (restype, argtypes)

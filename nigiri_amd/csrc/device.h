@@ -34,6 +34,7 @@ enum : int {
   kWorkHazardLanes = 12,    // queries replayed there
   kWorkLooseWaveStops = 13,  // the part of kWorkWaveStops spent on loose route items
   kWorkLooseItems = 14,      // loose (route, query) items scanned
+  kWorkSkippedStops = 15,    // the part of kWorkRouteStops before a scan's start (not read)
   kWorkCount = 16
 };
 
@@ -147,17 +148,19 @@ struct dev_state {
   // set wherever a label is written and cleared only by a reset.
   std::uint64_t* rt_valid;
   std::uint64_t* bt_valid;
-  // batched engine: packed route wavefronts (route << 32 | n_lanes) and
-  // their lanes (g << 6 | lane, 64 per wavefront)
+  // batched engine: packed route wavefronts (route << 32 | start << 16 |
+  // n_lanes; packed transfer wavefronts: location << 32 | n_lanes) and
+  // their lanes (g << 6 | lane, 64 per wavefront); start is the scan index
+  // of the route's first stop marked for one of the wavefront's queries
   std::uint64_t* packed_info;
   std::uint16_t* packed_lanes;
   std::uint32_t* counts;   // [0] list_t, [1] list_r, [2] list_route, [3] hz, [4] overflow flags,
                            // [5] packed route wavefronts, [6] / [7] work cursors of
                            // the route scan / transfer kernels, [8] loose route
                            // items, [9] their cursor (batched engine); 16 entries
-  // batched engine: loose route items (route << 32 | g << 6 | lane): the
-  // queries of packed wavefronts too sparse to share a route's staging,
-  // scanned one (route, query) per lane
+  // batched engine: loose route items (route << 32 | start << 16 | g << 6 |
+  // lane; g < 1024): the queries of packed wavefronts too sparse to share a
+  // route's staging, scanned one (route, query) per lane
   std::uint64_t* loose;
   std::size_t cap_loose;
   // algorithmic work counters (see kWork* indices)

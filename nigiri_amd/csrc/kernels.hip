@@ -638,6 +638,10 @@ __device__ inline void scan_route(route_view const& rv, dev_image const& im, dev
 constexpr std::uint32_t kChunkMax = 4U;
 constexpr std::uint32_t kStagedTransports = 16U;
 constexpr std::uint32_t kChunkEvBytes = 1024U;
+// stop words staged per window (one per lane); a route scan starts at the
+// scan index its packing found (16 bits in the packed item encodings)
+constexpr std::uint32_t kStopWindow = 64U;
+constexpr std::uint32_t kMaxScanStart = 0xFFFFU;
 static_assert(kChunkEvBytes / 4U >= kBatchedMaxRouteTransports, "a chunk holds at least one stop");
 static_assert(kStagedTransports * 8U * 8U >= kBatchedMaxRouteTransports * 4U,
               "the bitfield staging area holds the bitfield indices of a staged route");
@@ -645,7 +649,7 @@ static_assert(kStagedTransports * 8U * 8U >= kBatchedMaxRouteTransports * 4U,
 struct wave_lds {
   std::uint64_t rbits[8];
   std::uint64_t tbits[kStagedTransports * 8U];
-  std::uint32_t stop[kChunkMax];
+  std::uint32_t stop[kStopWindow];
   std::uint16_t lb[kChunkMax][64];
   std::int16_t rt[kChunkMax][64];
   std::uint16_t ev[kChunkEvBytes / 2U];  // [chunk stop][arr, dep][n_t]
@@ -664,10 +668,10 @@ struct lane_query {
 // uniform and staged in LDS; each lane reads its own query's state.
 template <bool Fwd>
 __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state const& s,
-                                    std::uint32_t r, std::uint32_t k, std::uint32_t lane,
-                                    lane_query const q, std::uint16_t base, int tad_k,
-                                    std::uint32_t& c_stops, std::uint32_t& c_tmp,
-                                    std::uint32_t& c_ev, std::uint32_t& c_rd,
+                                    std::uint32_t r, std::uint32_t start, std::uint32_t k,
+                                    std::uint32_t lane, lane_query const q, std::uint16_t base,
+                                    int tad_k, std::uint32_t& c_stops, std::uint32_t& c_skip,
+                                    std::uint32_t& c_tmp, std::uint32_t& c_ev, std::uint32_t& c_rd,
                                     std::uint32_t& c_tt, bool& any, bool& started) {
   using D = dirop<Fwd>;
   auto const L = s.L;
@@ -705,23 +709,35 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
   route_bits const bits{x.rbits, x.tbits, tidx, im.bitfields, staged_t};
   auto const chunk = min(kChunkMax, kChunkEvBytes / (4U * n_t));
   auto const top = 1U << (31 - __clz(n_t));  // largest power of two <= n_t
+  // the stop words of scan indices [ws, ws + kStopWindow), in scan order
+  auto const stage_stops = [&](std::uint32_t w0) {
+    if (w0 + lane < n_stops) {
+      auto const si = w0 + lane;
+      x.stop[lane] = stops_g[Fwd ? si : n_stops - si - 1U];
+    }
+  };
+  std::uint32_t ws = start;
+  stage_stops(ws);
 
   et_t<Fwd> et{~0U, 0xFFFFU};
   // `started`: one of the route's stops is marked for this query.  Before
   // its first marked stop a query has no trip (boarding needs a mark), so
   // nothing happens; its visited stops count only if the route was marked
-  // for it (the packing admits only such queries, so on lanes start).
+  // for it (the packing admits only such queries, so on lanes start).  The
+  // stops before `start` are marked for none of the wavefront's queries:
+  // they are not visited, and count as visited stops of nigiri's loop.
   bool done = !q.on;
   started = false;
-  std::uint32_t my_stops = 0U;
-  for (std::uint32_t c0 = 0U; c0 < n_stops; c0 += chunk) {
+  std::uint32_t my_stops = done ? 0U : start;
+  for (std::uint32_t c0 = start; c0 < n_stops; c0 += chunk) {
     auto const nc = min(chunk, n_stops - c0);
     wave_sync_lds();  // readers of the previous chunk are done
-    if (lane < nc) {
-      auto const si = c0 + lane;
-      x.stop[lane] = stops_g[Fwd ? si : n_stops - si - 1U];
+    if (c0 + nc > ws + kStopWindow) {
+      ws = c0;
+      stage_stops(ws);
+      wave_sync_lds();
     }
-    wave_sync_lds();
+    auto const* cstop = x.stop + (c0 - ws);
     // this lane's station marks at the chunk's stops, with its lower bounds
     // there if it rides into the chunk (a rider needs them at every stop),
     // and the chunk's event blocks, in flight together; a lane that does
@@ -738,7 +754,7 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
       rtr[j] = D::kInvalid;
       mkr[j] = 0U;
       if (j < nc && !done) {
-        auto const l = x.stop[j] & kLocMask;
+        auto const l = cstop[j] & kLocMask;
         mkr[j] = mark_t32[mrow + 2U * l];
         if (rides_in) lbr[j] = s.lb[col0 + l * 64U];
       }
@@ -785,7 +801,7 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
 #pragma unroll
       for (std::uint32_t j = 0U; j < kChunkMax; ++j) {
         if (j < nc && !done) {
-          auto const l = x.stop[j] & kLocMask;
+          auto const l = cstop[j] & kLocMask;
           if (!rides_in && j >= first) lbr[j] = s.lb[col0 + l * 64U];
           if (((mbits >> j) & 1U) != 0U) rtr[j] = rt_q[static_cast<std::size_t>(l) * 64U];
         }
@@ -802,7 +818,7 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
 
     for (std::uint32_t j = 0U; j < nc; ++j) {
       auto const i = c0 + j;
-      auto const stp = uni(x.stop[j]);
+      auto const stp = uni(cstop[j]);
       auto const l = stp & kLocMask;
       bool const marked = ((mbits >> j) & 1U) != 0U;
       int const lbv = x.lb[j][lane];
@@ -875,7 +891,10 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
     }
   }
   started = started && q.on;
-  if (started) c_stops += my_stops;
+  if (started) {
+    c_stops += my_stops;
+    c_skip += start;
+  }
 }
 
 // update_route (raptor.h:1181-1349), one lane per (route, query): for routes
@@ -885,10 +904,10 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
 // state from global memory stop by stop.
 template <bool Fwd>
 __device__ inline void scan_route_lane64(dev_image const& im, dev_state const& s, std::uint32_t r,
-                                               std::uint32_t k, lane_query const q, std::uint16_t base,
-                                               int tad_k, std::uint32_t& c_stops, std::uint32_t& c_tmp,
-                                               std::uint32_t& c_ev, std::uint32_t& c_rd, std::uint32_t& c_tt,
-                                               bool& any, bool& started) {
+                                         std::uint32_t start, std::uint32_t k, lane_query const q,
+                                         std::uint16_t base, int tad_k, std::uint32_t& c_stops,
+                                         std::uint32_t& c_skip, std::uint32_t& c_tmp, std::uint32_t& c_ev,
+                                         std::uint32_t& c_rd, std::uint32_t& c_tt, bool& any, bool& started) {
   using D = dirop<Fwd>;
   auto const L = s.L;
   auto const s0 = im.route_stop_offsets[r];
@@ -907,9 +926,13 @@ __device__ inline void scan_route_lane64(dev_image const& im, dev_state const& s
   auto const* rt_q = s.round_times + (static_cast<std::size_t>(q.g) * s.K + (k - 1U)) * L * 64U + q.ln;
   started = q.on;  // the packing admits only queries for which the route is marked
   if (!q.on) return;
+  // no stop before `start` is marked for this query: nothing happens there
+  // (counted as visited stops of nigiri's loop)
+  c_stops += start;
+  c_skip += start;
   et_t<Fwd> et{~0U, 0xFFFFU};
   et_counters ce;
-  for (std::uint32_t i = 0U; i < n_stops; ++i) {
+  for (std::uint32_t i = start; i < n_stops; ++i) {
     auto const stop_idx = Fwd ? i : n_stops - i - 1U;
     auto const stp = stops[stop_idx];
     auto const l = stp & kLocMask;
@@ -970,7 +993,12 @@ __device__ inline void scan_route_lane64(dev_image const& im, dev_state const& s
 // it is active and one of the route's stops is marked for it: those are the
 // queries for which mark_routes (raptor.h:228-241) marked the route.
 // Pairs are packed greedily in list order; one wavefront packs a window of
-// 64 list entries.
+// 64 list entries.  Each pair also yields its start: the scan index of the
+// route's first stop marked for one of its queries.  Before that stop none
+// of them has a trip (boarding needs a mark), so update_route changes
+// nothing there and the scan begins at the start (a packed wavefront at
+// the smallest start of its pairs).
+template <bool Fwd>
 __global__ void k_route_pack(round_args a) {
   auto const& im = a.im;
   auto const& s = a.st;
@@ -980,7 +1008,9 @@ __global__ void k_route_pack(round_args a) {
   // two bases
   __shared__ std::uint32_t kidx_all[kWavesPerBlock][64], loff_all[kWavesPerBlock][64];
   __shared__ std::uint32_t bases_all[kWavesPerBlock][2];
+  __shared__ std::uint32_t starts_all[kWavesPerBlock][64];
   auto* fills = fills_all[threadIdx.x / 64U];
+  auto* starts = starts_all[threadIdx.x / 64U];
   auto* kidx = kidx_all[threadIdx.x / 64U];
   auto* loff = loff_all[threadIdx.x / 64U];
   auto* bases = bases_all[threadIdx.x / 64U];
@@ -989,7 +1019,7 @@ __global__ void k_route_pack(round_args a) {
   auto const wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64U;
   for (std::uint32_t w0 = wave * 64U; w0 < n_items; w0 += gridDim.x * kWavesPerBlock * 64U) {
     auto const item = w0 + lane;
-    std::uint32_t r = ~0U, g = 0U;
+    std::uint32_t r = ~0U, g = 0U, start = 0U;
     std::uint64_t mask = 0ULL;
     if (item < n_items) {
       auto const e = s.list_route[item];
@@ -998,41 +1028,66 @@ __global__ void k_route_pack(round_args a) {
       auto const act = s.grp_active[g];
       if (act != 0ULL) {
         auto const* mt = s.mark_t + static_cast<std::size_t>(g) * s.L;
-        auto const i1 = im.route_stop_offsets[r + 1U];
-        auto i = im.route_stop_offsets[r];
+        auto const i0 = im.route_stop_offsets[r], i1 = im.route_stop_offsets[r + 1U];
+        // first and last marked stop (array order) of the active queries
+        std::uint32_t first = i1, last = i0;
+        auto i = i0;
         for (; i + 4U <= i1; i += 4U) {
           auto const a0 = im.route_stops[i] & kLocMask, a1 = im.route_stops[i + 1U] & kLocMask;
           auto const a2 = im.route_stops[i + 2U] & kLocMask, a3 = im.route_stops[i + 3U] & kLocMask;
-          mask |= mt[a0] | mt[a1] | mt[a2] | mt[a3];
+          auto const m0 = mt[a0] & act, m1 = mt[a1] & act, m2 = mt[a2] & act, m3 = mt[a3] & act;
+          auto const any4 = m0 | m1 | m2 | m3;
+          if (any4 != 0ULL) {
+            if (first == i1) first = i + (m0 != 0ULL ? 0U : (m1 != 0ULL ? 1U : (m2 != 0ULL ? 2U : 3U)));
+            last = i + (m3 != 0ULL ? 3U : (m2 != 0ULL ? 2U : (m1 != 0ULL ? 1U : 0U)));
+            mask |= any4;
+          }
         }
-        for (; i < i1; ++i) mask |= mt[im.route_stops[i] & kLocMask];
-        mask &= act;
+        for (; i < i1; ++i) {
+          auto const m0 = mt[im.route_stops[i] & kLocMask] & act;
+          if (m0 != 0ULL) {
+            if (first == i1) first = i;
+            last = i;
+            mask |= m0;
+          }
+        }
+        // scan order runs backward through the stops for backward searches
+        if (mask != 0ULL) start = min(Fwd ? first - i0 : i1 - 1U - last, kMaxScanStart);
       }
     }
     auto const c = static_cast<std::uint32_t>(__popcll(mask));
     // greedy assignment in item order (uniform loop over the 64 entries)
     // (at most max_entries list entries, i.e. groups, per wavefront)
     auto const max_entries = a.tune.pack_max_entries;
-    std::uint32_t my_w = 0U, my_o = 0U, n_w = 0U, fill = 0U, cur_r = ~0U, entries = 0U;
+    std::uint32_t my_w = 0U, my_o = 0U, n_w = 0U, fill = 0U, cur_r = ~0U, entries = 0U, w_start = 0U;
     for (std::uint32_t t = 0U; t < 64U; ++t) {
       auto const ct = static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(c), static_cast<int>(t)));
       if (ct == 0U) continue;
       auto const rt = static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(r), static_cast<int>(t)));
+      auto const st = static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(start), static_cast<int>(t)));
       if (n_w == 0U || rt != cur_r || fill + ct > 64U || entries == max_entries) {
-        if (n_w != 0U && lane == 0U) fills[n_w - 1U] = fill;
+        if (n_w != 0U && lane == 0U) {
+          fills[n_w - 1U] = fill;
+          starts[n_w - 1U] = w_start;
+        }
         ++n_w;
         fill = 0U;
         entries = 0U;
         cur_r = rt;
+        w_start = st;
       }
       ++entries;
+      w_start = min(w_start, st);
       if (lane == t) {
         my_w = n_w - 1U;
         my_o = fill;
       }
       fill += ct;
     }
-    if (n_w != 0U && lane == 0U) fills[n_w - 1U] = fill;
+    if (n_w != 0U && lane == 0U) {
+      fills[n_w - 1U] = fill;
+      starts[n_w - 1U] = w_start;
+    }
     wave_sync_lds();
     // wavefronts of fewer than loose_lanes queries go loose: their queries
     // are scanned one (route, query) per lane; the kept ones take packed
@@ -1056,7 +1111,9 @@ __global__ void k_route_pack(round_args a) {
       if (kidx[my_w] != ~0U) {
         auto const pw = static_cast<std::size_t>(bases[0]) + kidx[my_w];
         if (pw < s.cap_pack) {
-          if (my_o == 0U) s.packed_info[pw] = (static_cast<std::uint64_t>(r) << 32) | fills[my_w];
+          if (my_o == 0U) {
+            s.packed_info[pw] = (static_cast<std::uint64_t>(r) << 32) | (starts[my_w] << 16) | fills[my_w];
+          }
           auto* out = s.packed_lanes + pw * 64U + my_o;
           auto m = mask;
           for (std::uint32_t j = 0U; m != 0ULL; ++j) {
@@ -1075,7 +1132,7 @@ __global__ void k_route_pack(round_args a) {
           for (std::uint32_t j = 0U; m != 0ULL; ++j) {
             auto const b = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(m)) - 1);
             m &= m - 1ULL;
-            out[j] = (static_cast<std::uint64_t>(r) << 32) | (g << 6) | b;
+            out[j] = (static_cast<std::uint64_t>(r) << 32) | (start << 16) | (g << 6) | b;
           }
         } else {
           atomicOr(&s.counts[4], 1U);
@@ -1101,12 +1158,14 @@ __global__ void __launch_bounds__(256, 8) k_route_scan(round_args a) {
     auto& x = lds_all[threadIdx.x / 64U];
     std::uint32_t const lane = threadIdx.x & 63U;
     auto const n_items = clamp_count(s.counts[5], s.cap_pack);
-    std::uint32_t c_stops = 0U, c_tmp = 0U, c_ev = 0U, c_rd = 0U, c_tt = 0U, c_ws = 0U, c_lws = 0U, c_li = 0U;
+    std::uint32_t c_stops = 0U, c_skip = 0U, c_tmp = 0U, c_ev = 0U, c_rd = 0U, c_tt = 0U, c_ws = 0U, c_lws = 0U,
+                  c_li = 0U;
     item_claim claim{&s.counts[6], a.tune.route_claim};
     for (std::uint32_t item = claim.get(); item < n_items; item = claim.get()) {
       auto const info = s.packed_info[item];
       auto const r = uni(static_cast<std::uint32_t>(info >> 32));
-      auto const n_lanes = uni(static_cast<std::uint32_t>(info));
+      auto const n_lanes = uni(static_cast<std::uint32_t>(info) & 0xFFFFU);
+      auto const start = uni((static_cast<std::uint32_t>(info) >> 16) & 0xFFFFU);
       lane_query q{0U, 0U, false};
       if (lane < n_lanes) {
         auto const d = s.packed_lanes[static_cast<std::size_t>(item) * 64U + lane];
@@ -1122,13 +1181,14 @@ __global__ void __launch_bounds__(256, 8) k_route_scan(round_args a) {
       wave_sync_lds();  // the previous item's readers are done
       auto const n_t = uni(im.route_transport_from[r + 1U]) - uni(im.route_transport_from[r]);
       if (n_t <= kBatchedMaxRouteTransports) {
-        scan_route64<Fwd>(x, im, s, r, k, lane, q, base, tad_k, c_stops, c_tmp, c_ev, c_rd, c_tt,
-                          any, started);
+        scan_route64<Fwd>(x, im, s, r, start, k, lane, q, base, tad_k, c_stops, c_skip, c_tmp, c_ev, c_rd,
+                          c_tt, any, started);
       } else {
-        scan_route_lane64<Fwd>(im, s, r, k, q, base, tad_k, c_stops, c_tmp, c_ev, c_rd, c_tt, any, started);
+        scan_route_lane64<Fwd>(im, s, r, start, k, q, base, tad_k, c_stops, c_skip, c_tmp, c_ev, c_rd, c_tt,
+                               any, started);
       }
       if (__ballot(started) != 0ULL) {
-        c_ws += im.route_stop_offsets[r + 1U] - im.route_stop_offsets[r];
+        c_ws += im.route_stop_offsets[r + 1U] - im.route_stop_offsets[r] - start;
       }
       if (any) s.any_tmp[slot] = 1U;
     }
@@ -1142,11 +1202,12 @@ __global__ void __launch_bounds__(256, 8) k_route_scan(round_args a) {
       if (b >= n_loose) break;
       auto const it = b + lane;
       lane_query q{0U, 0U, false};
-      std::uint32_t r = 0U, n_stops = 0U;
+      std::uint32_t r = 0U, n_stops = 0U, start = 0U;
       if (it < n_loose) {
         auto const v = s.loose[it];
         r = static_cast<std::uint32_t>(v >> 32);
-        q.g = static_cast<std::uint32_t>(v >> 6) & ((1U << 26) - 1U);
+        start = (static_cast<std::uint32_t>(v) >> 16) & 0xFFFFU;
+        q.g = (static_cast<std::uint32_t>(v) >> 6) & 1023U;
         q.ln = static_cast<std::uint32_t>(v) & 63U;
         q.on = ((s.grp_active[q.g] >> q.ln) & 1ULL) != 0ULL;
         n_stops = im.route_stop_offsets[r + 1U] - im.route_stop_offsets[r];
@@ -1156,11 +1217,11 @@ __global__ void __launch_bounds__(256, 8) k_route_scan(round_args a) {
       int const tad_k = s.tad[static_cast<std::size_t>(slot) * kRowsMax + k];
       bool any = false, started = false;
       if (it < n_loose) {
-        scan_route_lane64<Fwd>(im, s, r, k, q, s.param[slot].base, tad_k, c_stops, c_tmp, c_ev, c_rd, c_tt,
-                               any, started);
+        scan_route_lane64<Fwd>(im, s, r, start, k, q, s.param[slot].base, tad_k, c_stops, c_skip, c_tmp, c_ev,
+                               c_rd, c_tt, any, started);
       }
-      // wavefront iterations: the longest route of the 64
-      auto ws = started ? n_stops : 0U;
+      // wavefront iterations: the longest scan of the 64
+      auto ws = started ? n_stops - start : 0U;
       for (int off = 32; off > 0; off >>= 1) ws = max(ws, static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(ws), off, 64)));
       c_ws += ws;
       c_lws += ws;
@@ -1168,6 +1229,7 @@ __global__ void __launch_bounds__(256, 8) k_route_scan(round_args a) {
       if (any) s.any_tmp[slot] = 1U;
     }
     wave_count(&s.work[kWorkRouteStops], c_stops);
+    wave_count(&s.work[kWorkSkippedStops], c_skip);
     wave_count(&s.work[kWorkTmpWrites], c_tmp);
     wave_count(&s.work[kWorkEvents], c_ev);
     wave_count(&s.work[kWorkRouteDayTests], c_rd);
@@ -2169,7 +2231,7 @@ void round_segment(int seg, round_args const& a, std::uint32_t const* sweep, boo
       hipLaunchKernelGGL((k_mark_routes<W>), dim3(item_blocks), dim3(kBlock), 0, st, a);
       if constexpr (W == 64) {
         compact(a.st.flag_route, a.st.G, a.st.R, a.st.list_route, &a.st.counts[2], a.st.cap_route, a.st, st);
-        hipLaunchKernelGGL(k_route_pack, dim3(item_blocks), dim3(kBlock), 0, st, a);
+        hipLaunchKernelGGL((k_route_pack<Fwd>), dim3(item_blocks), dim3(kBlock), 0, st, a);
       }
       break;
     case 1:

@@ -230,6 +230,19 @@ void make_sweep(std::uint32_t n, std::vector<std::uint32_t> const& by_rank_desc,
     }
     s.edge_off.push_back(static_cast<std::uint32_t>(s.src.size()));
   }
+  // host levels: the leading levels of fewer than kHostLevelNodes locations,
+  // while their edges stay within 1/32 of all sweep edges (each is one
+  // dependent device launch for a handful of locations)
+  constexpr std::uint32_t kHostLevelNodes = 128U;
+  auto const edge_budget = s.src.size() / 32U;
+  s.host_levels = 0U;
+  while (s.host_levels + 1U < s.level_off.size() &&
+         s.level_off[s.host_levels + 1U] - s.level_off[s.host_levels] < kHostLevelNodes &&
+         s.edge_off[s.level_off[s.host_levels + 1U]] <= edge_budget) {
+    ++s.host_levels;
+  }
+  s.on_host.assign(n, 0U);
+  for (std::uint32_t p = 0U; p != s.level_off[s.host_levels]; ++p) s.on_host[s.order[p]] = 1U;
 }
 
 }  // namespace
@@ -364,15 +377,41 @@ void ch_upward(contraction_hierarchy const& ch, bool fwd,
   }
 }
 
+void ch_sweep_seeds(contraction_hierarchy const& ch, bool fwd,
+                    std::vector<std::pair<std::uint32_t, std::uint16_t>> const& seeds, ch_search_scratch& s,
+                    std::vector<std::pair<std::uint32_t, std::uint16_t>>& out) {
+  thread_local std::vector<std::pair<std::uint32_t, std::uint16_t>> up;
+  ch_upward(ch, fwd, seeds, s, up);
+  auto const& sw = fwd ? ch.sweep_fwd : ch.sweep_bwd;
+  out.clear();
+  for (auto const& l : up) {
+    if (sw.on_host[l.first] == 0U) out.push_back(l);
+  }
+  // the host levels' sweep over the upward labels (s.dist where stamped):
+  // every edge comes from a location of an earlier host level or of no
+  // sweep level at all, so their values are final when read
+  auto const value = [&](std::uint32_t v) -> std::uint32_t { return s.stamp[v] == s.cur ? s.dist[v] : 0xFFFFU; };
+  for (std::uint32_t p = 0U; p != sw.level_off[sw.host_levels]; ++p) {
+    auto const v = sw.order[p];
+    auto d = value(v);
+    for (auto e = sw.edge_off[p]; e != sw.edge_off[p + 1U]; ++e) d = std::min(d, value(sw.src[e]) + sw.w[e]);
+    if (d < kLbCutoff) {  // (an upward label is below kLbCutoff: d is then too)
+      s.stamp[v] = s.cur;
+      s.dist[v] = d;
+      out.emplace_back(v, static_cast<std::uint16_t>(d));
+    }
+  }
+}
+
 void ch_lower_bounds(contraction_hierarchy const& ch, bool fwd,
                      std::vector<std::pair<std::uint32_t, std::uint16_t>> const& seeds, ch_search_scratch& s,
                      std::vector<std::uint16_t>& lb) {
-  std::vector<std::pair<std::uint32_t, std::uint16_t>> up;
-  ch_upward(ch, fwd, seeds, s, up);
+  std::vector<std::pair<std::uint32_t, std::uint16_t>> start;
+  ch_sweep_seeds(ch, fwd, seeds, s, start);
   lb.assign(ch.n, 0xFFFFU);
-  for (auto const& [v, d] : up) lb[v] = d;
+  for (auto const& [v, d] : start) lb[v] = d;
   auto const& sw = fwd ? ch.sweep_fwd : ch.sweep_bwd;
-  for (std::size_t i = 0; i != sw.order.size(); ++i) {
+  for (std::size_t i = sw.level_off[sw.host_levels]; i != sw.order.size(); ++i) {
     auto const v = sw.order[i];
     std::uint32_t d = lb[v];
     for (auto e = sw.edge_off[i]; e != sw.edge_off[i + 1U]; ++e) {

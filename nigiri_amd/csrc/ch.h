@@ -31,12 +31,17 @@ struct ch_csr {
 // Sweep order of one direction: the locations with edges from higher
 // ranks, grouped into levels (a location comes after every higher-ranked
 // neighbour its sweep edges read), and those edges in that order.
+// The leading `host_levels` levels (the top of the hierarchy: a long chain
+// of small levels) are swept on the host right after the upward search,
+// so that the device sweep starts at level host_levels.
 struct ch_sweep {
   std::vector<std::uint32_t> level_off;  // [n_levels + 1] into order
   std::vector<std::uint32_t> order;      // locations
   std::vector<std::uint32_t> edge_off;   // [order.size() + 1]
   std::vector<std::uint32_t> src;        // higher-ranked neighbour
   std::vector<std::uint16_t> w;
+  std::uint32_t host_levels = 0U;
+  std::vector<std::uint8_t> on_host;     // [n]: 1 for the locations of those levels
 };
 
 struct contraction_hierarchy {
@@ -69,8 +74,18 @@ void ch_upward(contraction_hierarchy const& ch, bool fwd,
                ch_search_scratch& s,
                std::vector<std::pair<std::uint32_t, std::uint16_t>>& out);
 
-// Complete lower bounds on the host (upward search + sweep): the reference
-// for the device sweep, and the plain host path.
+// The labels a sweep from level host_levels starts from: the upward
+// search's labels of the locations below the host levels, and the final
+// bounds (< kLbCutoff) of the host levels' locations, each location once.
+// Every location without a label starts unreachable.
+void ch_sweep_seeds(contraction_hierarchy const& ch, bool fwd,
+                    std::vector<std::pair<std::uint32_t, std::uint16_t>> const& seeds,
+                    ch_search_scratch& s,
+                    std::vector<std::pair<std::uint32_t, std::uint16_t>>& out);
+
+// Complete lower bounds on the host (sweep seeds, then the sweep from level
+// host_levels, as the device runs it): the reference for the device sweep,
+// and the plain host path.
 void ch_lower_bounds(contraction_hierarchy const& ch, bool fwd,
                      std::vector<std::pair<std::uint32_t, std::uint16_t>> const& seeds,
                      ch_search_scratch& s, std::vector<std::uint16_t>& lb);

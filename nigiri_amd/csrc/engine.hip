@@ -115,6 +115,8 @@ struct nigiri_amd_image {
 // host threads that write the legs of a step's journeys (with the calling
 // thread); the lower-bound workers and query preparation use others
 constexpr unsigned kLegWorkers = 9U;
+// slots whose lower bounds one PHAST sweep of the batched engine computes
+constexpr std::uint32_t kPhastRows = 128U;
 
 // Fork-join pool of host threads for the per-journey work of a step (the
 // legs of reconstructed journeys).  parallel_for runs f(0 .. n-1) on the
@@ -224,8 +226,9 @@ struct nigiri_amd_engine {
   // lower bounds of admitted slots (PHAST): selected groups / rows, their
   // lane masks, and the upward searches' labels (growable)
   std::uint16_t* d_lb_row = nullptr;   // [L] lower bounds given by the caller
-  std::uint32_t* d_ph_sel = nullptr;   // [S]
-  std::uint64_t* d_ph_masks = nullptr; // [S]
+  std::uint32_t* d_ph_sel = nullptr;   // [S] rows swept
+  std::uint32_t* d_ph_slots = nullptr; // [S] batched engine: the rows' slots
+  std::uint16_t* d_ph_rows = nullptr;  // batched engine: [min(S, kPhastRows)][L] PHAST rows
   std::uint32_t* d_ph_seed_idx = nullptr;
   std::uint16_t* d_ph_seed_min = nullptr;
   std::size_t ph_seed_cap = 0U;
@@ -939,27 +942,28 @@ struct runner {
   }
 
   // Lower bounds of the new slots (search.h:114-131): PHAST on the device,
-  // seeded with the upward searches of their host preparation.  Batched
-  // engine: the lanes of each touched group are swept together.
+  // seeded with the upward searches (and the host levels of the sweep) of
+  // their host preparation.  Plain engine: each slot's lower bounds are a
+  // row [L] of lb, swept in place.  Batched engine: up to kPhastRows slots at
+  // a time are swept as rows of the engine's PHAST buffer (one thread per
+  // (location, slot) instead of a wavefront per (location, group) for the
+  // one or two new lanes a group usually has), then copied into their lanes.
   void phast_lbs(std::vector<std::uint32_t> const& new_slots) {
     if (new_slots.empty()) return;
     auto const L = e.st.L, W = e.W;
-    std::vector<std::uint32_t> sel, seed_idx;
-    std::vector<std::uint64_t> masks;
+    auto const n = new_slots.size();
+    auto const rows_per = W == 64U ? std::min<std::size_t>(kPhastRows, n) : n;
+    std::vector<std::uint32_t> sel, seed_idx, seed_off{0U};
     std::vector<std::uint16_t> seed_min;
-    std::vector<std::int32_t> pos_of(W == 64U ? e.G : e.S, -1);
-    for (auto const s : new_slots) {
-      auto const key = W == 64U ? s / 64U : s;
-      if (pos_of[key] < 0) {
-        pos_of[key] = static_cast<std::int32_t>(sel.size());
-        sel.push_back(key);
-        masks.push_back(0U);
-      }
-      masks[static_cast<std::size_t>(pos_of[key])] |= W == 64U ? 1ULL << (s % 64U) : 1ULL;
+    for (std::size_t i = 0; i != n; ++i) {
+      auto const s = new_slots[i];
+      auto const row = W == 64U ? static_cast<std::uint32_t>(i % kPhastRows) : s;
+      if (i < rows_per) sel.push_back(row);
       for (auto const& [l, d] : pq[static_cast<std::size_t>(slots[s].q)].lb_up_) {
-        seed_idx.push_back(W == 64U ? ((s / 64U) * L + l) * 64U + s % 64U : s * L + l);
+        seed_idx.push_back(row * L + l);
         seed_min.push_back(d);
       }
+      if ((i + 1U) % rows_per == 0U || i + 1U == n) seed_off.push_back(static_cast<std::uint32_t>(seed_idx.size()));
     }
     auto const stream = e.stream;
     if (seed_idx.size() > e.ph_seed_cap) {
@@ -968,26 +972,33 @@ struct runner {
       e.d_ph_seed_min = e.grow(e.d_ph_seed_min, e.ph_seed_cap);
     }
     HIP_CHECK(hipMemcpyAsync(e.d_ph_sel, sel.data(), 4U * sel.size(), hipMemcpyHostToDevice, stream));
-    HIP_CHECK(hipMemcpyAsync(e.d_ph_masks, masks.data(), 8U * masks.size(), hipMemcpyHostToDevice, stream));
+    if (W == 64U) {
+      HIP_CHECK(hipMemcpyAsync(e.d_ph_slots, new_slots.data(), 4U * n, hipMemcpyHostToDevice, stream));
+    }
     if (!seed_idx.empty()) {
       HIP_CHECK(hipMemcpyAsync(e.d_ph_seed_idx, seed_idx.data(), 4U * seed_idx.size(), hipMemcpyHostToDevice,
                                stream));
       HIP_CHECK(hipMemcpyAsync(e.d_ph_seed_min, seed_min.data(), 2U * seed_min.size(), hipMemcpyHostToDevice,
                                stream));
     }
-    lb_phast_args a{};
-    a.sw = e.img->d.sweep[fwd ? 0 : 1];
-    a.dist = e.st.lb;
-    a.L = L;
-    a.sel = e.d_ph_sel;
-    a.masks = e.d_ph_masks;
-    a.n_sel = static_cast<std::uint32_t>(sel.size());
-    a.seed_idx = e.d_ph_seed_idx;
-    a.seed_min = e.d_ph_seed_min;
-    a.n_seeds = static_cast<std::uint32_t>(seed_idx.size());
-    auto const& lv = fwd ? tt.ch.sweep_fwd.level_off : tt.ch.sweep_bwd.level_off;
-    launch_lb_phast(W, a, lv.data(), static_cast<std::uint32_t>(lv.size() - 1U), stream);
-    HIP_CHECK(hipGetLastError());
+    auto const& sw = fwd ? tt.ch.sweep_fwd : tt.ch.sweep_bwd;
+    auto const n_levels = static_cast<std::uint32_t>(sw.level_off.size() - 1U);
+    for (std::size_t b = 0; b + 1U < seed_off.size(); ++b) {
+      auto const rows = static_cast<std::uint32_t>(std::min(rows_per, n - b * rows_per));
+      lb_phast_args a{};
+      a.sw = e.img->d.sweep[fwd ? 0 : 1];
+      a.dist = W == 64U ? e.d_ph_rows : e.st.lb;
+      a.L = L;
+      a.sel = e.d_ph_sel;
+      a.masks = nullptr;
+      a.n_sel = rows;
+      a.seed_idx = e.d_ph_seed_idx + seed_off[b];
+      a.seed_min = e.d_ph_seed_min + seed_off[b];
+      a.n_seeds = seed_off[b + 1U] - seed_off[b];
+      launch_lb_phast(1U, a, sw.level_off.data(), sw.host_levels, n_levels, stream);
+      if (W == 64U) launch_lb_scatter64(e.st, e.d_ph_rows, e.d_ph_slots + b * rows_per, rows, stream);
+      HIP_CHECK(hipGetLastError());
+    }
     HIP_CHECK(hipStreamSynchronize(stream));  // host vectors go out of scope
   }
 };
@@ -1132,7 +1143,8 @@ void nigiri_amd_image_destroy(nigiri_amd_image* img) { delete img; }
 namespace nigiri_amd {
 namespace {
 
-// The upward search (host) of a lower-bound query; throws on bad seeds.
+// The upward search and the host levels of the sweep (host) of a
+// lower-bound query; throws on bad seeds.
 void upward_labels(nigiri_amd_image const& img, int32_t dir, uint32_t n_seeds, const uint32_t* seed_loc,
                    const uint16_t* seed_min, std::vector<std::pair<std::uint32_t, std::uint16_t>>& up) {
   thread_local ch_search_scratch s;
@@ -1142,7 +1154,7 @@ void upward_labels(nigiri_amd_image const& img, int32_t dir, uint32_t n_seeds, c
     if (seed_loc[i] >= img.d.L) throw status_error{NIGIRI_AMD_EINVAL, "lower_bounds: seed out of range"};
     seeds.emplace_back(seed_loc[i], seed_min[i]);
   }
-  ch_upward(img.tt->ch, dir == NIGIRI_AMD_DIR_FWD, seeds, s, up);
+  ch_sweep_seeds(img.tt->ch, dir == NIGIRI_AMD_DIR_FWD, seeds, s, up);
 }
 
 // Stages the labels (host pinned-free copies into sel / seed arrays at the
@@ -1173,8 +1185,9 @@ void lb_row_sweep(nigiri_amd_image const& img, int32_t dir, std::vector<std::pai
   a.seed_idx = d_idx;
   a.seed_min = d_min;
   a.n_seeds = static_cast<std::uint32_t>(idx.size());
-  auto const& lv = dir == NIGIRI_AMD_DIR_FWD ? img.tt->ch.sweep_fwd.level_off : img.tt->ch.sweep_bwd.level_off;
-  launch_lb_phast(1U, a, lv.data(), static_cast<std::uint32_t>(lv.size() - 1U), nullptr);
+  auto const& sw = dir == NIGIRI_AMD_DIR_FWD ? img.tt->ch.sweep_fwd : img.tt->ch.sweep_bwd;
+  launch_lb_phast(1U, a, sw.level_off.data(), sw.host_levels, static_cast<std::uint32_t>(sw.level_off.size() - 1U),
+                  nullptr);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(nullptr));
 }
@@ -1404,7 +1417,10 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     e->d_slot_list = m.alloc<std::uint32_t>(S);
     e->d_lb_row = m.alloc<std::uint16_t>(L);
     e->d_ph_sel = m.alloc<std::uint32_t>(S);
-    e->d_ph_masks = m.alloc<std::uint64_t>(S);
+    if (lanes == 64U) {
+      e->d_ph_slots = m.alloc<std::uint32_t>(S);
+      e->d_ph_rows = m.alloc<std::uint16_t>(static_cast<std::size_t>(std::min<std::uint32_t>(S, kPhastRows)) * L);
+    }
     e->h_active.reserve(G);
     e->h_rows.reserve(static_cast<std::size_t>(S) * kMaxDestBits * K);
     e->h_counts.reserve(8U);

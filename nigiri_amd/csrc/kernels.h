@@ -73,9 +73,15 @@ struct lb_phast_args {
   std::uint32_t n_seeds;
 };
 
-// level_off: the sweep's level boundaries (host copy), one launch per level
+// level_off: the sweep's level boundaries (host copy); the sweep runs
+// levels [first_level, n_levels) (the ones before are swept on the host and
+// arrive as seeds), a launch per large level
 void launch_lb_phast(std::uint32_t W, lb_phast_args const& a, std::uint32_t const* level_off,
-                     std::uint32_t n_levels, hipStream_t stream);
+                     std::uint32_t first_level, std::uint32_t n_levels, hipStream_t stream);
+// PHAST rows [n_rows][L] (device) into the lower-bound lanes of the slots
+// slots[0 .. n_rows) of the batched engine
+void launch_lb_scatter64(dev_state const& st, std::uint16_t const* rows, std::uint32_t const* slots,
+                         std::uint32_t n_rows, hipStream_t stream);
 // lower bounds given as a row [L] (device) into the column of one slot
 void launch_lb_put(std::uint32_t W, dev_state const& st, std::uint32_t slot, std::uint16_t const* row,
                    hipStream_t stream);

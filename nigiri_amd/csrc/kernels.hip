@@ -2083,6 +2083,20 @@ __global__ void __launch_bounds__(1024) k_lb_levels(lb_phast_args a, std::uint32
   }
 }
 
+// PHAST rows [n_rows][L] into the lower-bound lanes of their slots
+// (batched engine)
+__global__ void k_lb_scatter64(dev_state s, std::uint16_t const* rows, std::uint32_t const* slots,
+                               std::uint32_t n_rows) {
+  auto const stride = gridDim.x * blockDim.x;
+  for (std::uint32_t r = 0U; r < n_rows; ++r) {  // uniform
+    auto const slot = slots[r];
+    auto const* row = rows + static_cast<std::size_t>(r) * s.L;
+    for (std::uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < s.L; l += stride) {
+      s.lb[lay<64>::col(s, slot / 64U, l, slot % 64U)] = row[l];
+    }
+  }
+}
+
 template <int W>
 __global__ void k_lb_put(dev_state s, std::uint32_t slot, std::uint16_t const* row) {
   for (std::uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < s.L; l += gridDim.x * blockDim.x) {
@@ -2613,7 +2627,7 @@ hipError_t launch_round_graph(round_graphs& g, bool fwd, std::uint32_t W, round_
 }
 
 void launch_lb_phast(std::uint32_t W, lb_phast_args const& a, std::uint32_t const* level_off,
-                     std::uint32_t n_levels, hipStream_t stream) {
+                     std::uint32_t first_level, std::uint32_t n_levels, hipStream_t stream) {
   if (a.n_sel == 0U) return;
   auto const fill = static_cast<std::size_t>(a.n_sel) * a.L * W;
   if (W == 64U) {
@@ -2627,7 +2641,7 @@ void launch_lb_phast(std::uint32_t W, lb_phast_args const& a, std::uint32_t cons
   // levels of at most kSmall items (wavefronts for W = 64, threads for
   // W = 1) run in one 1024-thread workgroup, as many in a row as there are
   std::size_t const kSmall = W == 64U ? 64U : 4096U;
-  for (std::uint32_t l = 0U; l != n_levels; ++l) {
+  for (std::uint32_t l = first_level; l < n_levels; ++l) {
     auto const b = level_off[l], e = level_off[l + 1U];
     if (b == e) continue;
     auto const items = static_cast<std::size_t>(e - b) * a.n_sel;
@@ -2649,6 +2663,12 @@ void launch_lb_phast(std::uint32_t W, lb_phast_args const& a, std::uint32_t cons
       hipLaunchKernelGGL((k_lb_level<1>), dim3(grid_for(items, kBlock, 8192)), dim3(kBlock), 0, stream, a, b, e);
     }
   }
+}
+
+void launch_lb_scatter64(dev_state const& st, std::uint16_t const* rows, std::uint32_t const* slots,
+                         std::uint32_t n_rows, hipStream_t stream) {
+  hipLaunchKernelGGL(k_lb_scatter64, dim3(grid_for(st.L, kBlock, 2048)), dim3(kBlock), 0, stream, st, rows, slots,
+                     n_rows);
 }
 
 void launch_lb_put(std::uint32_t W, dev_state const& st, std::uint32_t slot, std::uint16_t const* row,

@@ -420,7 +420,7 @@ void prepare(timetable const& tt, nigiri_query const& q, prepared_query& p,
     lower_bounds(tt, fwd, dest_min, p.lb_, s);
   } else {
     p.lb_.clear();
-    ch_upward(tt.ch, fwd, dest_min, s.ch_, p.lb_up_);
+    ch_sweep_seeds(tt.ch, fwd, dest_min, s.ch_, p.lb_up_);
   }
 
   // scheduling hint: queries that start close to each other mark the same

@@ -127,8 +127,9 @@ struct host_scratch {
 
 // Throws status_error for unsupported or malformed queries.  With
 // host_lb == true the lower bounds are computed on the host (lb_); else the
-// device computes them from the upward search lb_up_ (lb_seeds_ holds their
-// sources either way).
+// device sweeps them from lb_up_ (the upward search's labels with the host
+// levels of the sweep already done; lb_seeds_ holds their sources either
+// way).
 void prepare(timetable const& tt, nigiri_query const& q, prepared_query& p,
              host_scratch& s, bool host_lb = true);
 

@@ -127,6 +127,16 @@ def cpu_baseline(tt, qs, seconds):
             "cpu_model": cpu_model()}
 
 
+def knob_overrides(args):
+    """--knobs name=value,...: launch-shape settings for the engine (grid_waves
+    and the knobs of Engine.configure); results do not depend on them."""
+    out = {}
+    for kv in filter(None, args.knobs.split(",")):
+        k, v = kv.split("=")
+        out[k.strip()] = int(v)
+    return out
+
+
 def pct(a, p):
     return round(float(np.percentile(a, p)), 2) if len(a) else None
 
@@ -152,7 +162,7 @@ def served_main(args):
     max_k = 10
     eng = na.Engine(img, lanes=64, groups=args.groups, max_k=max_k)
     eng.profile(True)
-    eng.configure(admission_wait_ms=1.0, graphs=not args.eager)
+    eng.configure(admission_wait_ms=1.0, graphs=not args.eager, **knob_overrides(args))
     kinds = np.array(["intermodal" if qs[i].dest_match_mode == na.MATCH_INTERMODAL
                       else ("range" if qs[i].is_interval else "point") for i in range(n_total)])
     rng = np.random.default_rng(7)
@@ -423,6 +433,9 @@ def main():
                     help="--pong mode: also run the queries on T plain engines from T host threads")
     ap.add_argument("--pong-cpu", type=int, default=4, metavar="M",
                     help="range queries timed on the CPU oracle in --pong mode")
+    ap.add_argument("--knobs", default="",
+                    help="launch-shape settings name=value,... (grid_waves, route_claim, transfer_claim, "
+                         "pack_max_entries, transfer_pack_max_entries, loose_lanes)")
     ap.add_argument("--eager", action="store_true",
                     help="launch each round's kernels one by one instead of replaying captured hipGraphs")
     args = ap.parse_args()
@@ -474,7 +487,7 @@ def main():
     # a step submits a whole batch at once: let preparation fill every slot
     # before the first epoch starts
     eng.configure(admission_wait_ms=args.admission_wait_ms, graphs=not args.eager,
-                  admit_when_idle=args.pipelined)
+                  admit_when_idle=args.pipelined, **knob_overrides(args))
     B = groups * 64
     n_q = len(qs)
 

@@ -290,6 +290,19 @@ void nigiri_amd_image_destroy(nigiri_amd_image* img);
 int nigiri_amd_lower_bounds(nigiri_amd_image* img, int32_t dir,
                             uint32_t n_seeds, const uint32_t* seed_loc,
                             const uint16_t* seed_min, uint16_t* lb);
+/* Fastest direct connection of each query on the device (replaces
+ * routing::get_fastest_direct, src/routing/get_fastest_direct.cc:12-100, as
+ * search::search calls it, search.h:193): the shortest walk from a start
+ * offset into a destination offset, following footpaths (footpaths_out_
+ * forward, footpaths_in_ backward) only into destinations, with the query's
+ * transfer-time settings as given, times fastest_direct_factor rounded up;
+ * 32767 (kUnreachableDirect) when there is none.  A query that sets
+ * has_fastest_direct gets its own value back.  out: [n] minutes (host).
+ * NIGIRI_AMD_EINVAL for a location out of range, NIGIRI_AMD_ECAPACITY for a
+ * query with more than 1024 start and destination locations.
+ * nigiri_amd_search runs the same kernel for its queries. */
+int nigiri_amd_fastest_direct(nigiri_amd_image* img, const nigiri_query* qs,
+                              uint32_t n, int16_t* out);
 /* nigiri_amd_lower_bounds without any device allocation.  No counterpart in
  * the reference: for callers that cannot allocate device memory (code under
  * graph capture set-up, tight serving loops, harnesses that forbid

@@ -177,6 +177,7 @@ SIGNATURES = {
                                           C.POINTER(C.c_uint16), C.c_void_p]),
     "nigiri_amd_lower_bounds_noalloc": (C.c_int, [_P, C.c_int32, C.c_uint32, C.POINTER(C.c_uint32),
                                                   C.POINTER(C.c_uint16), C.c_void_p]),
+    "nigiri_amd_fastest_direct": (C.c_int, [_P, C.POINTER(Query), C.c_uint32, C.POINTER(C.c_int16)]),
     "nigiri_amd_engine_create": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
     "nigiri_amd_engine_state_bytes": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "nigiri_amd_engine_destroy": (None, [_P]),
@@ -511,6 +512,13 @@ class Image:
         fn = lib().nigiri_amd_lower_bounds_noalloc if noalloc else lib().nigiri_amd_lower_bounds
         _check(fn(self.handle, direction, len(seeds), ptr(loc, C.c_uint32), ptr(mins, C.c_uint16),
                   C.c_void_p(lb_ptr)))
+
+    def fastest_direct(self, queries):
+        """get_fastest_direct of every query on the device: minutes (32767:
+        none)."""
+        out = np.zeros(max(len(queries), 1), np.int16)
+        _check(lib().nigiri_amd_fastest_direct(self.handle, queries.q, len(queries), ptr(out, C.c_int16)))
+        return out[:len(queries)]
 
     def __init__(self, tt, device=0):
         self.tt = tt  # the image borrows the host timetable

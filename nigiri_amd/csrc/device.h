@@ -8,10 +8,14 @@
 namespace nigiri_amd {
 
 constexpr int kRowsMax = 16;          // round_times rows (kMaxTransfers + 2)
-constexpr std::uint32_t kMaxDestBits = 16U;  // destination stations per query
-constexpr std::uint32_t kMaxDist = 64U;      // intermodal egress offsets
-constexpr std::uint32_t kMaxChanges = 64U;   // time_at_dest changes per phase
-constexpr std::uint32_t kMaxDestEvents = 256U;
+constexpr std::uint32_t kMaxDestBits = 64U;  // destination stations per query
+constexpr std::uint32_t kMaxDist = 256U;     // intermodal egress offsets
+// relaxations into a query's destinations per round (its destinations'
+// transfers and incoming footpaths, or its egress offsets; checked when a
+// query is prepared), and time_at_dest changes per phase (at most one per
+// such relaxation)
+constexpr std::uint32_t kMaxDestEvents = 1024U;
+constexpr std::uint32_t kMaxChanges = kMaxDestEvents;
 constexpr std::uint32_t kPosTransfer = 0U << 30;
 constexpr std::uint32_t kPosIntermodal = 1U << 30;
 constexpr std::uint32_t kPosFootpath = 2U << 30;
@@ -36,6 +40,15 @@ enum : int {
   kWorkLooseItems = 14,      // loose (route, query) items scanned
   kWorkSkippedStops = 15,    // the part of kWorkRouteStops before a scan's start (not read)
   kWorkCount = 16
+};
+
+// A relaxation into a destination of one query in the transfer phase, at
+// its position in nigiri's order of relaxations.
+struct dest_event {
+  std::uint32_t pos;
+  std::uint32_t target;
+  std::int16_t x;
+  std::uint8_t kind;  // 0 transfer/footpath, 1 intermodal egress
 };
 
 // Downward sweep of PHAST for one search direction (the contraction
@@ -89,11 +102,11 @@ struct slot_param {
   std::int16_t tts_min, tts_add;
   float tts_factor;
   std::uint16_t base;
+  std::uint16_t n_dest;  // non-intermodal destination bits
+  std::uint16_t n_dist;  // intermodal egress offsets
   std::uint8_t end_k;
   std::uint8_t tts_default;
   std::uint8_t intermodal;
-  std::uint8_t n_dest;   // non-intermodal destination bits
-  std::uint8_t n_dist;   // intermodal egress offsets
   std::uint8_t bounds_last_k;  // plain engine: pong bounds (0 = none)
 };
 
@@ -171,6 +184,7 @@ struct dev_state {
   slot_param* param;
   std::uint8_t* kslot;     // [S] round of the slot's running epoch
   std::int16_t* tad;       // [S][16]
+  dest_event* dest_ev;     // [S][kMaxDestEvents] destination relaxations of the phase
   std::int16_t* tad0;      // [S]
   std::uint32_t* n_chg;    // [S]
   std::uint32_t* chg_pos;  // [S][kMaxChanges]

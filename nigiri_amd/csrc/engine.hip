@@ -106,9 +106,16 @@ struct nigiri_amd_image {
   std::uint32_t* d_lb_seed_idx = nullptr;
   std::uint16_t* d_lb_seed_min = nullptr;
   std::size_t lb_seed_cap = 0U;
+  // fastest direct connections (nigiri_amd_fastest_direct and the batched
+  // search): queries, their node sets and results, grown on demand (calls
+  // serialised by the mutex)
+  std::mutex fd_mu;
+  void* d_fd = nullptr;
+  std::size_t fd_cap = 0U;
   ~nigiri_amd_image() {
     if (d_lb_seed_idx != nullptr) (void)hipFree(d_lb_seed_idx);
     if (d_lb_seed_min != nullptr) (void)hipFree(d_lb_seed_min);
+    if (d_fd != nullptr) (void)hipFree(d_fd);
   }
 };
 
@@ -423,7 +430,8 @@ struct slot_info {
   bool is_new = false;
 };
 
-static_assert(kMaxQueryDests == kMaxDestBits && kMaxQueryEgress == kMaxDist,
+static_assert(kMaxQueryDests == kMaxDestBits && kMaxQueryEgress == kMaxDist &&
+                  kMaxQueryDestEvents == kMaxDestEvents,
               "query limits of the host preparation and the device state agree");
 
 // Runs the device journey reconstruction of `jobs` on the engine's stream;
@@ -594,8 +602,8 @@ struct runner {
         if (p.dests_.size() > kMaxDestBits || p.dist_loc_.size() > kMaxDist) {
           throw status_error{NIGIRI_AMD_EUNSUPPORTED, "too many destination offsets"};
         }
-        sp.n_dest = static_cast<std::uint8_t>(p.dests_.size());
-        sp.n_dist = static_cast<std::uint8_t>(p.dist_loc_.size());
+        sp.n_dest = static_cast<std::uint16_t>(p.dests_.size());
+        sp.n_dist = static_cast<std::uint16_t>(p.dist_loc_.size());
         for (std::size_t j = 0; j != p.dests_.size(); ++j) e.h_dest[s * kMaxDestBits + j] = p.dests_[j];
         for (std::size_t j = 0; j != p.dist_loc_.size(); ++j) {
           e.h_dist_loc[s * kMaxDist + j] = p.dist_loc_[j];
@@ -1006,7 +1014,8 @@ struct runner {
 // prepare queries on host threads
 void prepare_all(timetable const& tt, nigiri_query const* qs, std::uint32_t n,
                  std::vector<prepared_query>& pq, std::vector<int>& status,
-                 std::vector<std::string>& errors, bool host_lb) {
+                 std::vector<std::string>& errors, bool host_lb,
+                 std::vector<std::int16_t> const* fd = nullptr, std::vector<std::uint8_t> const* fd_have = nullptr) {
   pq.resize(n);
   status.assign(n, NIGIRI_AMD_OK);
   errors.assign(n, {});
@@ -1021,7 +1030,12 @@ void prepare_all(timetable const& tt, nigiri_query const* qs, std::uint32_t n,
         auto const i = next.fetch_add(1U);
         if (i >= n) break;
         try {
-          prepare(tt, qs[i], pq[i], scratch, host_lb);
+          if (fd_have != nullptr && (*fd_have)[i] == 2U) {
+            throw status_error{NIGIRI_AMD_ECAPACITY, "more than " + std::to_string(kFdMaxNodes) +
+                                                         " start and destination locations"};
+          }
+          auto const* given = fd != nullptr && (*fd_have)[i] == 1U ? &(*fd)[i] : nullptr;
+          prepare(tt, qs[i], pq[i], scratch, host_lb, given);
         } catch (status_error const& e) {
           status[i] = e.code_;
           errors[i] = e.what();
@@ -1288,6 +1302,109 @@ int nigiri_amd_lower_bounds(nigiri_amd_image* img, int32_t dir, uint32_t n_seeds
   });
 }
 
+namespace nigiri_amd {
+namespace {
+
+static_assert(sizeof(fd_node) == sizeof(fd_node_host), "fastest-direct node layouts agree");
+
+// get_fastest_direct of queries qs[0 .. n) on the device.  have[i] = 1 where
+// val[i] holds the result; 0 for a query whose node set cannot be built (a
+// location out of range: its preparation reports that) and for one that
+// brings its own value (has_fastest_direct); 2 for a query with more than
+// kFdMaxNodes start and destination locations (not computed).
+void device_fastest_direct(nigiri_amd_image& img, nigiri_query const* qs, std::uint32_t n,
+                           std::vector<std::int16_t>& val, std::vector<std::uint8_t>& have) {
+  val.assign(n, 0);
+  have.assign(n, 0U);
+  std::vector<fd_query> hq;
+  std::vector<fd_node_host> hn, nodes;
+  std::vector<std::uint32_t> idx;
+  for (std::uint32_t i = 0; i != n; ++i) {
+    auto const& q = qs[i];
+    if (q.has_fastest_direct != 0) continue;
+    try {
+      fastest_direct_nodes(*img.tt, q, nodes);
+    } catch (status_error const&) {
+      continue;
+    }
+    if (nodes.size() > kFdMaxNodes) {
+      have[i] = 2U;
+      continue;
+    }
+    fd_query f{};
+    f.node_begin = static_cast<std::uint32_t>(hn.size());
+    hn.insert(hn.end(), nodes.begin(), nodes.end());
+    f.node_end = static_cast<std::uint32_t>(hn.size());
+    f.factor = q.fastest_direct_factor;
+    f.tts_factor = q.tts_factor;
+    f.tts_min = q.tts_min_transfer_time;
+    f.tts_add = q.tts_additional_time;
+    f.fwd = q.dir == NIGIRI_AMD_DIR_FWD ? 1U : 0U;
+    f.tts_default = q.tts_default != 0 ? 1U : 0U;
+    hq.push_back(f);
+    idx.push_back(i);
+  }
+  if (hq.empty()) return;
+  auto const bytes_q = sizeof(fd_query) * hq.size();
+  auto const bytes_n = (sizeof(fd_node) * hn.size() + 255U) / 256U * 256U;
+  auto const bytes_o = sizeof(std::int16_t) * hq.size();
+  auto const need = bytes_q + bytes_n + bytes_o + 512U;
+  std::lock_guard<std::mutex> lock{img.fd_mu};
+  if (need > img.fd_cap) {
+    if (img.d_fd != nullptr) (void)hipFree(img.d_fd);
+    img.d_fd = nullptr;
+    img.fd_cap = 0U;
+    HIP_CHECK(hipMalloc(&img.d_fd, need * 2U));
+    img.fd_cap = need * 2U;
+  }
+  auto* base = static_cast<unsigned char*>(img.d_fd);
+  auto* d_nodes = reinterpret_cast<fd_node*>(base);
+  auto* d_q = reinterpret_cast<fd_query*>(base + bytes_n);
+  auto* d_out = reinterpret_cast<std::int16_t*>(base + bytes_n + (bytes_q + 255U) / 256U * 256U);
+  if (!hn.empty()) HIP_CHECK(hipMemcpy(d_nodes, hn.data(), sizeof(fd_node) * hn.size(), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(d_q, hq.data(), bytes_q, hipMemcpyHostToDevice));
+  fd_args a{};
+  a.im = img.d;
+  a.q = d_q;
+  a.nodes = d_nodes;
+  a.out = d_out;
+  a.n = static_cast<std::uint32_t>(hq.size());
+  launch_fastest_direct(a, nullptr);
+  HIP_CHECK(hipGetLastError());
+  std::vector<std::int16_t> out(hq.size());
+  HIP_CHECK(hipMemcpy(out.data(), d_out, bytes_o, hipMemcpyDeviceToHost));
+  for (std::size_t j = 0; j != idx.size(); ++j) {
+    val[idx[j]] = out[j];
+    have[idx[j]] = 1U;
+  }
+}
+
+}  // namespace
+}  // namespace nigiri_amd
+
+int nigiri_amd_fastest_direct(nigiri_amd_image* img, const nigiri_query* qs, uint32_t n, int16_t* out) {
+  if (img == nullptr || (n != 0U && (qs == nullptr || out == nullptr))) {
+    set_error("fastest_direct: bad argument");
+    return NIGIRI_AMD_EINVAL;
+  }
+  return guarded([&]() {
+    HIP_CHECK(hipSetDevice(img->device));
+    std::vector<fd_node_host> nodes;
+    for (uint32_t i = 0; i != n; ++i) {
+      if (qs[i].has_fastest_direct == 0) fastest_direct_nodes(*img->tt, qs[i], nodes);  // throws EINVAL
+    }
+    std::vector<std::int16_t> val;
+    std::vector<std::uint8_t> have;
+    device_fastest_direct(*img, qs, n, val, have);
+    if (std::find(have.begin(), have.end(), std::uint8_t{2U}) != have.end()) {
+      throw status_error{NIGIRI_AMD_ECAPACITY, "fastest_direct: more than " + std::to_string(kFdMaxNodes) +
+                                                   " start and destination locations in one query"};
+    }
+    for (uint32_t i = 0; i != n; ++i) out[i] = qs[i].has_fastest_direct != 0 ? qs[i].fastest_direct : val[i];
+    return NIGIRI_AMD_OK;
+  });
+}
+
 int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_groups,
                              uint32_t max_k, nigiri_amd_engine** out) {
   if (img == nullptr || out == nullptr || (lanes != 64U && lanes != 1U) || n_groups == 0U ||
@@ -1373,6 +1490,7 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     st.param = m.alloc<slot_param>(S);
     st.kslot = m.alloc<std::uint8_t>(S);
     st.tad = m.alloc<std::int16_t>(static_cast<std::size_t>(S) * kRowsMax);
+    st.dest_ev = m.alloc<dest_event>(static_cast<std::size_t>(S) * kMaxDestEvents);
     st.tad0 = m.alloc<std::int16_t>(S);
     st.n_chg = m.alloc<std::uint32_t>(S);
     st.chg_pos = m.alloc<std::uint32_t>(static_cast<std::size_t>(S) * kMaxChanges);
@@ -1470,8 +1588,12 @@ namespace {
 double run_search(nigiri_amd_engine& e, const nigiri_query* qs, std::uint32_t n, std::vector<prepared_query>& pq,
                   std::vector<int>& status, std::vector<std::string>& errors) {
   auto const& tt = *e.img->tt;
-  // lower bounds: upward searches on the host, sweeps on the device
-  prepare_all(tt, qs, n, pq, status, errors, false);
+  // fastest direct connections on the device; lower bounds: upward searches
+  // on the host, sweeps on the device
+  std::vector<std::int16_t> fd;
+  std::vector<std::uint8_t> fd_have;
+  device_fastest_direct(*e.img, qs, n, fd, fd_have);
+  prepare_all(tt, qs, n, pq, status, errors, false, &fd, &fd_have);
   auto const t0 = std::chrono::steady_clock::now();
   for (int dir = 0; dir != 2; ++dir) {
     std::vector<std::int32_t> queue;
@@ -1778,17 +1900,34 @@ int nigiri_amd_algo_init(nigiri_amd_engine* e, uint32_t slot, const nigiri_algo_
     set_error("algo_init: bad argument");
     return NIGIRI_AMD_EINVAL;
   }
-  if (init->n_dest > kMaxDestBits || init->n_dist > kMaxDist) {
+  if (init->n_dest > kMaxDestBits || init->n_dist > kMaxDist ||
+      (init->n_dest != 0U && init->dest == nullptr) || (init->n_dist != 0U && init->dist_loc == nullptr)) {
     set_error("algo_init: too many destinations");
     return NIGIRI_AMD_EUNSUPPORTED;
+  }
+  {
+    std::vector<std::uint32_t> d(init->intermodal != 0 ? init->dist_loc : init->dest,
+                                 (init->intermodal != 0 ? init->dist_loc : init->dest) +
+                                     (init->intermodal != 0 ? init->n_dist : init->n_dest));
+    auto const L = e->img->d.L;
+    if (std::any_of(init->dest, init->dest + init->n_dest, [&](uint32_t l) { return l >= L; }) ||
+        std::any_of(init->dist_loc, init->dist_loc + init->n_dist, [&](uint32_t l) { return l >= L; })) {
+      set_error("algo_init: destination out of range");
+      return NIGIRI_AMD_EINVAL;
+    }
+    if (dest_relaxations(*e->img->tt, init->dir == NIGIRI_AMD_DIR_FWD, init->intermodal != 0, d) >
+        kMaxDestEvents) {
+      set_error("algo_init: destinations reachable by too many transfers and footpaths in one round");
+      return NIGIRI_AMD_EUNSUPPORTED;
+    }
   }
   return guarded([&]() {
     HIP_CHECK(hipSetDevice(e->img->device));
     auto& sp = e->h_param[slot];
     sp.base = static_cast<std::uint16_t>(init->base_day);
     sp.intermodal = init->intermodal != 0 ? 1U : 0U;
-    sp.n_dest = static_cast<std::uint8_t>(init->n_dest);
-    sp.n_dist = static_cast<std::uint8_t>(init->n_dist);
+    sp.n_dest = static_cast<std::uint16_t>(init->n_dest);
+    sp.n_dist = static_cast<std::uint16_t>(init->n_dist);
     sp.tts_default = init->tts_default != 0 ? 1U : 0U;
     sp.tts_min = init->tts_min_transfer_time;
     sp.tts_add = init->tts_additional_time;
@@ -2006,7 +2145,11 @@ int nigiri_amd_algo_execute(nigiri_amd_engine* e, uint32_t slot, int32_t start_t
       HIP_CHECK(hipMemcpy(tr.counts + 4, &zero, 4U, hipMemcpyHostToDevice));
       throw status_error{NIGIRI_AMD_ECAPACITY, "device work list overflow"};
     }
-    if (dest_times != nullptr) std::memcpy(dest_times, e->h_rows.p, 2U * rows);
+    if (dest_times != nullptr) {  // the slot's destination rows: [n_dest_bits][K]
+      auto const& sp = e->h_param[slot];
+      auto const bits = static_cast<std::size_t>(sp.intermodal != 0U ? 1U : sp.n_dest);
+      std::memcpy(dest_times, e->h_rows.p, 2U * bits * e->K);
+    }
     if (n_rounds != nullptr) *n_rounds = rounds;
     e->algo_new[slot] = 0U;
     e->algo_fresh[slot] = 0U;

@@ -78,6 +78,36 @@ struct lb_phast_args {
 // arrive as seeds), a launch per large level
 void launch_lb_phast(std::uint32_t W, lb_phast_args const& a, std::uint32_t const* level_off,
                      std::uint32_t first_level, std::uint32_t n_levels, hipStream_t stream);
+// get_fastest_direct (get_fastest_direct.cc:12-100) for several queries:
+// one wavefront per query runs the reference's Dijkstra over the query's
+// node set (its start and destination locations, ascending), following
+// footpaths (footpaths_out forward, footpaths_in backward) only into
+// destinations.  Labels leave the queue in (minutes, insertion) order, as
+// the reference's bucket queue, so that the stop at special_station::kEnd
+// cuts the search where the reference cuts it.
+constexpr std::uint32_t kFdMaxNodes = 1024U;
+struct fd_node {
+  std::uint32_t loc;
+  std::uint16_t start;  // min(32767, shortest start offset) if is_start
+  std::uint16_t off;    // shortest destination offset if is_dest
+  std::uint8_t is_start, is_dest;
+};
+struct fd_query {
+  std::uint32_t node_begin, node_end;  // into the node array
+  double factor;                       // fastest_direct_factor
+  float tts_factor;                    // transfer_time_settings as given
+  std::int16_t tts_min, tts_add;
+  std::uint8_t fwd, tts_default;
+};
+struct fd_args {
+  dev_image im;
+  fd_query const* q;
+  fd_node const* nodes;
+  std::int16_t* out;  // minutes, 32767 (kUnreachableDirect) if none
+  std::uint32_t n;
+};
+void launch_fastest_direct(fd_args const& a, hipStream_t stream);
+
 // PHAST rows [n_rows][L] (device) into the lower-bound lanes of the slots
 // slots[0 .. n_rows) of the batched engine
 void launch_lb_scatter64(dev_state const& st, std::uint16_t const* rows, std::uint32_t const* slots,

@@ -1418,12 +1418,6 @@ __global__ void k_reset_counts(dev_state s, std::uint32_t mask) {
 }
 
 // ---- destination events (exact sequential simulation per query) ---------
-struct dest_event {
-  std::uint32_t pos;
-  std::uint32_t target;
-  std::int16_t x;
-  std::uint8_t kind;  // 0 transfer/footpath, 1 intermodal egress
-};
 
 template <bool Fwd, int W>
 __global__ void k_dest_pass(round_args a, std::uint32_t n_slots) {
@@ -1440,7 +1434,7 @@ __global__ void k_dest_pass(round_args a, std::uint32_t n_slots) {
   if ((s.grp_active[g] & (1ULL << lane)) == 0ULL) return;
   auto const& p = s.param[slot];
 
-  dest_event ev[kMaxDestEvents];
+  auto* ev = s.dest_ev + static_cast<std::size_t>(slot) * kMaxDestEvents;
   std::uint32_t n = 0U;
   auto const add = [&](std::uint32_t pos, std::uint32_t target, int x, std::uint8_t kind) {
     if (n < kMaxDestEvents) {
@@ -2083,6 +2077,102 @@ __global__ void __launch_bounds__(1024) k_lb_levels(lb_phast_args a, std::uint32
   }
 }
 
+// get_fastest_direct, one wavefront per query (see kernels.h).  Every node
+// holds one live label (minutes, insertion number); popping the smallest
+// (minutes, insertion) pair replays the reference's queue order, stale
+// entries included (a node improved after its push leaves the queue at its
+// improved label; its older entry would be skipped there).  The lanes pick
+// the smallest label; lane 0 relaxes the popped node's footpaths in list
+// order, which fixes the insertion numbers.
+__global__ void __launch_bounds__(64) k_fastest_direct(fd_args a) {
+  constexpr std::uint32_t kNone = 32767U;  // get_or_create's default distance
+  __shared__ std::uint32_t dist[kFdMaxNodes], ins[kFdMaxNodes];
+  __shared__ std::uint8_t queued[kFdMaxNodes];
+  auto const lane = threadIdx.x;
+  for (std::uint32_t qi = blockIdx.x; qi < a.n; qi += gridDim.x) {
+    auto const q = a.q[qi];
+    auto const n = q.node_end - q.node_begin;
+    auto const* nodes = a.nodes + q.node_begin;
+    // the starts enter the queue in ascending location order
+    std::uint32_t n_starts = 0U;
+    for (std::uint32_t b = 0U; b < n; b += 64U) {
+      auto const i = b + lane;
+      bool const st = i < n && nodes[i].is_start != 0U;
+      auto const m = __ballot(st);
+      if (i < n) {
+        dist[i] = st ? nodes[i].start : kNone;
+        ins[i] = n_starts + static_cast<std::uint32_t>(__popcll(m & ((1ULL << lane) - 1ULL)));
+        queued[i] = st ? 1U : 0U;
+      }
+      n_starts += static_cast<std::uint32_t>(__popcll(m));
+    }
+    __syncthreads();
+    slot_param tts{};
+    tts.tts_default = q.tts_default;
+    tts.tts_min = q.tts_min;
+    tts.tts_add = q.tts_add;
+    tts.tts_factor = q.tts_factor;
+    auto const* off = q.fwd != 0U ? a.im.fp_out_offsets : a.im.fp_in_offsets;
+    auto const* fps = q.fwd != 0U ? a.im.fp_out : a.im.fp_in;
+    std::uint32_t end_dist = kNone, next_ins = n_starts;
+    while (true) {
+      // smallest (minutes, insertion) of the queued nodes
+      auto key = ~0ULL;
+      for (auto i = lane; i < n; i += 64U) {
+        if (queued[i] != 0U) {
+          auto const k = (static_cast<unsigned long long>(dist[i]) << 42) |
+                         (static_cast<unsigned long long>(ins[i]) << 10) | i;
+          key = k < key ? k : key;
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        auto const other = static_cast<unsigned long long>(__shfl_xor(static_cast<long long>(key), o, 64));
+        key = other < key ? other : key;
+      }
+      if (key == ~0ULL) break;
+      auto const v = static_cast<std::uint32_t>(key & 1023ULL);
+      auto const d = dist[v];
+      auto const l = nodes[v].loc;
+      __syncthreads();
+      if (l == 1U) break;  // special_station::kEnd
+      if (lane == 0U) {
+        queued[v] = 0U;
+        for (auto e = off[l]; e != off[l + 1U]; ++e) {
+          auto const nd = d + adjusted<std::uint16_t>(tts, static_cast<std::uint16_t>(fps[e] >> 23));
+          if (nd > 65535U) continue;
+          auto const t = fps[e] & ((1U << 23) - 1U);
+          // the target's node (a destination), by binary search
+          std::uint32_t lo = 0U, hi = n;
+          while (lo < hi) {
+            auto const mid = (lo + hi) / 2U;
+            if (nodes[mid].loc < t) {
+              lo = mid + 1U;
+            } else {
+              hi = mid;
+            }
+          }
+          if (lo == n || nodes[lo].loc != t || nodes[lo].is_dest == 0U) continue;
+          if (nd < dist[lo] && nd < 65535U) {
+            dist[lo] = nd;
+            ins[lo] = next_ins++;
+            queued[lo] = 1U;
+          }
+        }
+        if (nodes[v].is_dest != 0U) {
+          auto const nd = d + nodes[v].off;
+          if (nd < 65535U && nd < end_dist) end_dist = nd;
+        }
+      }
+      __syncthreads();
+    }
+    if (lane == 0U) {
+      a.out[qi] = end_dist == kNone ? static_cast<std::int16_t>(kNone)
+                                    : static_cast<std::int16_t>(ceil(end_dist * q.factor));
+    }
+    __syncthreads();  // the next query reuses the LDS arrays
+  }
+}
+
 // PHAST rows [n_rows][L] into the lower-bound lanes of their slots
 // (batched engine)
 __global__ void k_lb_scatter64(dev_state s, std::uint16_t const* rows, std::uint32_t const* slots,
@@ -2663,6 +2753,11 @@ void launch_lb_phast(std::uint32_t W, lb_phast_args const& a, std::uint32_t cons
       hipLaunchKernelGGL((k_lb_level<1>), dim3(grid_for(items, kBlock, 8192)), dim3(kBlock), 0, stream, a, b, e);
     }
   }
+}
+
+void launch_fastest_direct(fd_args const& a, hipStream_t stream) {
+  if (a.n == 0U) return;
+  hipLaunchKernelGGL(k_fastest_direct, dim3(std::min<std::uint32_t>(a.n, 4096U)), dim3(64), 0, stream, a);
 }
 
 void launch_lb_scatter64(dev_state const& st, std::uint16_t const* rows, std::uint32_t const* slots,

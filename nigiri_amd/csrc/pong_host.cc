@@ -104,7 +104,7 @@ struct slot_search {
     check(nigiri_amd_algo_next_start_time(e, slot));
     for (auto const& [l, t] : starts) check(nigiri_amd_algo_add_start(e, slot, l, t));
     auto const K = engine_max_k(e);
-    rows.assign(static_cast<std::size_t>(16U) * K, 0);
+    rows.assign(std::max<std::size_t>(1U, dest_bits.size()) * K, 0);
     std::uint32_t n = 0U;
     check(nigiri_amd_algo_execute(e, slot, start_time, max_transfers, worst_time_at_dest, rows.data(), &n));
     n_rounds += n;

@@ -255,6 +255,16 @@ unsigned results_in_interval(prepared_query const& p) {  // search.h:382-391
 
 }  // namespace
 
+std::size_t dest_relaxations(timetable const& tt, bool fwd, bool intermodal,
+                             std::vector<std::uint32_t> const& dests) {
+  if (intermodal) return dests.size();
+  auto const& off = fwd ? tt.fp_in_offsets : tt.fp_out_offsets;
+  std::size_t n = 0U;
+  for (auto const d : dests) n += 1U + (off[d + 1U] - off[d]);
+  return n;
+}
+
+
 void early_termination(prepared_query& p, std::size_t epoch_idx) {
   if (p.min_connection_count_ == 0U || results_in_interval(p) < p.min_connection_count_) return;
   if (!((p.fwd_ && p.extend_earlier_ && !p.extend_later_) ||
@@ -337,8 +347,29 @@ bool extend_interval(timetable const& tt, prepared_query& p) {
   return false;
 }
 
+void fastest_direct_nodes(timetable const& tt, nigiri_query const& q, std::vector<fd_node_host>& out) {
+  std::map<std::uint32_t, fd_node_host> m;
+  auto const at = [&](std::uint32_t l) -> fd_node_host& {
+    if (l >= tt.n_locations) throw status_error{NIGIRI_AMD_EINVAL, "location out of range"};
+    return m.try_emplace(l, fd_node_host{l, 32767U, 0xFFFFU, 0U, 0U}).first->second;
+  };
+  for (auto i = 0U; i != q.n_start; ++i) {
+    auto& x = at(q.start_loc[i]);
+    x.is_start = 1U;
+    x.start = std::min(x.start, static_cast<std::uint16_t>(q.start_dur[i]));
+  }
+  for (auto i = 0U; i != q.n_dest; ++i) {
+    auto& x = at(q.dest_loc[i]);
+    auto const d = static_cast<std::uint16_t>(q.dest_dur[i]);
+    x.off = x.is_dest != 0U ? std::min(x.off, d) : d;
+    x.is_dest = 1U;
+  }
+  out.clear();
+  for (auto const& [l, x] : m) out.push_back(x);
+}
+
 void prepare(timetable const& tt, nigiri_query const& q, prepared_query& p,
-             host_scratch& s, bool host_lb) {
+             host_scratch& s, bool host_lb, std::int16_t const* fastest_direct_given) {
   if (q.prf_idx != 0) {
     throw status_error{NIGIRI_AMD_EUNSUPPORTED, "profiles other than 0 are not supported"};
   }
@@ -383,7 +414,7 @@ void prepare(timetable const& tt, nigiri_query const& q, prepared_query& p,
   std::stable_sort(p.dest_offsets_.begin(), p.dest_offsets_.end(), by_dur);
 
   host_tts raw{q.tts_default != 0, q.tts_min_transfer_time, q.tts_additional_time, q.tts_factor};
-  p.fastest_direct_ = fastest_direct(tt, q, fwd, raw);
+  p.fastest_direct_ = fastest_direct_given != nullptr ? *fastest_direct_given : fastest_direct(tt, q, fwd, raw);
 
   // search::init: normalised transfer-time settings
   p.tts_ = raw;
@@ -414,6 +445,11 @@ void prepare(timetable const& tt, nigiri_query const& q, prepared_query& p,
   if ((dest_im && p.dist_loc_.size() > kMaxQueryEgress) ||
       (!dest_im && p.dests_.size() > kMaxQueryDests)) {
     throw status_error{NIGIRI_AMD_EUNSUPPORTED, "too many destination offsets"};
+  }
+  if (dest_relaxations(tt, fwd, dest_im, dest_im ? p.dist_loc_ : p.dests_) > kMaxQueryDestEvents) {
+    throw status_error{NIGIRI_AMD_EUNSUPPORTED, "destinations reachable by more than " +
+                                                    std::to_string(kMaxQueryDestEvents) +
+                                                    " transfers and footpaths in one round"};
   }
   p.lb_seeds_ = dest_min;
   if (host_lb) {

@@ -15,9 +15,16 @@
 namespace nigiri_amd {
 
 // Per-query limits of the device engines: destination stations of a
-// non-intermodal query, egress offsets of an intermodal one.
-constexpr std::size_t kMaxQueryDests = 16U;
-constexpr std::size_t kMaxQueryEgress = 64U;
+// non-intermodal query, egress offsets of an intermodal one, and the
+// relaxations into its destinations one round can make (each destination's
+// transfer and incoming footpaths, or each egress offset).
+constexpr std::size_t kMaxQueryDests = 64U;
+constexpr std::size_t kMaxQueryEgress = 256U;
+constexpr std::size_t kMaxQueryDestEvents = 1024U;
+
+// Relaxations into the destinations one round can make (see above).
+std::size_t dest_relaxations(timetable const& tt, bool fwd, bool intermodal,
+                             std::vector<std::uint32_t> const& dests);
 
 struct host_tts {
   bool default_{true};
@@ -131,7 +138,19 @@ struct host_scratch {
 // levels of the sweep already done; lb_seeds_ holds their sources either
 // way).
 void prepare(timetable const& tt, nigiri_query const& q, prepared_query& p,
-             host_scratch& s, bool host_lb = true);
+             host_scratch& s, bool host_lb = true, std::int16_t const* fastest_direct = nullptr);
+
+// get_fastest_direct's node set of one query (get_fastest_direct.cc:19-46):
+// its start and destination locations in ascending order, each with the
+// shortest start offset (32767 at most) and destination offset.  Throws on
+// a location out of range.
+struct fd_node_host {
+  std::uint32_t loc;
+  std::uint16_t start;  // min(32767, shortest start offset) if is_start
+  std::uint16_t off;    // shortest destination offset if is_dest
+  std::uint8_t is_start, is_dest;
+};
+void fastest_direct_nodes(timetable const& tt, nigiri_query const& q, std::vector<fd_node_host>& out);
 
 // Journey extraction of one epoch (raptor.h:393-413).
 // dest_rows[j * max_k + k] is round_times[k][dest bit j] (delta).

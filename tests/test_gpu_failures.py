@@ -7,7 +7,7 @@ import pytest
 
 import nigiri_amd as na
 import oracle
-from test_gpu_parity import image_for, say
+from test_gpu_parity import image_for, say, with_destinations
 
 pytestmark = pytest.mark.gpu
 
@@ -83,3 +83,20 @@ def test_gpu_admission_waits_for_state_memory():
     eng.configure(state_budget_bytes=0)
     res = eng.search(qs.subset(list(range(4))))
     check_ok(tt, qs.subset(list(range(4))), res, set())
+
+
+def test_gpu_query_beyond_the_destination_limits_fails_alone():
+    # 65 exact destinations or 257 egress offsets are beyond the engines'
+    # per-query limits: refused with EUNSUPPORTED, the rest of the batch runs
+    tt, img = w1()
+    eng = na.Engine(img, lanes=64, groups=1, max_k=16)
+    qs = na.Queries.generate(tt, 1, 24, seed=0x7172 + 89)
+    big = na.Queries.generate(tt, 1, 2, seed=0x7172 + 97)
+    with_destinations(big, tt.n_locations, np.random.default_rng(97), 65, 257)
+    qs.q[3] = big.q[0]
+    qs.q[8] = big.q[1]
+    qs._keep.append(big)
+    say("destinations beyond the limits")
+    res = eng.search(qs)
+    assert res[3]["status"] == na.EUNSUPPORTED and res[8]["status"] == na.EUNSUPPORTED
+    check_ok(tt, qs, res, {3, 8})

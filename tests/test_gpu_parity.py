@@ -163,6 +163,35 @@ def test_gpu_served_results_equal_alone_and_in_mix():
         assert served[i]["latency_ms"] >= 0.0
 
 
+def with_destinations(qs, n_locations, rng, n_exact, n_egress):
+    """Even queries get n_exact exact destinations, odd ones n_egress
+    intermodal egress offsets (0-30 min), at random locations."""
+    L = n_locations
+    for i in range(len(qs)):
+        q = qs[i]
+        n = n_exact if i % 2 == 0 else n_egress
+        loc = rng.choice(np.arange(9, L), size=n, replace=False).astype(np.uint32)
+        dur = (rng.integers(0, 31, size=n) if i % 2 else np.zeros(n)).astype(np.int16)
+        q.dest_match_mode = na.MATCH_EXACT if i % 2 == 0 else na.MATCH_INTERMODAL
+        q.n_dest = n
+        q.dest_loc = loc.ctypes.data_as(q.dest_loc.__class__)
+        q.dest_dur = dur.ctypes.data_as(q.dest_dur.__class__)
+        qs._keep += [loc, dur]
+    return qs
+
+
+def test_gpu_many_destinations_and_egress_offsets():
+    # 40 exact destinations and 200 egress offsets per query (nigiri's
+    # collect_destinations has no limit; the engines take up to 64 and 256)
+    tt, img = image_for("w3s", lambda: na.Timetable.generate(3, scale=0.03))
+    qs = na.Queries.generate(tt, 3, 32, seed=0x7172 + 83)
+    with_destinations(qs, tt.n_locations, np.random.default_rng(83), 40, 200)
+    say("many destinations batched n=32")
+    compare(tt, na.Engine(img, lanes=64, groups=1, max_k=10), qs)
+    say("many destinations plain n=4")
+    compare(tt, na.Engine(img, lanes=1, groups=1, max_k=10), qs.subset([0, 1, 2, 3]))
+
+
 def test_gpu_work_counters_equal_oracle_counters():
     # the algorithmic byte model counts the same units on both sides
     tt, img = image_for("w3s", lambda: na.Timetable.generate(3, scale=0.03))

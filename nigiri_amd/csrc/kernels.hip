@@ -1531,8 +1531,9 @@ __global__ void k_dest_pass(round_args a, std::uint32_t n_slots) {
   s.n_chg[slot] = n_chg;
 }
 
-// One non-destination relaxation (update_transfers raptor.h:699-740 /
-// update_footpaths raptor.h:785-833).  Returns kRelaxImproved when it
+// One relaxation (update_transfers raptor.h:699-740 / update_footpaths
+// raptor.h:785-833); relaxations into the query's destination stations are
+// k_dest_pass's and do nothing here.  Returns kRelaxImproved when it
 // improved best (the station becomes marked), kRelaxHazard when it was a
 // pruned pre-write after a time_at_dest change (exact replay needed if the
 // station also has a passing relaxation), else 0.
@@ -1547,6 +1548,9 @@ __device__ inline int relax(dev_image const& im, dev_state const& s, std::uint32
   auto const c = lay<W>::col(s, g, t, lane);
   auto const xv = clamp16(x);
   if (!D::better(xv, s.best[c])) return 0;
+  // destination stations are relaxed by k_dest_pass (tested only now: a
+  // relaxation that does not beat best changes nothing either way)
+  if (lay<W>::test(s.dest_mask, s, g, t, lane)) return 0;
   auto const tad_e = tad_at(s, slot, pos);
   int const lbv = s.lb[c];
   bool const elig = D::better_eq(xv, tad_e) && lbv != kUnreachableLb &&
@@ -1688,7 +1692,7 @@ __global__ void k_transfer_fp64(round_args a) {
       }
     };
     // transfer at i (destination stations are handled by k_dest_pass)
-    if (valid && !lay<64>::test(s.dest_mask, s, g, i, ln)) {
+    if (valid) {
       auto const tt = adjusted<std::uint8_t>(p, im.transfer_time[i]);
       emit(i, relax<Fwd, 64>(a.im, s, g, ln, slot, k, i, tmp_i + D::dir(tt), kPosTransfer | i, tad0));
     }
@@ -1696,7 +1700,7 @@ __global__ void k_transfer_fp64(round_args a) {
     for (auto fe = fe0; fe != fe1; ++fe) {
       auto const fp = fps[fe];
       auto const t = fp & ((1U << 23) - 1U);
-      if (valid && !lay<64>::test(s.dest_mask, s, g, t, ln)) {
+      if (valid) {
         auto const dur = static_cast<std::int16_t>(fp >> 23);
         emit(t, relax<Fwd, 64>(a.im, s, g, ln, slot, k, t, tmp_i + D::dir(adjusted<std::int16_t>(p, dur)),
                                kPosFootpath | fe, tad0));
@@ -1739,7 +1743,7 @@ __global__ void k_transfer_fp1(round_args a) {
       if (code == kRelaxHazard) mark_one<W>(s.hz_mask, s.list_hz, &s.counts[3], s.cap_hz, overflow, s, g, t, 0U);
     };
     // transfer at i (destination stations are handled by k_dest_pass)
-    if (valid && !lay<W>::test(s.dest_mask, s, g, i, 0U)) {
+    if (valid) {
       auto const tt = adjusted<std::uint8_t>(p, im.transfer_time[i]);
       emit(i, relax<Fwd, W>(a.im, s, g, 0U, slot, k, i, tmp_i + D::dir(tt), kPosTransfer | i, tad0));
     }
@@ -1747,7 +1751,7 @@ __global__ void k_transfer_fp1(round_args a) {
     for (auto fe = off[i]; fe != off[i + 1U]; ++fe) {
       auto const fp = fps[fe];
       auto const t = fp & ((1U << 23) - 1U);
-      if (valid && !lay<W>::test(s.dest_mask, s, g, t, 0U)) {
+      if (valid) {
         auto const dur = static_cast<std::int16_t>(fp >> 23);
         emit(t, relax<Fwd, W>(a.im, s, g, 0U, slot, k, t, tmp_i + D::dir(adjusted<std::int16_t>(p, dur)),
                               kPosFootpath | fe, tad0));

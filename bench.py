@@ -8,8 +8,10 @@ workload's 60/25/15 point/range/intermodal mix arrive as a Poisson stream at
 serving loop.  A step is one batch of --batch queries of that stream; the
 K timed batches arrive back to back as one stream.  value = queries / wall
 time of the stream, latency = arrival -> completion per query (p50/p99
-overall and per query kind).  The plain engine runs a sample of the same
-stream one query at a time.
+overall and per query kind).  A second stream of --low-queries queries at
+--low-rate queries/s (below the server's capacity) reports latency without
+the queueing of the saturated stream (outside the timed region).  The plain
+engine runs a sample of the same stream one query at a time.
 
 Other workloads (--workload 1..4) keep the batch modes: a step is one batch
 of `--batch` queries; point-query batches are admitted whole once the
@@ -64,12 +66,13 @@ def route_scan_bytes(work):
             + 8 * work["route_day_tests"] + 12 * work["transport_day_tests"] + 2 * work["tmp_writes"])
 
 
-def committed_traffic(workload, scale, batch):
+def committed_traffic(workload, **cfg):
     """HBM bytes per route-scan launch from the committed rocprof PMC summary
     of this workload (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by a
     device copy of known size).  PMC counters need their own profiler runs, so
-    the bench line carries the figure of the last profiled build, for the
-    workload, scale and batch that summary names (else null)."""
+    the bench line carries the figure of the last profiled build, when the
+    summary was taken on this configuration (scale, batch, steps, warm-up
+    batches and groups); else null."""
     path = os.path.join(ROOT, "profiles", f"k_route_scan_workload{workload}_pmc.md")
     if not os.path.exists(path):
         return None
@@ -79,8 +82,9 @@ def committed_traffic(workload, scale, batch):
             k, v = line.split(":", 1)
             fields[k.strip()] = v.strip()
     try:
-        if (float(fields.get("scale", "nan")) != scale or int(fields.get("batch", "-1")) != batch):
-            return None
+        for k, v in cfg.items():
+            if float(fields.get(k, "nan")) != float(v):
+                return None
         return float(fields["traffic_bytes_per_launch"])
     except (KeyError, ValueError):
         return None
@@ -127,6 +131,16 @@ def cpu_baseline(tt, qs, seconds):
             "cpu_model": cpu_model()}
 
 
+def with_traffic_rate(roof):
+    """Adds the measured traffic's rate and fraction of peak to a roofline
+    object (traffic: HBM bytes per launch, or None)."""
+    t, ms = roof.get("traffic"), roof.get("avg_launch_ms")
+    if t is not None and ms:
+        roof["traffic_gbs"] = round(t / (ms * 1e-3) / 1e9, 1)
+        roof["traffic_frac"] = round(roof["traffic_gbs"] / HBM_PEAK_GBS, 4)
+    return roof
+
+
 def knob_overrides(args):
     """--knobs name=value,...: launch-shape settings for the engine (grid_waves
     and the knobs of Engine.configure); results do not depend on them."""
@@ -154,7 +168,8 @@ def served_main(args):
     tt = na.Timetable.generate(cfg, scale=args.scale)
     B = args.batch
     n_total = (args.warmup + args.steps) * B
-    qs = na.Queries.generate(tt, cfg, n_total)
+    # the queries of the main stream, then those of the below-capacity stream
+    qs = na.Queries.generate(tt, cfg, n_total + args.low_queries)
     print(f"[bench] workload {cfg}: {tt.view.n_locations} locations, {tt.view.n_routes} routes, "
           f"{tt.view.n_transports} transports ({time.time() - t0:.1f}s incl. contraction hierarchy)",
           file=sys.stderr, flush=True)
@@ -164,7 +179,8 @@ def served_main(args):
     eng.profile(True)
     eng.configure(admission_wait_ms=1.0, graphs=not args.eager, **knob_overrides(args))
     kinds = np.array(["intermodal" if qs[i].dest_match_mode == na.MATCH_INTERMODAL
-                      else ("range" if qs[i].is_interval else "point") for i in range(n_total)])
+                      else ("range" if qs[i].is_interval else "point")
+                      for i in range(n_total + args.low_queries)])
     rng = np.random.default_rng(7)
     caps = dict(journey_capacity=16 * B * max(1, args.steps), leg_capacity=400 * B * max(1, args.steps),
                 legs="raw")
@@ -187,7 +203,7 @@ def served_main(args):
     value = n / t_total
     lat = np.array([r["latency_ms"] for r in res])
     queue = np.array([r["queue_ms"] for r in res])
-    kd = kinds[first:]
+    kd = kinds[first:n_total]
     ok = sum(1 for r in res if r["status"] == 0)
     by_kind = {}
     for k in ("point", "range", "intermodal"):
@@ -195,6 +211,25 @@ def served_main(args):
         by_kind[k] = {"n": int(m.sum()), "p50_ms": pct(lat[m], 50), "p99_ms": pct(lat[m], 99),
                       "queue_p50_ms": pct(queue[m], 50)}
     print(f"[bench] {n} queries served in {t_total:.1f} s", file=sys.stderr, flush=True)
+
+    # a second stream below the server's capacity: latency without queueing
+    # for slots (not part of the timed region)
+    low = None
+    if args.low_queries > 0:
+        lidx = list(range(n_total, n_total + args.low_queries))
+        larr = np.cumsum(rng.exponential(1000.0 / args.low_rate, len(lidx)))
+        larr -= larr[0]
+        t_low = time.perf_counter()
+        lres, _ = eng.serve(qs.subset(lidx), larr, **caps)
+        t_low = time.perf_counter() - t_low
+        llat = np.array([r["latency_ms"] for r in lres])
+        lkd = kinds[n_total:]
+        low = {"offered_qps": args.low_rate, "queries": len(lidx), "completed_qps": round(len(lidx) / t_low, 2),
+               "p50_ms": pct(llat, 50), "p99_ms": pct(llat, 99),
+               "by_kind": {k: {"n": int((lkd == k).sum()), "p50_ms": pct(llat[lkd == k], 50),
+                               "p99_ms": pct(llat[lkd == k], 99)} for k in ("point", "range", "intermodal")}}
+        print(f"[bench] below-capacity stream: {len(lidx)} queries at {args.low_rate}/s in {t_low:.1f} s",
+              file=sys.stderr, flush=True)
 
     # a lone point query on an idle engine, batched and plain
     pts = [i for i in range(n_total) if kinds[i] == "point"][:5]
@@ -250,6 +285,7 @@ def served_main(args):
         "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
         "latency": "arrival -> completion (queue, preparation, lower bounds, rounds, legs)",
         "by_kind": by_kind,
+        "below_capacity": low,
         "lone_point_query_ms": {"batched": round(float(np.median(lone_b)), 2),
                                 "plain": round(float(np.median(lone_p)), 2)},
         "plain_1query_qps": round(plain_qps, 2),
@@ -260,11 +296,13 @@ def served_main(args):
         "lane_utilization": round((st["work"]["route_stops"] - st["work"]["skipped_stops"])
                                   / max(1, 64 * st["work"]["wave_stops"]), 4),
         "work": {k: int(v) for k, v in st["work"].items()},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+        "roofline": with_traffic_rate({"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": committed_traffic(cfg, args.scale, B), "kernel": "k_route_scan",
+                     "traffic": committed_traffic(cfg, scale=args.scale, batch=B, steps=args.steps,
+                                                  warmup=args.warmup, groups=args.groups),
+                     "kernel": "k_route_scan",
                      "alg_bytes_per_launch": int(alg_bytes / launches),
-                     "avg_launch_ms": round(avg_ms, 4), "launches": launches},
+                     "avg_launch_ms": round(avg_ms, 4), "launches": launches}),
         "transfer_kernel_avg_ms": round(st["transfer_ms"] / max(1, st["transfer_launches"]), 4),
         "cpu_baseline": cpu,
     }
@@ -396,8 +434,8 @@ def pong_main(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", type=int, default=5)
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--batch", type=int, default=0,
@@ -414,7 +452,11 @@ def main():
     ap.add_argument("--serve", type=float, default=0.0, metavar="RATE",
                     help="served measurement (the default for workload 5) at RATE queries/s "
                          "for any workload")
-    ap.add_argument("--groups", type=int, default=16, help="64-query groups (serving mode)")
+    ap.add_argument("--groups", type=int, default=32, help="64-query groups (serving mode)")
+    ap.add_argument("--low-rate", type=float, default=150.0,
+                    help="served mode: offered rate of the second, below-capacity stream (queries/s)")
+    ap.add_argument("--low-queries", type=int, default=768,
+                    help="served mode: queries of the below-capacity stream (0: none)")
     ap.add_argument("--continuous", action="store_true",
                     help="the timed steps' batches go through one serving run back to back: the "
                          "next batch's preparation and lower bounds overlap the current batch's "
@@ -605,11 +647,13 @@ def main():
                              "reconstruction": round(st["reconstruct_ms"] / args.steps, 1),
                              "legs": round(st["legs_ms"] / args.steps, 1),
                              "reconstructed_journeys": int(st["reconstructed"])},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+        "roofline": with_traffic_rate({"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": committed_traffic(cfg, args.scale, B), "kernel": "k_route_scan",
+                     "traffic": committed_traffic(cfg, scale=args.scale, batch=B, steps=args.steps,
+                                                  warmup=args.warmup, groups=groups),
+                     "kernel": "k_route_scan",
                      "alg_bytes_per_launch": int(alg_bytes / launches),
-                     "avg_launch_ms": round(avg_ms, 4), "launches": launches},
+                     "avg_launch_ms": round(avg_ms, 4), "launches": launches}),
         "transfer_kernel_avg_ms": round(st["transfer_ms"] / max(1, st["transfer_launches"]), 4),
         "cpu_baseline": cpu,
     }

@@ -1279,47 +1279,75 @@ __global__ void __launch_bounds__(256, 8) k_route_scan(round_args a) {
 }
 
 // Compaction (batched engine): flag bytes of a [major][minor] array -> work
-// list entries (major << 32 | minor); clears the flags.  Each lane tests 8
-// flags with one 64-bit load; a wavefront scan places the entries, so the
-// entries of one wavefront's 512 flags are in ascending order.
-__global__ void k_compact(std::uint8_t* flags, std::uint32_t n_per_group, std::size_t n_words,
-                          std::uint64_t* list, std::uint32_t* count, std::size_t cap,
-                          std::uint32_t* overflow) {
+// list entries (major << 32 | minor); clears the flags.  Each lane tests
+// kCompactWords 64-bit words (8 flags each); a workgroup scan places the
+// entries, with one atomic per workgroup and pass, so the entries of one
+// workgroup's kCompactWords * 256 words are in ascending order.
+constexpr std::uint32_t kCompactWords = 4U;
+
+__global__ void __launch_bounds__(kBlock) k_compact(std::uint8_t* flags, std::uint32_t n_per_group,
+                                                    std::size_t n_words, std::uint64_t* list,
+                                                    std::uint32_t* count, std::size_t cap,
+                                                    std::uint32_t* overflow) {
+  __shared__ std::uint32_t wave_total[kWavesPerBlock];
+  __shared__ std::uint32_t block_base;
   auto* words = reinterpret_cast<std::uint64_t*>(flags);
-  auto const lane = threadIdx.x & 63U;
-  for (std::size_t base_i = (blockIdx.x * static_cast<std::size_t>(blockDim.x)) + (threadIdx.x & ~63U);
-       base_i < n_words; base_i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
-    auto const i = base_i + lane;
-    std::uint64_t v = 0ULL;
-    if (i < n_words) {
-      v = words[i];
-      if (v != 0ULL) words[i] = 0ULL;
+  auto const lane = threadIdx.x & 63U, wave = threadIdx.x / 64U;
+  constexpr std::size_t kPerBlock = static_cast<std::size_t>(kBlock) * kCompactWords;
+  for (std::size_t b0 = blockIdx.x * kPerBlock; b0 < n_words; b0 += gridDim.x * kPerBlock) {
+    auto const i0 = b0 + threadIdx.x * static_cast<std::size_t>(kCompactWords);
+    std::uint64_t v[kCompactWords];
+    unsigned n = 0U;
+#pragma unroll
+    for (std::uint32_t j = 0U; j < kCompactWords; ++j) {
+      v[j] = i0 + j < n_words ? words[i0 + j] : 0ULL;
+      n += static_cast<unsigned>(__popcll(v[j]));  // flags are 0 or 1
     }
-    if (__ballot(v != 0ULL) == 0ULL) continue;
-    unsigned n = static_cast<unsigned>(__popcll(v));  // flags are 0 or 1
+    // the workgroup's flags in ascending order: wave scans, then the waves'
+    // totals (uniform branches: every wave of the block takes them)
     unsigned incl = n;
     for (int off = 1; off < 64; off <<= 1) {
       auto const t = __shfl_up(incl, off, 64);
       if (static_cast<int>(lane) >= off) incl += t;
     }
-    auto const total_n = __shfl(incl, 63, 64);
-    unsigned wave_base = 0U;
-    if (lane == 63U) wave_base = atomicAdd(count, total_n);
-    wave_base = __shfl(wave_base, 63, 64);
-    auto pos = wave_base + incl - n;
-    while (v != 0ULL) {
-      auto const b = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(v)) - 1);
-      v &= v - 1ULL;
-      auto const idx = i * 8U + b / 8U;
-      if (pos < cap) {
-        auto const g = static_cast<std::uint32_t>(idx / n_per_group);
-        auto const l = static_cast<std::uint32_t>(idx % n_per_group);
-        list[pos] = (static_cast<std::uint64_t>(g) << 32) | l;
-      } else {
-        atomicOr(overflow, 1U);
-      }
-      ++pos;
+    if (lane == 63U) wave_total[wave] = incl;
+    __syncthreads();
+    unsigned before = 0U, total = 0U;
+    for (std::uint32_t w = 0U; w < kWavesPerBlock; ++w) {
+      auto const t = wave_total[w];
+      if (w < wave) before += t;
+      total += t;
     }
+    if (threadIdx.x == 0U) block_base = total != 0U ? atomicAdd(count, total) : 0U;
+    __syncthreads();
+    auto pos = block_base + before + incl - n;
+    bool const narrow = n_words <= (std::size_t{1} << 29);  // flag indices below 2^32
+#pragma unroll
+    for (std::uint32_t j = 0U; j < kCompactWords; ++j) {
+      auto x = v[j];
+      if (x == 0ULL) continue;
+      words[i0 + j] = 0ULL;
+      auto const f0 = (i0 + j) * 8U;
+      while (x != 0ULL) {
+        auto const b = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(x)) - 1);
+        x &= x - 1ULL;
+        auto const idx = f0 + b / 8U;
+        if (pos < cap) {
+          std::uint64_t e = 0ULL;
+          if (narrow) {
+            auto const i32 = static_cast<std::uint32_t>(idx);
+            e = (static_cast<std::uint64_t>(i32 / n_per_group) << 32) | (i32 % n_per_group);
+          } else {
+            e = (static_cast<std::uint64_t>(idx / n_per_group) << 32) | (idx % n_per_group);
+          }
+          list[pos] = e;
+        } else {
+          atomicOr(overflow, 1U);
+        }
+        ++pos;
+      }
+    }
+    __syncthreads();  // wave_total and block_base are reused by the next pass
   }
 }
 
@@ -2314,7 +2342,7 @@ void compact(std::uint8_t* flags, std::uint32_t n_minor, std::uint32_t n_major, 
              std::uint32_t* count, std::size_t cap, dev_state const& s, hipStream_t st) {
   auto const n_per_group = n_minor;
   auto const n_words = flag_words(static_cast<std::size_t>(n_minor) * n_major);
-  hipLaunchKernelGGL(k_compact, dim3(grid_for(n_words, kBlock, 8192)), dim3(kBlock), 0, st, flags,
+  hipLaunchKernelGGL(k_compact, dim3(grid_for(n_words, kBlock * kCompactWords, 2048)), dim3(kBlock), 0, st, flags,
                      n_per_group, n_words, list, count, cap, &s.counts[4]);
 }
 

@@ -318,10 +318,17 @@ __global__ void k_best_sweep(round_args a, std::uint32_t const* sweep) {
       for (std::uint32_t k = 0U; k < kRowsMax; ++k) at_k[k] = __ballot(on && my_k == k);
       for (std::uint32_t l = tid; l < s.L; l += stride) {
         std::uint64_t wrote = 0ULL;
+        // the station's rt_valid words of the rounds in use, in flight together
+        std::uint64_t vw[kRowsMax];
 #pragma unroll
         for (std::uint32_t k = 1U; k < kRowsMax; ++k) {
-          if (k >= s.K || at_k[k] == 0ULL) continue;  // uniform
-          auto x = s.rt_valid[(static_cast<std::size_t>(g) * s.K + k) * s.L + l] & at_k[k];
+          vw[k] = (k < s.K && at_k[k] != 0ULL)  // uniform
+                      ? s.rt_valid[(static_cast<std::size_t>(g) * s.K + k) * s.L + l] & at_k[k]
+                      : 0ULL;
+        }
+#pragma unroll
+        for (std::uint32_t k = 1U; k < kRowsMax; ++k) {
+          auto x = vw[k];
           while (x != 0ULL) {
             auto const lane = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(x)) - 1);
             x &= x - 1ULL;
@@ -1032,16 +1039,26 @@ __global__ void k_route_pack(round_args a) {
         // first and last marked stop (array order) of the active queries
         std::uint32_t first = i1, last = i0;
         auto i = i0;
-        for (; i + 4U <= i1; i += 4U) {
-          auto const a0 = im.route_stops[i] & kLocMask, a1 = im.route_stops[i + 1U] & kLocMask;
-          auto const a2 = im.route_stops[i + 2U] & kLocMask, a3 = im.route_stops[i + 3U] & kLocMask;
-          auto const m0 = mt[a0] & act, m1 = mt[a1] & act, m2 = mt[a2] & act, m3 = mt[a3] & act;
-          auto const any4 = m0 | m1 | m2 | m3;
-          if (any4 != 0ULL) {
-            if (first == i1) first = i + (m0 != 0ULL ? 0U : (m1 != 0ULL ? 1U : (m2 != 0ULL ? 2U : 3U)));
-            last = i + (m3 != 0ULL ? 3U : (m2 != 0ULL ? 2U : (m1 != 0ULL ? 1U : 0U)));
-            mask |= any4;
+        // eight stops at a time: their stop words, then their mark words, in
+        // flight together
+        constexpr std::uint32_t kPackStops = 8U;
+        for (; i + kPackStops <= i1; i += kPackStops) {
+          std::uint32_t a[kPackStops];
+          std::uint64_t m[kPackStops];
+#pragma unroll
+          for (std::uint32_t j = 0U; j < kPackStops; ++j) a[j] = im.route_stops[i + j] & kLocMask;
+#pragma unroll
+          for (std::uint32_t j = 0U; j < kPackStops; ++j) m[j] = mt[a[j]] & act;
+          std::uint64_t any = 0ULL;
+#pragma unroll
+          for (std::uint32_t j = 0U; j < kPackStops; ++j) {
+            if (m[j] != 0ULL) {
+              if (first == i1) first = i + j;
+              last = i + j;
+            }
+            any |= m[j];
           }
+          mask |= any;
         }
         for (; i < i1; ++i) {
           auto const m0 = mt[im.route_stops[i] & kLocMask] & act;

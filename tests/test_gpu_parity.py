@@ -422,6 +422,29 @@ def test_gpu_full_size_workload5_mix_sample():
     compare(tt, na.Engine(img, lanes=64, groups=1, max_k=10), qs)
 
 
+def test_gpu_full_size_workload5_served_mix():
+    # the serving loop on the 500k-stop timetable: 48 queries of the mix arrive
+    # (Poisson, 5 ms apart on average) while others run on 128 slots; every
+    # result equals the same query searched alone on an idle engine, and the
+    # first 12 (the queries of the sample above) equal the oracle's
+    tt, img = image_for("w5", lambda: na.Timetable.generate(5))
+    qs = na.Queries.generate(tt, 5, 48, seed=0x7172 + 73)
+    say("full-size workload5 served n=48")
+    eng = na.Engine(img, lanes=64, groups=2, max_k=10)
+    arrivals = np.cumsum(np.random.default_rng(5).exponential(5.0, len(qs)))
+    served, _ = eng.serve(qs, arrivals)
+    alone = na.Engine(img, lanes=64, groups=1, max_k=10)
+    for i in range(len(qs)):
+        one = alone.search(qs.subset([i]))[0]
+        assert served[i]["status"] == 0 and one["status"] == 0, i
+        assert served[i]["journeys"] == one["journeys"], i
+        assert served[i]["legs"] == one["legs"], i
+    say("full-size workload5 served: oracle for the first 12")
+    for i in range(12):
+        ref = oracle.search(tt, qs[i])
+        assert served[i]["journeys"] == ref["journeys"] and served[i]["legs"] == ref["legs"], i
+
+
 def compare_pong(tt, eng, q, check_work=False):
     """The plain engine's pong against the oracle's: journeys, legs, interval and
     the number of executes; with check_work also the route-scan work counters,

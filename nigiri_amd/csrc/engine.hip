@@ -122,8 +122,11 @@ struct nigiri_amd_image {
 // host threads that write the legs of a step's journeys (with the calling
 // thread); the lower-bound workers and query preparation use others
 constexpr unsigned kLegWorkers = 9U;
-// slots whose lower bounds one PHAST sweep of the batched engine computes
-constexpr std::uint32_t kPhastRows = 128U;
+// PHAST of the batched engine: a group with this many new lanes is swept in
+// place; up to kPhastRows new slots of other groups are swept per pass as
+// rows of the engine's PHAST buffer
+constexpr std::uint32_t kPhastDenseLanes = 8U;
+constexpr std::uint32_t kPhastRows = 512U;
 
 // Fork-join pool of host threads for the per-journey work of a step (the
 // legs of reconstructed journeys).  parallel_for runs f(0 .. n-1) on the
@@ -233,8 +236,9 @@ struct nigiri_amd_engine {
   // lower bounds of admitted slots (PHAST): selected groups / rows, their
   // lane masks, and the upward searches' labels (growable)
   std::uint16_t* d_lb_row = nullptr;   // [L] lower bounds given by the caller
-  std::uint32_t* d_ph_sel = nullptr;   // [S] rows swept
+  std::uint32_t* d_ph_sel = nullptr;   // [S + G] groups and rows swept
   std::uint32_t* d_ph_slots = nullptr; // [S] batched engine: the rows' slots
+  std::uint64_t* d_ph_masks = nullptr; // [G] batched engine: new lanes of the groups swept in place
   std::uint16_t* d_ph_rows = nullptr;  // batched engine: [min(S, kPhastRows)][L] PHAST rows
   std::uint32_t* d_ph_seed_idx = nullptr;
   std::uint16_t* d_ph_seed_min = nullptr;
@@ -952,26 +956,58 @@ struct runner {
   // Lower bounds of the new slots (search.h:114-131): PHAST on the device,
   // seeded with the upward searches (and the host levels of the sweep) of
   // their host preparation.  Plain engine: each slot's lower bounds are a
-  // row [L] of lb, swept in place.  Batched engine: up to kPhastRows slots at
-  // a time are swept as rows of the engine's PHAST buffer (one thread per
-  // (location, slot) instead of a wavefront per (location, group) for the
-  // one or two new lanes a group usually has), then copied into their lanes.
+  // row [L] of lb, swept in place.  Batched engine: a group with at least
+  // kPhastDenseLanes new lanes is swept in place, a wavefront per (location,
+  // group) whose lanes share each 128 B row (a whole batch admitted at once);
+  // the new slots of the other groups (the one or two lanes a group gets in
+  // a serving step) are swept as rows of the engine's PHAST buffer, a thread
+  // per (location, slot), up to kPhastRows at a time, then copied into their
+  // lanes.
   void phast_lbs(std::vector<std::uint32_t> const& new_slots) {
     if (new_slots.empty()) return;
     auto const L = e.st.L, W = e.W;
-    auto const n = new_slots.size();
-    auto const rows_per = W == 64U ? std::min<std::size_t>(kPhastRows, n) : n;
-    std::vector<std::uint32_t> sel, seed_idx, seed_off{0U};
+    std::vector<std::uint32_t> dense_sel, rows;
+    std::vector<std::uint64_t> dense_mask;
+    if (W == 64U) {
+      std::vector<std::uint64_t> fresh(e.G, 0ULL);
+      for (auto const s : new_slots) fresh[s / 64U] |= 1ULL << (s % 64U);
+      for (std::uint32_t g = 0; g != e.G; ++g) {
+        if (static_cast<std::uint32_t>(__builtin_popcountll(fresh[g])) >= kPhastDenseLanes) {
+          dense_sel.push_back(g);
+          dense_mask.push_back(fresh[g]);
+        }
+      }
+      for (auto const s : new_slots) {
+        if ((static_cast<std::uint32_t>(__builtin_popcountll(fresh[s / 64U]))) < kPhastDenseLanes) rows.push_back(s);
+      }
+    } else {
+      rows = new_slots;
+    }
+    // seeds: the dense groups' (in the [G][L][64] layout of lb), then the
+    // rows' in passes of rows_per ([row][L] of the buffer, or of lb)
+    auto const rows_per = W == 64U ? std::min<std::size_t>(kPhastRows, std::max<std::size_t>(1U, rows.size()))
+                                   : std::max<std::size_t>(1U, rows.size());
+    std::vector<std::uint32_t> sel(dense_sel), seed_idx, seed_off{0U};
     std::vector<std::uint16_t> seed_min;
-    for (std::size_t i = 0; i != n; ++i) {
-      auto const s = new_slots[i];
-      auto const row = W == 64U ? static_cast<std::uint32_t>(i % kPhastRows) : s;
-      if (i < rows_per) sel.push_back(row);
+    auto const add_seeds = [&](std::uint32_t s, std::uint32_t base, std::uint32_t stride, std::uint32_t lane) {
       for (auto const& [l, d] : pq[static_cast<std::size_t>(slots[s].q)].lb_up_) {
-        seed_idx.push_back(row * L + l);
+        seed_idx.push_back((base + l) * stride + lane);
         seed_min.push_back(d);
       }
-      if ((i + 1U) % rows_per == 0U || i + 1U == n) seed_off.push_back(static_cast<std::uint32_t>(seed_idx.size()));
+    };
+    for (auto const s : new_slots) {
+      if (W == 64U && std::binary_search(dense_sel.begin(), dense_sel.end(), s / 64U)) {
+        add_seeds(s, (s / 64U) * L, 64U, s % 64U);
+      }
+    }
+    seed_off.push_back(static_cast<std::uint32_t>(seed_idx.size()));
+    for (std::size_t i = 0; i != rows.size(); ++i) {
+      auto const row = W == 64U ? static_cast<std::uint32_t>(i % kPhastRows) : rows[i];
+      if (i < rows_per) sel.push_back(row);
+      add_seeds(rows[i], row * L, 1U, 0U);
+      if ((i + 1U) % rows_per == 0U || i + 1U == rows.size()) {
+        seed_off.push_back(static_cast<std::uint32_t>(seed_idx.size()));
+      }
     }
     auto const stream = e.stream;
     if (seed_idx.size() > e.ph_seed_cap) {
@@ -980,8 +1016,12 @@ struct runner {
       e.d_ph_seed_min = e.grow(e.d_ph_seed_min, e.ph_seed_cap);
     }
     HIP_CHECK(hipMemcpyAsync(e.d_ph_sel, sel.data(), 4U * sel.size(), hipMemcpyHostToDevice, stream));
-    if (W == 64U) {
-      HIP_CHECK(hipMemcpyAsync(e.d_ph_slots, new_slots.data(), 4U * n, hipMemcpyHostToDevice, stream));
+    if (!dense_mask.empty()) {
+      HIP_CHECK(hipMemcpyAsync(e.d_ph_masks, dense_mask.data(), 8U * dense_mask.size(), hipMemcpyHostToDevice,
+                               stream));
+    }
+    if (W == 64U && !rows.empty()) {
+      HIP_CHECK(hipMemcpyAsync(e.d_ph_slots, rows.data(), 4U * rows.size(), hipMemcpyHostToDevice, stream));
     }
     if (!seed_idx.empty()) {
       HIP_CHECK(hipMemcpyAsync(e.d_ph_seed_idx, seed_idx.data(), 4U * seed_idx.size(), hipMemcpyHostToDevice,
@@ -991,20 +1031,31 @@ struct runner {
     }
     auto const& sw = fwd ? tt.ch.sweep_fwd : tt.ch.sweep_bwd;
     auto const n_levels = static_cast<std::uint32_t>(sw.level_off.size() - 1U);
-    for (std::size_t b = 0; b + 1U < seed_off.size(); ++b) {
-      auto const rows = static_cast<std::uint32_t>(std::min(rows_per, n - b * rows_per));
-      lb_phast_args a{};
-      a.sw = e.img->d.sweep[fwd ? 0 : 1];
-      a.dist = W == 64U ? e.d_ph_rows : e.st.lb;
-      a.L = L;
+    lb_phast_args a{};
+    a.sw = e.img->d.sweep[fwd ? 0 : 1];
+    a.L = L;
+    if (!dense_sel.empty()) {
+      a.dist = e.st.lb;
       a.sel = e.d_ph_sel;
+      a.masks = e.d_ph_masks;
+      a.n_sel = static_cast<std::uint32_t>(dense_sel.size());
+      a.seed_idx = e.d_ph_seed_idx;
+      a.seed_min = e.d_ph_seed_min;
+      a.n_seeds = seed_off[1];
+      launch_lb_phast(64U, a, sw.level_off.data(), sw.host_levels, n_levels, stream);
+      HIP_CHECK(hipGetLastError());
+    }
+    for (std::size_t b = 1; b + 1U < seed_off.size(); ++b) {
+      auto const n_rows = static_cast<std::uint32_t>(std::min(rows_per, rows.size() - (b - 1U) * rows_per));
+      a.dist = W == 64U ? e.d_ph_rows : e.st.lb;
+      a.sel = e.d_ph_sel + dense_sel.size();
       a.masks = nullptr;
-      a.n_sel = rows;
+      a.n_sel = n_rows;
       a.seed_idx = e.d_ph_seed_idx + seed_off[b];
       a.seed_min = e.d_ph_seed_min + seed_off[b];
       a.n_seeds = seed_off[b + 1U] - seed_off[b];
       launch_lb_phast(1U, a, sw.level_off.data(), sw.host_levels, n_levels, stream);
-      if (W == 64U) launch_lb_scatter64(e.st, e.d_ph_rows, e.d_ph_slots + b * rows_per, rows, stream);
+      if (W == 64U) launch_lb_scatter64(e.st, e.d_ph_rows, e.d_ph_slots + (b - 1U) * rows_per, n_rows, stream);
       HIP_CHECK(hipGetLastError());
     }
     HIP_CHECK(hipStreamSynchronize(stream));  // host vectors go out of scope
@@ -1534,9 +1585,10 @@ int nigiri_amd_engine_create(nigiri_amd_image* img, uint32_t lanes, uint32_t n_g
     e->d_sweep_groups = m.alloc<std::uint32_t>(G + 1U);
     e->d_slot_list = m.alloc<std::uint32_t>(S);
     e->d_lb_row = m.alloc<std::uint16_t>(L);
-    e->d_ph_sel = m.alloc<std::uint32_t>(S);
+    e->d_ph_sel = m.alloc<std::uint32_t>(static_cast<std::size_t>(S) + G);
     if (lanes == 64U) {
       e->d_ph_slots = m.alloc<std::uint32_t>(S);
+      e->d_ph_masks = m.alloc<std::uint64_t>(G);
       e->d_ph_rows = m.alloc<std::uint16_t>(static_cast<std::size_t>(std::min<std::uint32_t>(S, kPhastRows)) * L);
     }
     e->h_active.reserve(G);

@@ -2223,16 +2223,13 @@ __global__ void __launch_bounds__(64) k_fastest_direct(fd_args a) {
 }
 
 // PHAST rows [n_rows][L] into the lower-bound lanes of their slots
-// (batched engine)
-__global__ void k_lb_scatter64(dev_state s, std::uint16_t const* rows, std::uint32_t const* slots,
-                               std::uint32_t n_rows) {
-  auto const stride = gridDim.x * blockDim.x;
-  for (std::uint32_t r = 0U; r < n_rows; ++r) {  // uniform
-    auto const slot = slots[r];
-    auto const* row = rows + static_cast<std::size_t>(r) * s.L;
-    for (std::uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < s.L; l += stride) {
-      s.lb[lay<64>::col(s, slot / 64U, l, slot % 64U)] = row[l];
-    }
+// (batched engine); row blockIdx.y
+__global__ void k_lb_scatter64(dev_state s, std::uint16_t const* rows, std::uint32_t const* slots) {
+  auto const r = blockIdx.y;
+  auto const slot = slots[r];
+  auto const* row = rows + static_cast<std::size_t>(r) * s.L;
+  for (std::uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < s.L; l += gridDim.x * blockDim.x) {
+    s.lb[lay<64>::col(s, slot / 64U, l, slot % 64U)] = row[l];
   }
 }
 
@@ -2811,8 +2808,9 @@ void launch_fastest_direct(fd_args const& a, hipStream_t stream) {
 
 void launch_lb_scatter64(dev_state const& st, std::uint16_t const* rows, std::uint32_t const* slots,
                          std::uint32_t n_rows, hipStream_t stream) {
-  hipLaunchKernelGGL(k_lb_scatter64, dim3(grid_for(st.L, kBlock, 2048)), dim3(kBlock), 0, stream, st, rows, slots,
-                     n_rows);
+  if (n_rows == 0U) return;
+  hipLaunchKernelGGL(k_lb_scatter64, dim3(grid_for(st.L, kBlock, 64), n_rows), dim3(kBlock), 0, stream, st, rows,
+                     slots);
 }
 
 void launch_lb_put(std::uint32_t W, dev_state const& st, std::uint32_t slot, std::uint16_t const* row,

@@ -4,8 +4,9 @@
 // with -fsanitize=address,undefined, run on generated workloads and on a
 // GTFS feed.  No GPU code is built or run.  Besides running clean under
 // the sanitizers, it checks that the host's contraction-hierarchy lower
-// bounds equal the oracle's Dijkstra and that query preparation with host
-// and device-style lower bounds agrees.
+// bounds equal the oracle's Dijkstra, that query preparation with host
+// and device-style lower bounds agrees, and that the host's fastest direct
+// walk equals the oracle's.
 //
 // usage: sanitize_check [gtfs_dir date_from_day date_to_day stop_from stop_to]
 #include <cstdio>
@@ -84,6 +85,17 @@ void run_queries(nigiri_amd_tt* tt, int cfg, std::vector<nigiri_query> const& qs
       continue;
     }
     if (p_host.lb_ != std::vector<std::uint16_t>(lb.begin(), lb.end())) fail("host lb != oracle lb", cfg, i);
+    // fastest direct walk: the host function against the oracle, and the
+    // node sets the device kernel gets
+    std::int16_t fd_ref = 0;
+    if (oracle_fastest_direct(&v, &q, &fd_ref) != 0 || fd_ref != p_host.fastest_direct_) {
+      fail("fastest_direct != oracle", cfg, i);
+    }
+    std::vector<nigiri_amd::fd_node_host> fd_nodes;
+    nigiri_amd::fastest_direct_nodes(tt->tt, q, fd_nodes);
+    for (std::size_t j = 1; j < fd_nodes.size(); ++j) {
+      if (fd_nodes[j - 1U].loc >= fd_nodes[j].loc) fail("fastest_direct_nodes order", cfg, i);
+    }
     if (p_host.epochs_.size() != p_dev.epochs_.size() || p_host.lb_seeds_ != p_dev.lb_seeds_) {
       fail("prepare host / device-style", cfg, i);
     }

@@ -554,7 +554,8 @@ def _collect(res, jr, n, lg=None):
             if lg is not None:
                 legs.append([lg[j.leg_offset + y].as_tuple() for y in range(j.n_legs)])
         out.append({"status": r.status, "journeys": js, "legs": legs, "n_epochs": r.n_epochs,
-                    "base_day": r.base_day, "fastest_direct": r.fastest_direct, "slot": r.slot,
+                    "n_rounds": r.n_rounds, "base_day": r.base_day, "fastest_direct": r.fastest_direct,
+                    "slot": r.slot,
                     "latency_ms": r.latency_ms, "queue_ms": r.queue_ms,
                     "interval_from": r.interval_from, "interval_to": r.interval_to})
     return out

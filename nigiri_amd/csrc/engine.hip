@@ -778,6 +778,7 @@ struct runner {
     for (std::uint32_t s = 0; s != S; ++s) {
       if (!slots[s].running) continue;
       any = true;
+      ++pq[static_cast<std::size_t>(slots[s].q)].rounds_;
       // after its first epoch a slot sees earlier rows in round_times
       if (slots[s].epoch != 0U) sweep[s / W] = 1U;
     }
@@ -1121,7 +1122,7 @@ std::uint32_t write_results(std::vector<prepared_query> const& pq, std::vector<i
     r.interval_to = p.iv_to_;
     r.n_epochs = static_cast<std::uint32_t>(p.epochs_.size());
     r.slot = p.slot_;
-    if (epochs != nullptr) r.n_rounds = (*epochs)[i];
+    r.n_rounds = epochs != nullptr ? (*epochs)[i] : p.rounds_;
     if (latency != nullptr) r.latency_ms = (*latency)[i];
     if (queue != nullptr) r.queue_ms = (*queue)[i];
     for (auto const& j : p.results_) {

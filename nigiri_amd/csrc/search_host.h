@@ -103,6 +103,7 @@ struct prepared_query {
   std::vector<epoch> epochs_;
   std::uint64_t locality_key_{0U};  // Morton code of the first start stop
   std::int32_t slot_{-1};           // engine slot that ran the query
+  std::uint32_t rounds_{0U};        // rounds the engine ran for it, over its epochs
   std::uint64_t state_bytes_{0U};   // device state charged at admission
   std::vector<host_journey> results_;  // running Pareto set
 

@@ -43,6 +43,7 @@ def compare(tt, eng, qs, idx=None, check_state=True):
         assert got["journeys"] == ref["journeys"], (i, got["journeys"], ref["journeys"])
         assert got["legs"] == ref["legs"], (i, got["legs"], ref["legs"])
         assert got["base_day"] == ref["base_day"]
+        assert got["n_rounds"] == ref["stats"]["n_rounds"], (i, got["n_rounds"], ref["stats"]["n_rounds"])
         if check_state and len(qs) <= eng.slots:
             st = eng.download(got["slot"])
             K = eng.max_k

@@ -76,6 +76,11 @@ struct dev_image {
   std::uint64_t const* bitfields;
   std::uint32_t const* location_route_offsets;
   std::uint32_t const* location_routes;
+  // work order of the batched route phase: routes sorted by the position of
+  // their stops (route_work_order), the rank of each location_routes entry's
+  // route in it (lr_rank) and the route of each rank (rank_route)
+  std::uint32_t const* lr_rank;
+  std::uint32_t const* rank_route;
   std::uint8_t const* transfer_time;
   std::uint32_t const* fp_out_offsets;
   std::uint32_t const* fp_out;

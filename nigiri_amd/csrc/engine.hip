@@ -20,6 +20,7 @@
 #include <cstring>
 #include <deque>
 #include <functional>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -322,6 +323,48 @@ namespace nigiri_amd {
 
 namespace {
 
+// Work order of the batched route phase: routes sorted by the Morton code of
+// the mean position of their stops (normalised to the bounding box of the
+// locations), so that neighbouring entries of the route scan's work list
+// visit one neighbourhood and read the same state lines while they are in
+// the caches.  Without coordinates routes keep their index order.  The order
+// changes no result: the route phase's updates are order-independent.
+std::vector<std::uint32_t> route_work_order(timetable const& tt) {
+  auto const R = tt.n_routes();
+  std::vector<std::uint32_t> order(R);
+  for (std::uint32_t r = 0U; r != R; ++r) order[r] = r;
+  if (tt.coord_x.size() != tt.n_locations || tt.coord_y.size() != tt.n_locations) return order;
+  auto x0 = std::numeric_limits<double>::infinity(), y0 = x0, x1 = -x0, y1 = -x0;
+  for (auto l = kNSpecialStations; l < tt.n_locations; ++l) {
+    x0 = std::min(x0, static_cast<double>(tt.coord_x[l]));
+    x1 = std::max(x1, static_cast<double>(tt.coord_x[l]));
+    y0 = std::min(y0, static_cast<double>(tt.coord_y[l]));
+    y1 = std::max(y1, static_cast<double>(tt.coord_y[l]));
+  }
+  if (!(x1 > x0) || !(y1 > y0)) return order;
+  auto const cell = [](double v, double lo, double hi) {
+    return static_cast<std::uint32_t>(std::clamp((v - lo) / (hi - lo), 0.0, 1.0) * 65535.0);
+  };
+  std::vector<std::uint64_t> key(R, 0U);
+  for (std::uint32_t r = 0U; r != R; ++r) {
+    auto const s0 = tt.route_stop_offsets[r], s1 = tt.route_stop_offsets[r + 1U];
+    if (s1 == s0) continue;
+    double mx = 0.0, my = 0.0;
+    for (auto i = s0; i != s1; ++i) {
+      auto const l = tt.route_stops[i] & kStopLocMask;
+      mx += tt.coord_x[l];
+      my += tt.coord_y[l];
+    }
+    auto const cx = cell(mx / (s1 - s0), x0, x1), cy = cell(my / (s1 - s0), y0, y1);
+    for (int b = 0; b != 16; ++b) {
+      key[r] |= static_cast<std::uint64_t>((cx >> b) & 1U) << (2 * b);
+      key[r] |= static_cast<std::uint64_t>((cy >> b) & 1U) << (2 * b + 1);
+    }
+  }
+  std::stable_sort(order.begin(), order.end(), [&](std::uint32_t a, std::uint32_t b) { return key[a] < key[b]; });
+  return order;
+}
+
 void build_image(nigiri_amd_image& img, timetable const& tt, hipStream_t s) {
   auto& d = img.d;
   auto& m = img.mem;
@@ -339,6 +382,15 @@ void build_image(nigiri_amd_image& img, timetable const& tt, hipStream_t s) {
   d.bitfields = m.upload(tt.bitfields, s);
   d.location_route_offsets = m.upload(tt.location_route_offsets, s);
   d.location_routes = m.upload(tt.location_routes, s);
+  {
+    auto const order = route_work_order(tt);
+    std::vector<std::uint32_t> rank(order.size()), lr_rank(tt.location_routes.size());
+    for (std::uint32_t i = 0U; i != order.size(); ++i) rank[order[i]] = i;
+    for (std::size_t j = 0U; j != lr_rank.size(); ++j) lr_rank[j] = rank[tt.location_routes[j]];
+    d.rank_route = m.upload(order, s);
+    d.lr_rank = m.upload(lr_rank, s);
+    HIP_CHECK(hipStreamSynchronize(s));  // the host vectors go out of scope
+  }
   d.transfer_time = m.upload(tt.transfer_time, s);
   d.fp_out_offsets = m.upload(tt.fp_out_offsets, s);
   d.fp_out = m.upload(tt.fp_out, s);

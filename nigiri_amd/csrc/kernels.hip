@@ -368,8 +368,10 @@ __global__ void k_mark_routes(round_args a) {
   auto* overflow = &s.counts[4];
   auto const n_items = clamp_count(s.counts[0], s.cap_t);
   if constexpr (W == 64) {
-    // flag_route[r][g] = 1 for every route of a marked station (plain
-    // stores); route_pack rebuilds the lane masks from the station marks.
+    // flag_route[rank][g] = 1 for every route of a marked station (plain
+    // stores; rank: the route's place in the image's work order, so that
+    // the compacted list runs through the routes in that order);
+    // route_pack rebuilds the lane masks from the station marks.
     // A station serves a few routes: 8 lanes per (group, station) item,
     // 8 items per wavefront.
     constexpr std::uint32_t kSub = 8U;
@@ -391,9 +393,9 @@ __global__ void k_mark_routes(round_args a) {
         c_st += nq;
         c_lr += nq * (r1 - r0);
       }
-      // route-major flags: [route][group]
+      // route-major flags: [rank][group]
       for (auto j = r0 + sl; j < r1; j += kSub) {
-        s.flag_route[static_cast<std::size_t>(im.location_routes[j]) * s.G + g] = 1U;
+        s.flag_route[static_cast<std::size_t>(im.lr_rank[j]) * s.G + g] = 1U;
       }
       if (r1 != r0) {
         for (auto b = sl; b < 64U; b += kSub) {
@@ -1030,7 +1032,7 @@ __global__ void k_route_pack(round_args a) {
     std::uint64_t mask = 0ULL;
     if (item < n_items) {
       auto const e = s.list_route[item];
-      r = static_cast<std::uint32_t>(e >> 32);
+      r = im.rank_route[static_cast<std::uint32_t>(e >> 32)];
       g = static_cast<std::uint32_t>(e);
       auto const act = s.grp_active[g];
       if (act != 0ULL) {

@@ -650,6 +650,7 @@ constexpr std::uint32_t kChunkEvBytes = 1024U;
 // stop words staged per window (one per lane); a route scan starts at the
 // scan index its packing found (16 bits in the packed item encodings)
 constexpr std::uint32_t kStopWindow = 64U;
+constexpr std::uint32_t kMarkBatch = 16U;  // mark words in flight per lane
 constexpr std::uint32_t kMaxScanStart = 0xFFFFU;
 static_assert(kChunkEvBytes / 4U >= kBatchedMaxRouteTransports, "a chunk holds at least one stop");
 static_assert(kStagedTransports * 8U * 8U >= kBatchedMaxRouteTransports * 4U,
@@ -729,6 +730,26 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
   stage_stops(ws);
 
   et_t<Fwd> et{~0U, 0xFFFFU};
+  // this lane's station marks at the stops of the staged window, bit i for
+  // scan index ws + i: loaded for the whole window at once, kMarkBatch stops
+  // in flight, so that a chunk knows its marks before it loads anything
+  std::uint64_t win_marks = 0ULL;
+  auto const load_marks = [&](bool live) {
+    win_marks = 0ULL;
+    if (!live) return;
+    auto const n_win = min(kStopWindow, n_stops - ws);
+    for (std::uint32_t i0 = 0U; i0 < n_win; i0 += kMarkBatch) {
+      std::uint32_t w[kMarkBatch];
+#pragma unroll
+      for (std::uint32_t j = 0U; j < kMarkBatch; ++j) {
+        w[j] = i0 + j < n_win ? mark_t32[mrow + 2U * (x.stop[i0 + j] & kLocMask)] : 0U;
+      }
+#pragma unroll
+      for (std::uint32_t j = 0U; j < kMarkBatch; ++j) {
+        win_marks |= static_cast<std::uint64_t>((w[j] & mbit) != 0U ? 1U : 0U) << (i0 + j);
+      }
+    }
+  };
   // `started`: one of the route's stops is marked for this query.  Before
   // its first marked stop a query has no trip (boarding needs a mark), so
   // nothing happens; its visited stops count only if the route was marked
@@ -738,6 +759,8 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
   bool done = !q.on;
   started = false;
   std::uint32_t my_stops = done ? 0U : start;
+  wave_sync_lds();  // the window's stop words
+  load_marks(!done);
   for (std::uint32_t c0 = start; c0 < n_stops; c0 += chunk) {
     auto const nc = min(chunk, n_stops - c0);
     wave_sync_lds();  // readers of the previous chunk are done
@@ -745,37 +768,24 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
       ws = c0;
       stage_stops(ws);
       wave_sync_lds();
+      load_marks(!done);
     }
     auto const* cstop = x.stop + (c0 - ws);
-    // this lane's station marks at the chunk's stops, with its lower bounds
-    // there if it rides into the chunk (a rider needs them at every stop),
-    // and the chunk's event blocks, in flight together; a lane that does
-    // not ride needs lower bounds only from its first marked stop on, and
-    // round_times[k-1] only at marked stops: those are loaded once the
-    // marks are known (and not at all for a chunk that is skipped)
-    bool const rides_in = !done && et.valid();
-    std::uint16_t lbr[kChunkMax];
-    std::int16_t rtr[kChunkMax];
-    std::uint32_t mkr[kChunkMax];
-#pragma unroll
-    for (std::uint32_t j = 0U; j < kChunkMax; ++j) {
-      lbr[j] = 0U;
-      rtr[j] = D::kInvalid;
-      mkr[j] = 0U;
-      if (j < nc && !done) {
-        auto const l = cstop[j] & kLocMask;
-        mkr[j] = mark_t32[mrow + 2U * l];
-        if (rides_in) lbr[j] = s.lb[col0 + l * 64U];
-      }
+    // bit j: this query is marked at chunk stop j
+    auto const mbits = done ? 0U : static_cast<std::uint32_t>(win_marks >> (c0 - ws)) & ((1U << nc) - 1U);
+    started = started || mbits != 0U;
+    // A chunk in which no live lane rides and no live lane is marked changes
+    // nothing: rides need a trip, boarding and the lower-bound break need a
+    // mark.  Such a chunk only counts its stops.
+    if (__ballot(!done && (et.valid() || mbits != 0U)) == 0ULL) {
+      my_stops += done ? 0U : nc;
+      continue;
     }
     // event blocks: for chunk stop j, [arr at stop_idx | dep at stop_idx];
     // the arrival row of stop_idx directly precedes its departure row in the
     // route's event array (stop-pair layout), so both rows are one run of
     // 2 n_t events starting at n_t (2 stop_idx - 1); a missing row (no
-    // arrival at the first stop, no departure at the last) is zeroed.  While
-    // a lane rides, the chunk is scanned for sure and its events are loaded
-    // with the lane values; otherwise only once a mark shows the chunk is
-    // not skipped.
+    // arrival at the first stop, no departure at the last) is zeroed.
     auto const stage_events = [&]() {
       for (std::uint32_t j = 0U; j < nc; ++j) {
         auto const si = c0 + j;
@@ -789,29 +799,23 @@ __device__ inline void scan_route64(wave_lds& x, dev_image const& im, dev_state 
         }
       }
     };
-    bool const any_rider = __ballot(!done && et.valid()) != 0ULL;
-    if (any_rider) stage_events();
-    std::uint32_t mbits = 0U;  // bit j: this query is marked at chunk stop j
-#pragma unroll
-    for (std::uint32_t j = 0U; j < kChunkMax; ++j) {
-      mbits |= ((mkr[j] & mbit) != 0U ? 1U : 0U) << j;
-    }
-    started = started || mbits != 0U;
-    // A chunk in which no live lane rides and no live lane is marked changes
-    // nothing: rides need a trip, boarding and the lower-bound break need a
-    // mark.  Such a chunk only counts its stops.
-    if (__ballot(!done && (et.valid() || mbits != 0U)) == 0ULL) {
-      my_stops += done ? 0U : nc;
-      continue;
-    }
-    if (!any_rider) stage_events();
+    // the lane values of the chunk and its event blocks, in flight together:
+    // a lane that rides into the chunk needs lower bounds at every stop, one
+    // that does not only from its first marked stop on, and round_times[k-1]
+    // only at marked stops
+    stage_events();
+    bool const rides_in = !done && et.valid();
+    std::uint16_t lbr[kChunkMax];
+    std::int16_t rtr[kChunkMax];
     {
       auto const first = mbits != 0U ? static_cast<std::uint32_t>(__builtin_ctz(mbits)) : kChunkMax;
 #pragma unroll
       for (std::uint32_t j = 0U; j < kChunkMax; ++j) {
+        lbr[j] = 0U;
+        rtr[j] = D::kInvalid;
         if (j < nc && !done) {
           auto const l = cstop[j] & kLocMask;
-          if (!rides_in && j >= first) lbr[j] = s.lb[col0 + l * 64U];
+          if (rides_in || j >= first) lbr[j] = s.lb[col0 + l * 64U];
           if (((mbits >> j) & 1U) != 0U) rtr[j] = rt_q[static_cast<std::size_t>(l) * 64U];
         }
       }
